@@ -1,0 +1,53 @@
+/* Status codes, tensor kinds and contraction flags shared by the
+ * sparsebackends_amd C ABI (include/sparsebackends_amd.h).
+ *
+ * The reference raises Julia exceptions; each exception class it raises on
+ * the contraction path maps to one status code here, so a caller can tell
+ * the classes apart exactly as a Julia `@test_throws` would:
+ *   AssertionError  (@assert, e.g. SparseBackends/src/blocksparse/storage.jl:126-132)
+ *   ErrorException  (error(), e.g. tensoralgebra/contract_bs_bs.jl:603,621)
+ *   ArgumentError   (e.g. blocksparse/conversions.jl:232-234, coo/storage.jl:141-142)
+ *   KeyError        (Dict lookup of a missing label, e.g. contract.jl:100 mapB[rlab])
+ *   MethodError     (no kernel method for the operand kinds, contract.jl:100,102)
+ *   BoundsError     (storage.jl:28, coo/storage.jl:26)
+ */
+#ifndef SB_STATUS_H
+#define SB_STATUS_H
+
+#define SB_OK            0
+#define SB_E_ASSERT      1  /* AssertionError */
+#define SB_E_ERROR       2  /* ErrorException */
+#define SB_E_ARGUMENT    3  /* ArgumentError */
+#define SB_E_KEY         4  /* KeyError */
+#define SB_E_METHOD      5  /* MethodError: no kernel for these operand kinds */
+#define SB_E_BOUNDS      6  /* BoundsError */
+#define SB_E_DEVICE      7  /* HIP runtime / launch failure */
+#define SB_E_NOMEM       8  /* device or host allocation failed */
+#define SB_E_HANDLE      9  /* null handle or handle of the wrong kind */
+#define SB_E_LIMIT      10  /* a size beyond what the library supports */
+
+/* tensor kinds (the reference's storage types) */
+#define SB_KIND_BLOCKSPARSE 1  /* NewBlockSparseSorted  (blocksparse/storage.jl:3-9) */
+#define SB_KIND_COO         2  /* COOTensor             (coo/storage.jl:1-6) */
+#define SB_KIND_DENSE       3  /* plain Array (column-major) */
+#define SB_KIND_ALIASED     4  /* AliasedBlockSparse    (aliased/storage.jl:46-59) */
+
+/* contraction flags.  With flags == 0 every kernel follows the reference
+ * code, including its documented divergences from the dense mathematics. */
+/* multi-label BS x BS merge-join: join on the order the keys are sorted in,
+ * instead of the reference's first-axis-first comparison
+ * (tensoralgebra/contract_bs_bs.jl:430-442,687-699), which can skip
+ * matches when two or more shared labels sit in the key prefix. */
+#define SB_FLAG_EXACT_JOIN        (1u << 0)
+/* COO x Dense: create no output block whose contribution is all zero
+ * (the reference creates one for every entry, contract_coo_dense.jl:113-122
+ * in the included file; its test coo_dense.jl:164-189 expects none). */
+#define SB_FLAG_SKIP_ZERO_SLICES  (1u << 1)
+
+/* precision form of the block GEMMs (bits 8..11).  Storage stays float64. */
+#define SB_PREC_MASK    (0xFu << 8)
+#define SB_PREC_FP64    (0u << 8)   /* FP64 MFMA, the drop-in */
+#define SB_PREC_FP32    (1u << 8)   /* operands rounded to FP32, FP32 MFMA, FP32 accumulate */
+#define SB_PREC_BF16X3  (2u << 8)   /* operands split in three bf16 terms, FP32 accumulate */
+
+#endif
