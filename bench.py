@@ -1,0 +1,208 @@
+"""Benchmark: two-site H_eff*psi on one MI355X.
+
+Workload (default): the Hubbard chain of the precision configuration
+(t=1, U=4, local dim 4, U(1)xU(1) charges (n, 2Sz), MPO bond dim 6,
+chi=2048 in 520 sectors sized by a 2D Gaussian of width 7), one two-site
+H_eff*psi = four contract! calls on NewBlockSparseSorted tensors
+(psi*L*W1*W2*R).  Synthetic data: random psi / environments on the
+charge-allowed entries, seed 0; the MPO is the Hubbard MPO.
+
+A step is one H_eff*psi: every step recomputes all four contractions
+(operand permutes + grouped block products) from the inputs resident in
+HBM.  The block structure is planned once (sb_plan_create), as a Krylov
+solve keeps it fixed; the planning time is reported separately.
+
+value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
+block pairs with the actual sector extents) / time per step, in TFLOP/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c1]
+                       [--prec fp64|fp32|bf16x3] [--no-cpu] [--calibrate-copy MB]
+"""
+from __future__ import annotations
+
+import argparse
+import copy
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 6.0}  # dense MFMA TFLOP/s (bf16x3: 6 bf16 products)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c4", choices=["c4", "c1"])
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="also run one plain device copy of this many MB (rocprof traffic calibration)")
+    return ap.parse_args()
+
+
+def make_problem(cfg):
+    from sparsebackends_amd import heff as H
+    return H.config4_problem(0) if cfg == "c4" else H.config1_problem(0)
+
+
+def rel_err(x, y):
+    n = np.linalg.norm(y)
+    return float(np.linalg.norm(x - y) / (n if n else 1.0))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (this build measures one GPU's share; "
+              "see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+    import sparsebackends_amd as sb
+
+    flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec]
+    t0 = time.perf_counter()
+    prob = make_problem(a.config)
+    t_gen = time.perf_counter() - t0
+    alg = prob.algorithmic_flops()["total"]
+    pad = prob.padded_flops()["total"]
+    prob.upload().outputs()
+    sb.synchronize()
+    t0 = time.perf_counter()
+    plans = prob.make_plans(flags)
+    sb.synchronize()
+    t_plan = time.perf_counter() - t0
+    stats = [p.stats() for p in plans]
+    opb = [p.operand_bytes() for p in plans]
+    for _ in range(a.warmup):
+        prob.execute()
+    sb.synchronize()
+    sb.profile(True)
+    sb.profile_reset()
+    sb.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        prob.execute()
+    sb.synchronize()
+    dt = time.perf_counter() - t0
+    prof = sb.profile_read()
+    sb.profile(False)
+    ms_step = 1e3 * dt / a.steps
+    value = alg / (dt / a.steps) / 1e12
+
+    # dominant kernel: the kernel name with the largest total time
+    dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
+    dom_name, dom_rec = dom
+    launches_per_step = dom_rec["count"] / a.steps
+    avg_ms = dom_rec["total_ms"] / dom_rec["count"]
+    if dom_name == "grouped_gemm":
+        # per launch: the algorithmic flops of the four block-product launches, averaged
+        gemm_alg = alg / launches_per_step
+        ai = alg / sum(o["a"] + o["b"] + o["c"] for o in opb)
+        algo_bytes = sum(o["a"] + o["b"] + o["c"] for o in opb) / launches_per_step
+        if ai * HBM_PEAK_GBS / 1e3 < PEAK[a.prec]:
+            roof = dict(bound="hbm", achieved=algo_bytes / (avg_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s")
+        else:
+            roof = dict(bound="mfma", achieved=gemm_alg / (avg_ms * 1e-3) / 1e12, peak=PEAK[a.prec], unit="TFLOP/s")
+        roof["note"] = (f"average over the {int(launches_per_step)} grouped_gemm launches of a step; "
+                        f"arithmetic intensity {ai:.2f} FLOP/B (algorithmic flops / distinct operand bytes)")
+    else:
+        perm_bytes = sum(s["perm_bytes"] for s in stats) / max(1.0, launches_per_step)
+        roof = dict(bound="hbm", achieved=perm_bytes / (avg_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                    note=f"average over the {int(launches_per_step)} {dom_name} launches of a step")
+    roof["kernel"] = dom_name
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["avg_launch_ms"] = avg_ms
+
+    out = {
+        "metric": "H_eff·ψ TFLOP/s FP64 vs mixed at |ΔE|≤1e-10; DMRG solves/s + p99 latency",
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 (f64 storage)", "bf16x3": "bf16x3->f32 (f64 storage)"}[a.prec],
+        "data": "synthetic: random psi/L/R on the charge-allowed entries (seed 0), exact model MPO",
+        "config": {
+            "workload": ("two-site H_eff*psi, Hubbard t=1 U=4, d=4, U(1)xU(1), MPO w=6, chi=2048 in "
+                         f"{prob.Bl.nsec}+{prob.Br.nsec} sectors (2D Gaussian sigma=7)") if a.config == "c4" else
+            "two-site H_eff*psi, Heisenberg J=1, U(1) Sz, MPO w=5, chi=256 in 17 sectors",
+            "format": "sector + in-sector axes, dense tail padded to the largest sector",
+            "psi_blocks": int(len(prob.psi_parts["ids"])),
+            "algorithmic_gflop_per_step": alg / 1e9,
+            "padded_gflop_per_step": pad / 1e9,
+            "block_pairs_per_step": int(sum(s["npairs"] for s in stats)),
+            "plan_s": t_plan,
+            "generate_s": t_gen,
+        },
+        "kernels": {k: dict(v, per_step_ms=v["total_ms"] / a.steps) for k, v in prof.items()},
+        "roofline": roof,
+    }
+
+    if not a.no_mixed and a.prec == "fp64":
+        # the precision forms on the same problem, with their error vs FP64
+        ref = prob.Hpsi.parts()["data"]
+        mixed = {}
+        for name, fl in (("fp32", sb.PREC_FP32), ("bf16x3", sb.PREC_BF16X3)):
+            pm = copy.copy(prob).upload().outputs()  # same host data, fresh device tensors
+            pm.make_plans(fl)
+            pm.execute()
+            sb.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(max(3, a.steps // 2)):
+                pm.execute()
+            sb.synchronize()
+            d2 = (time.perf_counter() - t0) / max(3, a.steps // 2)
+            mixed[name] = dict(tflops=alg / d2 / 1e12, ms_per_step=d2 * 1e3,
+                               rel_err_vs_fp64=rel_err(pm.Hpsi.parts()["data"], ref))
+            del pm
+        out["mixed"] = mixed
+
+    if not a.no_cpu:
+        # the oracle (reference algorithm restated in C++, OpenBLAS 0.3.29 one
+        # thread) on one full H_eff*psi of the same problem: ~10-20 s of CPU work
+        from oracle import oracle as O
+        from oracle.heff_oracle import HeffOracle
+        O.use_blas(True, threads=1)
+        h = HeffOracle(prob)
+        t0 = time.perf_counter()
+        hp = h.run()
+        tc = time.perf_counter() - t0
+        cpu = dict(value=alg / tc / 1e12, unit="TFLOP/s", cores=1, kind="port",
+                   sample="one full two-site H_eff*psi of the same workload (4 contract! calls), "
+                          "oracle C++ restatement with OpenBLAS 0.3.29 dgemm on 1 thread",
+                   seconds=tc)
+        q = hp.parts()
+        p = prob.Hpsi.parts()
+        if np.array_equal(p["keys"], q["keys"]) and np.array_equal(p["ids"], q["ids"]):
+            cpu["gpu_vs_cpu_rel_err"] = rel_err(p["data"], q["data"])
+        else:
+            cpu["gpu_vs_cpu_rel_err"] = "block lists differ"
+        out["cpu_baseline"] = cpu
+
+    if a.calibrate_copy > 0:
+        n = int(a.calibrate_copy * 1e6 / 8)
+        T = sb.DenseTensor.from_numpy(np.zeros(n))
+        sb.synchronize()
+        T.permutedims([1])  # identity permute = one plain copy kernel of n*8 bytes each way
+        sb.synchronize()
+        out["calibration_copy_bytes"] = n * 8
+
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,164 @@
+/* sparsebackends_amd: block-sparse tensor contraction for MI355X (gfx950).
+ *
+ * C ABI of libsparsebackends_amd.so.  It replaces the storage types and the
+ * contract/permute entry points of the reference backend
+ * (Ritvik1sharma/SparseBackends, Julia), so a Julia `ccall` (or any FFI) can
+ * drive it from the reference's ITensors boundary (wrapped_contract,
+ * SparseBackends/src/tensor_wrappers.jl:614-676).
+ *
+ * Conventions (those of the reference, kept so results compare bit for bit):
+ *  - tensors are device-resident; handles own their device memory;
+ *  - keys and ids are 1-based; a block-sparse key is the tuple of the first
+ *    P = N - N2 coordinates, passed row-wise (nblocks x P int64);
+ *  - keys of a NewBlockSparseSorted are sorted by the column-major linear
+ *    index of the prefix (axis 1 fastest); ids are payload slots;
+ *  - payloads and dense arrays are float64, column-major; block `id` lives at
+ *    data[(id-1)*blksize .. id*blksize);
+ *  - a label is (id, plev) as two uint64 (tensor_wrappers.jl:577-580),
+ *    so label arrays hold 2*rank uint64;
+ *  - every call returns SB_OK or one status of sb_status.h, and
+ *    sb_last_error() describes the last failure of the calling thread.
+ * All work is ordered on the context's stream; a call that returns host data
+ * synchronises that stream.  One context per host thread; calls on one
+ * context must not overlap.
+ */
+#ifndef SPARSEBACKENDS_AMD_H
+#define SPARSEBACKENDS_AMD_H
+
+#include <stdint.h>
+
+#include "sb_status.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sb_ctx_s* sb_ctx_t;      /* device + stream + device-memory cache */
+typedef struct sb_tensor_s* sb_tensor_t; /* a device-resident tensor of one storage kind */
+typedef struct sb_plan_s* sb_plan_t;     /* a contraction with its block structure fixed */
+
+/* ------------------------------------------------------------ runtime */
+const char* sb_last_error(void);
+int sb_version(void);
+/* New: the reference runs in one Julia process on the CPU; a context binds a
+ * HIP device and a stream (per-thread scratch of tensor_wrappers.jl:582-598). */
+int sb_ctx_create(int device, sb_ctx_t* out);
+int sb_ctx_destroy(sb_ctx_t ctx);
+/* run on a caller's hipStream_t (NULL: the context's own stream) */
+int sb_ctx_set_stream(sb_ctx_t ctx, void* hip_stream);
+int sb_ctx_synchronize(sb_ctx_t ctx);
+/* device bytes held by live tensors / kept in the cache for reuse */
+int sb_ctx_memory(sb_ctx_t ctx, int64_t* in_use, int64_t* cached);
+int sb_ctx_trim(sb_ctx_t ctx);
+/* New (the reference's only tracing is the SB_TRACE prints, e.g.
+ * contract_bs_dense.jl:112-114): time every numeric step of plan executions
+ * with HIP events on the context's stream, aggregated per kernel name */
+int sb_ctx_profile(sb_ctx_t ctx, int enable);
+int sb_ctx_profile_reset(sb_ctx_t ctx);
+/* synchronises; writes lines "name count total_ms min_ms max_ms" into buf
+ * (NUL-terminated, at most cap bytes); *needed = bytes the full text takes */
+int sb_ctx_profile_read(sb_ctx_t ctx, char* buf, int64_t cap, int64_t* needed);
+
+int sb_tensor_free(sb_tensor_t t);
+int sb_tensor_kind(sb_tensor_t t, int* kind);
+
+/* ------------------------------------------------------------ NewBlockSparseSorted */
+/* NewBlockSparseSorted{T,N,N2}(dims) (blocksparse/storage.jl:86-93): no blocks */
+int sb_bs_new(sb_ctx_t ctx, int N, int N2, const int64_t* dims, sb_tensor_t* out);
+/* NewBlockSparseSorted{T,N,N2,P}(dims, blksize, keys, ids, data) (storage.jl:3-13):
+ * host arrays uploaded as given (keys nblocks x P, ids nblocks, data nblocks*blksize) */
+int sb_bs_from_host(sb_ctx_t ctx, int N, int N2, const int64_t* dims, int64_t nblocks, const int64_t* keys,
+                    const int64_t* ids, const double* data, sb_tensor_t* out);
+int sb_bs_info(sb_tensor_t t, int* N, int* N2, int64_t* dims, int64_t* nblocks, int64_t* blksize);
+/* the fields keys/ids/data copied to host arrays */
+int sb_bs_to_host(sb_ctx_t ctx, sb_tensor_t t, int64_t* keys, int64_t* ids, double* data);
+/* device pointers of the fields (int32 keys/ids, float64 payload), valid until the tensor changes */
+int sb_bs_device_ptrs(sb_tensor_t t, const int32_t** keys, const int32_t** ids, double** data);
+/* getindex / setindex! (storage.jl:100-124); I holds N 1-based coordinates */
+int sb_bs_getindex(sb_ctx_t ctx, sb_tensor_t t, const int64_t* I, double* v);
+int sb_bs_setindex(sb_ctx_t ctx, sb_tensor_t t, double v, const int64_t* I);
+/* sort!(::NewBlockSparseSorted) (storage.jl:403-410): lexicographic keys, ids follow */
+int sb_bs_sort_lex(sb_ctx_t ctx, sb_tensor_t t);
+/* is_dense (storage.jl:413-433) */
+int sb_bs_is_dense(sb_tensor_t t, int* out);
+/* blocksparse_from_dense(A, Val(N2); atol, rtol, dropzeros) (blocksparse/conversions.jl:21-75) */
+int sb_bs_from_dense(sb_ctx_t ctx, sb_tensor_t dense, int N2, double atol, double rtol, int dropzeros,
+                     sb_tensor_t* out);
+/* to_dense(B; init, merged_axes) (conversions.jl:83-187); groups: ngroups pairs of 1-based axes */
+int sb_bs_to_dense(sb_ctx_t ctx, sb_tensor_t t, double init, int ngroups, const int64_t* groups, sb_tensor_t* out);
+/* fuse_two_axes! (conversions.jl:299-314): may permute t in place (fuse_dense_axes!) */
+int sb_bs_fuse_two_axes(sb_ctx_t ctx, sb_tensor_t t, int64_t ax1, int64_t ax2, sb_tensor_t* out);
+
+/* ------------------------------------------------------------ COOTensor */
+/* COOTensor{T,N}(dims) (coo/storage.jl:14-15) */
+int sb_coo_new(sb_ctx_t ctx, int N, const int64_t* dims, sb_tensor_t* out);
+/* COOTensor{T,N}(dims, keys, vals, dirty) (coo/storage.jl:10-12); keys nnz x N */
+int sb_coo_from_host(sb_ctx_t ctx, int N, const int64_t* dims, int64_t nnz, const int64_t* keys, const double* vals,
+                     int dirty, sb_tensor_t* out);
+int sb_coo_info(sb_tensor_t t, int* N, int64_t* dims, int64_t* nnz, int* dirty);
+int sb_coo_to_host(sb_ctx_t ctx, sb_tensor_t t, int64_t* keys, double* vals);
+int sb_coo_device_ptrs(sb_tensor_t t, const int32_t** keys, double** vals);
+/* getindex / setindex!(...; dirty) (coo/storage.jl:49-60, 84-134) */
+int sb_coo_getindex(sb_ctx_t ctx, sb_tensor_t t, const int64_t* I, double* v);
+int sb_coo_setindex(sb_ctx_t ctx, sb_tensor_t t, double v, const int64_t* I, int dirty);
+/* sort!(::COOTensor) = sort_coo! (coo/storage.jl:182-196) */
+int sb_coo_sort(sb_ctx_t ctx, sb_tensor_t t);
+/* coo_from_dense(A; atol, rtol) (coo/conversions.jl:13-26) and to_dense (:4-10) */
+int sb_coo_from_dense(sb_ctx_t ctx, sb_tensor_t dense, double atol, double rtol, sb_tensor_t* out);
+int sb_coo_to_dense(sb_ctx_t ctx, sb_tensor_t t, sb_tensor_t* out);
+/* fuse_axes_coo (coo/conversions.jl:174-223) */
+int sb_coo_fuse_axes(sb_ctx_t ctx, sb_tensor_t t, int64_t ax1, int64_t ax2, sb_tensor_t* out);
+
+/* ------------------------------------------------------------ dense arrays */
+int sb_dense_from_host(sb_ctx_t ctx, int N, const int64_t* dims, const double* data, sb_tensor_t* out);
+/* wrap (borrow != 0, caller keeps the memory alive) or copy caller device memory */
+int sb_dense_from_device(sb_ctx_t ctx, int N, const int64_t* dims, double* dptr, int borrow, sb_tensor_t* out);
+int sb_dense_info(sb_tensor_t t, int* N, int64_t* dims);
+int sb_dense_to_host(sb_ctx_t ctx, sb_tensor_t t, double* data);
+int sb_dense_device_ptr(sb_tensor_t t, double** data);
+
+/* ------------------------------------------------------------ permutes */
+/* Base.permutedims: NewBlockSparseSorted (storage.jl:188-264), COOTensor
+ * (coo/storage.jl:153-166, keys only, marks dirty), dense arrays; perm 1-based */
+int sb_permutedims(sb_ctx_t ctx, sb_tensor_t t, const int64_t* perm, sb_tensor_t* out);
+/* permutedims! (storage.jl:271-392; coo/storage.jl:168-180) */
+int sb_permutedims_inplace(sb_ctx_t ctx, sb_tensor_t t, const int64_t* perm);
+
+/* ------------------------------------------------------------ contraction */
+/* contract!(C, labelsC, A, labelsA, B, labelsB) (tensoralgebra/contract.jl:37-104).
+ * C was made by sb_bs_new / sb_coo_new with its dims and N2 and is overwritten.
+ * A and B are not modified.  Operand kinds follow the reference's methods:
+ * BS x BS, BS x Dense, COO x COO, COO x BS, COO x Dense (sparse operand first,
+ * as wrapped_contract arranges, tensor_wrappers.jl:561-573).  flags: sb_status.h. */
+int sb_contract(sb_ctx_t ctx, sb_tensor_t C, const uint64_t* labelsC, int nC, sb_tensor_t A, const uint64_t* labelsA,
+                int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags);
+/* contract_bs_bs_to_dense! (contract_bs_bs.jl:736-754): P_C = 0 sink into a dense C */
+int sb_contract_bs_bs_to_dense(sb_ctx_t ctx, sb_tensor_t Cdense, const uint64_t* labelsC, int nC, sb_tensor_t A,
+                               const uint64_t* labelsA, int nA, sb_tensor_t B, const uint64_t* labelsB, int nB,
+                               uint32_t flags);
+
+/* New: the reference recomputes the block structure on every call.  A plan
+ * runs sb_contract once (C gets its keys/ids and values) and keeps the numeric
+ * phase; sb_plan_execute recomputes C's values from the operands' current
+ * values, which must keep their block structure (same keys, ids and payload
+ * buffers).  For the families whose output blocks depend on values (rank-1 over a
+ * dense-tail index, COO products, the zero-slice flag) the block list of the
+ * first call is kept.  Plans of one context may be executed back to back and
+ * captured into a hipGraph by the caller. */
+int sb_plan_create(sb_ctx_t ctx, sb_tensor_t C, const uint64_t* labelsC, int nC, sb_tensor_t A,
+                   const uint64_t* labelsA, int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags,
+                   sb_plan_t* out);
+int sb_plan_execute(sb_ctx_t ctx, sb_plan_t plan);
+/* flops: sum of 2*M*K*N over contributing block pairs (actual extents);
+ * gemm_bytes / perm_bytes: algorithmic bytes of the block products / permutes */
+int sb_plan_stats(sb_plan_t plan, double* flops, double* gemm_bytes, double* perm_bytes, int64_t* npairs,
+                  int64_t* nblocks_out);
+/* distinct payload bytes of the block-product operands (A, B) and result C */
+int sb_plan_operand_bytes(sb_plan_t plan, double* a_bytes, double* b_bytes, double* c_bytes);
+int sb_plan_free(sb_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

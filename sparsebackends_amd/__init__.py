@@ -1,0 +1,17 @@
+"""sparsebackends_amd: the block-sparse contraction backend of
+Ritvik1sharma/SparseBackends rebuilt for MI355X (gfx950).
+
+Storage types and contract!/permutedims mirror the reference's Julia API
+(SparseBackends/src/SparseBackends.jl:4-8); all work runs in
+libsparsebackends_amd.so (HIP), reached through its C ABI.
+"""
+from ._abi import SBError, load, LIB_PATH
+from .storage import (NewBlockSparseSorted, COOTensor, DenseTensor, blocksparse_from_dense, coo_from_dense,
+                      to_dense, context, synchronize, profile, profile_reset, profile_read)
+from .tensoralgebra import (contract_, contract_bs_bs_to_dense_, Plan, labels_array, EXACT_JOIN, SKIP_ZERO_SLICES,
+                            PREC_FP64, PREC_FP32, PREC_BF16X3)
+
+__all__ = ["SBError", "load", "LIB_PATH", "NewBlockSparseSorted", "COOTensor", "DenseTensor",
+           "blocksparse_from_dense", "coo_from_dense", "to_dense", "context", "synchronize", "profile", "profile_reset", "profile_read", "contract_",
+           "contract_bs_bs_to_dense_", "Plan", "labels_array", "EXACT_JOIN", "SKIP_ZERO_SLICES", "PREC_FP64",
+           "PREC_FP32", "PREC_BF16X3"]

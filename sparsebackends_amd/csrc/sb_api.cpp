@@ -1,0 +1,455 @@
+// The C ABI (include/sparsebackends_amd.h): handle management, host/device
+// transfer, and the thin wrappers that map C++ exceptions to status codes.
+#include <cstring>
+#include <mutex>
+#include <set>
+
+#include "../../include/sparsebackends_amd.h"
+#include "sb_contract.h"
+
+using namespace sb;
+
+static thread_local std::string g_err;
+
+#define SB_TRY(...)                   \
+  try {                               \
+    __VA_ARGS__;                      \
+    return SB_OK;                     \
+  } catch (const sb::Error& e) {      \
+    g_err = e.msg;                    \
+    return e.code;                    \
+  } catch (const std::bad_alloc&) {   \
+    g_err = "host out of memory";     \
+    return SB_E_NOMEM;                \
+  } catch (const std::exception& e) { \
+    g_err = e.what();                 \
+    return SB_E_ERROR;                \
+  }
+
+static Ctx* C_(sb_ctx_t c) {
+  if (!c) fail(SB_E_HANDLE, "null context");
+  Ctx* ctx = reinterpret_cast<Ctx*>(c);
+  HIP_CHECK(hipSetDevice(ctx->device));
+  return ctx;
+}
+static Tensor* T_(sb_tensor_t t, int kind = 0) { return as_tensor(reinterpret_cast<void*>(t), kind); }
+static sb_tensor_t H_(TP&& t) { return reinterpret_cast<sb_tensor_t>(t.release()); }
+static std::vector<i64> vec(const int64_t* p, int n) { return std::vector<i64>(p, p + n); }
+static Labels labs(const uint64_t* p, int n) {
+  if (n < 0) fail(SB_E_ARGUMENT, "negative label count");
+  Labels l(n);
+  for (int i = 0; i < n; ++i) l[i] = Label{p[2 * i], p[2 * i + 1]};
+  return l;
+}
+
+static void up_i32(Ctx* ctx, DBuf<i32>& dst, const int64_t* src, i64 n) {
+  std::vector<i32> h(n);
+  for (i64 i = 0; i < n; ++i) {
+    if (src[i] < INT32_MIN || src[i] > INT32_MAX) fail(SB_E_LIMIT, "key or id outside int32");
+    h[i] = (i32)src[i];
+  }
+  dst.alloc(ctx, n ? n : 1);
+  if (n) HIP_CHECK(hipMemcpyAsync(dst.get(), h.data(), n * sizeof(i32), hipMemcpyHostToDevice, ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+static void down_i32(Ctx* ctx, int64_t* dst, const i32* src, i64 n) {
+  std::vector<i32> h(n);
+  if (n) HIP_CHECK(hipMemcpyAsync(h.data(), src, n * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (i64 i = 0; i < n; ++i) dst[i] = h[i];
+}
+
+extern "C" {
+
+const char* sb_last_error(void) { return g_err.c_str(); }
+int sb_version(void) { return 10000; }
+
+int sb_ctx_create(int device, sb_ctx_t* out) {
+  SB_TRY({
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(SB_E_ARGUMENT, "no such HIP device");
+    *out = reinterpret_cast<sb_ctx_t>(new Ctx(device));
+  })
+}
+int sb_ctx_destroy(sb_ctx_t c) { SB_TRY(delete C_(c)) }
+int sb_ctx_set_stream(sb_ctx_t c, void* s) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->own_stream) HIP_CHECK(hipStreamDestroy(ctx->stream));
+    if (s) {
+      ctx->stream = static_cast<hipStream_t>(s);
+      ctx->own_stream = false;
+    } else {
+      HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      ctx->own_stream = true;
+    }
+  })
+}
+int sb_ctx_synchronize(sb_ctx_t c) { SB_TRY(HIP_CHECK(hipStreamSynchronize(C_(c)->stream))) }
+int sb_ctx_memory(sb_ctx_t c, int64_t* in_use, int64_t* cached) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    *in_use = (int64_t)ctx->cache.bytes_in_use();
+    *cached = (int64_t)ctx->cache.bytes_cached();
+  })
+}
+int sb_ctx_trim(sb_ctx_t c) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->cache.trim();
+  })
+}
+
+int sb_ctx_profile(sb_ctx_t c, int enable) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!enable) ctx->prof_collect();
+    ctx->profile = enable != 0;
+  })
+}
+int sb_ctx_profile_reset(sb_ctx_t c) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    ctx->prof_collect();
+    ctx->prof.clear();
+  })
+}
+int sb_ctx_profile_read(sb_ctx_t c, char* buf, int64_t cap, int64_t* needed) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    ctx->prof_collect();
+    std::string s;
+    char line[256];
+    for (auto& kv : ctx->prof) {
+      snprintf(line, sizeof line, "%s %ld %.6f %.6f %.6f\n", kv.first.c_str(), kv.second.count, kv.second.ms,
+               kv.second.min_ms, kv.second.max_ms);
+      s += line;
+    }
+    if (needed) *needed = (int64_t)s.size() + 1;
+    if (buf && cap > 0) {
+      size_t n = std::min((size_t)(cap - 1), s.size());
+      std::memcpy(buf, s.data(), n);
+      buf[n] = 0;
+    }
+  })
+}
+
+int sb_tensor_free(sb_tensor_t t) { SB_TRY(delete T_(t)) }
+int sb_tensor_kind(sb_tensor_t t, int* kind) { SB_TRY(*kind = T_(t)->kind) }
+
+// ---------------------------------------------------------------- block-sparse
+int sb_bs_new(sb_ctx_t c, int N, int N2, const int64_t* dims, sb_tensor_t* out) {
+  SB_TRY(*out = H_(bs_new(C_(c), vec(dims, N), N2)))
+}
+int sb_bs_from_host(sb_ctx_t c, int N, int N2, const int64_t* dims, int64_t nblocks, const int64_t* keys,
+                    const int64_t* ids, const double* data, sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    TP t = bs_make(ctx, vec(dims, N), N2);
+    int P = N - N2;
+    t->n = nblocks;
+    up_i32(ctx, t->keys, keys, nblocks * P);
+    up_i32(ctx, t->ids, ids, nblocks);
+    for (i64 i = 0; i < nblocks; ++i)
+      if (ids[i] < 1 || ids[i] > nblocks) fail(SB_E_BOUNDS, "block id outside 1:nblocks");
+    // keys sorted by column-major lin and unique? (else the tensor is marked
+    // unsorted and permutes re-sort it, as the reference's sortperm would)
+    bool sorted = true;
+    for (i64 i = 1; i < nblocks && sorted; ++i) {
+      for (int d = P - 1; d >= 0; --d) {
+        i64 a = keys[(i - 1) * P + d], b = keys[i * P + d];
+        if (a < b) break;
+        if (a > b || d == 0) {
+          sorted = false;
+          break;
+        }
+      }
+    }
+    t->dirty = !sorted;
+    t->data.alloc(ctx, nblocks * t->blksize);
+    if (nblocks * t->blksize)
+      HIP_CHECK(hipMemcpyAsync(t->data.get(), data, nblocks * t->blksize * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = H_(std::move(t));
+  })
+}
+int sb_bs_info(sb_tensor_t h, int* N, int* N2, int64_t* dims, int64_t* nblocks, int64_t* blksize) {
+  SB_TRY({
+    Tensor* t = T_(h, BS);
+    *N = t->N();
+    *N2 = t->N2;
+    if (dims)
+      for (int i = 0; i < t->N(); ++i) dims[i] = t->dims[i];
+    *nblocks = t->n;
+    *blksize = t->blksize;
+  })
+}
+int sb_bs_to_host(sb_ctx_t c, sb_tensor_t h, int64_t* keys, int64_t* ids, double* data) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h, BS);
+    if (keys) down_i32(ctx, keys, t->keys.get(), t->n * t->P());
+    if (ids) down_i32(ctx, ids, t->ids.get(), t->n);
+    if (data && t->n * t->blksize) {
+      HIP_CHECK(hipMemcpyAsync(data, t->data.get(), t->n * t->blksize * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+  })
+}
+int sb_bs_device_ptrs(sb_tensor_t h, const int32_t** keys, const int32_t** ids, double** data) {
+  SB_TRY({
+    Tensor* t = T_(h, BS);
+    if (keys) *keys = t->keys.get();
+    if (ids) *ids = t->ids.get();
+    if (data) *data = t->data.get();
+  })
+}
+int sb_bs_getindex(sb_ctx_t c, sb_tensor_t h, const int64_t* I, double* v) {
+  SB_TRY(*v = bs_getindex(C_(c), *T_(h, BS), I))
+}
+int sb_bs_setindex(sb_ctx_t c, sb_tensor_t h, double v, const int64_t* I) {
+  SB_TRY(bs_setindex(C_(c), *T_(h, BS), v, I))
+}
+int sb_bs_sort_lex(sb_ctx_t c, sb_tensor_t h) { SB_TRY(bs_sort_lex(C_(c), *T_(h, BS))) }
+int sb_bs_is_dense(sb_tensor_t h, int* out) { SB_TRY(*out = bs_is_dense(*T_(h, BS)) ? 1 : 0) }
+int sb_bs_from_dense(sb_ctx_t c, sb_tensor_t d, int N2, double atol, double rtol, int dropzeros, sb_tensor_t* out) {
+  SB_TRY(*out = H_(bs_from_dense(C_(c), *T_(d, DENSE), N2, atol, rtol, dropzeros != 0)))
+}
+int sb_bs_to_dense(sb_ctx_t c, sb_tensor_t h, double init, int ngroups, const int64_t* groups, sb_tensor_t* out) {
+  SB_TRY({
+    std::vector<std::pair<i64, i64>> g;
+    for (int i = 0; i < ngroups; ++i) g.push_back({groups[2 * i], groups[2 * i + 1]});
+    *out = H_(bs_to_dense(C_(c), *T_(h, BS), init, g));
+  })
+}
+int sb_bs_fuse_two_axes(sb_ctx_t c, sb_tensor_t h, int64_t ax1, int64_t ax2, sb_tensor_t* out) {
+  SB_TRY(*out = H_(bs_fuse_two_axes(C_(c), *T_(h, BS), ax1, ax2)))
+}
+
+// ---------------------------------------------------------------- COO
+int sb_coo_new(sb_ctx_t c, int N, const int64_t* dims, sb_tensor_t* out) { SB_TRY(*out = H_(coo_new(C_(c), vec(dims, N)))) }
+int sb_coo_from_host(sb_ctx_t c, int N, const int64_t* dims, int64_t nnz, const int64_t* keys, const double* vals,
+                     int dirty, sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    TP t = coo_new(ctx, vec(dims, N));
+    t->n = nnz;
+    up_i32(ctx, t->keys, keys, nnz * N);
+    t->data.alloc(ctx, nnz);
+    if (nnz) HIP_CHECK(hipMemcpyAsync(t->data.get(), vals, nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    t->dirty = dirty != 0;
+    *out = H_(std::move(t));
+  })
+}
+int sb_coo_info(sb_tensor_t h, int* N, int64_t* dims, int64_t* nnz, int* dirty) {
+  SB_TRY({
+    Tensor* t = T_(h, COO);
+    *N = t->N();
+    if (dims)
+      for (int i = 0; i < t->N(); ++i) dims[i] = t->dims[i];
+    *nnz = t->n;
+    *dirty = t->dirty ? 1 : 0;
+  })
+}
+int sb_coo_to_host(sb_ctx_t c, sb_tensor_t h, int64_t* keys, double* vals) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h, COO);
+    if (keys) down_i32(ctx, keys, t->keys.get(), t->n * t->N());
+    if (vals && t->n) {
+      HIP_CHECK(hipMemcpyAsync(vals, t->data.get(), t->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+  })
+}
+int sb_coo_device_ptrs(sb_tensor_t h, const int32_t** keys, double** vals) {
+  SB_TRY({
+    Tensor* t = T_(h, COO);
+    if (keys) *keys = t->keys.get();
+    if (vals) *vals = t->data.get();
+  })
+}
+int sb_coo_getindex(sb_ctx_t c, sb_tensor_t h, const int64_t* I, double* v) {
+  SB_TRY(*v = coo_getindex(C_(c), *T_(h, COO), I))
+}
+int sb_coo_setindex(sb_ctx_t c, sb_tensor_t h, double v, const int64_t* I, int dirty) {
+  SB_TRY(coo_setindex(C_(c), *T_(h, COO), v, I, dirty != 0))
+}
+int sb_coo_sort(sb_ctx_t c, sb_tensor_t h) { SB_TRY(coo_sort(C_(c), *T_(h, COO))) }
+int sb_coo_from_dense(sb_ctx_t c, sb_tensor_t d, double atol, double rtol, sb_tensor_t* out) {
+  SB_TRY(*out = H_(coo_from_dense(C_(c), *T_(d, DENSE), atol, rtol)))
+}
+int sb_coo_to_dense(sb_ctx_t c, sb_tensor_t h, sb_tensor_t* out) { SB_TRY(*out = H_(coo_to_dense(C_(c), *T_(h, COO)))) }
+int sb_coo_fuse_axes(sb_ctx_t c, sb_tensor_t h, int64_t ax1, int64_t ax2, sb_tensor_t* out) {
+  SB_TRY(*out = H_(coo_fuse_axes(C_(c), *T_(h, COO), ax1, ax2)))
+}
+
+// ---------------------------------------------------------------- dense
+int sb_dense_from_host(sb_ctx_t c, int N, const int64_t* dims, const double* data, sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    TP t = dense_new(ctx, vec(dims, N));
+    if (t->numel())
+      HIP_CHECK(hipMemcpyAsync(t->data.get(), data, t->numel() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = H_(std::move(t));
+  })
+}
+int sb_dense_from_device(sb_ctx_t c, int N, const int64_t* dims, double* dptr, int borrow, sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    TP t;
+    if (borrow) {
+      t.reset(new Tensor);
+      t->ctx = ctx;
+      t->kind = DENSE;
+      t->dims = vec(dims, N);
+      t->N2 = N;
+      t->blksize = t->numel();
+      t->n = 1;
+      t->borrowed = dptr;
+    } else {
+      t = dense_new(ctx, vec(dims, N));
+      if (t->numel())
+        HIP_CHECK(hipMemcpyAsync(t->data.get(), dptr, t->numel() * sizeof(double), hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+    }
+    *out = H_(std::move(t));
+  })
+}
+int sb_dense_info(sb_tensor_t h, int* N, int64_t* dims) {
+  SB_TRY({
+    Tensor* t = T_(h, DENSE);
+    *N = t->N();
+    if (dims)
+      for (int i = 0; i < t->N(); ++i) dims[i] = t->dims[i];
+  })
+}
+int sb_dense_to_host(sb_ctx_t c, sb_tensor_t h, double* data) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h, DENSE);
+    if (t->numel())
+      HIP_CHECK(hipMemcpyAsync(data, t->dptr(), t->numel() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  })
+}
+int sb_dense_device_ptr(sb_tensor_t h, double** data) { SB_TRY(*data = T_(h, DENSE)->dptr()) }
+
+// ---------------------------------------------------------------- permutes
+int sb_permutedims(sb_ctx_t c, sb_tensor_t h, const int64_t* perm, sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h);
+    std::vector<i64> p = vec(perm, t->N());
+    if (t->kind == BS)
+      *out = H_(bs_permute(ctx, *t, p));
+    else if (t->kind == COO)
+      *out = H_(coo_permute(ctx, *t, p));
+    else if (t->kind == DENSE) {
+      std::vector<i64> s(p);
+      std::sort(s.begin(), s.end());
+      for (int i = 0; i < t->N(); ++i)
+        if (s[i] != i + 1) fail(SB_E_ARGUMENT, "perm must be a permutation of 1:N");
+      *out = H_(dense_permute(ctx, *t, p));
+    } else
+      fail(SB_E_METHOD, "permutedims: unsupported storage kind");
+  })
+}
+int sb_permutedims_inplace(sb_ctx_t c, sb_tensor_t h, const int64_t* perm) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h);
+    std::vector<i64> p = vec(perm, t->N());
+    if (t->kind == BS)
+      bs_permute_inplace(ctx, *t, p);
+    else if (t->kind == COO) {
+      TP r = coo_permute(ctx, *t, p);
+      t->dims = r->dims;
+      t->keys = std::move(r->keys);
+      t->dirty = true;
+    } else
+      fail(SB_E_METHOD, "permutedims!: unsupported storage kind");
+  })
+}
+
+// ---------------------------------------------------------------- contraction
+int sb_contract(sb_ctx_t c, sb_tensor_t C, const uint64_t* labelsC, int nC, sb_tensor_t A, const uint64_t* labelsA,
+                int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    auto pl = contract_plan(ctx, *T_(C), labs(labelsC, nC), *T_(A), labs(labelsA, nA), *T_(B), labs(labelsB, nB),
+                            flags);
+    pl->run_pending();
+    // the plan's scratch returns to the cache; stream order keeps reuse safe
+  })
+}
+
+int sb_contract_bs_bs_to_dense(sb_ctx_t c, sb_tensor_t Cd, const uint64_t* labelsC, int nC, sb_tensor_t A,
+                               const uint64_t* labelsA, int nA, sb_tensor_t B, const uint64_t* labelsB, int nB,
+                               uint32_t flags) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* D = T_(Cd, DENSE);
+    Tensor* a = T_(A);
+    Tensor* b = T_(B);
+    if (a->kind != BS || b->kind != BS) fail(SB_E_METHOD, "MethodError: contract_bs_bs_to_dense! needs BS x BS");
+    int NC = D->N();
+    TP Cbs = bs_new(ctx, D->dims, NC);
+    auto pl = contract_plan(ctx, *Cbs, labs(labelsC, nC), *a, labs(labelsA, nA), *b, labs(labelsB, nB), flags);
+    pl->run_pending();
+    if (Cbs->n > 0) {
+      HIP_CHECK(hipMemcpyAsync(D->dptr(), Cbs->data.get(), D->numel() * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    }
+  })
+}
+
+int sb_plan_create(sb_ctx_t c, sb_tensor_t C, const uint64_t* labelsC, int nC, sb_tensor_t A,
+                   const uint64_t* labelsA, int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags,
+                   sb_plan_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    auto pl = contract_plan(ctx, *T_(C), labs(labelsC, nC), *T_(A), labs(labelsA, nA), *T_(B), labs(labelsB, nB),
+                            flags);
+    pl->run_pending();
+    *out = reinterpret_cast<sb_plan_t>(pl.release());
+  })
+}
+int sb_plan_execute(sb_ctx_t c, sb_plan_t p) {
+  SB_TRY({
+    C_(c);
+    if (!p) fail(SB_E_HANDLE, "null plan");
+    reinterpret_cast<Plan*>(p)->run();
+  })
+}
+int sb_plan_stats(sb_plan_t p, double* flops, double* gemm_bytes, double* perm_bytes, int64_t* npairs,
+                  int64_t* nblocks_out) {
+  SB_TRY({
+    if (!p) fail(SB_E_HANDLE, "null plan");
+    Plan* pl = reinterpret_cast<Plan*>(p);
+    if (flops) *flops = pl->flops;
+    if (gemm_bytes) *gemm_bytes = pl->gemm_bytes;
+    if (perm_bytes) *perm_bytes = pl->perm_bytes;
+    if (npairs) *npairs = pl->npairs;
+    if (nblocks_out) *nblocks_out = pl->nblocks_out;
+  })
+}
+int sb_plan_operand_bytes(sb_plan_t p, double* a_bytes, double* b_bytes, double* c_bytes) {
+  SB_TRY({
+    if (!p) fail(SB_E_HANDLE, "null plan");
+    Plan* pl = reinterpret_cast<Plan*>(p);
+    if (a_bytes) *a_bytes = pl->a_bytes;
+    if (b_bytes) *b_bytes = pl->b_bytes;
+    if (c_bytes) *c_bytes = pl->c_bytes;
+  })
+}
+int sb_plan_free(sb_plan_t p) { SB_TRY(delete reinterpret_cast<Plan*>(p)) }
+
+}  // extern "C"

@@ -1,0 +1,1284 @@
+// contract! on the device: label analysis and dispatch exactly as the
+// reference (SparseBackends/src/tensoralgebra/contract.jl:37-104), then for
+// each kernel family a symbolic phase (key permutes, merge-joins, output
+// block allocation; all integer kernels on the GPU, bit-exact) and a numeric
+// phase (operand permutes + grouped block products).
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <set>
+
+#include "sb_contract.h"
+
+namespace sb {
+
+using LabelMap = std::map<Label, int>;
+
+static LabelMap make_map(const Labels& l) {
+  LabelMap m;
+  for (size_t i = 0; i < l.size(); ++i) m[l[i]] = (int)i + 1;
+  return m;
+}
+static bool has(const LabelMap& m, const Label& l) { return m.count(l) != 0; }
+static int at(const LabelMap& m, const Label& l) {
+  auto it = m.find(l);
+  if (it == m.end()) fail(SB_E_KEY, "KeyError: label not found in tensor labels");
+  return it->second;
+}
+static bool contains(const Labels& v, const Label& l) { return std::find(v.begin(), v.end(), l) != v.end(); }
+static Labels sub(const Labels& l, size_t lo, size_t hi) { return Labels(l.begin() + lo, l.begin() + hi); }
+static Labels cat(const Labels& a, const Labels& b) {
+  Labels r(a);
+  r.insert(r.end(), b.begin(), b.end());
+  return r;
+}
+static bool same_set(const Labels& a, const Labels& b) {
+  std::set<Label> x(a.begin(), a.end()), y(b.begin(), b.end());
+  return x == y;
+}
+static bool is_identity(const std::vector<i64>& p) {
+  for (size_t i = 0; i < p.size(); ++i)
+    if (p[i] != (i64)i + 1) return false;
+  return true;
+}
+static Labels permute_labels(const Labels& l, const std::vector<i64>& p) {
+  Labels r(p.size());
+  for (size_t i = 0; i < p.size(); ++i) r[i] = l[p[i] - 1];
+  return r;
+}
+
+// An operand as seen by a kernel: the current tensor (maybe a permuted copy
+// whose payload the plan fills), its labels and label map.
+struct Oper {
+  const Tensor* t;
+  Labels labels;
+  LabelMap map;
+};
+
+// ---------------------------------------------------------------- planned permutes
+// permutedims(::NewBlockSparseSorted) split into its symbolic part (keys,
+// ids, source offsets; run now) and its numeric part (payload gather; a plan step)
+static const Tensor* plan_bs_permute(Plan& pl, const Tensor& A, const std::vector<i64>& perm) {
+  Ctx* ctx = pl.ctx;
+  int N = A.N(), P = A.P(), N2 = A.N2;
+  std::vector<i64> dimsB(N);
+  for (int j = 0; j < N; ++j) dimsB[j] = A.dims[perm[j] - 1];
+  TP B = bs_make(ctx, dimsB, N2);
+  i64 n = A.n;
+  B->n = n;
+  if (n > 0) {
+    std::vector<int> sel(P);
+    bool prefix_id = true;
+    for (int j = 0; j < P; ++j) {
+      sel[j] = (int)perm[j];
+      prefix_id = prefix_id && perm[j] == j + 1;
+    }
+    B->keys.alloc(ctx, (size_t)n * (P ? P : 1));
+    B->ids.alloc(ctx, n);
+    B->data.alloc(ctx, (size_t)n * B->blksize);
+    k_iota(ctx, B->ids.get(), n, 1);
+    DBuf<i64> src(ctx, n);
+    if (P > 0) {
+      DBuf<i32> nk(ctx, (size_t)n * P), idx(ctx, n);
+      DBuf<u64> lin(ctx, n);
+      k_remap_keys(ctx, A.keys.get(), n, P, sel.data(), P, dimsB.data(), nk.get(), lin.get(), idx.get());
+      if (!(prefix_id && !A.dirty)) sort_pairs(ctx, lin.get(), idx.get(), n, bits_for((u64)prod(dimsB, 0, P) - 1));
+      k_gather_rows(ctx, nk.get(), P, idx.get(), n, B->keys.get());
+      k_src_offsets(ctx, A.ids.get(), idx.get(), n, A.blksize, src.get());
+    } else {
+      k_block_offsets(ctx, A.ids.get(), n, A.blksize, src.get());
+    }
+    std::vector<i64> tdims(A.dims.begin() + P, A.dims.end());
+    std::vector<int> pblk(N2);
+    for (int i = 0; i < N2; ++i) pblk[i] = (int)(perm[P + i] - P - 1);
+    const double* in = A.data.get();
+    double* out = B->data.get();
+    i64* so = pl.keep(std::move(src));
+    i64 bs = B->blksize;
+    pl.steps.push_back({"permute_blocks", [ctx, in, so, n, N2, tdims, pblk, out]() {
+                          permute_blocks(ctx, in, so, n, N2, tdims.data(), pblk.data(), out);
+                        }});
+    pl.perm_bytes += 2.0 * n * bs * sizeof(double);
+  }
+  const Tensor* r = B.get();
+  pl.keep_tensor(std::move(B));
+  return r;
+}
+
+static const Tensor* plan_dense_permute(Plan& pl, const Tensor& Bt, const std::vector<i64>& perm) {
+  Ctx* ctx = pl.ctx;
+  int N = Bt.N();
+  std::vector<i64> nd(N);
+  std::vector<int> p0(N);
+  for (int j = 0; j < N; ++j) {
+    nd[j] = Bt.dims[perm[j] - 1];
+    p0[j] = (int)perm[j] - 1;
+  }
+  if (N > MAXR) fail(SB_E_LIMIT, "dense permute of rank above 8");
+  TP R = dense_new(ctx, nd);
+  const double* in = Bt.dptr();
+  double* out = R->data.get();
+  std::vector<i64> dims = Bt.dims;
+  pl.steps.push_back({"permute_dense", [ctx, in, N, dims, p0, out]() {
+                        permute_blocks(ctx, in, nullptr, 1, N, dims.data(), p0.data(), out);
+                      }});
+  pl.perm_bytes += 2.0 * Bt.numel() * sizeof(double);
+  const Tensor* r = R.get();
+  pl.keep_tensor(std::move(R));
+  return r;
+}
+
+// _find_permutation_for_tensor, single label (contract_bs_bs.jl:33-45)
+static std::vector<i64> find_perm_for_tensor(const Tensor& T, const LabelMap& m, const Label& rlab) {
+  int P = T.P(), N = T.N();
+  int axisr = at(m, rlab);
+  std::vector<i64> perm;
+  if (axisr <= P) {
+    for (int a = 1; a <= P; ++a)
+      if (a != axisr) perm.push_back(a);
+    perm.push_back(axisr);
+    for (int a = P + 1; a <= N; ++a) perm.push_back(a);
+  } else {
+    for (int a = 1; a <= P; ++a) perm.push_back(a);
+    for (int a = P + 1; a <= N; ++a)
+      if (a != axisr) perm.push_back(a);
+    perm.push_back(axisr);
+  }
+  return perm;
+}
+
+// _permute_r_to_last_prefix (contract_bs_bs.jl:77-95)
+static void permute_r_to_last_prefix(Plan& pl, Oper& X, const Label& rlab) {
+  int axisr = at(X.map, rlab);
+  SB_ASSERT(axisr <= X.t->P(), "rlab must be in prefix to use prefix kernel");
+  if (axisr == X.t->P()) return;
+  auto perm = find_perm_for_tensor(*X.t, X.map, rlab);
+  X.t = plan_bs_permute(pl, *X.t, perm);
+  X.labels = permute_labels(X.labels, perm);
+  X.map = make_map(X.labels);
+}
+
+// ---------------------------------------------------------------- shared pipeline pieces
+struct Pairs {
+  i64 n = 0;
+  DBuf<i32> a, b;
+};
+
+// join-equal ranges -> pairs (A-major, B-minor within each A row: the reference traversal order)
+static Pairs pairs_from_ranges(Ctx* ctx, DBuf<i32>& lo, DBuf<i32>& hi, i64 nA) {
+  Pairs P;
+  if (nA == 0) return P;
+  DBuf<i64> cnt(ctx, nA), off(ctx, nA);
+  k_range_counts(ctx, lo.get(), hi.get(), nA, cnt.get());
+  P.n = exclusive_scan_total(ctx, cnt.get(), off.get(), nA);
+  if (P.n >= (1ll << 31)) fail(SB_E_LIMIT, "more than 2^31 block pairs in one contraction");
+  P.a.alloc(ctx, P.n ? P.n : 1);
+  P.b.alloc(ctx, P.n ? P.n : 1);
+  k_emit_pairs(ctx, lo.get(), off.get(), cnt.get(), nA, P.a.get(), P.b.get());
+  return P;
+}
+
+// keys of C for every pair, then output-block allocation (_ensure_block! order)
+static void allocate_from_pairs(Plan& pl, Tensor& C, const Pairs& pr, const Tensor* A, int PAw, const Tensor* B,
+                                int PBw, const std::vector<int>& src, Alloc& al) {
+  Ctx* ctx = pl.ctx;
+  int PC = C.P();
+  DBuf<i32> ck(ctx, (size_t)(pr.n ? pr.n : 1) * (PC ? PC : 1));
+  DBuf<u64> lin(ctx, pr.n ? pr.n : 1);
+  if (pr.n && PC > 0)
+    k_pair_keys(ctx, pr.a.get(), pr.b.get(), pr.n, A ? A->keys.get() : nullptr, PAw, B ? B->keys.get() : nullptr,
+                PBw, src.data(), PC, C.dims.data(), ck.get(), lin.get());
+  else if (pr.n)
+    k_zero(ctx, lin.get(), pr.n * sizeof(u64));
+  u64 maxlin = PC > 0 ? (u64)prod(C.dims, 0, PC) - 1 : 0;
+  allocate_blocks(ctx, pr.n, PC, ck, lin, maxlin, al);
+  // install the block list in C (overwrite semantics: empty!(C.keys/ids/data))
+  C.n = al.nC;
+  C.dirty = false;
+  C.keys = std::move(al.keys);
+  C.ids.alloc(ctx, al.nC ? al.nC : 1);
+  if (al.nC)
+    HIP_CHECK(hipMemcpyAsync(C.ids.get(), al.ids.get(), al.nC * sizeof(i32), hipMemcpyDeviceToDevice, ctx->stream));
+  C.data.alloc(ctx, (size_t)al.nC * C.blksize);
+  pl.nblocks_out = al.nC;
+  pl.npairs = pr.n;
+}
+
+// per-pair offsets: off[p] = blockoff(T)[idx[p]]
+static i64* pair_offsets(Plan& pl, const Tensor& T, const i32* idx, i64 np) {
+  Ctx* ctx = pl.ctx;
+  DBuf<i64> bo = block_offsets(ctx, T);
+  DBuf<i64> o(ctx, np ? np : 1);
+  if (np) k_take_i64(ctx, bo.get(), idx, np, o.get());
+  return pl.keep(std::move(o));
+}
+// per-group offsets of C blocks in group (key) order
+static i64* group_offsets(Plan& pl, const Tensor& C) {
+  DBuf<i64> o = block_offsets(pl.ctx, C);
+  return pl.keep(std::move(o));
+}
+
+// ---------------------------------------------------------------- grouping of C dense labels
+enum Mode { AthenB, BthenA, Interleaved };
+// _cdense_grouping_and_orders (contract_bs_bs.jl:470-505)
+static Mode cdense_grouping(const Labels& Cd, const Labels& keepA, const Labels& keepB, Labels& dA, Labels& dB) {
+  std::vector<int> origin(Cd.size());
+  for (size_t i = 0; i < Cd.size(); ++i) {
+    if (contains(keepA, Cd[i]))
+      origin[i] = 1;
+    else if (contains(keepB, Cd[i]))
+      origin[i] = 0;
+    else
+      fail(SB_E_ERROR, "Cdense label not found in keepA/keepB");
+  }
+  int firstB = -1, lastA = -1, firstA = -1, lastB = -1;
+  for (int i = 0; i < (int)Cd.size(); ++i) {
+    if (origin[i]) {
+      if (firstA < 0) firstA = i;
+      lastA = i;
+    } else {
+      if (firstB < 0) firstB = i;
+      lastB = i;
+    }
+  }
+  Mode m;
+  if (firstB < 0 || firstA < 0)
+    m = AthenB;
+  else if (lastA < firstB)
+    m = AthenB;
+  else if (lastB < firstA)
+    m = BthenA;
+  else
+    m = Interleaved;
+  dA.clear();
+  dB.clear();
+  for (auto& l : Cd) {
+    if (contains(keepA, l)) dA.push_back(l);
+    if (contains(keepB, l)) dB.push_back(l);
+  }
+  return m;
+}
+
+static void add_gemm_step(Plan& pl, unsigned prec, const GemmDesc& d, const double* A, const i64* aoff,
+                          const double* B, const i64* boff, double* C, const i64* coff, const i32* seg,
+                          const i32* order, i64 ng, const unsigned char* kmask) {
+  Ctx* ctx = pl.ctx;
+  pl.steps.push_back({"grouped_gemm", [=]() { grouped_gemm(ctx, prec, d, A, aoff, B, boff, C, coff, seg, order, ng, kmask); }});
+}
+
+// ================================================================ BS x BS
+// contract_shared! BS x BS (contract_bs_bs.jl:572-730)
+static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared,
+                         unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_bs_shared";
+  for (auto& l : shared) SB_ASSERT(!contains(labelsC, l), "shared label must be reduced (must not appear in labelsC)");
+  int PA = A.t->P(), PB = B.t->P(), NA = A.t->N(), NB = B.t->N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    SB_ASSERT(has(A.map, l) && has(B.map, l), "shared label must exist in both A and B");
+    int ap = at(A.map, l), bp = at(B.map, l);
+    bool apf = ap <= PA, bpf = bp <= PB;
+    if (apf && bpf)
+      sp.push_back(l);
+    else if (!apf && !bpf)
+      sd.push_back(l);
+    else
+      fail(SB_E_ERROR, "shared label crosses prefix/dense; not supported");
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), Bd0 = sub(B.labels, PB, NB);
+  Labels red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : Bd0)
+    if (!contains(sd, l)) keepB0.push_back(l);
+  int PC = C.P(), NC = C.N();
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; would require permuting C (unsupported)");
+  SB_ASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA covers keepA");
+  SB_ASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB covers keepB");
+  auto build_perm = [&](Oper& X, bool red_first, const Labels& keep) {
+    int P = X.t->P();
+    std::vector<i64> spax;
+    for (auto& l : sp) spax.push_back(at(X.map, l));
+    std::vector<i64> perm;
+    for (int a = 1; a <= P; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) perm.push_back(a);
+    for (auto a : spax) perm.push_back(a);
+    std::vector<i64> ka, ra;
+    for (auto& l : keep) ka.push_back(at(X.map, l) - P);
+    for (auto& l : red) ra.push_back(at(X.map, l) - P);
+    SB_ASSERT((int)(ka.size() + ra.size()) == X.t->N2, "dense permutation covers the tail");
+    for (auto a : (red_first ? ra : ka)) perm.push_back(P + a);
+    for (auto a : (red_first ? ka : ra)) perm.push_back(P + a);
+    return perm;
+  };
+  auto pA = build_perm(A, false, dA);
+  auto pB = build_perm(B, true, dB);
+  if (!(is_identity(pA) && !A.t->dirty)) {
+    A.t = plan_bs_permute(pl, *A.t, pA);
+    A.labels = permute_labels(A.labels, pA);
+    A.map = make_map(A.labels);
+  }
+  if (!(is_identity(pB) && !B.t->dirty)) {
+    B.t = plan_bs_permute(pl, *B.t, pB);
+    B.labels = permute_labels(B.labels, pB);
+    B.map = make_map(B.labels);
+  }
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l) && at(A.map, l) <= PA)
+      src[j] = at(A.map, l);
+    else if (has(B.map, l) && at(B.map, l) <= PB)
+      src[j] = -at(B.map, l);
+    else
+      fail(SB_E_ERROR, "C prefix label must come from A/B prefix");
+  }
+  std::vector<int> jA, jB;
+  std::vector<i64> spd;
+  for (auto& l : sp) {
+    jA.push_back(at(A.map, l));
+    jB.push_back(at(B.map, l));
+    spd.push_back(A.t->dims[at(A.map, l) - 1]);
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 M = prod(TA.dims, PA, PA + dA.size());
+  i64 K = prod(TA.dims, PA + dA.size(), NA);
+  i64 N = prod(TB.dims, PB + red.size(), NB);
+  if (!red.empty()) SB_ASSERT(K == prod(TB.dims, PB, PB + red.size()), "Reduced dense extents mismatch between A and B");
+  SB_ASSERT(C.blksize == M * N, "C.blksize must equal M*N");
+  i64 nA = TA.n, nB = TB.n;
+  DBuf<i32> lo(ctx, nA ? nA : 1), hi(ctx, nA ? nA : 1);
+  int nsp = (int)sp.size();
+  if (nA && nB) {
+    if (nsp == 0) {
+      // join on nothing: every A block meets every B block
+      DBuf<u64> z(ctx, nA), zb(ctx, nB);
+      k_zero(ctx, z.get(), nA * sizeof(u64));
+      k_zero(ctx, zb.get(), nB * sizeof(u64));
+      k_equal_ranges(ctx, z.get(), nA, zb.get(), nB, lo.get(), hi.get());
+    } else if (nsp >= 2 && !(flags & SB_FLAG_EXACT_JOIN)) {
+      // the reference's merge walk compares the first shared axis first
+      k_walk_ranges(ctx, 2, TA.keys.get(), PA, nA, TB.keys.get(), PB, nB, jA.data(), jB.data(), nsp, lo.get(),
+                    hi.get());
+    } else {
+      DBuf<u64> ja(ctx, nA), jb(ctx, nB);
+      k_cols_lin(ctx, TA.keys.get(), nA, PA, jA.data(), nsp, spd.data(), ja.get());
+      k_cols_lin(ctx, TB.keys.get(), nB, PB, jB.data(), nsp, spd.data(), jb.get());
+      k_equal_ranges(ctx, ja.get(), nA, jb.get(), nB, lo.get(), hi.get());
+    }
+  }
+  Pairs pr = (nA && nB) ? pairs_from_ranges(ctx, lo, hi, nA) : Pairs();
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, PA, &TB, PB, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * M * K * N * np;
+  pl.gemm_bytes += 8.0 * (double)np * (M * K + K * N) + 8.0 * ng * M * N;
+  pl.a_bytes += 8.0 * TA.n * TA.blksize;
+  pl.b_bytes += 8.0 * TB.n * TB.blksize;
+  pl.c_bytes += 8.0 * ng * C.blksize;
+  if (!np) return;
+  i64* aoff = pair_offsets(pl, TA, pr.a.get(), np);
+  i64* boff = pair_offsets(pl, TB, pr.b.get(), np);
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  pl.keep(std::move(pr.a));
+  pl.keep(std::move(pr.b));
+  GemmDesc d{M, N, K, 1, M, 1, K, mode == AthenB ? 1 : N, mode == AthenB ? M : 1};
+  add_gemm_step(pl, flags, d, TA.data.get(), aoff, TB.data.get(), boff, C.data.get(), coff, seg, order, ng, nullptr);
+}
+
+// contract_prefix_outer_bb! (contract_bs_bs.jl:173-263)
+static void prefix_outer_bb(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_bs_prefix_outer";
+  SB_ASSERT(!contains(labelsC, rlab), "rlab must be reduced (not in labelsC)");
+  int PA = A.t->P(), PB = B.t->P(), PC = C.P(), NA = A.t->N(), NB = B.t->N(), NC = C.N();
+  Labels Ad = sub(A.labels, PA, NA), Bd = sub(B.labels, PB, NB), Cd = sub(labelsC, PC, NC);
+  for (auto& l : Ad) SB_ASSERT(!contains(Bd, l), "outer kernel assumes disjoint dense labels");
+  int dense_order;
+  if (Cd == cat(Ad, Bd))
+    dense_order = 1;
+  else if (Cd == cat(Bd, Ad))
+    dense_order = 2;
+  else
+    fail(SB_E_ERROR, "C dense must be [A_dense...,B_dense...] or [B_dense..., A_dense...]");
+  SB_ASSERT(C.blksize == A.t->blksize * B.t->blksize, "C.blksize must equal A.blksize * B.blksize");
+  int axisAr = at(A.map, rlab), axisBr = at(B.map, rlab);
+  SB_ASSERT(axisAr == PA, "axisAr == PA");
+  SB_ASSERT(axisBr == PB, "axisBr == PB");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    if (has(A.map, labelsC[j]))
+      src[j] = at(A.map, labelsC[j]);
+    else {
+      SB_ASSERT(has(B.map, labelsC[j]), "Label not found in A or B");
+      src[j] = -at(B.map, labelsC[j]);
+    }
+    if ((src[j] > 0 && src[j] > PA) || (src[j] < 0 && -src[j] > PB)) fail(SB_E_BOUNDS, "BoundsError: key tuple");
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 nA = TA.n, nB = TB.n;
+  DBuf<i32> lo(ctx, nA ? nA : 1), hi(ctx, nA ? nA : 1);
+  if (nA && nB) {
+    int ca = PA, cb = PB;
+    if (TA.dirty || TB.dirty) {
+      // keys not in column-major order: the reference's run walk as written
+      k_walk_ranges(ctx, 0, TA.keys.get(), PA, nA, TB.keys.get(), PB, nB, &ca, &cb, 1, lo.get(), hi.get());
+    } else {
+      i64 rd = TA.dims[PA - 1];
+      DBuf<u64> ja(ctx, nA), jb(ctx, nB);
+      k_cols_lin(ctx, TA.keys.get(), nA, PA, &ca, 1, &rd, ja.get());
+      k_cols_lin(ctx, TB.keys.get(), nB, PB, &cb, 1, &rd, jb.get());
+      k_equal_ranges(ctx, ja.get(), nA, jb.get(), nB, lo.get(), hi.get());
+    }
+  }
+  Pairs pr = (nA && nB) ? pairs_from_ranges(ctx, lo, hi, nA) : Pairs();
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, PA, &TB, PB, src, al);
+  i64 np = pr.n, ng = al.nC, ma = TA.blksize, nb = TB.blksize;
+  pl.flops += 2.0 * ma * nb * np;
+  pl.gemm_bytes += 8.0 * (double)np * (ma + nb) + 8.0 * ng * ma * nb;
+  if (!np) return;
+  i64* aoff = pair_offsets(pl, TA, pr.a.get(), np);
+  i64* boff = pair_offsets(pl, TB, pr.b.get(), np);
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  pl.keep(std::move(pr.a));
+  pl.keep(std::move(pr.b));
+  const double *Ad_ = TA.data.get(), *Bd_ = TB.data.get();
+  double* Cd_ = C.data.get();
+  if (dense_order == 1)
+    pl.steps.push_back({"outer_groups", [=]() { outer_groups(ctx, ma, nb, Ad_, aoff, Bd_, boff, Cd_, coff, seg, order, ng); }});
+  else
+    pl.steps.push_back({"outer_groups", [=]() { outer_groups(ctx, nb, ma, Bd_, boff, Ad_, aoff, Cd_, coff, seg, order, ng); }});
+}
+
+// contract_dense_bb! (contract_bs_bs.jl:288-423)
+static void dense_bb(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab, unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_bs_dense_r";
+  SB_ASSERT(!contains(labelsC, rlab), "rlab must be reduced (not in labelsC)");
+  int PA = A.t->P(), PB = B.t->P(), NA = A.t->N(), NB = B.t->N();
+  int axisAr = at(A.map, rlab);
+  SB_ASSERT(axisAr > PA, "rlab must be in A dense tail");
+  int axisBr = at(B.map, rlab);
+  SB_ASSERT(axisBr > PB, "rlab must be in B dense tail");
+  LabelMap mapA0 = A.map, mapB0 = B.map;  // prefix sourcing uses the unpermuted maps
+  if (axisAr != NA) {
+    auto p = find_perm_for_tensor(*A.t, A.map, rlab);
+    A.t = plan_bs_permute(pl, *A.t, p);
+    A.labels = permute_labels(A.labels, p);
+  }
+  if (axisBr != NB) {
+    auto p = find_perm_for_tensor(*B.t, B.map, rlab);
+    B.t = plan_bs_permute(pl, *B.t, p);
+    B.labels = permute_labels(B.labels, p);
+  }
+  SB_ASSERT(A.labels.back() == rlab, "labelsA[end] == rlab");
+  SB_ASSERT(B.labels.back() == rlab, "labelsB[end] == rlab");
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 R = TA.dims[NA - 1];
+  SB_ASSERT(TB.dims[NB - 1] == R, "dimsB_dense[end] == R");
+  i64 chunkA = TA.blksize / R, chunkB = TB.blksize / R;
+  int PC = C.P(), NC = C.N();
+  SB_ASSERT(C.N2 == (TA.N2 - 1) + (TB.N2 - 1), "N2C == (N2A-1) + (N2B-1)");
+  SB_ASSERT(C.blksize == chunkA * chunkB, "C.blksize == chunkA * chunkB");
+  Labels Awo = sub(A.labels, PA, NA - 1), Bwo = sub(B.labels, PB, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool AB;
+  if (Cd == cat(Awo, Bwo))
+    AB = true;
+  else if (Cd == cat(Bwo, Awo))
+    AB = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense..., B_dense...] or [B_dense..., A_dense...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    if (has(mapA0, labelsC[j])) {
+      int ax = at(mapA0, labelsC[j]);
+      SB_ASSERT(ax <= PA, "ax <= PA");
+      src[j] = ax;
+    } else {
+      int ax = at(mapB0, labelsC[j]);
+      SB_ASSERT(ax <= PB, "ax <= PB");
+      src[j] = -ax;
+    }
+  }
+  i64 nA = TA.n, nB = TB.n;
+  Pairs pr;
+  DBuf<unsigned char> kmask;
+  if (nA && nB) {
+    // the payloads of the permuted operands are needed now (block existence
+    // depends on values): run the permute steps recorded so far
+    pl.run_pending();
+    DBuf<i64> offA = block_offsets(ctx, TA), offB = block_offsets(ctx, TB);
+    DBuf<unsigned char> fA(ctx, nA * R), fB(ctx, nB * R);
+    slice_nonzero(ctx, TA.data.get(), offA.get(), nA, R, chunkA, fA.get());
+    slice_nonzero(ctx, TB.data.get(), offB.get(), nB, R, chunkB, fB.get());
+    i64 np0 = nA * nB;
+    if (np0 >= (1ll << 31)) fail(SB_E_LIMIT, "more than 2^31 block pairs in one contraction");
+    pr.a.alloc(ctx, np0);
+    pr.b.alloc(ctx, np0);
+    k_cross_pairs(ctx, nA, nB, pr.a.get(), pr.b.get());
+    DBuf<unsigned char> keep(ctx, np0);
+    k_rank1_exist(ctx, fA.get(), fB.get(), pr.a.get(), pr.b.get(), np0, R, keep.get(), nullptr);
+    pr.n = compact_pairs(ctx, keep.get(), pr.a, pr.b, np0);
+    if (pr.n) {
+      kmask.alloc(ctx, pr.n * R);
+      DBuf<unsigned char> keep2(ctx, pr.n);
+      k_rank1_exist(ctx, fA.get(), fB.get(), pr.a.get(), pr.b.get(), pr.n, R, keep2.get(), kmask.get());
+    }
+  }
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, PA, &TB, PB, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * chunkA * chunkB * R * np;
+  pl.gemm_bytes += 8.0 * (double)np * (chunkA + chunkB) * R + 8.0 * ng * chunkA * chunkB;
+  if (!np) return;
+  i64* aoff = pair_offsets(pl, TA, pr.a.get(), np);
+  i64* boff = pair_offsets(pl, TB, pr.b.get(), np);
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  unsigned char* km = pl.keep(std::move(kmask));
+  pl.keep(std::move(pr.a));
+  pl.keep(std::move(pr.b));
+  // A block (chunkA x R), B block (chunkB x R): C[i,j] = sum_r A[i,r] B[j,r]
+  GemmDesc d{chunkA, chunkB, R, 1, chunkA, chunkB, 1, AB ? 1 : chunkB, AB ? chunkA : 1};
+  // existence was decided on the current values; replays keep that block list
+  add_gemm_step(pl, flags, d, TA.data.get(), aoff, TB.data.get(), boff, C.data.get(), coff, seg, order, ng, km);
+}
+
+// ================================================================ BS x Dense
+// contract_prefix_outer_bd! (contract_bs_dense.jl:34-204)
+static void prefix_outer_bd(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_dense_prefix_outer";
+  SB_ASSERT(!contains(labelsC, rlab), "rlab must be reduced (not in labelsC)");
+  int axisAr = at(A.map, rlab);
+  SB_ASSERT(axisAr <= A.t->P(), "rlab must be in A prefix/head");
+  int NB = B.t->N();
+  int axisBr = at(B.map, rlab);
+  SB_ASSERT(1 <= axisBr && axisBr <= NB, "rlab must be a dim of dense B");
+  permute_r_to_last_prefix(pl, A, rlab);
+  int PA = A.t->P(), NA = A.t->N(), PC = C.P(), NC = C.N();
+  SB_ASSERT(at(A.map, rlab) == PA, "mapA[rlab] == PA");
+  const Tensor* Bt = B.t;
+  i64 R = Bt->dims[axisBr - 1];
+  if (axisBr != NB) {
+    std::vector<i64> p;
+    for (int d = 1; d <= NB; ++d)
+      if (d != axisBr) p.push_back(d);
+    p.push_back(axisBr);
+    Bt = plan_dense_permute(pl, *Bt, p);
+  }
+  i64 chunkB = Bt->numel() / R;
+  Labels Ad = sub(A.labels, PA, NA), Bwo, Cd = sub(labelsC, PC, NC);
+  for (auto& l : B.labels)
+    if (l != rlab) Bwo.push_back(l);
+  bool AB;
+  if (Cd == cat(Ad, Bwo))
+    AB = true;
+  else if (Cd == cat(Bwo, Ad))
+    AB = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense..., B_wo_r...] or [B_wo_r..., A_dense...]");
+  const Tensor& TA = *A.t;
+  i64 chunkA = TA.blksize;
+  SB_ASSERT(C.blksize == chunkA * chunkB, "C.blksize must equal A.blksize * (prod(B dims without r))");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    SB_ASSERT(has(A.map, labelsC[j]), "C prefix label not in A");
+    int ax = at(A.map, labelsC[j]);
+    SB_ASSERT(ax <= PA && labelsC[j] != rlab, "C prefix must be from A prefix excluding rlab");
+    src[j] = ax;
+  }
+  i64 nA = TA.n;
+  // groups in lexicographic (stable) order of the C keys (:121-126): the pair
+  // list is A's blocks in that order, so first-creation order = lex order
+  Pairs pr;
+  pr.n = nA;
+  if (nA) {
+    pr.a.alloc(ctx, nA);
+    k_iota(ctx, pr.a.get(), nA, 0);
+    if (PC > 0) {
+      std::vector<int> cols(PC);
+      std::vector<i64> cd(PC);
+      for (int j = 0; j < PC; ++j) {
+        cols[j] = src[PC - 1 - j];
+        cd[j] = C.dims[PC - 1 - j];
+      }
+      DBuf<u64> lex(ctx, nA);
+      k_cols_lin(ctx, TA.keys.get(), nA, PA, cols.data(), PC, cd.data(), lex.get());
+      sort_pairs(ctx, lex.get(), pr.a.get(), nA, bits_for((u64)prod(C.dims, 0, PC)));
+    }
+    // the staging buffer of the reference holds at most R blocks per group
+  }
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, PA, nullptr, 0, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * chunkA * chunkB * np;
+  pl.gemm_bytes += 8.0 * (double)np * (chunkA + chunkB) + 8.0 * ng * chunkA * chunkB;
+  if (!np) return;
+  {
+    // nb <= R per group (the reference stages at most R blocks, :135-157)
+    std::vector<i32> ss(ng + 1);
+    HIP_CHECK(hipMemcpyAsync(ss.data(), al.seg_start.get(), (ng + 1) * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (i64 s = 0; s < ng; ++s) SB_ASSERT(ss[s + 1] - ss[s] <= R, "group size <= R (staging buffer)");
+  }
+  i64* aoff = pair_offsets(pl, TA, pr.a.get(), np);
+  DBuf<i64> bo(ctx, np);
+  // B column of pair p: rv = A key's r (last prefix axis)
+  k_rv_offsets(ctx, TA.keys.get(), PA, PA, pr.a.get(), nullptr, nullptr, np, chunkB, bo.get());
+  i64* boff = pl.keep(std::move(bo));
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  pl.keep(std::move(pr.a));
+  const double *Ad_ = TA.data.get(), *Bd_ = Bt->dptr();
+  double* Cd_ = C.data.get();
+  pl.steps.push_back({"kahan_outer", [=]() { kahan_outer(ctx, AB, chunkA, chunkB, Ad_, aoff, Bd_, boff, Cd_, coff, seg, order, ng); }});
+}
+
+// contract_dense_bd! (contract_bs_dense.jl:248-392)
+static void dense_bd(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab, unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_dense_dense_r";
+  SB_ASSERT(!contains(labelsC, rlab), "rlab must be reduced (not in labelsC)");
+  int PA = A.t->P(), NA = A.t->N(), NB = B.t->N();
+  int axisAr = at(A.map, rlab);
+  SB_ASSERT(axisAr > PA, "rlab must be in A dense tail");
+  int axisBr = at(B.map, rlab);
+  SB_ASSERT(1 <= axisBr && axisBr <= NB, "rlab must be an axis of B");
+  LabelMap mapA0 = A.map;
+  if (axisAr != NA) {
+    auto p = find_perm_for_tensor(*A.t, A.map, rlab);
+    A.t = plan_bs_permute(pl, *A.t, p);
+    A.labels = permute_labels(A.labels, p);
+  }
+  SB_ASSERT(A.labels.back() == rlab, "labelsA[end] == rlab");
+  const Tensor* Bt = B.t;
+  Labels Bl = B.labels;
+  if (axisBr != NB) {
+    std::vector<i64> p;
+    for (int d = 1; d <= NB; ++d)
+      if (d != axisBr) p.push_back(d);
+    p.push_back(axisBr);
+    Bt = plan_dense_permute(pl, *Bt, p);
+    Bl = permute_labels(B.labels, p);
+  }
+  SB_ASSERT(Bl.back() == rlab, "labelsB[end] == rlab");
+  const Tensor& TA = *A.t;
+  i64 R = TA.dims[NA - 1];
+  SB_ASSERT(Bt->dims[NB - 1] == R, "dimsB_dense[end] == R");
+  i64 chunkA = TA.blksize / R;
+  i64 chunkB = prod(Bt->dims, 0, NB - 1);
+  int PC = C.P(), NC = C.N();
+  SB_ASSERT(C.N2 == (TA.N2 - 1) + (NB - 1), "N2C == (N2A - 1) + (NB - 1)");
+  SB_ASSERT(C.blksize == chunkA * chunkB, "C.blksize == chunkA * chunkB");
+  Labels Awo = sub(A.labels, PA, NA - 1), Bwo = sub(Bl, 0, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool AB;
+  if (Cd == cat(Awo, Bwo))
+    AB = true;
+  else if (Cd == cat(Bwo, Awo))
+    AB = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense..., B_dense...] or [B_dense..., A_dense...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    int ax = at(mapA0, labelsC[j]);
+    SB_ASSERT(ax <= PA, "C sparse prefix label must come from A sparse head");
+    src[j] = ax;
+  }
+  i64 nA = TA.n;
+  Pairs pr;
+  DBuf<unsigned char> kmask;
+  if (nA) {
+    pl.run_pending();  // existence depends on the permuted payloads
+    DBuf<i64> offA = block_offsets(ctx, TA);
+    DBuf<unsigned char> fA(ctx, nA * R), fB(ctx, R);
+    slice_nonzero(ctx, TA.data.get(), offA.get(), nA, R, chunkA, fA.get());
+    slice_nonzero(ctx, Bt->dptr(), nullptr, 1, R, chunkB, fB.get());
+    pr.a.alloc(ctx, nA);
+    k_iota(ctx, pr.a.get(), nA, 0);
+    DBuf<unsigned char> keep(ctx, nA);
+    k_rank1_exist(ctx, fA.get(), fB.get(), pr.a.get(), nullptr, nA, R, keep.get(), nullptr);
+    pr.n = compact_pairs(ctx, keep.get(), pr.a, pr.b, nA);
+    if (pr.n) {
+      kmask.alloc(ctx, pr.n * R);
+      DBuf<unsigned char> keep2(ctx, pr.n);
+      k_rank1_exist(ctx, fA.get(), fB.get(), pr.a.get(), nullptr, pr.n, R, keep2.get(), kmask.get());
+    }
+  }
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, PA, nullptr, 0, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * chunkA * chunkB * R * np;
+  pl.gemm_bytes += 8.0 * (double)np * chunkA * R + 8.0 * chunkB * R + 8.0 * ng * chunkA * chunkB;
+  if (!np) return;
+  i64* aoff = pair_offsets(pl, TA, pr.a.get(), np);
+  DBuf<i64> bz(ctx, np);
+  k_zero(ctx, bz.get(), np * sizeof(i64));
+  i64* boff = pl.keep(std::move(bz));
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  unsigned char* km = pl.keep(std::move(kmask));
+  pl.keep(std::move(pr.a));
+  GemmDesc d{chunkA, chunkB, R, 1, chunkA, chunkB, 1, AB ? 1 : chunkB, AB ? chunkA : 1};
+  add_gemm_step(pl, flags, d, TA.data.get(), aoff, Bt->dptr(), boff, C.data.get(), coff, seg, order, ng, km);
+}
+
+// contract_shared! BS x Dense (contract_bs_dense.jl:411-619).  The dense
+// operand is permuted to [red..., keepB..., shared_prefix...] so that every
+// shared-prefix slice is one contiguous K x N matrix (the reference uses
+// [shared_prefix..., red..., keepB...] and strided slices; same values).
+static void shared_bs_dense(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared,
+                            unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_dense_shared";
+  for (auto& l : shared) SB_ASSERT(!contains(labelsC, l), "shared label must be reduced (must not appear in labelsC)");
+  int PA = A.t->P(), NA = A.t->N(), NB = B.t->N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    SB_ASSERT(has(A.map, l), "shared label must exist in A");
+    SB_ASSERT(has(B.map, l), "shared label must exist in B");
+    (at(A.map, l) <= PA ? sp : sd).push_back(l);
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : B.labels)
+    if (!contains(shared, l)) keepB0.push_back(l);
+  int PC = C.P(), NC = C.N();
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; would require permuting C (unsupported)");
+  SB_ASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA covers keepA");
+  SB_ASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB covers keepB");
+  std::vector<i64> pA;
+  {
+    std::vector<i64> spax;
+    for (auto& l : sp) spax.push_back(at(A.map, l));
+    for (int a = 1; a <= PA; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) pA.push_back(a);
+    for (auto a : spax) pA.push_back(a);
+    std::vector<i64> ka, ra;
+    for (auto& l : dA) ka.push_back(at(A.map, l) - PA);
+    for (auto& l : red) ra.push_back(at(A.map, l) - PA);
+    SB_ASSERT((int)(ka.size() + ra.size()) == A.t->N2, "dense permutation covers the tail");
+    for (auto a : ka) pA.push_back(PA + a);
+    for (auto a : ra) pA.push_back(PA + a);
+  }
+  std::vector<i64> pBref;  // the reference's order, checked for its length assert
+  for (auto& l : sp) pBref.push_back(at(B.map, l));
+  for (auto& l : red) pBref.push_back(at(B.map, l));
+  for (auto& l : dB) pBref.push_back(at(B.map, l));
+  SB_ASSERT((int)pBref.size() == NB, "B perm length mismatch; labelsB must match B ndims");
+  std::vector<i64> pB;
+  for (auto& l : red) pB.push_back(at(B.map, l));
+  for (auto& l : dB) pB.push_back(at(B.map, l));
+  for (auto& l : sp) pB.push_back(at(B.map, l));
+  if (!is_identity(pA)) {
+    A.t = plan_bs_permute(pl, *A.t, pA);
+    A.labels = permute_labels(A.labels, pA);
+    A.map = make_map(A.labels);
+  }
+  const Tensor* Bt = B.t;
+  if (!is_identity(pB)) Bt = plan_dense_permute(pl, *Bt, pB);
+  int nsp = (int)sp.size(), nred = (int)red.size(), nkA = (int)dA.size();
+  std::vector<int> csrc(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    SB_ASSERT(has(A.map, l), "C prefix label must exist in A");
+    int ap = at(A.map, l);
+    SB_ASSERT(ap <= PA, "C prefix label must come from A prefix");
+    SB_ASSERT(!contains(sp, l), "C prefix label cannot be a reduced shared_prefix label");
+    csrc[j] = ap;
+  }
+  const Tensor& TA = *A.t;
+  i64 M = nkA == 0 ? 1 : prod(TA.dims, PA, PA + nkA);
+  i64 K = nred == 0 ? 1 : prod(TA.dims, PA + nkA, NA);
+  i64 N = 1;
+  for (auto& l : dB) N *= B.t->dims[at(B.map, l) - 1];
+  if (nred > 0) {
+    i64 Kb = 1;
+    for (auto& l : red) Kb *= B.t->dims[at(B.map, l) - 1];
+    SB_ASSERT(K == Kb, "Reduced dense extents mismatch between A and B");
+  }
+  SB_ASSERT(C.blksize == M * N, "C.blksize must equal M*N");
+  i64 nA = TA.n;
+  // pairs = A blocks in order; B slice offset = sp_lin * K * N
+  Pairs pr;
+  pr.n = nA;
+  if (nA) {
+    pr.a.alloc(ctx, nA);
+    k_iota(ctx, pr.a.get(), nA, 0);
+  }
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, PA, nullptr, 0, csrc, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * M * K * N * np;
+  pl.gemm_bytes += 8.0 * (double)np * M * K + 8.0 * (double)Bt->numel() + 8.0 * ng * M * N;
+  pl.a_bytes += 8.0 * TA.n * TA.blksize;
+  pl.b_bytes += 8.0 * (double)Bt->numel();
+  pl.c_bytes += 8.0 * ng * C.blksize;
+  if (!np) return;
+  i64* aoff = pair_offsets(pl, TA, pr.a.get(), np);
+  DBuf<i64> bo(ctx, np);
+  if (nsp > 0) {
+    std::vector<int> jA;
+    std::vector<i64> spd;
+    for (auto& l : sp) {
+      jA.push_back(at(A.map, l));
+      spd.push_back(TA.dims[at(A.map, l) - 1]);
+    }
+    DBuf<u64> sl(ctx, np);
+    k_cols_lin(ctx, TA.keys.get(), np, PA, jA.data(), nsp, spd.data(), sl.get());
+    k_lin_scale(ctx, sl.get(), np, K * N, bo.get());
+  } else {
+    k_zero(ctx, bo.get(), np * sizeof(i64));
+  }
+  i64* boff = pl.keep(std::move(bo));
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  pl.keep(std::move(pr.a));
+  GemmDesc d{M, N, K, 1, M, 1, K, mode == AthenB ? 1 : N, mode == AthenB ? M : 1};
+  add_gemm_step(pl, flags, d, TA.data.get(), aoff, Bt->dptr(), boff, C.data.get(), coff, seg, order, ng, nullptr);
+}
+
+// ================================================================ COO kernels
+// permutedims + sort! of a COO operand: keys permuted and sorted now, the
+// values gathered by a plan step
+static const Tensor* plan_coo_permute_sort(Plan& pl, const Tensor& A, const std::vector<i64>& perm) {
+  Ctx* ctx = pl.ctx;
+  DBuf<i32> idx;
+  TP t = coo_permute_sorted(ctx, A, perm, idx);
+  i64 n = t->n;
+  if (n) {
+    const double* in = A.data.get();
+    double* out = t->data.get();
+    i32* ip = pl.keep(std::move(idx));
+    pl.steps.push_back({"coo_gather", [=]() { k_take_f64(ctx, in, ip, n, out); }});
+  }
+  const Tensor* r = t.get();
+  pl.keep_tensor(std::move(t));
+  return r;
+}
+
+// ensure_rlab_last_and_sorted (contract_coo.jl:48-82)
+struct CooOper {
+  const Tensor* t;
+  Labels labels;
+  LabelMap map;
+};
+static void coo_rlab_last_sorted(Plan& pl, CooOper& X, const Label& rlab) {
+  int N = X.t->N();
+  if (X.labels.back() == rlab) {
+    X.map = make_map(X.labels);
+    return;  // fast exit: no permute, no sort
+  }
+  Labels l2;
+  for (auto& l : X.labels)
+    if (l != rlab) l2.push_back(l);
+  l2.push_back(rlab);
+  LabelMap m = make_map(X.labels);
+  std::vector<i64> perm(N);
+  for (int j = 0; j < N; ++j) perm[j] = at(m, l2[j]);
+  X.t = plan_coo_permute_sort(pl, *X.t, perm);
+  X.labels = l2;
+  X.map = make_map(l2);
+}
+
+// B range of each A entry for a join on one column; mode 0: the reference's
+// run-wise walk (contract_coo_bs.jl:74-122), 1: element-step walk (contract_coo.jl:139-167)
+static void coo_join(Ctx* ctx, int mode, const Tensor& A, int wA, int colA, const Tensor& B, int wB, int colB,
+                     DBuf<i32>& lo, DBuf<i32>& hi) {
+  i64 nA = A.n, nB = B.n;
+  bool sorted = column_sorted(ctx, A.keys.get(), wA, colA, nA) && column_sorted(ctx, B.keys.get(), wB, colB, nB);
+  if (sorted) {
+    i64 big = 1ll << 31;
+    DBuf<u64> ja(ctx, nA), jb(ctx, nB);
+    k_cols_lin(ctx, A.keys.get(), nA, wA, &colA, 1, &big, ja.get());
+    k_cols_lin(ctx, B.keys.get(), nB, wB, &colB, 1, &big, jb.get());
+    k_equal_ranges(ctx, ja.get(), nA, jb.get(), nB, lo.get(), hi.get());
+  } else {
+    k_walk_ranges(ctx, mode, A.keys.get(), wA, nA, B.keys.get(), wB, nB, &colA, &colB, 1, lo.get(), hi.get());
+  }
+}
+
+// contract! COO x COO (contract_coo.jl:93-170): every nonzero product is
+// pushed, duplicates are not coalesced, C is left dirty
+static void contract_coo_coo(Plan& pl, Tensor& C, const Labels& labelsC, CooOper& A, CooOper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "coo_coo";
+  coo_rlab_last_sorted(pl, A, rlab);
+  coo_rlab_last_sorted(pl, B, rlab);
+  int NC = C.N(), NA = A.t->N(), NB = B.t->N();
+  std::vector<int> src(NC);
+  for (int t = 0; t < NC; ++t) {
+    if (has(A.map, labelsC[t]))
+      src[t] = at(A.map, labelsC[t]);
+    else
+      src[t] = -at(B.map, labelsC[t]);
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 nA = TA.n, nB = TB.n;
+  C.keys.release();
+  C.data.release();
+  C.n = 0;
+  C.dirty = true;
+  if (!nA || !nB) return;
+  DBuf<i32> lo(ctx, nA), hi(ctx, nA);
+  coo_join(ctx, 1, TA, NA, NA, TB, NB, NB, lo, hi);
+  Pairs pr = pairs_from_ranges(ctx, lo, hi, nA);
+  if (!pr.n) return;
+  pl.run_pending();  // operand values (gathered by the sort steps) decide which products are kept
+  DBuf<double> prod_(ctx, pr.n);
+  DBuf<unsigned char> keep(ctx, pr.n);
+  k_coo_products(ctx, TA.data.get(), TB.data.get(), pr.a.get(), pr.b.get(), pr.n, prod_.get(), keep.get());
+  // keep the surviving pairs and their products (stable)
+  DBuf<i32> idx(ctx, pr.n), none;
+  k_iota(ctx, idx.get(), pr.n, 0);
+  i64 m = compact_pairs(ctx, keep.get(), idx, none, pr.n);
+  C.n = m;
+  pl.npairs = pr.n;
+  pl.nblocks_out = m;
+  pl.flops += (double)pr.n;
+  pl.gemm_bytes += 8.0 * (2.0 * pr.n) + (8.0 + 4.0 * NC) * m;
+  if (!m) return;
+  DBuf<i32> pa2(ctx, m), pb2(ctx, m);
+  k_gather_rows(ctx, pr.a.get(), 1, idx.get(), m, pa2.get());
+  k_gather_rows(ctx, pr.b.get(), 1, idx.get(), m, pb2.get());
+  C.keys.alloc(ctx, (size_t)m * NC);
+  C.data.alloc(ctx, m);
+  DBuf<u64> lin(ctx, m);
+  std::vector<i64> bigd(NC, 1);
+  k_pair_keys(ctx, pa2.get(), pb2.get(), m, TA.keys.get(), NA, TB.keys.get(), NB, src.data(), NC, bigd.data(),
+              C.keys.get(), lin.get());
+  i32* pa = pl.keep(std::move(pa2));
+  i32* pb = pl.keep(std::move(pb2));
+  const double *va = TA.data.get(), *vb = TB.data.get();
+  double* out = C.data.get();
+  DBuf<unsigned char> kk(ctx, m);
+  unsigned char* kp = pl.keep(std::move(kk));
+  pl.steps.push_back({"coo_products", [=]() { k_coo_products(ctx, va, vb, pa, pb, m, out, kp); }});
+}
+
+// contract_prefix! COO x BS (contract_coo_bs.jl:51-124)
+static void coo_bs_prefix(Plan& pl, Tensor& C, const Labels& labelsC, CooOper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "coo_bs_prefix";
+  if (C.N2 != B.t->N2) fail(SB_E_METHOD, "MethodError: contract_prefix! needs C and B with the same N2");
+  int NA = A.t->N(), PB = B.t->P(), PC = C.P();
+  SB_ASSERT(PC == NA + PB - 2, "PC == NA + PB - 2");
+  int axisAr = at(A.map, rlab), axisBr = at(B.map, rlab);
+  SB_ASSERT(axisBr <= PB, "axisBr <= PB");
+  const Tensor &TA = *A.t, &TB = *B.t;
+  SB_ASSERT(C.blksize == TB.blksize, "length(Cvec) == length(Bvec)");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    if (has(A.map, labelsC[j]))
+      src[j] = at(A.map, labelsC[j]);
+    else {
+      int ax = at(B.map, labelsC[j]);
+      SB_ASSERT(ax <= PB, "ax <= PB");
+      src[j] = -ax;
+    }
+  }
+  i64 nA = TA.n, nB = TB.n;
+  DBuf<i32> lo(ctx, nA ? nA : 1), hi(ctx, nA ? nA : 1);
+  if (nA && nB) coo_join(ctx, 0, TA, NA, axisAr, TB, PB, axisBr, lo, hi);
+  Pairs pr = (nA && nB) ? pairs_from_ranges(ctx, lo, hi, nA) : Pairs();
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, NA, &TB, PB, src, al);
+  i64 np = pr.n, ng = al.nC, L = C.blksize;
+  pl.flops += 2.0 * L * np;
+  pl.gemm_bytes += 8.0 * (double)np * (L + 1) + 8.0 * ng * L;
+  if (!np) return;
+  DBuf<double> alpha(ctx, np);
+  DBuf<i64> xo(ctx, np);
+  i64* xoff = pair_offsets(pl, TB, pr.b.get(), np);
+  i32* pa = pl.keep(std::move(pr.a));
+  double* al_ = pl.keep(std::move(alpha));
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  pl.keep(std::move(pr.b));
+  const double *va = TA.data.get(), *X = TB.data.get();
+  double* Cd = C.data.get();
+  pl.steps.push_back({"axpy_groups", [=]() {
+                        k_take_f64(ctx, va, pa, np, al_);
+                        axpy_groups(ctx, L, al_, X, xoff, Cd, coff, seg, order, ng);
+                      }});
+}
+
+// contract_dense_index! COO x BS (contract_coo_bs.jl:140-241)
+static void coo_bs_dense_index(Plan& pl, Tensor& C, const Labels& labelsC, CooOper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "coo_bs_dense_r";
+  SB_ASSERT(!contains(labelsC, rlab), "rlab must be reduced (not in labelsC)");
+  int PB = B.t->P(), NB = B.t->N(), N2 = B.t->N2, NA = A.t->N(), PC = C.P(), NC = C.N();
+  SB_ASSERT(C.N2 == N2 - 1, "N2C == N2 - 1");
+  int axisBr = at(B.map, rlab);
+  SB_ASSERT(axisBr > PB, "rlab must be in B dense tail");
+  int redpos = axisBr - PB;
+  int axisAr = at(A.map, rlab);
+  SB_ASSERT(axisAr == NA, "Expected A to have rlab as last axis already (axisAr == NA)");
+  Labels Bd = sub(B.labels, PB, NB), Bdp = Bd;
+  LabelMap mapB0 = B.map;
+  if (redpos != N2) {
+    std::vector<i64> pd, pg;
+    for (int d = 1; d <= N2; ++d)
+      if (d != redpos) pd.push_back(d);
+    pd.push_back(redpos);
+    for (int a = 1; a <= PB; ++a) pg.push_back(a);
+    for (auto d : pd) pg.push_back(PB + d);
+    B.t = plan_bs_permute(pl, *B.t, pg);
+    for (int i = 0; i < N2; ++i) Bdp[i] = Bd[pd[i] - 1];
+  }
+  SB_ASSERT(Bdp.back() == rlab, "labelsB_dense_p[end] == rlab");
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 dim_r = TB.dims[NB - 1];
+  i64 chunk = prod(TB.dims, PB, NB - 1);
+  SB_ASSERT(sub(labelsC, PC, NC) == sub(Bdp, 0, N2 - 1), "collect(labelsC[PC+1:NC]) == expectedC_dense");
+  for (int i = 0; i < C.N2; ++i) SB_ASSERT(C.dims[PC + i] == TB.dims[PB + i], "C dense dims == B dense dims");
+  SB_ASSERT(C.blksize == chunk, "C.blksize == chunklen");
+  SB_ASSERT(TB.blksize == chunk * dim_r, "Bp.blksize == chunklen * dim_r");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    if (has(A.map, labelsC[j]))
+      src[j] = at(A.map, labelsC[j]);
+    else {
+      int ax = at(mapB0, labelsC[j]);
+      SB_ASSERT(ax <= PB, "Label in C prefix must come from B prefix (not dense)");
+      src[j] = -ax;
+    }
+  }
+  i64 nA = TA.n, nB = TB.n;
+  if (nA) {
+    std::vector<i32> hk((size_t)nA * NA);
+    HIP_CHECK(hipMemcpyAsync(hk.data(), TA.keys.get(), hk.size() * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (i64 i = 0; i < nA; ++i) {
+      i64 rv = hk[i * NA + NA - 1];
+      SB_ASSERT(1 <= rv && rv <= dim_r, "Reduction index rv out of bounds for dim_r");
+    }
+  }
+  Pairs pr;
+  DBuf<i64> blkoff = block_offsets(ctx, TB);
+  if (nA && nB) {
+    pl.run_pending();  // existence depends on the (permuted) payload of B
+    i64 np0 = nA * nB;
+    if (np0 >= (1ll << 31)) fail(SB_E_LIMIT, "more than 2^31 entry-block pairs in one contraction");
+    pr.a.alloc(ctx, np0);
+    pr.b.alloc(ctx, np0);
+    k_cross_pairs(ctx, nA, nB, pr.a.get(), pr.b.get());
+    DBuf<unsigned char> keep(ctx, np0);
+    k_scaled_slice_nonzero(ctx, TA.data.get(), TA.keys.get(), NA, TB.data.get(), blkoff.get(), pr.a.get(),
+                           pr.b.get(), np0, chunk, keep.get());
+    pr.n = compact_pairs(ctx, keep.get(), pr.a, pr.b, np0);
+  }
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, NA, &TB, PB, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * chunk * np;
+  pl.gemm_bytes += 8.0 * (double)np * (chunk + 1) + 8.0 * ng * chunk;
+  if (!np) return;
+  DBuf<i64> xo(ctx, np);
+  k_rv_offsets(ctx, TA.keys.get(), NA, NA, pr.a.get(), blkoff.get(), pr.b.get(), np, chunk, xo.get());
+  i64* xoff = pl.keep(std::move(xo));
+  DBuf<double> alpha(ctx, np);
+  double* al_ = pl.keep(std::move(alpha));
+  i32* pa = pl.keep(std::move(pr.a));
+  pl.keep(std::move(pr.b));
+  pl.keep(std::move(blkoff));
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  const double *va = TA.data.get(), *X = TB.data.get();
+  double* Cd = C.data.get();
+  pl.steps.push_back({"axpy_groups", [=]() {
+                        k_take_f64(ctx, va, pa, np, al_);
+                        axpy_groups(ctx, chunk, al_, X, xoff, Cd, coff, seg, order, ng);
+                      }});
+}
+
+// contract! COO x Dense (contract_coo_dense.jl:48-125)
+static void contract_coo_dense(Plan& pl, Tensor& C, const Labels& labelsC, CooOper& A, Oper& B, const Label& rlab,
+                               unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "coo_dense";
+  int PC = C.P(), NC = C.N(), NA = A.t->N(), NB = B.t->N();
+  {  // aligned_A_to_Cprefix (contract_coo_dense.jl:4-18)
+    Labels desired = sub(labelsC, 0, PC);
+    desired.push_back(rlab);
+    LabelMap m = make_map(A.labels);
+    SB_ASSERT((int)desired.size() == NA, "length(desired) == NA");
+    std::vector<i64> perm(NA);
+    for (int j = 0; j < NA; ++j) perm[j] = at(m, desired[j]);
+    A.t = plan_coo_permute_sort(pl, *A.t, perm);
+    A.labels = desired;
+    A.map = make_map(desired);
+  }
+  SB_ASSERT(A.labels.back() == rlab, "labelsA[end] == rlab");
+  Labels dnr;
+  for (int j = PC; j < NC; ++j)
+    if (labelsC[j] != rlab) dnr.push_back(labelsC[j]);
+  int rpos = -1;
+  for (int i = 0; i < NB; ++i)
+    if (B.labels[i] == rlab) {
+      rpos = i + 1;
+      break;
+    }
+  SB_ASSERT(rpos > 0, "rlab not found in labelsB");
+  std::vector<i64> perm(NB, 0);
+  for (size_t k = 0; k < dnr.size(); ++k) {
+    int pos = -1;
+    for (int i = 0; i < NB; ++i)
+      if (B.labels[i] == dnr[k]) {
+        pos = i + 1;
+        break;
+      }
+    SB_ASSERT(pos > 0, "label not found in labelsB");
+    if ((int)k >= NB) fail(SB_E_BOUNDS, "BoundsError: perm");
+    perm[k] = pos;
+  }
+  perm[NB - 1] = rpos;
+  const Tensor* Bt = B.t;
+  Labels Bl = B.labels;
+  if (!is_identity(perm)) {
+    for (auto p : perm)
+      if (p < 1) fail(SB_E_ARGUMENT, "invalid permutation of B");
+    std::vector<i64> s(perm);
+    std::sort(s.begin(), s.end());
+    for (int i = 0; i < NB; ++i)
+      if (s[i] != i + 1) fail(SB_E_ARGUMENT, "invalid permutation of B");
+    Bt = plan_dense_permute(pl, *Bt, perm);
+    Bl = permute_labels(B.labels, perm);
+  }
+  SB_ASSERT(Bl.back() == rlab, "labelsB[end] == rlab");
+  SB_ASSERT(!contains(labelsC, rlab), "rlab must be reduced (not in labelsC)");
+  Labels ep = sub(A.labels, 0, NA - 1), ed = sub(Bl, 0, NB - 1);
+  SB_ASSERT(PC == (int)ep.size(), "PC mismatch");
+  SB_ASSERT(C.N2 == (int)ed.size(), "N2 mismatch");
+  SB_ASSERT(sub(labelsC, 0, PC) == ep, "C prefix labels must be A labels without r");
+  SB_ASSERT(sub(labelsC, PC, NC) == ed, "C dense labels must be B labels without r");
+  i64 R = Bt->dims[NB - 1];
+  i64 chunk = prod(Bt->dims, 0, NB - 1);
+  SB_ASSERT(C.blksize == chunk, "C.blksize must equal prod(B dims without r)");
+  const Tensor& TA = *A.t;
+  i64 nA = TA.n;
+  Pairs pr;
+  if (nA) {
+    pr.a.alloc(ctx, nA);
+    k_iota(ctx, pr.a.get(), nA, 0);
+    DBuf<unsigned char> keep(ctx, nA);
+    k_rv_inrange(ctx, TA.keys.get(), NA, NA, nA, R, keep.get());
+    if (flags & SB_FLAG_SKIP_ZERO_SLICES) {
+      pl.run_pending();
+      k_coo_dense_nonzero(ctx, TA.data.get(), TA.keys.get(), NA, NA, nA, Bt->dptr(), chunk, keep.get());
+    }
+    pr.n = compact_pairs(ctx, keep.get(), pr.a, pr.b, nA);
+  }
+  std::vector<int> src(PC);
+  std::iota(src.begin(), src.end(), 1);
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, &TA, NA, nullptr, 0, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * chunk * np;
+  pl.gemm_bytes += 8.0 * (double)np * (chunk + 1) + 8.0 * ng * chunk;
+  if (!np) return;
+  DBuf<i64> xo(ctx, np);
+  k_rv_offsets(ctx, TA.keys.get(), NA, NA, pr.a.get(), nullptr, nullptr, np, chunk, xo.get());
+  i64* xoff = pl.keep(std::move(xo));
+  DBuf<double> alpha(ctx, np);
+  double* al_ = pl.keep(std::move(alpha));
+  i32* pa = pl.keep(std::move(pr.a));
+  i64* coff = group_offsets(pl, C);
+  i32* seg = pl.keep(std::move(al.seg_start));
+  i32* order = pl.keep(std::move(al.order));
+  const double *va = TA.data.get(), *X = Bt->dptr();
+  double* Cd = C.data.get();
+  pl.steps.push_back({"axpy_groups", [=]() {
+                        k_take_f64(ctx, va, pa, np, al_);
+                        axpy_groups(ctx, chunk, al_, X, xoff, Cd, coff, seg, order, ng);
+                      }});
+}
+
+// ================================================================ dispatcher
+std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, const Tensor& A,
+                                    const Labels& labelsA, const Tensor& B, const Labels& labelsB, unsigned flags) {
+  SB_ASSERT(labelsA.size() == A.dims.size(), "labelsA length must match A rank");
+  SB_ASSERT(labelsB.size() == B.dims.size(), "labelsB length must match B rank");
+  SB_ASSERT(labelsC.size() == C.dims.size(), "labelsC length must match C rank");
+  auto pl = std::make_unique<Plan>();
+  pl->ctx = ctx;
+  LabelMap mapA = make_map(labelsA), mapB = make_map(labelsB);
+  Labels shared, reduced;
+  for (auto& l : labelsA)
+    if (has(mapB, l)) shared.push_back(l);
+  for (auto& l : shared)
+    if (!contains(labelsC, l)) reduced.push_back(l);
+  Oper Ao{&A, labelsA, mapA}, Bo{&B, labelsB, mapB};
+  if (reduced.size() == 1) {
+    SB_ASSERT(shared.size() == 1, "Expected exactly 1 shared label");
+    const Label& r = reduced[0];
+    i64 da = A.dims[at(mapA, r) - 1], db = B.dims[at(mapB, r) - 1];
+    SB_ASSERT(da == db, "Dim mismatch on shared label");
+    SB_ASSERT(!contains(labelsC, r), "Reduction label must NOT appear in output labels");
+    if (C.kind == BS && A.kind == BS && B.kind == BS) {
+      // contract! BS x BS (contract_bs_bs.jl:126-151)
+      int axisBr = at(mapB, r);
+      if (axisBr <= B.P()) {
+        permute_r_to_last_prefix(*pl, Ao, r);
+        permute_r_to_last_prefix(*pl, Bo, r);
+        prefix_outer_bb(*pl, C, labelsC, Ao, Bo, r);
+      } else {
+        dense_bb(*pl, C, labelsC, Ao, Bo, r, flags);
+      }
+    } else if (C.kind == BS && A.kind == BS && B.kind == DENSE) {
+      // contract! BS x Dense (contract_bs_dense.jl:207-245)
+      SB_ASSERT(has(mapA, r), "rlab not found in mapA");
+      SB_ASSERT(has(mapB, r), "rlab not found in mapB");
+      if (at(mapA, r) <= A.P()) {
+        permute_r_to_last_prefix(*pl, Ao, r);
+        prefix_outer_bd(*pl, C, labelsC, Ao, Bo, r);
+      } else {
+        dense_bd(*pl, C, labelsC, Ao, Bo, r, flags);
+      }
+    } else if (C.kind == COO && A.kind == COO && B.kind == COO) {
+      CooOper Ac{&A, labelsA, mapA}, Bc{&B, labelsB, mapB};
+      contract_coo_coo(*pl, C, labelsC, Ac, Bc, r);
+    } else if (C.kind == BS && A.kind == COO && B.kind == BS) {
+      // contract! COO x BS (contract_coo_bs.jl:14-46)
+      CooOper Ac{&A, labelsA, mapA};
+      coo_rlab_last_sorted(*pl, Ac, r);
+      if (at(mapB, r) <= B.P()) {
+        permute_r_to_last_prefix(*pl, Bo, r);
+        coo_bs_prefix(*pl, C, labelsC, Ac, Bo, r);
+      } else {
+        coo_bs_dense_index(*pl, C, labelsC, Ac, Bo, r);
+      }
+    } else if (C.kind == BS && A.kind == COO && B.kind == DENSE) {
+      CooOper Ac{&A, labelsA, mapA};
+      contract_coo_dense(*pl, C, labelsC, Ac, Bo, r, flags);
+    } else {
+      fail(SB_E_METHOD, "MethodError: no single-label contract! method for these storage kinds");
+    }
+  } else {
+    if (C.kind == BS && A.kind == BS && B.kind == BS)
+      shared_bs_bs(*pl, C, labelsC, Ao, Bo, shared, flags);
+    else if (C.kind == BS && A.kind == BS && B.kind == DENSE)
+      shared_bs_dense(*pl, C, labelsC, Ao, Bo, shared, flags);
+    else
+      fail(SB_E_METHOD, "MethodError: no contract_shared! method for these storage kinds");
+  }
+  return pl;
+}
+
+}  // namespace sb

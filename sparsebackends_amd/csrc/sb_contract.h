@@ -1,0 +1,75 @@
+// Contraction plans: the symbolic phase runs once at plan time; the numeric
+// phase is a fixed list of kernel launches over fixed buffers, replayable
+// whenever the operands' values change but their block structure does not.
+#pragma once
+
+#include <functional>
+#include <memory>
+
+#include "sb_ops.h"
+
+namespace sb {
+
+struct Label {
+  uint64_t id, plev;
+  bool operator==(const Label& o) const { return id == o.id && plev == o.plev; }
+  bool operator!=(const Label& o) const { return !(*this == o); }
+  bool operator<(const Label& o) const { return id < o.id || (id == o.id && plev < o.plev); }
+};
+using Labels = std::vector<Label>;
+
+struct Step {
+  const char* name;  // kernel family, for timing
+  std::function<void()> fn;
+  void operator()() const { fn(); }
+};
+
+struct Plan {
+  Ctx* ctx = nullptr;
+  std::vector<Step> steps;
+  std::vector<std::shared_ptr<void>> hold;
+  // accounting of the numeric phase (per execution)
+  double flops = 0;        // sum of 2*M*K*N over contributing pairs (actual extents)
+  double gemm_bytes = 0;   // algorithmic bytes of the block products (operands read once + C written)
+  double perm_bytes = 0;   // bytes moved by operand permutes (read + write)
+  i64 npairs = 0, nblocks_out = 0;
+  std::string family;
+
+  template <class T>
+  T* keep(DBuf<T>&& b) {
+    auto sp = std::make_shared<DBuf<T>>(std::move(b));
+    hold.push_back(sp);
+    return sp->get();
+  }
+  void keep_tensor(TP&& t) { hold.push_back(std::shared_ptr<Tensor>(t.release())); }
+  size_t done = 0;  // steps [0, done) already ran during planning
+  double a_bytes = 0, b_bytes = 0, c_bytes = 0;  // distinct payload bytes of the block-product operands
+  void run_step(const Step& s) const {
+    if (!ctx->profile) {
+      s();
+      return;
+    }
+    hipEvent_t a = ctx->event(), b = ctx->event();
+    (void)hipEventRecord(a, ctx->stream);
+    s();
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->prof_pending.push_back({s.name, a, b});
+  }
+  // every step (a replay with refreshed operand values)
+  void run() const {
+    for (auto& s : steps) run_step(s);
+  }
+  // the steps not yet run (planning needs some values mid-way)
+  void run_pending() {
+    for (size_t i = done; i < steps.size(); ++i) run_step(steps[i]);
+    done = steps.size();
+  }
+};
+
+// contract!(C, labelsC, A, labelsA, B, labelsB) (SparseBackends/src/tensoralgebra/contract.jl:37-104).
+// C is overwritten: its keys/ids are set at plan time, its payload by every
+// plan execution.  The plan keeps pointers to A's and B's payloads.
+std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, const Tensor& A,
+                                    const Labels& labelsA, const Tensor& B, const Labels& labelsB, unsigned flags);
+
+}  // namespace sb

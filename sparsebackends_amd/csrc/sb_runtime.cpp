@@ -1,0 +1,130 @@
+// Runtime: error state, contexts (device + stream), caching device allocator.
+#include <cstdio>
+#include <cstdlib>
+#include <algorithm>
+
+#include "sb_internal.h"
+
+namespace sb {
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+i64 prod(const std::vector<i64>& v, size_t lo, size_t hi) {
+  i64 p = 1;
+  for (size_t i = lo; i < hi; ++i) p *= v[i];
+  return p;
+}
+
+// ---------------------------------------------------------------- allocator
+size_t DeviceCache::cls(size_t b) {
+  if (b <= 256) return 256;
+  // round up to 1/8 steps of the next power of two (bounded waste)
+  size_t p = 256;
+  while (p < b) p <<= 1;
+  size_t step = p / 8;
+  return ((b + step - 1) / step) * step;
+}
+
+void* DeviceCache::get(size_t bytes) {
+  size_t c = cls(bytes);
+  auto it = free_.lower_bound(c);
+  if (it != free_.end() && it->first <= c + c / 4) {
+    void* p = it->second;
+    size_t sz = it->first;
+    free_.erase(it);
+    cached_ -= sz;
+    live_[p] = sz;
+    in_use_ += sz;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim();
+    HIP_CHECK(hipMalloc(&p, c));
+  }
+  live_[p] = c;
+  in_use_ += c;
+  return p;
+}
+
+void DeviceCache::put(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_.emplace(it->second, p);
+  cached_ += it->second;
+  in_use_ -= it->second;
+  live_.erase(it);
+}
+
+void DeviceCache::trim() {
+  // caller guarantees no queued work uses cached blocks (device synchronised)
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+  cached_ = 0;
+}
+
+DeviceCache::~DeviceCache() {
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(device_);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  for (auto& kv : live_) (void)hipFree(kv.first);
+  (void)hipSetDevice(cur);
+}
+
+Ctx::Ctx(int dev) : device(dev), cache(dev) {
+  HIP_CHECK(hipSetDevice(dev));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) num_cu = n;
+}
+
+Ctx::~Ctx() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& r : prof_pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : event_pool) (void)hipEventDestroy(e);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+hipEvent_t Ctx::event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::prof_collect() {
+  HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto& r : prof_pending) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    ProfAgg& g = prof[r.name];
+    g.count += 1;
+    g.ms += ms;
+    g.min_ms = std::min(g.min_ms, (double)ms);
+    g.max_ms = std::max(g.max_ms, (double)ms);
+    event_pool.push_back(r.a);
+    event_pool.push_back(r.b);
+  }
+  prof_pending.clear();
+}
+
+Tensor* as_tensor(void* h, int kind) {
+  if (!h) fail(SB_E_HANDLE, "null tensor handle");
+  Tensor* t = static_cast<Tensor*>(h);
+  if (kind && t->kind != kind) fail(SB_E_HANDLE, "tensor handle of the wrong storage kind");
+  return t;
+}
+
+}  // namespace sb

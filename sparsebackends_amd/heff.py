@@ -1,0 +1,368 @@
+"""Two-site H_eff*psi of DMRG on symmetry-sector tensors, expressed as four
+reference contract! calls on NewBlockSparseSorted tensors.
+
+Sector structure (U(1) or U(1)xU(1) charges) is mapped onto the reference's
+prefix-sparse format: every bond index becomes a sector axis (in the key
+prefix) plus an in-sector axis (in the dense tail, padded to the largest
+sector of the bond); physical and MPO-link indices stay in the dense tail.
+The MPO site tensor is one dense block (P = 0).
+
+The contraction order is psi*L*W1*W2*R (the order ITensorMPS uses for
+product(ProjMPO, psi)); each step is contract_shared! BS x BS
+(tensoralgebra/contract_bs_bs.jl:572-730) with at most one shared key axis.
+
+    psi  [Ql, Qr | l, s1, s2, r]
+    L    [Ql', Ql | l', wl, l]          R [Qr', Qr | r', wr, r]
+    W1   [ | wl, s1', s1, wm]           W2 [ | wm, s2', s2, wr]
+    T1 = psi*L   [Ql', Qr | s1, s2, r, l', wl]
+    T2 = T1*W1   [Ql', Qr | s2, r, l', s1', wm]
+    T3 = T2*W2   [Ql', Qr | r, l', s1', s2', wr]
+    Hpsi = T3*R  [Ql', Qr' | l', s1', s2', r']
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .storage import NewBlockSparseSorted
+from .tensoralgebra import Plan, contract_
+
+# ------------------------------------------------------------------ sectors
+
+
+def largest_remainder(weights, total):
+    """Integer dims proportional to weights summing to total (largest remainder)."""
+    w = np.asarray(weights, dtype=np.float64)
+    raw = w / w.sum() * total
+    base = np.floor(raw).astype(np.int64)
+    rem = int(total - base.sum())
+    order = np.argsort(-(raw - base), kind="stable")
+    base[order[:rem]] += 1
+    return base
+
+
+class Bond:
+    """A bond index split into charge sectors: charges[i] (tuple), dims[i]."""
+
+    def __init__(self, charges, dims):
+        keep = [i for i, d in enumerate(dims) if d > 0]
+        self.charges = [tuple(charges[i]) for i in keep]
+        self.dims = [int(dims[i]) for i in keep]
+        self.index = {q: i for i, q in enumerate(self.charges)}
+
+    @property
+    def nsec(self):
+        return len(self.charges)
+
+    @property
+    def dmax(self):
+        return max(self.dims)
+
+    @property
+    def dim(self):
+        return sum(self.dims)
+
+
+def qadd(a, b):
+    return tuple(x + y for x, y in zip(a, b))
+
+
+def qsub(a, b):
+    return tuple(x - y for x, y in zip(a, b))
+
+
+# ------------------------------------------------------------------ models
+
+
+class SiteModel:
+    """Local space: charges of the d states; MPO W[wl, s', s, wr] (dense) and
+    the charge Qlink[w] of every MPO link state, with W nonzero only where
+    q(s') - q(s) = Qlink[wl] - Qlink[wr]."""
+
+    def __init__(self, name, charges, W, Qlink):
+        self.name = name
+        self.charges = [tuple(c) for c in charges]
+        self.W = np.asarray(W, dtype=np.float64)
+        self.Qlink = [tuple(q) for q in Qlink]
+        self.d = len(charges)
+        self.w = self.W.shape[0]
+
+
+def heisenberg_model(J=1.0, Jz=None):
+    """S=1/2 XXZ chain MPO, w=5: index 0 = done, 4 = start; charge q = 2Sz."""
+    Jz = J if Jz is None else Jz
+    I = np.eye(2)
+    Sz = np.diag([0.5, -0.5])
+    Sp = np.array([[0.0, 1.0], [0.0, 0.0]])  # |up><dn|, state 0 = up
+    Sm = Sp.T
+    W = np.zeros((5, 2, 2, 5))
+    W[0, :, :, 0] = I
+    W[1, :, :, 0] = Sp
+    W[2, :, :, 0] = Sm
+    W[3, :, :, 0] = Sz
+    W[4, :, :, 1] = 0.5 * J * Sm
+    W[4, :, :, 2] = 0.5 * J * Sp
+    W[4, :, :, 3] = Jz * Sz
+    W[4, :, :, 4] = I
+    return SiteModel("heisenberg", [(1,), (-1,)], W, [(0,), (2,), (-2,), (0,), (0,)])
+
+
+def j1j2_model(J1=1.0, J2=0.0):
+    """S=1/2 J1-J2 chain MPO (w=8): nearest and next-nearest Heisenberg couplings."""
+    I = np.eye(2)
+    Sz = np.diag([0.5, -0.5])
+    Sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    Sm = Sp.T
+    W = np.zeros((8, 2, 2, 8))
+    # 0 done, 7 start; 1..3 carry (S+, S-, Sz) one site; 4..6 carry them two sites
+    W[0, :, :, 0] = I
+    W[1, :, :, 0] = Sp
+    W[2, :, :, 0] = Sm
+    W[3, :, :, 0] = Sz
+    W[4, :, :, 1] = I
+    W[5, :, :, 2] = I
+    W[6, :, :, 3] = I
+    W[7, :, :, 1] = 0.5 * J1 * Sm
+    W[7, :, :, 2] = 0.5 * J1 * Sp
+    W[7, :, :, 3] = J1 * Sz
+    W[7, :, :, 4] = 0.5 * J2 * Sm
+    W[7, :, :, 5] = 0.5 * J2 * Sp
+    W[7, :, :, 6] = J2 * Sz
+    W[7, :, :, 7] = I
+    return SiteModel("j1j2", [(1,), (-1,)], W, [(0,), (2,), (-2,), (0,), (2,), (-2,), (0,), (0,)])
+
+
+def hubbard_model(t=1.0, U=4.0):
+    """Hubbard chain MPO (w=6) with Jordan-Wigner strings; states |0>,|up>,|dn>,|updn>
+    with charges (n, 2Sz); index 0 = done, 5 = start."""
+    d = 4
+    # c_up, c_dn on the site basis [0, up, dn, updn] with up ordered before dn
+    cu = np.zeros((d, d))
+    cd = np.zeros((d, d))
+    cu[0, 1] = 1.0   # c_up |up> = |0>
+    cu[2, 3] = 1.0   # c_up |updn> = |dn>
+    cd[0, 2] = 1.0   # c_dn |dn> = |0>
+    cd[1, 3] = -1.0  # c_dn |updn> = -|up>  (passes c_up^dag)
+    F = np.diag([1.0, -1.0, -1.0, 1.0])
+    I = np.eye(d)
+    nu = cu.T @ cu
+    nd = cd.T @ cd
+    W = np.zeros((6, d, d, 6))
+    W[0, :, :, 0] = I
+    W[5, :, :, 5] = I
+    W[5, :, :, 0] = U * nu @ nd
+    # c^dag_{i s} c_{i+1 s} and h.c., strings on the left operator
+    W[5, :, :, 1] = -t * cu.T @ F
+    W[5, :, :, 2] = -t * cd.T @ F
+    W[5, :, :, 3] = t * cu @ F
+    W[5, :, :, 4] = t * cd @ F
+    W[1, :, :, 0] = cu
+    W[2, :, :, 0] = cd
+    W[3, :, :, 0] = cu.T
+    W[4, :, :, 0] = cd.T
+    charges = [(0, 0), (1, 1), (1, -1), (2, 0)]
+    Qlink = [(0, 0), (-1, -1), (-1, 1), (1, 1), (1, -1), (0, 0)]
+    return SiteModel("hubbard", charges, W, Qlink)
+
+
+# ------------------------------------------------------------------ sector layouts of the configs
+
+
+def heisenberg_bond(chi=256, qmax=8, width=32.0):
+    """q in {-qmax..qmax}, dims proportional to exp(-q^2/width), summing to chi."""
+    qs = list(range(-qmax, qmax + 1))
+    dims = largest_remainder([np.exp(-q * q / width) for q in qs], chi)
+    return Bond([(q,) for q in qs], dims)
+
+
+def hubbard_bond(chi=2048, n0=0, sigma=7.0, span=3.0):
+    """(n, 2Sz) sectors around (n0, 0) with dims from a 2D Gaussian of width sigma."""
+    r = int(np.ceil(span * sigma))
+    qs, w = [], []
+    for n in range(n0 - r, n0 + r + 1):
+        for s in range(-r, r + 1):
+            if (n - s) % 2:
+                continue  # 2Sz has the parity of n
+            qs.append((n, s))
+            w.append(np.exp(-((n - n0) ** 2 + s ** 2) / (2 * sigma * sigma)))
+    dims = largest_remainder(w, chi)
+    return Bond(qs, dims)
+
+
+# ------------------------------------------------------------------ problem
+
+
+def _lab(name):
+    return (int.from_bytes(("h:" + name).encode()[:8].ljust(8, b"\0"), "little"), 0)
+
+
+class HeffProblem:
+    """A two-site H_eff*psi at one bond: left bond sectors Bl, right Br,
+    site model m.  Random psi / L / R of the implied block structure (random
+    values on the charge-allowed entries, zero elsewhere and in the padding)."""
+
+    def __init__(self, model, Bl, Br, seed=0, device=0):
+        self.m, self.Bl, self.Br = model, Bl, Br
+        self.device = device
+        rng = np.random.default_rng(seed)
+        d, w = model.d, model.w
+        Dl, Dr = Bl.dmax, Br.dmax
+        self.Dl, self.Dr = Dl, Dr
+        qs, Ql = model.charges, model.Qlink
+        # psi [Ql, Qr | l, s1, s2, r] : q(r) = q(l) + q(s1) + q(s2)
+        psi = {}
+        for il, ql in enumerate(Bl.charges):
+            for ir, qr in enumerate(Br.charges):
+                m = np.zeros((d, d))
+                for s1 in range(d):
+                    for s2 in range(d):
+                        m[s1, s2] = qadd(qadd(ql, qs[s1]), qs[s2]) == qr
+                if m.any():
+                    psi[(il + 1, ir + 1)] = m
+        self.psi_parts = self._parts([Bl.nsec, Br.nsec], (Dl, d, d, Dr), psi,
+                                     lambda k: (Bl.dims[k[0] - 1], d, d, Br.dims[k[1] - 1]),
+                                     lambda k, blk: blk * psi[k][None, :, :, None], rng)
+
+        # environments: X(x', w, x) nonzero where q(x') - q(x) = -Qlink(w)
+        def env(B):
+            blocks = {}
+            for ip, qp in enumerate(B.charges):
+                for i, q in enumerate(B.charges):
+                    m = np.array([qsub(qp, q) == tuple(-c for c in Ql[x]) for x in range(w)], dtype=np.float64)
+                    if m.any():
+                        blocks[(ip + 1, i + 1)] = m
+            return blocks
+        Lb, Rb = env(Bl), env(Br)
+        self.L_parts = self._parts([Bl.nsec, Bl.nsec], (Dl, w, Dl), Lb,
+                                   lambda k: (Bl.dims[k[0] - 1], w, Bl.dims[k[1] - 1]),
+                                   lambda k, blk: blk * Lb[k][None, :, None], rng)
+        self.R_parts = self._parts([Br.nsec, Br.nsec], (Dr, w, Dr), Rb,
+                                   lambda k: (Br.dims[k[0] - 1], w, Br.dims[k[1] - 1]),
+                                   lambda k, blk: blk * Rb[k][None, :, None], rng)
+        # W [ | wl, s', s, wr] : one dense block
+        self.W_parts = dict(dims=(w, d, d, w), N2=4, keys=np.zeros((1, 0), dtype=np.int64),
+                            ids=np.array([1]), data=np.asfortranarray(model.W).reshape(-1, order="F"))
+        L_ = _lab
+        self.l_psi = [L_("Ql"), L_("Qr"), L_("l"), L_("s1"), L_("s2"), L_("r")]
+        self.l_L = [L_("Qlp"), L_("Ql"), L_("lp"), L_("wl"), L_("l")]
+        self.l_W1 = [L_("wl"), L_("s1p"), L_("s1"), L_("wm")]
+        self.l_W2 = [L_("wm"), L_("s2p"), L_("s2"), L_("wr")]
+        self.l_R = [L_("Qrp"), L_("Qr"), L_("rp"), L_("wr"), L_("r")]
+        self.l_T1 = [L_("Qlp"), L_("Qr"), L_("s1"), L_("s2"), L_("r"), L_("lp"), L_("wl")]
+        self.l_T2 = [L_("Qlp"), L_("Qr"), L_("s2"), L_("r"), L_("lp"), L_("s1p"), L_("wm")]
+        self.l_T3 = [L_("Qlp"), L_("Qr"), L_("r"), L_("lp"), L_("s1p"), L_("s2p"), L_("wr")]
+        self.l_out = [L_("Qlp"), L_("Qrp"), L_("lp"), L_("s1p"), L_("s2p"), L_("rp")]
+        nl, nr = Bl.nsec, Br.nsec
+        self.dims_T1 = (nl, nr, d, d, Dr, Dl, w)
+        self.dims_T2 = (nl, nr, d, Dr, Dl, d, w)
+        self.dims_T3 = (nl, nr, Dr, Dl, d, d, w)
+        self.dims_out = (nl, nr, Dl, d, d, Dr)
+
+    @staticmethod
+    def _parts(pdims, tdims, blocks, ext, mask, rng):
+        """random payload on the actual extents ext(key), masked; padding zero"""
+        keys = sorted(blocks, key=lambda k: tuple(reversed(k)))  # column-major order of the prefix
+        bs = int(np.prod(tdims))
+        data = np.zeros(len(keys) * bs)
+        for i, k in enumerate(keys):
+            e = ext(k)
+            blk = np.zeros(tdims, order="F")
+            sub = np.zeros(tdims)
+            sl = tuple(slice(0, x) for x in e)
+            sub[sl] = rng.standard_normal(e)
+            blk[...] = mask(k, sub)
+            data[i * bs:(i + 1) * bs] = blk.reshape(-1, order="F")
+        return dict(dims=tuple(pdims) + tuple(tdims), N2=len(tdims),
+                    keys=np.array(keys, dtype=np.int64).reshape(len(keys), len(pdims)),
+                    ids=np.arange(1, len(keys) + 1), data=data)
+
+    def upload(self):
+        mk = lambda p: NewBlockSparseSorted.from_parts(p["dims"], p["N2"], p["keys"], p["ids"], p["data"],  # noqa
+                                                       device=self.device)
+        self.psi, self.L, self.R = mk(self.psi_parts), mk(self.L_parts), mk(self.R_parts)
+        self.W1 = mk(self.W_parts)
+        self.W2 = mk(self.W_parts)
+        return self
+
+    def outputs(self):
+        new = lambda dims, N2: NewBlockSparseSorted.new(dims, N2, device=self.device)  # noqa
+        self.T1 = new(self.dims_T1, 5)
+        self.T2 = new(self.dims_T2, 5)
+        self.T3 = new(self.dims_T3, 5)
+        self.Hpsi = new(self.dims_out, 4)
+        return self
+
+    def contract_eager(self, flags=0):
+        """Hpsi through four contract! calls (block structure recomputed each call)."""
+        contract_(self.T1, self.l_T1, self.psi, self.l_psi, self.L, self.l_L, flags)
+        contract_(self.T2, self.l_T2, self.T1, self.l_T1, self.W1, self.l_W1, flags)
+        contract_(self.T3, self.l_T3, self.T2, self.l_T2, self.W2, self.l_W2, flags)
+        contract_(self.Hpsi, self.l_out, self.T3, self.l_T3, self.R, self.l_R, flags)
+        return self.Hpsi
+
+    def make_plans(self, flags=0):
+        """The four contractions as plans (structure fixed, values recomputed per execute)."""
+        self.plans = [Plan(self.T1, self.l_T1, self.psi, self.l_psi, self.L, self.l_L, flags)]
+        self.plans.append(Plan(self.T2, self.l_T2, self.T1, self.l_T1, self.W1, self.l_W1, flags))
+        self.plans.append(Plan(self.T3, self.l_T3, self.T2, self.l_T2, self.W2, self.l_W2, flags))
+        self.plans.append(Plan(self.Hpsi, self.l_out, self.T3, self.l_T3, self.R, self.l_R, flags))
+        return self.plans
+
+    def execute(self):
+        for p in self.plans:
+            p.execute()
+
+    # ---------------------------------------------------------- accounting
+    def algorithmic_flops(self):
+        """sum of 2*M*K*N over the matched block pairs of the four steps, with
+        the actual sector extents in place of the padded ones"""
+        d, w = self.m.d, self.m.w
+        dl, dr = self.Bl.dims, self.Br.dims
+        psi = [tuple(k) for k in self.psi_parts["keys"]]
+        L_by = {}
+        for k in self.L_parts["keys"]:
+            L_by.setdefault(int(k[1]), []).append(int(k[0]))
+        R_by = {}
+        for k in self.R_parts["keys"]:
+            R_by.setdefault(int(k[1]), []).append(int(k[0]))
+        f = [0.0, 0.0, 0.0, 0.0]
+        T1 = set()
+        for (il, ir) in psi:
+            for ilp in L_by.get(il, []):
+                f[0] += 2.0 * (d * d * dr[ir - 1]) * dl[il - 1] * (dl[ilp - 1] * w)
+                T1.add((ilp, ir))
+        for (ilp, ir) in T1:
+            f[1] += 2.0 * (d * dr[ir - 1] * dl[ilp - 1]) * (d * w) * (d * w)
+            f[2] += 2.0 * (dr[ir - 1] * dl[ilp - 1] * d) * (d * w) * (d * w)
+            for irp in R_by.get(ir, []):
+                f[3] += 2.0 * (dl[ilp - 1] * d * d) * (dr[ir - 1] * w) * dr[irp - 1]
+        return dict(total=sum(f), steps=f)
+
+    def padded_flops(self):
+        """the same sum with the padded extents the kernels execute"""
+        d, w, Dl, Dr = self.m.d, self.m.w, self.Dl, self.Dr
+        psi = [tuple(k) for k in self.psi_parts["keys"]]
+        L_by = {}
+        for k in self.L_parts["keys"]:
+            L_by.setdefault(int(k[1]), []).append(int(k[0]))
+        R_by = {}
+        for k in self.R_parts["keys"]:
+            R_by.setdefault(int(k[1]), []).append(int(k[0]))
+        n1 = sum(len(L_by.get(il, [])) for (il, ir) in psi)
+        T1 = {(ilp, ir) for (il, ir) in psi for ilp in L_by.get(il, [])}
+        n4 = sum(len(R_by.get(ir, [])) for (ilp, ir) in T1)
+        f = [2.0 * (d * d * Dr) * Dl * (Dl * w) * n1, 2.0 * (d * Dr * Dl) * (d * w) ** 2 * len(T1),
+             2.0 * (Dr * Dl * d) * (d * w) ** 2 * len(T1), 2.0 * (Dl * d * d) * (Dr * w) * Dr * n4]
+        return dict(total=sum(f), steps=f)
+
+
+def config1_problem(seed=0, device=0):
+    """Heisenberg S=1/2, J=1, chi=256 in Sz sectors q in {-8..8}, dims ~ exp(-q^2/32)."""
+    B = heisenberg_bond(256)
+    return HeffProblem(heisenberg_model(1.0), B, B, seed=seed, device=device)
+
+
+def config4_problem(seed=0, sigma=7.0, device=0):
+    """Hubbard t=1, U=4, chi=2048 in (n, 2Sz) sectors with a 2D Gaussian law of width sigma."""
+    Bl = hubbard_bond(2048, n0=0, sigma=sigma)
+    Br = hubbard_bond(2048, n0=2, sigma=sigma)
+    return HeffProblem(hubbard_model(1.0, 4.0), Bl, Br, seed=seed, device=device)

@@ -1,0 +1,355 @@
+"""Device-resident storage types mirroring the reference's
+NewBlockSparseSorted (SparseBackends/src/blocksparse/storage.jl:3-9),
+COOTensor (coo/storage.jl:1-6) and plain dense arrays.
+
+Field names, 1-based keys/ids and column-major payloads are the reference's.
+Reading .keys/.ids/.data copies them to the host (numpy).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import numpy as np
+
+from ._abi import load, check
+
+_ctx_lock = threading.Lock()
+_contexts = {}
+
+
+def context(device=0):
+    """The default context (HIP device + stream) of this thread and device."""
+    key = (threading.get_ident(), device)
+    with _ctx_lock:
+        if key not in _contexts:
+            h = C.c_void_p()
+            check(load().sb_ctx_create(device, C.byref(h)))
+            _contexts[key] = h
+        return _contexts[key]
+
+
+def synchronize(device=0):
+    check(load().sb_ctx_synchronize(context(device)))
+
+
+def profile(enable=True, device=0):
+    """Time every numeric plan step with HIP events on the library's stream."""
+    check(load().sb_ctx_profile(context(device), 1 if enable else 0))
+
+
+def profile_reset(device=0):
+    check(load().sb_ctx_profile_reset(context(device)))
+
+
+def profile_read(device=0):
+    """{kernel name: dict(count, total_ms, min_ms, max_ms)} (synchronises)."""
+    need = C.c_int64()
+    check(load().sb_ctx_profile_read(context(device), None, 0, C.byref(need)))
+    buf = C.create_string_buffer(int(need.value) + 1)
+    check(load().sb_ctx_profile_read(context(device), buf, len(buf), C.byref(need)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, tot, mn, mx = line.split()
+        out[name] = dict(count=int(cnt), total_ms=float(tot), min_ms=float(mn), max_ms=float(mx))
+    return out
+
+
+def _i64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class _Tensor:
+    kind = 0
+
+    def __init__(self, handle, device=0):
+        self._h = handle if isinstance(handle, C.c_void_p) else C.c_void_p(handle)
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def ctx(self):
+        return context(self.device)
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().sb_tensor_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def permutedims(self, perm):
+        """Base.permutedims (new tensor); perm is 1-based."""
+        out = C.c_void_p()
+        p = _i64(perm)
+        check(load().sb_permutedims(self.ctx, self._h, _ptr(p, C.c_int64), C.byref(out)))
+        return type(self)(out, self.device)
+
+    def permutedims_(self, perm):
+        """permutedims! (in place)."""
+        p = _i64(perm)
+        check(load().sb_permutedims_inplace(self.ctx, self._h, _ptr(p, C.c_int64)))
+        return self
+
+
+class DenseTensor(_Tensor):
+    """A dense column-major float64 array on the device."""
+    kind = 3
+
+    @staticmethod
+    def from_numpy(arr, device=0):
+        a = np.asarray(arr, dtype=np.float64)
+        flat = np.ascontiguousarray(a.reshape(-1, order="F"))
+        d = _i64(a.shape)
+        out = C.c_void_p()
+        check(load().sb_dense_from_host(context(device), a.ndim, _ptr(d, C.c_int64), _ptr(flat, C.c_double),
+                                        C.byref(out)))
+        return DenseTensor(out, device)
+
+    @staticmethod
+    def from_device_ptr(ptr, shape, borrow=True, device=0):
+        """Wrap (borrow) or copy caller device memory, e.g. torch's tensor.data_ptr()."""
+        d = _i64(shape)
+        out = C.c_void_p()
+        check(load().sb_dense_from_device(context(device), len(shape), _ptr(d, C.c_int64), C.c_void_p(ptr),
+                                          1 if borrow else 0, C.byref(out)))
+        return DenseTensor(out, device)
+
+    @property
+    def dims(self):
+        N = C.c_int()
+        d = np.zeros(16, dtype=np.int64)
+        check(load().sb_dense_info(self._h, C.byref(N), _ptr(d, C.c_int64)))
+        return tuple(int(x) for x in d[:N.value])
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        check(load().sb_dense_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def numpy(self):
+        shape = self.dims
+        flat = np.zeros(int(np.prod(shape)) if shape else 1, dtype=np.float64)
+        check(load().sb_dense_to_host(self.ctx, self._h, _ptr(flat, C.c_double)))
+        return flat.reshape(shape, order="F")
+
+
+class NewBlockSparseSorted(_Tensor):
+    """Prefix-sparse block tensor: the first P = N - N2 axes are sparse keys
+    sorted by their column-major linear index; the last N2 axes form one dense
+    block per key (blocksparse/storage.jl:3-13)."""
+    kind = 1
+
+    @staticmethod
+    def new(dims, N2, device=0):
+        """NewBlockSparseSorted{T,N,N2}(dims): no blocks (storage.jl:86-93)."""
+        d = _i64(dims)
+        out = C.c_void_p()
+        check(load().sb_bs_new(context(device), len(dims), N2, _ptr(d, C.c_int64), C.byref(out)))
+        return NewBlockSparseSorted(out, device)
+
+    @staticmethod
+    def from_parts(dims, N2, keys, ids, data, device=0):
+        d = _i64(dims)
+        k = _i64(keys).reshape(-1)
+        i = _i64(ids)
+        x = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
+        out = C.c_void_p()
+        check(load().sb_bs_from_host(context(device), len(dims), N2, _ptr(d, C.c_int64), len(i), _ptr(k, C.c_int64),
+                                     _ptr(i, C.c_int64), _ptr(x, C.c_double), C.byref(out)))
+        return NewBlockSparseSorted(out, device)
+
+    def info(self):
+        N, N2, nb, bs = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
+        d = np.zeros(16, dtype=np.int64)
+        check(load().sb_bs_info(self._h, C.byref(N), C.byref(N2), _ptr(d, C.c_int64), C.byref(nb), C.byref(bs)))
+        return tuple(int(x) for x in d[:N.value]), N2.value, nb.value, bs.value
+
+    @property
+    def dims(self):
+        return self.info()[0]
+
+    @property
+    def N2(self):
+        return self.info()[1]
+
+    @property
+    def nblocks(self):
+        return self.info()[2]
+
+    @property
+    def blksize(self):
+        return self.info()[3]
+
+    def parts(self):
+        """dims, N2, blksize, keys (nblocks x P), ids, data copied to the host."""
+        dims, N2, nb, bs = self.info()
+        P = len(dims) - N2
+        keys = np.zeros((nb, P), dtype=np.int64)
+        ids = np.zeros(nb, dtype=np.int64)
+        data = np.zeros(nb * bs, dtype=np.float64)
+        check(load().sb_bs_to_host(self.ctx, self._h, _ptr(keys, C.c_int64), _ptr(ids, C.c_int64),
+                                   _ptr(data, C.c_double)))
+        return dict(dims=dims, N2=N2, blksize=bs, keys=keys, ids=ids, data=data)
+
+    @property
+    def keys(self):
+        return self.parts()["keys"]
+
+    @property
+    def ids(self):
+        return self.parts()["ids"]
+
+    @property
+    def data(self):
+        return self.parts()["data"]
+
+    def device_ptrs(self):
+        k, i, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(load().sb_bs_device_ptrs(self._h, C.byref(k), C.byref(i), C.byref(d)))
+        return k.value, i.value, d.value
+
+    def __getitem__(self, I):
+        a = _i64(I)
+        v = C.c_double()
+        check(load().sb_bs_getindex(self.ctx, self._h, _ptr(a, C.c_int64), C.byref(v)))
+        return v.value
+
+    def __setitem__(self, I, v):
+        a = _i64(I)
+        check(load().sb_bs_setindex(self.ctx, self._h, float(v), _ptr(a, C.c_int64)))
+
+    def sort_(self):
+        """sort! (storage.jl:403-410): lexicographic keys, ids follow, data stays."""
+        check(load().sb_bs_sort_lex(self.ctx, self._h))
+        return self
+
+    def is_dense(self):
+        o = C.c_int()
+        check(load().sb_bs_is_dense(self._h, C.byref(o)))
+        return bool(o.value)
+
+    def to_dense(self, init=0.0, merged_axes=None):
+        """to_dense(B; init, merged_axes) (blocksparse/conversions.jl:83-187) -> numpy."""
+        return self.to_dense_tensor(init, merged_axes).numpy()
+
+    def to_dense_tensor(self, init=0.0, merged_axes=None):
+        g = _i64(merged_axes if merged_axes else np.zeros((0, 2))).reshape(-1)
+        out = C.c_void_p()
+        check(load().sb_bs_to_dense(self.ctx, self._h, float(init), len(g) // 2, _ptr(g, C.c_int64), C.byref(out)))
+        return DenseTensor(out, self.device)
+
+    def fuse_two_axes_(self, ax1, ax2):
+        """fuse_two_axes! (conversions.jl:299-314)."""
+        out = C.c_void_p()
+        check(load().sb_bs_fuse_two_axes(self.ctx, self._h, ax1, ax2, C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
+
+class COOTensor(_Tensor):
+    """Element-sparse tensor: keys (nnz x N), vals, dirty flag (coo/storage.jl:1-6)."""
+    kind = 2
+
+    @staticmethod
+    def new(dims, device=0):
+        d = _i64(dims)
+        out = C.c_void_p()
+        check(load().sb_coo_new(context(device), len(dims), _ptr(d, C.c_int64), C.byref(out)))
+        return COOTensor(out, device)
+
+    @staticmethod
+    def from_parts(dims, keys, vals, dirty=False, device=0):
+        d = _i64(dims)
+        k = _i64(keys).reshape(-1)
+        v = np.ascontiguousarray(np.asarray(vals, dtype=np.float64))
+        out = C.c_void_p()
+        check(load().sb_coo_from_host(context(device), len(dims), _ptr(d, C.c_int64), len(v), _ptr(k, C.c_int64),
+                                      _ptr(v, C.c_double), 1 if dirty else 0, C.byref(out)))
+        return COOTensor(out, device)
+
+    def parts(self):
+        N, nnz, dirty = C.c_int(), C.c_int64(), C.c_int()
+        d = np.zeros(16, dtype=np.int64)
+        check(load().sb_coo_info(self._h, C.byref(N), _ptr(d, C.c_int64), C.byref(nnz), C.byref(dirty)))
+        keys = np.zeros((nnz.value, N.value), dtype=np.int64)
+        vals = np.zeros(nnz.value, dtype=np.float64)
+        check(load().sb_coo_to_host(self.ctx, self._h, _ptr(keys, C.c_int64), _ptr(vals, C.c_double)))
+        return dict(dims=tuple(int(x) for x in d[:N.value]), keys=keys, vals=vals, dirty=bool(dirty.value))
+
+    @property
+    def dims(self):
+        return self.parts()["dims"]
+
+    @property
+    def keys(self):
+        return self.parts()["keys"]
+
+    @property
+    def vals(self):
+        return self.parts()["vals"]
+
+    @property
+    def dirty(self):
+        return self.parts()["dirty"]
+
+    def __getitem__(self, I):
+        a = _i64(I)
+        v = C.c_double()
+        check(load().sb_coo_getindex(self.ctx, self._h, _ptr(a, C.c_int64), C.byref(v)))
+        return v.value
+
+    def setindex_(self, v, *I, dirty=False):
+        """setindex!(A, v, I...; dirty) (coo/storage.jl:84-134)."""
+        a = _i64(I)
+        check(load().sb_coo_setindex(self.ctx, self._h, float(v), _ptr(a, C.c_int64), 1 if dirty else 0))
+        return self
+
+    def __setitem__(self, I, v):
+        self.setindex_(v, *I)
+
+    def sort_(self):
+        check(load().sb_coo_sort(self.ctx, self._h))
+        return self
+
+    def to_dense(self):
+        out = C.c_void_p()
+        check(load().sb_coo_to_dense(self.ctx, self._h, C.byref(out)))
+        return DenseTensor(out, self.device).numpy()
+
+    def fuse_axes(self, ax1, ax2):
+        """fuse_axes_coo (coo/conversions.jl:174-223)."""
+        out = C.c_void_p()
+        check(load().sb_coo_fuse_axes(self.ctx, self._h, ax1, ax2, C.byref(out)))
+        return COOTensor(out, self.device)
+
+
+def blocksparse_from_dense(arr, N2, atol=0.0, rtol=0.0, dropzeros=True, device=0):
+    """blocksparse_from_dense(A, Val(N2); atol, rtol, dropzeros) (blocksparse/conversions.jl:21-75)."""
+    D = arr if isinstance(arr, DenseTensor) else DenseTensor.from_numpy(arr, device)
+    out = C.c_void_p()
+    check(load().sb_bs_from_dense(D.ctx, D.handle, N2, float(atol), float(rtol), 1 if dropzeros else 0,
+                                  C.byref(out)))
+    return NewBlockSparseSorted(out, D.device)
+
+
+def coo_from_dense(arr, atol=1e-12, rtol=0.0, device=0):
+    """coo_from_dense(A; atol=1e-12, rtol=0) (coo/conversions.jl:13-26)."""
+    D = arr if isinstance(arr, DenseTensor) else DenseTensor.from_numpy(arr, device)
+    out = C.c_void_p()
+    check(load().sb_coo_from_dense(D.ctx, D.handle, float(atol), float(rtol), C.byref(out)))
+    return COOTensor(out, D.device)
+
+
+def to_dense(t, **kw):
+    """to_dense for any storage kind -> numpy."""
+    if isinstance(t, DenseTensor):
+        return t.numpy()
+    return t.to_dense(**kw)

@@ -1,0 +1,96 @@
+"""contract! and contraction plans over the device storage types
+(SparseBackends/src/tensoralgebra/contract.jl:37-104)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._abi import load, check
+from .storage import DenseTensor
+
+# flags (include/sb_status.h)
+EXACT_JOIN = 1 << 0
+SKIP_ZERO_SLICES = 1 << 1
+PREC_FP64 = 0 << 8
+PREC_FP32 = 1 << 8
+PREC_BF16X3 = 2 << 8
+
+
+def labels_array(labels):
+    """A label is (id, plev) (tensor_wrappers.jl:577-580) -> uint64 array of 2*n."""
+    a = np.zeros(2 * len(labels), dtype=np.uint64)
+    for i, (lid, plev) in enumerate(labels):
+        a[2 * i] = lid
+        a[2 * i + 1] = plev
+    return a
+
+
+def _as_operand(x, device):
+    if isinstance(x, np.ndarray):
+        return DenseTensor.from_numpy(x, device)
+    return x
+
+
+def contract_(C_, labelsC, A, labelsA, B, labelsB, flags=0):
+    """contract!(C, labelsC, A, labelsA, B, labelsB): C (made with
+    NewBlockSparseSorted.new / COOTensor.new) is overwritten; A and B are not
+    modified.  B may be a numpy array (uploaded as a dense operand)."""
+    dev = C_.device
+    A = _as_operand(A, dev)
+    B = _as_operand(B, dev)
+    lc, la, lb = labels_array(labelsC), labels_array(labelsA), labels_array(labelsB)
+    u = C.POINTER(C.c_uint64)
+    check(load().sb_contract(C_.ctx, C_.handle, lc.ctypes.data_as(u), len(labelsC), A.handle, la.ctypes.data_as(u),
+                             len(labelsA), B.handle, lb.ctypes.data_as(u), len(labelsB), flags))
+    return C_
+
+
+def contract_bs_bs_to_dense_(Cd, labelsC, A, labelsA, B, labelsB, flags=0):
+    """contract_bs_bs_to_dense! (contract_bs_bs.jl:736-754): P_C = 0 sink."""
+    lc, la, lb = labels_array(labelsC), labels_array(labelsA), labels_array(labelsB)
+    u = C.POINTER(C.c_uint64)
+    check(load().sb_contract_bs_bs_to_dense(Cd.ctx, Cd.handle, lc.ctypes.data_as(u), len(labelsC), A.handle,
+                                            la.ctypes.data_as(u), len(labelsA), B.handle, lb.ctypes.data_as(u),
+                                            len(labelsB), flags))
+    return Cd
+
+
+class Plan:
+    """contract! with the block structure fixed: creation runs the contraction
+    once; execute() recomputes C's values from the operands' current values."""
+
+    def __init__(self, C_, labelsC, A, labelsA, B, labelsB, flags=0):
+        dev = C_.device
+        self._ops = (C_, _as_operand(A, dev), _as_operand(B, dev))  # keep alive: the plan holds their buffers
+        lc, la, lb = labels_array(labelsC), labels_array(labelsA), labels_array(labelsB)
+        u = C.POINTER(C.c_uint64)
+        h = C.c_void_p()
+        check(load().sb_plan_create(C_.ctx, C_.handle, lc.ctypes.data_as(u), len(labelsC), self._ops[1].handle,
+                                    la.ctypes.data_as(u), len(labelsA), self._ops[2].handle, lb.ctypes.data_as(u),
+                                    len(labelsB), flags, C.byref(h)))
+        self._h = h
+        self.ctx = C_.ctx
+
+    def execute(self):
+        check(load().sb_plan_execute(self.ctx, self._h))
+
+    def stats(self):
+        f, gb, pb = C.c_double(), C.c_double(), C.c_double()
+        npairs, nblk = C.c_int64(), C.c_int64()
+        check(load().sb_plan_stats(self._h, C.byref(f), C.byref(gb), C.byref(pb), C.byref(npairs), C.byref(nblk)))
+        return dict(flops=f.value, gemm_bytes=gb.value, perm_bytes=pb.value, npairs=npairs.value,
+                    nblocks_out=nblk.value)
+
+    def operand_bytes(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(load().sb_plan_operand_bytes(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(a=a.value, b=b.value, c=c.value)
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().sb_plan_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass

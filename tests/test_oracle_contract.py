@@ -143,3 +143,67 @@ def test_dispatch_errors():
     with pytest.raises(O.OracleError) as e:
         O.contract(Cc, labs(["z"]), Ac, labs(["a", "b"]), Bc, labs(["a", "b"]))
     assert e.value.kind in ("MethodError", "AssertionError")
+
+
+# ---------------------------------------------------------------- random brute force
+from gen_cases import gen_shared, gen_single, gauss  # noqa: E402
+
+
+def _orc_case(c, flags=0):
+    mk = {"bs": lambda a, n2: O.BS.from_dense(a, n2), "coo": lambda a, n2: O.COO.from_dense(a, atol=0.0),
+          "dense": lambda a, n2: O.Dense.new(a)}
+    Ct = O.BS.new(c["cdims"], c["N2C"]) if c["ck"] == "bs" else O.COO.new(c["cdims"])
+    At = mk[c["ak"]](c["A"], c["N2A"])
+    Bt = mk[c["bk"]](c["B"], c["N2B"])
+    O.contract(Ct, labs(c["lc"]), At, labs(c["la"]), Bt, labs(c["lb"]), flags)
+    return Ct
+
+
+def _coalesced_dense(Ct, shape):
+    p = Ct.parts()
+    out = np.zeros(shape, order="F")
+    for k, v in zip(p["keys"], p["vals"]):
+        out[tuple(int(x) - 1 for x in k)] += v
+    return out
+
+
+@pytest.mark.parametrize("b_kind", ["bs", "dense"])
+@pytest.mark.parametrize("seed", range(40))
+def test_random_shared_bruteforce(b_kind, seed):
+    rng = np.random.default_rng(1000 + seed)
+    c = gen_shared(rng, b_kind)
+    c.update(ck="bs", ak="bs", bk=b_kind)
+    ref = einsum_contract(c["A"], c["la"], c["B"], c["lb"], c["lc"])
+    Ct = _orc_case(c, EXACT_JOIN)
+    assert np.array_equal(Ct.to_dense(), ref)
+    # block list: exactly the keys the reference creates (every pair / every A block)
+    Ct0 = _orc_case(c, 0)
+    if b_kind == "dense":
+        assert np.array_equal(Ct0.to_dense(), ref)
+
+
+FAMS = ["prefix_bb", "dense_bb", "prefix_bd", "dense_bd", "coo_coo", "coo_bs_prefix", "coo_bs_dense", "coo_dense"]
+
+
+@pytest.mark.parametrize("fam", FAMS)
+@pytest.mark.parametrize("seed", range(20))
+def test_random_single_bruteforce(fam, seed):
+    rng = np.random.default_rng(2000 + seed)
+    c = gen_single(rng, fam)
+    ref = einsum_contract(c["A"], c["la"], c["B"], c["lb"], c["lc"])
+    Ct = _orc_case(c)
+    got = _coalesced_dense(Ct, c["cdims"]) if c["ck"] == "coo" else Ct.to_dense()
+    assert np.array_equal(got, ref)
+
+
+def test_kahan_matches_triple_loop_bound():
+    """test_kernel_precision.jl:47-70 style: the Kahan path stays within a few
+    ulps of an exactly rounded (math.fsum) outer-product sum"""
+    import math
+    rng = np.random.default_rng(42)
+    for _ in range(5):
+        c = gen_single(rng, "prefix_bd", gen=gauss)
+        Ct = _orc_case(c)
+        ref = einsum_contract(c["A"], c["la"], c["B"], c["lb"], c["lc"])
+        got = Ct.to_dense()
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-14)
