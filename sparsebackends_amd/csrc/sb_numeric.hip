@@ -189,7 +189,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GArgs g, const double* _
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        i64 m = m0 + (wm * FM + i) * 16 + 4 * (lane >> 4) + v;
+        // output row of register v: f64 16x16x4 holds rows v*4 + lane/16 (four
+        // groups of one row); the f32 / bf16 16x16 forms hold rows 4*(lane/16) + v
+        i64 m = m0 + (wm * FM + i) * 16 + (PREC == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
         i64 n = n0 + (wn * FN + j) * 16 + (lane & 15);
         if (m < g.M && n < g.N) Cb[m * g.scm + n * g.scn] = PREC == 0 ? accd[i][j][v] : (double)accf[i][j][v];
       }
