@@ -15,7 +15,7 @@ solve keeps it fixed; the planning time is reported separately.
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c1]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
                        [--prec fp64|fp32|bf16x3] [--no-cpu] [--calibrate-copy MB]
 """
 from __future__ import annotations
@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c4", choices=["c4", "c1"])
+    ap.add_argument("--workload", default="hubbard-chi2048", choices=["hubbard-chi2048", "heisenberg-chi256"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -52,7 +52,7 @@ def parse():
 
 def make_problem(cfg):
     from sparsebackends_amd import heff as H
-    return H.config4_problem(0) if cfg == "c4" else H.config1_problem(0)
+    return H.hubbard_chi2048_problem(0) if cfg == "hubbard-chi2048" else H.heisenberg_chi256_problem(0)
 
 
 def rel_err(x, y):
@@ -64,13 +64,13 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (this build measures one GPU's share; "
-              "see DESIGN.md for how the path would shard)", file=sys.stderr)
+              "a multi-GPU run would shard the output block keys across GPUs)", file=sys.stderr)
         sys.exit(2)
     import sparsebackends_amd as sb
 
     flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec]
     t0 = time.perf_counter()
-    prob = make_problem(a.config)
+    prob = make_problem(a.workload)
     t_gen = time.perf_counter() - t0
     alg = prob.algorithmic_flops()["total"]
     pad = prob.padded_flops()["total"]
@@ -138,7 +138,7 @@ def main():
         "data": "synthetic: random psi/L/R on the charge-allowed entries (seed 0), exact model MPO",
         "config": {
             "workload": ("two-site H_eff*psi, Hubbard t=1 U=4, d=4, U(1)xU(1), MPO w=6, chi=2048 in "
-                         f"{prob.Bl.nsec}+{prob.Br.nsec} sectors (2D Gaussian sigma=7)") if a.config == "c4" else
+                         f"{prob.Bl.nsec}+{prob.Br.nsec} sectors (2D Gaussian sigma=7)") if a.workload == "hubbard-chi2048" else
             "two-site H_eff*psi, Heisenberg J=1, U(1) Sz, MPO w=5, chi=256 in 17 sectors",
             "format": "sector + in-sector axes, dense tail padded to the largest sector",
             "psi_blocks": int(len(prob.psi_parts["ids"])),

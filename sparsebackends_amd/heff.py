@@ -355,13 +355,13 @@ class HeffProblem:
         return dict(total=sum(f), steps=f)
 
 
-def config1_problem(seed=0, device=0):
+def heisenberg_chi256_problem(seed=0, device=0):
     """Heisenberg S=1/2, J=1, chi=256 in Sz sectors q in {-8..8}, dims ~ exp(-q^2/32)."""
     B = heisenberg_bond(256)
     return HeffProblem(heisenberg_model(1.0), B, B, seed=seed, device=device)
 
 
-def config4_problem(seed=0, sigma=7.0, device=0):
+def hubbard_chi2048_problem(seed=0, sigma=7.0, device=0):
     """Hubbard t=1, U=4, chi=2048 in (n, 2Sz) sectors with a 2D Gaussian law of width sigma."""
     Bl = hubbard_bond(2048, n0=0, sigma=sigma)
     Br = hubbard_bond(2048, n0=2, sigma=sigma)
