@@ -50,4 +50,10 @@
 #define SB_PREC_FP32    (1u << 8)   /* operands rounded to FP32, FP32 MFMA, FP32 accumulate */
 #define SB_PREC_BF16X3  (2u << 8)   /* operands split in three bf16 terms, FP32 accumulate */
 
+/* tile shape of the BS x BS block products (bits 16..19): 0 chooses from the
+ * operands' block boxes at plan time; 1 + c forces shape c of
+ * 16x128, 128x16, 32x32, 64x32, 32x64, 64x64, 16x64, 64x16 (speed only) */
+#define SB_TILE_SHIFT   16
+#define SB_TILE_MASK    (0xFu << SB_TILE_SHIFT)
+
 #endif

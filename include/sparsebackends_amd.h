@@ -72,8 +72,14 @@ int sb_bs_from_host(sb_ctx_t ctx, int N, int N2, const int64_t* dims, int64_t nb
 int sb_bs_info(sb_tensor_t t, int* N, int* N2, int64_t* dims, int64_t* nblocks, int64_t* blksize);
 /* the fields keys/ids/data copied to host arrays */
 int sb_bs_to_host(sb_ctx_t ctx, sb_tensor_t t, int64_t* keys, int64_t* ids, double* data);
-/* device pointers of the fields (int32 keys/ids, float64 payload), valid until the tensor changes */
+/* device pointers of the fields (int32 keys/ids, float64 payload), valid until the tensor changes.
+ * Asking for the payload pointer marks the payload as written from outside:
+ * later contractions rescan it for zero padding on every use. */
 int sb_bs_device_ptrs(sb_tensor_t t, const int32_t** keys, const int32_t** ids, double** data);
+/* assignment to the data field (A.data .= x on the struct of storage.jl:3-9):
+ * replace the payload (nblocks*blksize values, host or device memory) keeping
+ * keys and ids */
+int sb_bs_set_payload(sb_ctx_t ctx, sb_tensor_t t, const double* data, int from_device);
 /* getindex / setindex! (storage.jl:100-124); I holds N 1-based coordinates */
 int sb_bs_getindex(sb_ctx_t ctx, sb_tensor_t t, const int64_t* I, double* v);
 int sb_bs_setindex(sb_ctx_t ctx, sb_tensor_t t, double v, const int64_t* I);

@@ -41,6 +41,7 @@ SIGNATURES = {
     "sb_bs_info": [P, I32P, I32P, I64P, I64P, I64P],
     "sb_bs_to_host": [P, P, I64P, I64P, F64P],
     "sb_bs_device_ptrs": [P, PP, PP, PP],
+    "sb_bs_set_payload": [P, P, P, C.c_int],
     "sb_bs_getindex": [P, P, I64P, F64P],
     "sb_bs_setindex": [P, P, C.c_double, I64P],
     "sb_bs_sort_lex": [P, P],

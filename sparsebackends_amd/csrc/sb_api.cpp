@@ -206,16 +206,42 @@ int sb_bs_device_ptrs(sb_tensor_t h, const int32_t** keys, const int32_t** ids, 
     Tensor* t = T_(h, BS);
     if (keys) *keys = t->keys.get();
     if (ids) *ids = t->ids.get();
-    if (data) *data = t->data.get();
+    if (data) {
+      *data = t->data.get();
+      t->external = true;  // writes through the pointer are invisible to the block boxes
+    }
+  })
+}
+int sb_bs_set_payload(sb_ctx_t c, sb_tensor_t h, const double* data, int from_device) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h, BS);
+    i64 cnt = t->n * t->blksize;
+    if (cnt && !data) fail(SB_E_ARGUMENT, "null payload");
+    t->touch();
+    if (cnt)
+      HIP_CHECK(hipMemcpyAsync(t->data.get(), data, cnt * sizeof(double),
+                               from_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    if (!from_device) HIP_CHECK(hipStreamSynchronize(ctx->stream));
   })
 }
 int sb_bs_getindex(sb_ctx_t c, sb_tensor_t h, const int64_t* I, double* v) {
   SB_TRY(*v = bs_getindex(C_(c), *T_(h, BS), I))
 }
 int sb_bs_setindex(sb_ctx_t c, sb_tensor_t h, double v, const int64_t* I) {
-  SB_TRY(bs_setindex(C_(c), *T_(h, BS), v, I))
+  SB_TRY({
+    Tensor* t = T_(h, BS);
+    t->touch();
+    bs_setindex(C_(c), *t, v, I);
+  })
 }
-int sb_bs_sort_lex(sb_ctx_t c, sb_tensor_t h) { SB_TRY(bs_sort_lex(C_(c), *T_(h, BS))) }
+int sb_bs_sort_lex(sb_ctx_t c, sb_tensor_t h) {
+  SB_TRY({
+    Tensor* t = T_(h, BS);
+    t->touch();
+    bs_sort_lex(C_(c), *t);
+  })
+}
 int sb_bs_is_dense(sb_tensor_t h, int* out) { SB_TRY(*out = bs_is_dense(*T_(h, BS)) ? 1 : 0) }
 int sb_bs_from_dense(sb_ctx_t c, sb_tensor_t d, int N2, double atol, double rtol, int dropzeros, sb_tensor_t* out) {
   SB_TRY(*out = H_(bs_from_dense(C_(c), *T_(d, DENSE), N2, atol, rtol, dropzeros != 0)))
@@ -367,8 +393,10 @@ int sb_permutedims_inplace(sb_ctx_t c, sb_tensor_t h, const int64_t* perm) {
     Ctx* ctx = C_(c);
     Tensor* t = T_(h);
     std::vector<i64> p = vec(perm, t->N());
-    if (t->kind == BS)
+    if (t->kind == BS) {
+      t->touch();
       bs_permute_inplace(ctx, *t, p);
+    }
     else if (t->kind == COO) {
       TP r = coo_permute(ctx, *t, p);
       t->dims = r->dims;

@@ -1,0 +1,559 @@
+// Box-aware grouped block products for BS x BS contract_shared!
+// (the reference: permute A and B, then mul! per pair, contract_bs_bs.jl:572-730).
+//
+// Two ideas replace the reference's permute-then-GEMM:
+//  * no operand permutes: the block product reads A, B and writes C through
+//    per-axis strides of their own tail layouts (index decode tables in LDS),
+//    so the payloads are never copied;
+//  * boxes: for every block and tail axis, one past the last index that holds
+//    a nonzero.  Entries beyond a box are zero, so their products are zero and
+//    are skipped: each C block is computed only inside the box spanned by its
+//    operands, and the rest of the block is kept at zero.  This is the common
+//    case of symmetry sectors stored in a block padded to the largest sector.
+//    A block holding a non-finite value widens the partner's extents to the
+//    full block, so NaN/Inf propagate exactly as through the full products.
+//
+// Per execution: box_prepare (group boxes, per-pair K extents, tile counts,
+// clearing of C entries that fall outside a shrunken box) then box_gemm, a
+// persistent kernel that walks the (group, tile) work list.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "sb_kernels.h"
+
+namespace sb {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+static inline unsigned nb256(i64 n) { return (unsigned)((n + 255) / 256); }
+
+// ---------------------------------------------------------------- operand boxes
+__global__ __launch_bounds__(256) void box_scan_kernel(const double* __restrict__ data, const i32* __restrict__ ids,
+                                                       i64 n, i64 bs, int N2, IArr td, i32* __restrict__ box,
+                                                       i32* __restrict__ flag) {
+  __shared__ int sm[MAXR];
+  __shared__ int sf;
+  const int tid = threadIdx.x;
+  for (i64 i = blockIdx.x; i < n; i += gridDim.x) {
+    if (tid < MAXR) sm[tid] = 0;
+    if (tid == 0) sf = 0;
+    __syncthreads();
+    const double* b = data + (i64)(ids[i] - 1) * bs;
+    int lm[MAXR];
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a) lm[a] = 0;
+    int lf = 0;
+    for (i64 e = tid; e < bs; e += 256) {
+      double v = b[e];
+      if (v != 0.0) {  // NaN compares unequal, so it counts as nonzero
+        i64 x = e;
+#pragma unroll
+        for (int a = 0; a < MAXR; ++a)
+          if (a < N2) {
+            i64 q = x / td.v[a];
+            int r = (int)(x - q * td.v[a]);
+            lm[a] = max(lm[a], r + 1);
+            x = q;
+          }
+        if (!isfinite(v)) lf = 1;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < MAXR; ++a)
+      if (a < N2 && lm[a]) atomicMax(&sm[a], lm[a]);
+    if (lf) atomicOr(&sf, 1);
+    __syncthreads();
+    if (tid < N2) box[i * N2 + tid] = sm[tid];
+    if (tid == 0) flag[i] = sf;
+    __syncthreads();
+  }
+}
+
+void box_scan(Ctx* ctx, const double* data, const i32* ids, i64 n, i64 bs, int N2, const i64* tdims, i32* box,
+              i32* flag) {
+  if (n == 0) return;
+  IArr td{};
+  for (int a = 0; a < N2; ++a) td.v[a] = tdims[a];
+  unsigned g = (unsigned)std::min<i64>(n, 65536);
+  hipLaunchKernelGGL(box_scan_kernel, dim3(g), dim3(256), 0, ctx->stream, data, ids, n, bs, N2, td, box, flag);
+  HIP_CHECK(hipGetLastError());
+}
+
+__global__ void fill_rows_kernel(i32* __restrict__ p, i64 n, int w, IArr row) {
+  i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * w) return;
+  p[t] = (i32)row.v[t % w];
+}
+void k_fill_rows(Ctx* ctx, i32* p, i64 n, int w, const i64* row) {
+  if (n * w == 0) return;
+  IArr r{};
+  for (int a = 0; a < w; ++a) r.v[a] = row[a];
+  hipLaunchKernelGGL(fill_rows_kernel, dim3(nb256(n * w)), dim3(256), 0, ctx->stream, p, n, w, r);
+  HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- preparation
+// One thread per group: the group's M / N boxes (max over its contributing
+// pairs), every pair's K extents (min of the two operands, widened where an
+// operand holds a non-finite value), the new C box, whether the old C box
+// reaches beyond it, and the number of tiles.
+__global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ seg, const i32* __restrict__ arow,
+                                  const i32* __restrict__ brow, const i32* __restrict__ abox,
+                                  const i32* __restrict__ aflag, const i32* __restrict__ bbox,
+                                  const i32* __restrict__ bflag, i32* __restrict__ gbox, i32* __restrict__ kbox,
+                                  i32* __restrict__ kc, i32* __restrict__ cbox, i32* __restrict__ cflag,
+                                  i32* __restrict__ cnew, i32* __restrict__ clear, i32* __restrict__ tiles, int BM,
+                                  int BN) {
+  i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  int mb[MAXAX], nb[MAXAX];
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a) mb[a] = nb[a] = 0;
+  bool fullM = false, fullN = false;
+  for (i32 pp = seg[g]; pp < seg[g + 1]; ++pp) {
+    const i32 ar = arow[pp], br = brow[pp];
+    const bool fa = aflag[ar] != 0, fb = bflag[br] != 0;
+    int kcount = 1;
+#pragma unroll
+    for (int q = 0; q < MAXAX; ++q)
+      if (q < d.nk) {
+        int ea = abox[(i64)ar * d.NA2 + d.k_axa[q]], eb = bbox[(i64)br * d.NB2 + d.k_axb[q]];
+        int e = min(ea, eb);
+        if (fa) e = max(e, ea);
+        if (fb) e = max(e, eb);
+        kbox[(i64)pp * d.nk + q] = e;
+        kcount *= e;
+      }
+    kc[pp] = kcount;
+    if (kcount == 0) continue;  // contributes nothing
+    fullM = fullM || fb;
+    fullN = fullN || fa;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a) {
+      if (a < d.nm) mb[a] = max(mb[a], abox[(i64)ar * d.NA2 + d.m_ax[a]]);
+      if (a < d.nn) nb[a] = max(nb[a], bbox[(i64)br * d.NB2 + d.n_ax[a]]);
+    }
+  }
+  int Mc = 1, Nc = 1;
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a) {
+    if (a < d.nm) {
+      if (fullM) mb[a] = d.m_ext[a];
+      Mc *= mb[a];
+    }
+    if (a < d.nn) {
+      if (fullN) nb[a] = d.n_ext[a];
+      Nc *= nb[a];
+    }
+  }
+  if (Mc == 0 || Nc == 0) {
+    Mc = Nc = 0;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a) mb[a] = nb[a] = 0;
+  }
+  const int nmn = d.nm + d.nn;
+  int cb[MAXAX];
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a) {
+    if (a < d.nm) {
+      gbox[g * nmn + a] = mb[a];
+      cb[d.m_cax[a]] = mb[a];
+    }
+    if (a < d.nn) {
+      gbox[g * nmn + d.nm + a] = nb[a];
+      cb[d.n_cax[a]] = nb[a];
+    }
+  }
+  bool old_empty = false, beyond = false;
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a)
+    if (a < d.NC2) {
+      int o = cbox[g * d.NC2 + a];
+      old_empty = old_empty || o == 0;
+      beyond = beyond || o > cb[a];
+    }
+  const bool need = !old_empty && beyond;
+  clear[g] = need ? 1 : 0;
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a)
+    if (a < d.NC2) {
+      cnew[g * d.NC2 + a] = cb[a];
+      if (!need) cbox[g * d.NC2 + a] = cb[a];
+    }
+  cflag[g] = 0;
+  tiles[g] = (Mc && Nc) ? ((Mc + BM - 1) / BM) * ((Nc + BN - 1) / BN) : 0;
+}
+
+__global__ void tile_groups_kernel(const i32* __restrict__ toff, i64 ng, i32* __restrict__ tg) {
+  i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  for (i32 t = toff[g]; t < toff[g + 1]; ++t) tg[t] = (i32)g;
+}
+__global__ void first_zero_kernel(i32* p) { p[0] = 0; }
+
+// zero the entries of C block g inside its old box and outside its new one,
+// then install the new box
+__global__ __launch_bounds__(256) void box_clear_kernel(BoxDesc d, i64 ng, const i32* __restrict__ clear,
+                                                        const i32* __restrict__ cnew, i32* __restrict__ cbox,
+                                                        const i64* __restrict__ coff, double* __restrict__ C) {
+  const int tid = threadIdx.x;
+  for (i64 g = blockIdx.x; g < ng; g += gridDim.x) {
+    if (!clear[g]) continue;
+    int ob[MAXAX], nbx[MAXAX];
+    i64 vol = 1;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a)
+      if (a < d.NC2) {
+        ob[a] = cbox[g * d.NC2 + a];
+        nbx[a] = cnew[g * d.NC2 + a];
+        vol *= ob[a];
+      }
+    double* base = C + coff[g];
+    for (i64 e = tid; e < vol; e += 256) {
+      i64 x = e, off = 0, st = 1;
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < MAXAX; ++a)
+        if (a < d.NC2) {
+          i64 q = x / ob[a];
+          int r = (int)(x - q * ob[a]);
+          inside = inside && r < nbx[a];
+          off += r * st;
+          st *= d.c_ext[a];
+          x = q;
+        }
+      if (!inside) base[off] = 0.0;
+    }
+    __syncthreads();
+    if (tid < d.NC2) cbox[g * d.NC2 + tid] = nbx[tid];
+    __syncthreads();
+  }
+}
+
+size_t box_scan_tmp_bytes(Ctx* ctx, i64 ng) {
+  size_t tb = 0;
+  HIP_CHECK(rocprim::inclusive_scan(nullptr, tb, (const i32*)nullptr, (i32*)nullptr, (size_t)std::max<i64>(ng, 1),
+                                    rocprim::plus<i32>(), ctx->stream));
+  return tb ? tb : 1;
+}
+
+void box_prepare(Ctx* ctx, const BoxGemmArgs& g, const i32* abox, const i32* aflag, const i32* bbox,
+                 const i32* bflag, i32* cbox, i32* cflag, double* C) {
+  if (g.ng == 0) return;
+  const BoxTile t = box_tile(g.cfg);
+  hipLaunchKernelGGL(box_groups_kernel, dim3(nb256(g.ng)), dim3(256), 0, ctx->stream, g.d, g.ng, g.seg, g.arow,
+                     g.brow, abox, aflag, bbox, bflag, g.gbox, g.kbox, g.kc, cbox, cflag, g.cnew, g.clear, g.tiles,
+                     t.BM, t.BN);
+  HIP_CHECK(hipGetLastError());
+  if (g.tile_group) {  // the work list (absent while the tile shape is being chosen)
+    hipLaunchKernelGGL(first_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, g.toff);
+    HIP_CHECK(hipGetLastError());
+    size_t tb = g.scan_tmp_bytes;
+    HIP_CHECK(rocprim::inclusive_scan(g.scan_tmp, tb, g.tiles, g.toff + 1, (size_t)g.ng, rocprim::plus<i32>(),
+                                      ctx->stream));
+    hipLaunchKernelGGL(tile_groups_kernel, dim3(nb256(g.ng)), dim3(256), 0, ctx->stream, g.toff, g.ng,
+                       g.tile_group);
+    HIP_CHECK(hipGetLastError());
+  }
+  unsigned cg = (unsigned)std::min<i64>(g.ng, 2048);
+  hipLaunchKernelGGL(box_clear_kernel, dim3(cg), dim3(256), 0, ctx->stream, g.d, g.ng, g.clear, g.cnew, cbox, g.coff,
+                     C);
+  HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- block products
+// index x of a compressed box -> (offset in operand 1, offset in operand 2),
+// axis 0 fastest
+__device__ inline void decode2(int x, int n, const int* ext, const int* s1, const int* s2, int& o1, int& o2) {
+  int a1 = 0, a2 = 0;
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a)
+    if (a < n) {
+      int q = x / ext[a];
+      int r = x - q * ext[a];
+      a1 += r * s1[a];
+      a2 += r * s2[a];
+      x = q;
+    }
+  o1 = a1;
+  o2 = a2;
+}
+
+__device__ inline unsigned short bf16_rne_b(float x) {
+  unsigned u = __float_as_uint(x);
+  unsigned r = ((u >> 16) & 1u) + 0x7fffu;
+  return (unsigned short)((u + r) >> 16);
+}
+__device__ inline float bf16_to_f_b(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ inline void split3_b(float x, unsigned short& h, unsigned short& m, unsigned short& l) {
+  h = bf16_rne_b(x);
+  float r1 = x - bf16_to_f_b(h);
+  m = bf16_rne_b(r1);
+  float r2 = r1 - bf16_to_f_b(m);
+  l = bf16_rne_b(r2);
+}
+
+// One workgroup (4 waves) per (group, tile) of the work list, persistent over
+// the list.  The K dimension of a tile is the concatenation of the K boxes of
+// the group's pairs in the reference's pair order; a table in LDS maps each K
+// position to its absolute A and B element offsets.  PREC as grouped_gemm.
+template <int BM, int BN, int WM, int WN, int PREC>
+__global__ __launch_bounds__(256) void box_gemm_kernel(BoxDesc d, i64 ng, const i32* __restrict__ seg,
+                                                       const i64* __restrict__ aoff, const i64* __restrict__ boff,
+                                                       const i64* __restrict__ coff, const i32* __restrict__ gbox,
+                                                       const i32* __restrict__ kbox, const i32* __restrict__ kc,
+                                                       const i32* __restrict__ toff, const i32* __restrict__ tgrp,
+                                                       const double* __restrict__ A, const double* __restrict__ B,
+                                                       double* __restrict__ C, i32* __restrict__ cflag, int amf,
+                                                       int bnf) {
+  constexpr int BK = 16, KT = 512;
+  constexpr int FM = BM / (16 * WM), FN = BN / (16 * WN);
+  static_assert(FM >= 1 && FN >= 1 && WM * WN == 4, "tile");
+  using T = typename std::conditional<PREC == 0, double, float>::type;
+  __shared__ T As[BK][BM + 2];
+  __shared__ T Bs[BK][BN + 2];
+  __shared__ int rowA[BM], rowC[BM], colB[BN], colC[BN];
+  __shared__ i64 tabA[KT], tabB[KT];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave % WM, wn = wave / WM;
+  const int nmn = d.nm + d.nn;
+  const i32 total = toff[ng];
+  for (i32 w = blockIdx.x; w < total; w += gridDim.x) {
+    const i32 g = tgrp[w];
+    const i32 lt = w - toff[g];
+    int mb[MAXAX], nbx[MAXAX];
+    int Mc = 1, Nc = 1;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a) {
+      mb[a] = a < d.nm ? gbox[(i64)g * nmn + a] : 1;
+      nbx[a] = a < d.nn ? gbox[(i64)g * nmn + d.nm + a] : 1;
+      Mc *= mb[a];
+      Nc *= nbx[a];
+    }
+    const int tiles_m = (Mc + BM - 1) / BM;
+    const int tm = lt % tiles_m, tn = lt / tiles_m;
+    const int m0 = tm * BM, n0 = tn * BN;
+    __syncthreads();  // the previous tile is done with the LDS tables
+    for (int r = tid; r < BM; r += 256) {
+      int oa = -1, oc = -1;
+      if (m0 + r < Mc) decode2(m0 + r, d.nm, mb, d.m_sa, d.m_sc, oa, oc);
+      rowA[r] = oa;
+      rowC[r] = oc;
+    }
+    for (int c = tid; c < BN; c += 256) {
+      int ob = -1, oc = -1;
+      if (n0 + c < Nc) decode2(n0 + c, d.nn, nbx, d.n_sb, d.n_sc, ob, oc);
+      colB[c] = ob;
+      colC[c] = oc;
+    }
+    d4 accd[FM][FN];
+    f4 accf[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        accd[i][j] = d4{0, 0, 0, 0};
+        accf[i][j] = f4{0, 0, 0, 0};
+      }
+    const i32 p1 = seg[g + 1];
+    i32 pp = seg[g];
+    int kin = 0;
+    while (pp < p1) {
+      __syncthreads();
+      int fill = 0;
+      while (pp < p1 && fill < KT) {
+        const int kcp = kc[pp];
+        const int take = min(kcp - kin, KT - fill);
+        if (take > 0) {
+          int kb[MAXAX];
+#pragma unroll
+          for (int a = 0; a < MAXAX; ++a) kb[a] = a < d.nk ? kbox[(i64)pp * d.nk + a] : 1;
+          const i64 ao = aoff[pp], bo = boff[pp];
+          for (int t = tid; t < take; t += 256) {
+            int oa, ob;
+            decode2(kin + t, d.nk, kb, d.k_sa, d.k_sb, oa, ob);
+            tabA[fill + t] = ao + oa;
+            tabB[fill + t] = bo + ob;
+          }
+        }
+        fill += take;
+        kin += take;
+        if (kin >= kcp) {
+          ++pp;
+          kin = 0;
+        }
+      }
+      __syncthreads();
+      for (int k0 = 0; k0 < fill; k0 += BK) {
+        if (k0) __syncthreads();
+        for (int idx = tid; idx < BM * BK; idx += 256) {
+          int m, k;
+          if (amf) {
+            m = idx % BM;
+            k = idx / BM;
+          } else {
+            k = idx % BK;
+            m = idx / BK;
+          }
+          double v = 0.0;
+          const int ra = rowA[m];
+          if (ra >= 0 && k0 + k < fill) v = A[tabA[k0 + k] + ra];
+          As[k][m] = (T)v;
+        }
+        for (int idx = tid; idx < BK * BN; idx += 256) {
+          int n, k;
+          if (bnf) {
+            n = idx % BN;
+            k = idx / BN;
+          } else {
+            k = idx % BK;
+            n = idx / BK;
+          }
+          double v = 0.0;
+          const int cb = colB[n];
+          if (cb >= 0 && k0 + k < fill) v = B[tabB[k0 + k] + cb];
+          Bs[k][n] = (T)v;
+        }
+        __syncthreads();
+        if constexpr (PREC == 0) {
+#pragma unroll
+          for (int kk = 0; kk < BK / 4; ++kk) {
+            double a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + (lane >> 4)][(wm * FM + i) * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + (lane >> 4)][(wn * FN + j) * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j)
+                accd[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], accd[i][j], 0, 0, 0);
+          }
+        } else if constexpr (PREC == 1) {
+#pragma unroll
+          for (int kk = 0; kk < BK / 4; ++kk) {
+            float a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + (lane >> 4)][(wm * FM + i) * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + (lane >> 4)][(wn * FN + j) * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j)
+                accf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], accf[i][j], 0, 0, 0);
+          }
+        } else {
+          // bf16x3: both operands split in three bf16 terms; the six products of
+          // order <= 2 accumulate in FP32
+          s4 ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              unsigned short h, m, l;
+              split3_b(As[4 * (lane >> 4) + v][(wm * FM + i) * 16 + (lane & 15)], h, m, l);
+              ah[i][v] = (short)h;
+              am[i][v] = (short)m;
+              al[i][v] = (short)l;
+            }
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              unsigned short h, m, l;
+              split3_b(Bs[4 * (lane >> 4) + v][(wn * FN + j) * 16 + (lane & 15)], h, m, l);
+              bh[j][v] = (short)h;
+              bm[j][v] = (short)m;
+              bl[j][v] = (short)l;
+            }
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+              f4 c = accf[i][j];
+              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bm[j], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bm[j], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bh[j], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], c, 0, 0, 0);
+              accf[i][j] = c;
+            }
+        }
+      }
+    }
+    double* Cb = C + coff[g];
+    int bad = 0;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          // output row of register v: f64 16x16x4 holds rows 4*v + lane/16;
+          // the f32 / bf16 16x16 forms hold rows 4*(lane/16) + v
+          const int ml = (wm * FM + i) * 16 + (PREC == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
+          const int nl = (wn * FN + j) * 16 + (lane & 15);
+          const int rc = rowC[ml], cc = colC[nl];
+          if (rc >= 0 && cc >= 0) {
+            const double val = PREC == 0 ? accd[i][j][v] : (double)accf[i][j][v];
+            Cb[rc + cc] = val;
+            bad |= !isfinite(val);
+          }
+        }
+    if (bad) atomicOr(&cflag[g], 1);
+  }
+}
+
+static const BoxTile kTiles[] = {
+    {16, 128}, {128, 16}, {32, 32}, {64, 32}, {32, 64}, {64, 64}, {16, 64}, {64, 16},
+};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+BoxTile box_tile(int cfg) {
+  SB_ASSERT(cfg >= 0 && cfg < kNumTiles, "box tile configuration");
+  return kTiles[cfg];
+}
+int box_tile_configs() { return kNumTiles; }
+
+template <int PREC>
+static void launch_box(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
+  const BoxTile t = kTiles[g.cfg];
+  i64 per_cu = 4;
+  i64 nwg = std::min<i64>(g.max_tiles, (i64)ctx->num_cu * per_cu);
+  if (nwg <= 0) return;
+  dim3 grid((unsigned)nwg), blk(256);
+#define SB_BOX_LAUNCH(BM_, BN_, WM_, WN_)                                                                        \
+  hipLaunchKernelGGL((box_gemm_kernel<BM_, BN_, WM_, WN_, PREC>), grid, blk, 0, ctx->stream, g.d, g.ng, g.seg, \
+                     g.aoff, g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.tile_group, A, B, C, cflag, g.amf,  \
+                     g.bnf)
+  switch (g.cfg) {
+    case 0: SB_BOX_LAUNCH(16, 128, 1, 4); break;
+    case 1: SB_BOX_LAUNCH(128, 16, 4, 1); break;
+    case 2: SB_BOX_LAUNCH(32, 32, 2, 2); break;
+    case 3: SB_BOX_LAUNCH(64, 32, 2, 2); break;
+    case 4: SB_BOX_LAUNCH(32, 64, 2, 2); break;
+    case 5: SB_BOX_LAUNCH(64, 64, 2, 2); break;
+    case 6: SB_BOX_LAUNCH(16, 64, 1, 4); break;
+    case 7: SB_BOX_LAUNCH(64, 16, 4, 1); break;
+    default: fail(SB_E_ASSERT, "box tile configuration");
+  }
+#undef SB_BOX_LAUNCH
+  HIP_CHECK(hipGetLastError());
+}
+
+void box_gemm(Ctx* ctx, unsigned prec, const BoxGemmArgs& g, const double* A, const double* B, double* C,
+              i32* cflag) {
+  if (g.ng == 0 || g.max_tiles == 0) return;
+  switch (prec & SB_PREC_MASK) {
+    case SB_PREC_FP64: launch_box<0>(ctx, g, A, B, C, cflag); break;
+    case SB_PREC_FP32: launch_box<1>(ctx, g, A, B, C, cflag); break;
+    case SB_PREC_BF16X3: launch_box<2>(ctx, g, A, B, C, cflag); break;
+    default: fail(SB_E_ARGUMENT, "unknown precision form");
+  }
+}
+
+}  // namespace sb

@@ -4,6 +4,7 @@
 // block allocation; all integer kernels on the GPU, bit-exact) and a numeric
 // phase (operand permutes + grouped block products).
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <numeric>
 #include <set>
@@ -179,15 +180,15 @@ static Pairs pairs_from_ranges(Ctx* ctx, DBuf<i32>& lo, DBuf<i32>& hi, i64 nA) {
 }
 
 // keys of C for every pair, then output-block allocation (_ensure_block! order)
-static void allocate_from_pairs(Plan& pl, Tensor& C, const Pairs& pr, const Tensor* A, int PAw, const Tensor* B,
+static void allocate_from_pairs(Plan& pl, Tensor& C, const Pairs& pr, const i32* keysA, int PAw, const i32* keysB,
                                 int PBw, const std::vector<int>& src, Alloc& al) {
   Ctx* ctx = pl.ctx;
   int PC = C.P();
   DBuf<i32> ck(ctx, (size_t)(pr.n ? pr.n : 1) * (PC ? PC : 1));
   DBuf<u64> lin(ctx, pr.n ? pr.n : 1);
   if (pr.n && PC > 0)
-    k_pair_keys(ctx, pr.a.get(), pr.b.get(), pr.n, A ? A->keys.get() : nullptr, PAw, B ? B->keys.get() : nullptr,
-                PBw, src.data(), PC, C.dims.data(), ck.get(), lin.get());
+    k_pair_keys(ctx, pr.a.get(), pr.b.get(), pr.n, keysA, PAw, keysB, PBw, src.data(), PC, C.dims.data(), ck.get(),
+                lin.get());
   else if (pr.n)
     k_zero(ctx, lin.get(), pr.n * sizeof(u64));
   u64 maxlin = PC > 0 ? (u64)prod(C.dims, 0, PC) - 1 : 0;
@@ -202,6 +203,10 @@ static void allocate_from_pairs(Plan& pl, Tensor& C, const Pairs& pr, const Tens
   C.data.alloc(ctx, (size_t)al.nC * C.blksize);
   pl.nblocks_out = al.nC;
   pl.npairs = pr.n;
+}
+static void allocate_from_pairs(Plan& pl, Tensor& C, const Pairs& pr, const Tensor* A, int PAw, const Tensor* B,
+                                int PBw, const std::vector<int>& src, Alloc& al) {
+  allocate_from_pairs(pl, C, pr, A ? A->keys.get() : nullptr, PAw, B ? B->keys.get() : nullptr, PBw, src, al);
 }
 
 // per-pair offsets: off[p] = blockoff(T)[idx[p]]
@@ -267,9 +272,333 @@ static void add_gemm_step(Plan& pl, unsigned prec, const GemmDesc& d, const doub
 }
 
 // ================================================================ BS x BS
-// contract_shared! BS x BS (contract_bs_bs.jl:572-730)
+// The keys of a block-sparse operand with its prefix reordered by pperm
+// (1-based) and sorted by the column-major lin of the new prefix, exactly as
+// permutedims leaves them (storage.jl:188-264); the payload stays where it
+// is: view row i is row row[i] of the tensor (row == null: the identity).
+struct KeyView {
+  const i32* keys = nullptr;
+  const i32* row = nullptr;
+  Labels labels;  // prefix labels in view order, then the tail labels in tensor order
+  LabelMap map;
+};
+static KeyView key_view(Plan& pl, const Oper& X, const std::vector<i64>& pperm) {
+  Ctx* ctx = pl.ctx;
+  const Tensor& T = *X.t;
+  int P = T.P(), N = T.N();
+  KeyView v;
+  for (int j = 0; j < P; ++j) v.labels.push_back(X.labels[pperm[j] - 1]);
+  for (int j = P; j < N; ++j) v.labels.push_back(X.labels[j]);
+  v.map = make_map(v.labels);
+  if ((is_identity(pperm) && !T.dirty) || T.n == 0 || P == 0) {
+    v.keys = T.keys.get();
+    return v;
+  }
+  i64 n = T.n;
+  std::vector<int> sel(P);
+  std::vector<i64> pd(P);
+  for (int j = 0; j < P; ++j) {
+    sel[j] = (int)pperm[j];
+    pd[j] = T.dims[pperm[j] - 1];
+  }
+  DBuf<i32> nk(ctx, (size_t)n * P), idx(ctx, n), sorted(ctx, (size_t)n * P);
+  DBuf<u64> lin(ctx, n);
+  k_remap_keys(ctx, T.keys.get(), n, P, sel.data(), P, pd.data(), nk.get(), lin.get(), idx.get());
+  sort_pairs(ctx, lin.get(), idx.get(), n, bits_for((u64)prod(pd, 0, P) - 1));
+  k_gather_rows(ctx, nk.get(), P, idx.get(), n, sorted.get());
+  v.keys = pl.keep(std::move(sorted));
+  v.row = pl.keep(std::move(idx));
+  return v;
+}
+
+// boxes of a block-sparse tensor, recomputed from its payload unless the last
+// change of the payload left them valid
+static void ensure_box(Ctx* ctx, const Tensor& T) {
+  if (T.box_trusted()) return;
+  if (T.bflag.n < (size_t)T.n) T.bflag.alloc(ctx, T.n ? T.n : 1);
+  if (T.box.n < (size_t)(T.n * T.N2)) T.box.alloc(ctx, (size_t)(T.n * T.N2 ? T.n * T.N2 : 1));
+  std::vector<i64> td(T.dims.begin() + T.P(), T.dims.end());
+  box_scan(ctx, T.data.get(), T.ids.get(), T.n, T.blksize, T.N2, td.data(), T.box.get(), T.bflag.get());
+  T.box_version = T.version;
+}
+
+static void shared_bs_bs_permuted(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B,
+                                  const Labels& shared, unsigned flags);
+
+// contract_shared! BS x BS (contract_bs_bs.jl:572-730) without operand
+// permutes: the symbolic phase (key views, merge-join, output allocation)
+// follows the reference; the numeric phase is the box-aware block product.
 static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared,
                          unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "bs_bs_shared";
+  for (auto& l : shared) SB_ASSERT(!contains(labelsC, l), "shared label must be reduced (must not appear in labelsC)");
+  int PA = A.t->P(), PB = B.t->P(), NA = A.t->N(), NB = B.t->N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    SB_ASSERT(has(A.map, l) && has(B.map, l), "shared label must exist in both A and B");
+    int ap = at(A.map, l), bp = at(B.map, l);
+    bool apf = ap <= PA, bpf = bp <= PB;
+    if (apf && bpf)
+      sp.push_back(l);
+    else if (!apf && !bpf)
+      sd.push_back(l);
+    else
+      fail(SB_E_ERROR, "shared label crosses prefix/dense; not supported");
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), Bd0 = sub(B.labels, PB, NB);
+  Labels red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : Bd0)
+    if (!contains(sd, l)) keepB0.push_back(l);
+  int PC = C.P(), NC = C.N();
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; would require permuting C (unsupported)");
+  SB_ASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA covers keepA");
+  SB_ASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB covers keepB");
+  // the reduced extents must agree axis by axis for the strided product (the
+  // reference compares their products only); blocks must index with int32
+  bool strided = A.t->N2 <= MAXAX && B.t->N2 <= MAXAX && C.N2 <= MAXAX && A.t->blksize < (1ll << 31) &&
+                 B.t->blksize < (1ll << 31) && C.blksize < (1ll << 31);
+  for (auto& l : red) strided = strided && A.t->dims[at(A.map, l) - 1] == B.t->dims[at(B.map, l) - 1];
+  if (!strided) {
+    shared_bs_bs_permuted(pl, C, labelsC, A, B, shared, flags);
+    return;
+  }
+  // prefix order of the reference's permuted operands: unshared axes, then the shared ones
+  auto prefix_perm = [&](const Oper& X) {
+    int P = X.t->P();
+    std::vector<i64> spax, perm;
+    for (auto& l : sp) spax.push_back(at(X.map, l));
+    for (int a = 1; a <= P; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) perm.push_back(a);
+    for (auto a : spax) perm.push_back(a);
+    return perm;
+  };
+  KeyView VA = key_view(pl, A, prefix_perm(A)), VB = key_view(pl, B, prefix_perm(B));
+  const Tensor &TA = *A.t, &TB = *B.t;
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(VA.map, l) && at(VA.map, l) <= PA)
+      src[j] = at(VA.map, l);
+    else if (has(VB.map, l) && at(VB.map, l) <= PB)
+      src[j] = -at(VB.map, l);
+    else
+      fail(SB_E_ERROR, "C prefix label must come from A/B prefix");
+  }
+  std::vector<int> jA, jB;
+  std::vector<i64> spd;
+  for (auto& l : sp) {
+    jA.push_back(at(VA.map, l));
+    jB.push_back(at(VB.map, l));
+    spd.push_back(TA.dims[at(A.map, l) - 1]);
+  }
+  // tail geometry: M = A-kept axes, N = B-kept axes, K = reduced axes
+  auto tail_strides = [](const Tensor& T) {
+    std::vector<i64> s(T.N2);
+    i64 acc = 1;
+    for (int a = 0; a < T.N2; ++a) {
+      s[a] = acc;
+      acc *= T.dims[T.P() + a];
+    }
+    return s;
+  };
+  std::vector<i64> sA = tail_strides(TA), sB = tail_strides(TB), sC = tail_strides(C);
+  struct Ax {
+    int ext, s1, s2, c1, c2;  // extent, strides in operand 1/2, box columns in operand 1/2
+  };
+  auto cpos = [&](const Label& l) { return (int)(std::find(Cd.begin(), Cd.end(), l) - Cd.begin()); };
+  std::vector<Ax> mx, nx, kx;
+  for (auto& l : dA) {
+    int ta = at(A.map, l) - PA - 1, tc = cpos(l);
+    mx.push_back({(int)TA.dims[PA + ta], (int)sA[ta], (int)sC[tc], ta, tc});
+  }
+  for (auto& l : dB) {
+    int tb = at(B.map, l) - PB - 1, tc = cpos(l);
+    nx.push_back({(int)TB.dims[PB + tb], (int)sB[tb], (int)sC[tc], tb, tc});
+  }
+  for (auto& l : red) {
+    int ta = at(A.map, l) - PA - 1, tb = at(B.map, l) - PB - 1;
+    kx.push_back({(int)TA.dims[PA + ta], (int)sA[ta], (int)sB[tb], ta, tb});
+  }
+  // decode order: smallest operand stride first (the fastest-varying index of a tile)
+  auto by_s1 = [](const Ax& x, const Ax& y) { return x.s1 < y.s1; };
+  std::stable_sort(mx.begin(), mx.end(), by_s1);
+  std::stable_sort(nx.begin(), nx.end(), by_s1);
+  std::stable_sort(kx.begin(), kx.end(), by_s1);
+  i64 M = 1, K = 1, N = 1;
+  for (auto& x : mx) M *= x.ext;
+  for (auto& x : kx) K *= x.ext;
+  for (auto& x : nx) N *= x.ext;
+  SB_ASSERT(C.blksize == M * N, "C.blksize must equal M*N");
+  i64 nA = TA.n, nB = TB.n;
+  DBuf<i32> lo(ctx, nA ? nA : 1), hi(ctx, nA ? nA : 1);
+  int nsp = (int)sp.size();
+  if (nA && nB) {
+    if (nsp == 0) {
+      // join on nothing: every A block meets every B block
+      DBuf<u64> z(ctx, nA), zb(ctx, nB);
+      k_zero(ctx, z.get(), nA * sizeof(u64));
+      k_zero(ctx, zb.get(), nB * sizeof(u64));
+      k_equal_ranges(ctx, z.get(), nA, zb.get(), nB, lo.get(), hi.get());
+    } else if (nsp >= 2 && !(flags & SB_FLAG_EXACT_JOIN)) {
+      // the reference's merge walk compares the first shared axis first
+      k_walk_ranges(ctx, 2, VA.keys, PA, nA, VB.keys, PB, nB, jA.data(), jB.data(), nsp, lo.get(), hi.get());
+    } else {
+      DBuf<u64> ja(ctx, nA), jb(ctx, nB);
+      k_cols_lin(ctx, VA.keys, nA, PA, jA.data(), nsp, spd.data(), ja.get());
+      k_cols_lin(ctx, VB.keys, nB, PB, jB.data(), nsp, spd.data(), jb.get());
+      k_equal_ranges(ctx, ja.get(), nA, jb.get(), nB, lo.get(), hi.get());
+    }
+  }
+  Pairs pr = (nA && nB) ? pairs_from_ranges(ctx, lo, hi, nA) : Pairs();
+  Alloc al;
+  allocate_from_pairs(pl, C, pr, VA.keys, PA, VB.keys, PB, src, al);
+  i64 np = pr.n, ng = al.nC;
+  pl.flops += 2.0 * M * K * N * np;
+  pl.gemm_bytes += 8.0 * (double)np * (M * K + K * N) + 8.0 * ng * M * N;
+  pl.a_bytes += 8.0 * TA.n * TA.blksize;
+  pl.b_bytes += 8.0 * TB.n * TB.blksize;
+  pl.c_bytes += 8.0 * ng * C.blksize;
+  if (!ng) return;
+  // C's boxes: unknown payload until the first execution clears it
+  C.bflag.alloc(ctx, ng);
+  C.box.alloc(ctx, (size_t)(ng * C.N2 ? ng * C.N2 : 1));
+  // pairs in group order: tensor rows and payload offsets of their blocks
+  DBuf<i32> ar(ctx, np ? np : 1), br(ctx, np ? np : 1);
+  if (np) {
+    DBuf<i32> t(ctx, np);
+    k_gather_rows(ctx, pr.a.get(), 1, al.order.get(), np, t.get());
+    if (VA.row)
+      k_gather_rows(ctx, VA.row, 1, t.get(), np, ar.get());
+    else
+      std::swap(ar, t);
+    DBuf<i32> u(ctx, np);
+    k_gather_rows(ctx, pr.b.get(), 1, al.order.get(), np, u.get());
+    if (VB.row)
+      k_gather_rows(ctx, VB.row, 1, u.get(), np, br.get());
+    else
+      std::swap(br, u);
+  }
+  DBuf<i64> ao(ctx, np ? np : 1), bo(ctx, np ? np : 1);
+  if (np) {
+    k_src_offsets(ctx, TA.ids.get(), ar.get(), np, TA.blksize, ao.get());
+    k_src_offsets(ctx, TB.ids.get(), br.get(), np, TB.blksize, bo.get());
+  }
+  BoxGemmArgs g;
+  BoxDesc& d = g.d;
+  std::memset(&d, 0, sizeof(d));
+  d.nm = (int)mx.size();
+  d.nn = (int)nx.size();
+  d.nk = (int)kx.size();
+  d.NA2 = TA.N2;
+  d.NB2 = TB.N2;
+  d.NC2 = C.N2;
+  SB_ASSERT(d.nm + d.nn == C.N2, "C tail = A-kept + B-kept axes");
+  for (int i = 0; i < d.nm; ++i) {
+    d.m_ext[i] = mx[i].ext;
+    d.m_sa[i] = mx[i].s1;
+    d.m_sc[i] = mx[i].s2;
+    d.m_ax[i] = mx[i].c1;
+    d.m_cax[i] = mx[i].c2;
+  }
+  for (int i = 0; i < d.nn; ++i) {
+    d.n_ext[i] = nx[i].ext;
+    d.n_sb[i] = nx[i].s1;
+    d.n_sc[i] = nx[i].s2;
+    d.n_ax[i] = nx[i].c1;
+    d.n_cax[i] = nx[i].c2;
+  }
+  for (int i = 0; i < d.nk; ++i) {
+    d.k_ext[i] = kx[i].ext;
+    d.k_sa[i] = kx[i].s1;
+    d.k_sb[i] = kx[i].s2;
+    d.k_axa[i] = kx[i].c1;
+    d.k_axb[i] = kx[i].c2;
+  }
+  for (int a = 0; a < C.N2; ++a) d.c_ext[a] = (int)C.dims[PC + a];
+  g.ng = ng;
+  g.np = np;
+  g.amf = d.nm > 0 && (d.nk == 0 || d.m_sa[0] < d.k_sa[0]);
+  g.bnf = d.nn > 0 && (d.nk == 0 || d.n_sb[0] < d.k_sb[0]);
+  g.seg = pl.keep(std::move(al.seg_start));
+  g.arow = pl.keep(std::move(ar));
+  g.brow = pl.keep(std::move(br));
+  g.aoff = pl.keep(std::move(ao));
+  g.boff = pl.keep(std::move(bo));
+  g.coff = group_offsets(pl, C);
+  g.gbox = pl.keep(DBuf<i32>(ctx, (size_t)ng * std::max(1, d.nm + d.nn)));
+  g.kbox = pl.keep(DBuf<i32>(ctx, (size_t)std::max<i64>(1, np * d.nk)));
+  g.kc = pl.keep(DBuf<i32>(ctx, np ? np : 1));
+  g.cnew = pl.keep(DBuf<i32>(ctx, (size_t)ng * std::max(1, C.N2)));
+  g.clear = pl.keep(DBuf<i32>(ctx, ng));
+  g.tiles = pl.keep(DBuf<i32>(ctx, ng));
+  g.toff = pl.keep(DBuf<i32>(ctx, ng + 1));
+  g.scan_tmp_bytes = box_scan_tmp_bytes(ctx, ng);
+  g.scan_tmp = pl.keep(DBuf<unsigned char>(ctx, g.scan_tmp_bytes));
+  std::vector<i64> cfull(C.dims.begin() + PC, C.dims.end());
+  const Tensor *pA = &TA, *pB = &TB;
+  Tensor* pC = &C;
+  // tile shape: forced by the flags, or the configuration with the least
+  // padded work over the group boxes of the current operands (any shape is
+  // correct; this is speed only)
+  const unsigned forced = (flags & SB_TILE_MASK) >> SB_TILE_SHIFT;
+  if (forced) {
+    if ((int)forced > box_tile_configs()) fail(SB_E_ARGUMENT, "unknown tile shape in flags");
+    g.cfg = (int)forced - 1;
+  } else {
+    ensure_box(ctx, TA);
+    ensure_box(ctx, TB);
+    k_fill_rows(ctx, C.box.get(), ng, C.N2, cfull.data());
+    box_prepare(ctx, g, TA.box.get(), TA.bflag.get(), TB.box.get(), TB.bflag.get(), C.box.get(), C.bflag.get(),
+                C.data.get());
+    C.box_version = C.version;  // entries outside the boxes are zero now
+    std::vector<i32> hb((size_t)ng * std::max(1, d.nm + d.nn));
+    HIP_CHECK(hipMemcpyAsync(hb.data(), g.gbox, hb.size() * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double best = 1e300;
+    for (int c = 0; c < box_tile_configs(); ++c) {
+      BoxTile t = box_tile(c);
+      double cost = 0;
+      for (i64 q = 0; q < ng; ++q) {
+        i64 mc = 1, nc = 1;
+        for (int a = 0; a < d.nm; ++a) mc *= hb[q * (d.nm + d.nn) + a];
+        for (int a = 0; a < d.nn; ++a) nc *= hb[q * (d.nm + d.nn) + d.nm + a];
+        if (mc && nc) cost += (double)((mc + t.BM - 1) / t.BM) * ((nc + t.BN - 1) / t.BN) * (t.BM * t.BN + 1024);
+      }
+      if (cost < best) {
+        best = cost;
+        g.cfg = c;
+      }
+    }
+  }
+  BoxTile t = box_tile(g.cfg);
+  g.max_tiles = ng * ((M + t.BM - 1) / t.BM) * ((N + t.BN - 1) / t.BN);
+  SB_ASSERT(g.max_tiles < (1ll << 31), "box_gemm: work list too large");
+  g.tile_group = pl.keep(DBuf<i32>(ctx, (size_t)std::max<i64>(1, g.max_tiles)));
+  pl.steps.push_back({"box_scan", [ctx, pA, pB]() {
+                        ensure_box(ctx, *pA);
+                        ensure_box(ctx, *pB);
+                      }});
+  pl.steps.push_back({"box_prepare", [ctx, g, pA, pB, pC, cfull]() {
+                        if (!pC->box_trusted()) k_fill_rows(ctx, pC->box.get(), g.ng, pC->N2, cfull.data());
+                        box_prepare(ctx, g, pA->box.get(), pA->bflag.get(), pB->box.get(), pB->bflag.get(),
+                                    pC->box.get(), pC->bflag.get(), pC->data.get());
+                      }});
+  const double *Ad = TA.data.get(), *Bd = TB.data.get();
+  double* Cdp = C.data.get();
+  i32* cflag = C.bflag.get();
+  pl.steps.push_back({"grouped_gemm", [ctx, flags, g, Ad, Bd, Cdp, cflag]() { box_gemm(ctx, flags, g, Ad, Bd, Cdp, cflag); }});
+  pl.out_box = true;
+}
+
+// contract_shared! BS x BS through permuted operand copies (the reference's
+// own route, contract_bs_bs.jl:572-730); used where the strided product does
+// not apply
+static void shared_bs_bs_permuted(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B,
+                                  const Labels& shared, unsigned flags) {
   Ctx* ctx = pl.ctx;
   pl.family = "bs_bs_shared";
   for (auto& l : shared) SB_ASSERT(!contains(labelsC, l), "shared label must be reduced (must not appear in labelsC)");
@@ -1218,6 +1547,8 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
   SB_ASSERT(labelsC.size() == C.dims.size(), "labelsC length must match C rank");
   auto pl = std::make_unique<Plan>();
   pl->ctx = ctx;
+  C.touch();  // C's block list and payload are replaced
+  pl->out = &C;
   LabelMap mapA = make_map(labelsA), mapB = make_map(labelsB);
   Labels shared, reduced;
   for (auto& l : labelsA)

@@ -44,6 +44,15 @@ struct Plan {
   void keep_tensor(TP&& t) { hold.push_back(std::shared_ptr<Tensor>(t.release())); }
   size_t done = 0;  // steps [0, done) already ran during planning
   double a_bytes = 0, b_bytes = 0, c_bytes = 0;  // distinct payload bytes of the block-product operands
+  // the output tensor: every execution changes its payload; plans whose
+  // last step leaves valid block boxes in it say so in out_box
+  Tensor* out = nullptr;
+  bool out_box = false;
+  void finish() const {
+    if (!out) return;
+    out->touch();
+    if (out_box) out->box_version = out->version;
+  }
   void run_step(const Step& s) const {
     if (!ctx->profile) {
       s();
@@ -58,11 +67,13 @@ struct Plan {
   // every step (a replay with refreshed operand values)
   void run() const {
     for (auto& s : steps) run_step(s);
+    finish();
   }
   // the steps not yet run (planning needs some values mid-way)
   void run_pending() {
     for (size_t i = done; i < steps.size(); ++i) run_step(steps[i]);
     done = steps.size();
+    finish();
   }
 };
 

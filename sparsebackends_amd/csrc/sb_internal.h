@@ -151,6 +151,19 @@ struct Tensor {
   // aliased block-sparse (aliased/storage.jl:46-59)
   DBuf<double> scalars;
   i64 ntemplates = 0;
+  // Block boxes (BS): per block and tail axis, one past the last index with a
+  // nonzero entry (n x N2), and a flag per block marking non-finite values.
+  // They describe the payload while box_version == version and no device
+  // pointer to the payload was handed out (external).  Every change of the
+  // payload or block list advances version.
+  mutable DBuf<i32> box, bflag;
+  mutable i64 box_version = -1;
+  i64 version = 0;
+  bool external = false;
+  void touch() { ++version; }
+  bool box_trusted() const {
+    return kind == BS && !external && box_version == version && bflag.n >= (size_t)n && box.n >= (size_t)(n * N2);
+  }
 
   int N() const { return (int)dims.size(); }
   int P() const { return (int)dims.size() - N2; }

@@ -107,6 +107,57 @@ void outer_groups(Ctx*, i64 m, i64 n, const double* A, const i64* aoff, const do
 // per (row, r) nonzero flags of slices: flag[i*R + r] = any(X[off_i + r*chunk .. +chunk] != 0)
 void slice_nonzero(Ctx*, const double* X, const i64* off, i64 nrows, i64 R, i64 chunk, unsigned char* flag);
 
+// ---- box-aware block products (sb_boxgemm.hip) ---------------------------
+// A box row holds, per tail axis of a block, one past the last index with a
+// nonzero entry; a flag row marks blocks holding a non-finite value.
+constexpr int MAXAX = MAXR;
+struct BoxDesc {
+  int nm, nn, nk;        // axes of the M (A kept), N (B kept) and K (reduced) index spaces
+  int NA2, NB2, NC2;     // tail ranks of A, B, C (box row widths)
+  int m_ext[MAXAX], n_ext[MAXAX], k_ext[MAXAX];  // full extents
+  int m_sa[MAXAX], m_sc[MAXAX];  // element strides of the M axes in an A block / a C block
+  int n_sb[MAXAX], n_sc[MAXAX];  // of the N axes in a B block / a C block
+  int k_sa[MAXAX], k_sb[MAXAX];  // of the K axes in an A block / a B block
+  int m_ax[MAXAX], n_ax[MAXAX];  // box column (tail axis) of each M axis in A, N axis in B
+  int k_axa[MAXAX], k_axb[MAXAX];
+  int m_cax[MAXAX], n_cax[MAXAX];  // box column of each M / N axis in C
+  int c_ext[MAXAX];                // C tail extents in C's axis order
+};
+struct BoxTile {
+  int BM, BN;
+};
+BoxTile box_tile(int cfg);
+int box_tile_configs();
+struct BoxGemmArgs {
+  BoxDesc d;
+  i64 ng = 0, np = 0, max_tiles = 0;
+  int cfg = 0, amf = 0, bnf = 0;
+  // fixed per plan: pairs in group order
+  const i32* seg = nullptr;   // ng + 1
+  const i64* aoff = nullptr;  // per pair: element offset of the A block
+  const i64* boff = nullptr;
+  const i32* arow = nullptr;  // per pair: box row of the A block
+  const i32* brow = nullptr;
+  const i64* coff = nullptr;  // per group: element offset of the C block
+  // per execution
+  i32 *gbox = nullptr, *kbox = nullptr, *kc = nullptr, *cnew = nullptr, *clear = nullptr;
+  i32 *tiles = nullptr, *toff = nullptr, *tile_group = nullptr;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+// boxes and flags of n blocks (block i at (ids[i]-1)*bs, tail extents tdims)
+void box_scan(Ctx*, const double* data, const i32* ids, i64 n, i64 bs, int N2, const i64* tdims, i32* box,
+              i32* flag);
+// every row of an n x w array set to row[0..w)
+void k_fill_rows(Ctx*, i32* p, i64 n, int w, const i64* row);
+size_t box_scan_tmp_bytes(Ctx*, i64 ng);
+// group boxes, pair K extents, C boxes (cbox old -> new, entries outside a shrunken box zeroed) and,
+// when tile_group is set, the (group, tile) work list
+void box_prepare(Ctx*, const BoxGemmArgs&, const i32* abox, const i32* aflag, const i32* bbox, const i32* bflag,
+                 i32* cbox, i32* cflag, double* C);
+// C_g[box] = sum over pairs and K boxes of A_p * B_p (flags non-finite outputs in cflag)
+void box_gemm(Ctx*, unsigned prec, const BoxGemmArgs&, const double* A, const double* B, double* C, i32* cflag);
+
 // element offsets of blocks: off[i] = (ids[i]-1)*bs  (ids == null: i*bs)
 void k_block_offsets(Ctx*, const i32* ids, i64 n, i64 bs, i64* off);
 // off[p] = base[idx[p]] (+ add)

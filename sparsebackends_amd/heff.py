@@ -10,14 +10,17 @@ The MPO site tensor is one dense block (P = 0).
 The contraction order is psi*L*W1*W2*R (the order ITensorMPS uses for
 product(ProjMPO, psi)); each step is contract_shared! BS x BS
 (tensoralgebra/contract_bs_bs.jl:572-730) with at most one shared key axis.
+Within each dense tail the small physical / MPO axes come first and the
+padded in-sector axes last, so the nonzero part of a block is a few long
+contiguous runs.
 
-    psi  [Ql, Qr | l, s1, s2, r]
-    L    [Ql', Ql | l', wl, l]          R [Qr', Qr | r', wr, r]
+    psi  [Ql, Qr | s1, s2, l, r]
+    L    [Ql', Ql | wl, l', l]          R [Qr', Qr | wr, r', r]
     W1   [ | wl, s1', s1, wm]           W2 [ | wm, s2', s2, wr]
-    T1 = psi*L   [Ql', Qr | s1, s2, r, l', wl]
-    T2 = T1*W1   [Ql', Qr | s2, r, l', s1', wm]
-    T3 = T2*W2   [Ql', Qr | r, l', s1', s2', wr]
-    Hpsi = T3*R  [Ql', Qr' | l', s1', s2', r']
+    T1 = psi*L   [Ql', Qr | s1, s2, r, wl, l']
+    T2 = T1*W1   [Ql', Qr | s1', wm, s2, r, l']
+    T3 = T2*W2   [Ql', Qr | s2', wr, s1', r, l']
+    Hpsi = T3*R  [Ql', Qr' | s1', s2', l', r']
 """
 from __future__ import annotations
 
@@ -208,7 +211,7 @@ class HeffProblem:
         Dl, Dr = Bl.dmax, Br.dmax
         self.Dl, self.Dr = Dl, Dr
         qs, Ql = model.charges, model.Qlink
-        # psi [Ql, Qr | l, s1, s2, r] : q(r) = q(l) + q(s1) + q(s2)
+        # psi [Ql, Qr | s1, s2, l, r] : q(r) = q(l) + q(s1) + q(s2)
         psi = {}
         for il, ql in enumerate(Bl.charges):
             for ir, qr in enumerate(Br.charges):
@@ -218,11 +221,11 @@ class HeffProblem:
                         m[s1, s2] = qadd(qadd(ql, qs[s1]), qs[s2]) == qr
                 if m.any():
                     psi[(il + 1, ir + 1)] = m
-        self.psi_parts = self._parts([Bl.nsec, Br.nsec], (Dl, d, d, Dr), psi,
-                                     lambda k: (Bl.dims[k[0] - 1], d, d, Br.dims[k[1] - 1]),
-                                     lambda k, blk: blk * psi[k][None, :, :, None], rng)
+        self.psi_parts = self._parts([Bl.nsec, Br.nsec], (d, d, Dl, Dr), psi,
+                                     lambda k: (d, d, Bl.dims[k[0] - 1], Br.dims[k[1] - 1]),
+                                     lambda k, blk: blk * psi[k][:, :, None, None], rng)
 
-        # environments: X(x', w, x) nonzero where q(x') - q(x) = -Qlink(w)
+        # environments: X[w, x', x] nonzero where q(x') - q(x) = -Qlink(w)
         def env(B):
             blocks = {}
             for ip, qp in enumerate(B.charges):
@@ -232,30 +235,30 @@ class HeffProblem:
                         blocks[(ip + 1, i + 1)] = m
             return blocks
         Lb, Rb = env(Bl), env(Br)
-        self.L_parts = self._parts([Bl.nsec, Bl.nsec], (Dl, w, Dl), Lb,
-                                   lambda k: (Bl.dims[k[0] - 1], w, Bl.dims[k[1] - 1]),
-                                   lambda k, blk: blk * Lb[k][None, :, None], rng)
-        self.R_parts = self._parts([Br.nsec, Br.nsec], (Dr, w, Dr), Rb,
-                                   lambda k: (Br.dims[k[0] - 1], w, Br.dims[k[1] - 1]),
-                                   lambda k, blk: blk * Rb[k][None, :, None], rng)
+        self.L_parts = self._parts([Bl.nsec, Bl.nsec], (w, Dl, Dl), Lb,
+                                   lambda k: (w, Bl.dims[k[0] - 1], Bl.dims[k[1] - 1]),
+                                   lambda k, blk: blk * Lb[k][:, None, None], rng)
+        self.R_parts = self._parts([Br.nsec, Br.nsec], (w, Dr, Dr), Rb,
+                                   lambda k: (w, Br.dims[k[0] - 1], Br.dims[k[1] - 1]),
+                                   lambda k, blk: blk * Rb[k][:, None, None], rng)
         # W [ | wl, s', s, wr] : one dense block
         self.W_parts = dict(dims=(w, d, d, w), N2=4, keys=np.zeros((1, 0), dtype=np.int64),
                             ids=np.array([1]), data=np.asfortranarray(model.W).reshape(-1, order="F"))
         L_ = _lab
-        self.l_psi = [L_("Ql"), L_("Qr"), L_("l"), L_("s1"), L_("s2"), L_("r")]
-        self.l_L = [L_("Qlp"), L_("Ql"), L_("lp"), L_("wl"), L_("l")]
+        self.l_psi = [L_("Ql"), L_("Qr"), L_("s1"), L_("s2"), L_("l"), L_("r")]
+        self.l_L = [L_("Qlp"), L_("Ql"), L_("wl"), L_("lp"), L_("l")]
         self.l_W1 = [L_("wl"), L_("s1p"), L_("s1"), L_("wm")]
         self.l_W2 = [L_("wm"), L_("s2p"), L_("s2"), L_("wr")]
-        self.l_R = [L_("Qrp"), L_("Qr"), L_("rp"), L_("wr"), L_("r")]
-        self.l_T1 = [L_("Qlp"), L_("Qr"), L_("s1"), L_("s2"), L_("r"), L_("lp"), L_("wl")]
-        self.l_T2 = [L_("Qlp"), L_("Qr"), L_("s2"), L_("r"), L_("lp"), L_("s1p"), L_("wm")]
-        self.l_T3 = [L_("Qlp"), L_("Qr"), L_("r"), L_("lp"), L_("s1p"), L_("s2p"), L_("wr")]
-        self.l_out = [L_("Qlp"), L_("Qrp"), L_("lp"), L_("s1p"), L_("s2p"), L_("rp")]
+        self.l_R = [L_("Qrp"), L_("Qr"), L_("wr"), L_("rp"), L_("r")]
+        self.l_T1 = [L_("Qlp"), L_("Qr"), L_("s1"), L_("s2"), L_("r"), L_("wl"), L_("lp")]
+        self.l_T2 = [L_("Qlp"), L_("Qr"), L_("s1p"), L_("wm"), L_("s2"), L_("r"), L_("lp")]
+        self.l_T3 = [L_("Qlp"), L_("Qr"), L_("s2p"), L_("wr"), L_("s1p"), L_("r"), L_("lp")]
+        self.l_out = [L_("Qlp"), L_("Qrp"), L_("s1p"), L_("s2p"), L_("lp"), L_("rp")]
         nl, nr = Bl.nsec, Br.nsec
-        self.dims_T1 = (nl, nr, d, d, Dr, Dl, w)
-        self.dims_T2 = (nl, nr, d, Dr, Dl, d, w)
-        self.dims_T3 = (nl, nr, Dr, Dl, d, d, w)
-        self.dims_out = (nl, nr, Dl, d, d, Dr)
+        self.dims_T1 = (nl, nr, d, d, Dr, w, Dl)
+        self.dims_T2 = (nl, nr, d, w, d, Dr, Dl)
+        self.dims_T3 = (nl, nr, d, w, d, Dr, Dl)
+        self.dims_out = (nl, nr, d, d, Dl, Dr)
 
     @staticmethod
     def _parts(pdims, tdims, blocks, ext, mask, rng):
@@ -353,6 +356,31 @@ class HeffProblem:
         f = [2.0 * (d * d * Dr) * Dl * (Dl * w) * n1, 2.0 * (d * Dr * Dl) * (d * w) ** 2 * len(T1),
              2.0 * (Dr * Dl * d) * (d * w) ** 2 * len(T1), 2.0 * (Dl * d * d) * (Dr * w) * Dr * n4]
         return dict(total=sum(f), steps=f)
+
+    def algorithmic_bytes(self):
+        """float64 bytes each step must move at the least: its two operands read
+        once and its result written once, every block counted with the actual
+        sector extents (no padding)"""
+        d, w = self.m.d, self.m.w
+        dl, dr = self.Bl.dims, self.Br.dims
+        psi = [tuple(map(int, k)) for k in self.psi_parts["keys"]]
+        Lk = [tuple(map(int, k)) for k in self.L_parts["keys"]]
+        Rk = [tuple(map(int, k)) for k in self.R_parts["keys"]]
+        L_by, R_by = {}, {}
+        for (a, b) in Lk:
+            L_by.setdefault(b, []).append(a)
+        for (a, b) in Rk:
+            R_by.setdefault(b, []).append(a)
+        T1 = {(a, ir) for (il, ir) in psi for a in L_by.get(il, [])}
+        out = {(a, c) for (a, ir) in T1 for c in R_by.get(ir, [])}
+        b_psi = sum(d * d * dl[a - 1] * dr[b - 1] for (a, b) in psi)
+        b_L = sum(w * dl[a - 1] * dl[b - 1] for (a, b) in Lk)
+        b_R = sum(w * dr[a - 1] * dr[b - 1] for (a, b) in Rk)
+        b_T = sum(d * d * w * dl[a - 1] * dr[b - 1] for (a, b) in T1)
+        b_out = sum(d * d * dl[a - 1] * dr[c - 1] for (a, c) in out)
+        b_W = w * d * d * w
+        steps = [8.0 * (b_psi + b_L + b_T), 8.0 * (b_T + b_W + b_T), 8.0 * (b_T + b_W + b_T), 8.0 * (b_T + b_R + b_out)]
+        return dict(total=sum(steps), steps=steps)
 
 
 def heisenberg_chi256_problem(seed=0, device=0):

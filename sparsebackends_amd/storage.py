@@ -217,6 +217,19 @@ class NewBlockSparseSorted(_Tensor):
         check(load().sb_bs_device_ptrs(self._h, C.byref(k), C.byref(i), C.byref(d)))
         return k.value, i.value, d.value
 
+    def set_data_(self, data):
+        """A.data .= data (the payload field, storage.jl:3-9): keys and ids stay."""
+        x = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
+        if x.size != self.nblocks * self.blksize:
+            raise ValueError("payload size must be nblocks * blksize")
+        check(load().sb_bs_set_payload(self.ctx, self._h, x.ctypes.data, 0))
+        return self
+
+    def set_data_device_(self, ptr):
+        """the payload copied from device memory at ptr (nblocks * blksize float64)."""
+        check(load().sb_bs_set_payload(self.ctx, self._h, ptr, 1))
+        return self
+
     def __getitem__(self, I):
         a = _i64(I)
         v = C.c_double()

@@ -4,8 +4,8 @@ A sector-format block-sparse tensor (key prefix = sector numbers, dense tail
 = in-sector coordinates padded to the largest sector) is expanded to a plain
 dense array over the full bond dimensions; H_eff*psi is then one einsum:
 
-    Hpsi[l', s1', s2', r'] = sum L[l', wl, l] W[wl, s1', s1, wm] W[wm, s2', s2, wr]
-                                 R[r', wr, r] psi[l, s1, s2, r]
+    Hpsi[s1', s2', l', r'] = sum L[wl, l', l] W[wl, s1', s1, wm] W[wm, s2', s2, wr]
+                                 R[wr, r', r] psi[s1, s2, l, r]
 """
 import numpy as np
 
@@ -48,16 +48,17 @@ def expand(parts, bonds, tail_map):
 def heff_dense(prob):
     """(Hpsi over the full bonds, the four dense operands) of a HeffProblem's host parts"""
     Bl, Br = prob.Bl, prob.Br
-    psi, _ = expand(prob.psi_parts, [Bl, Br], [0, None, None, 1])
-    L, _ = expand(prob.L_parts, [Bl, Bl], [0, None, 1])
-    R, _ = expand(prob.R_parts, [Br, Br], [0, None, 1])
+    psi, _ = expand(prob.psi_parts, [Bl, Br], [None, None, 0, 1])  # [s1, s2, l, r]
+    L, _ = expand(prob.L_parts, [Bl, Bl], [None, 0, 1])  # [wl, l', l]
+    R, _ = expand(prob.R_parts, [Br, Br], [None, 0, 1])  # [wr, r', r]
     W = prob.m.W
     # pairwise, in the chain's order (one 5-operand einsum is far slower)
-    T1 = np.einsum("awb,byue->awyue", L, psi)
+    T1 = np.einsum("wab,yube->awyue", L, psi)
     T2 = np.einsum("awyue,wxyv->axvue", T1, W)
     T3 = np.einsum("axvue,vzuq->axzqe", T2, W)
-    return np.einsum("axzqe,cqe->axzc", T3, R)
+    H = np.einsum("axzqe,qce->axzc", T3, R)  # [l', s1', s2', r']
+    return np.ascontiguousarray(H.transpose(1, 2, 0, 3))  # [s1', s2', l', r']
 
 
 def expand_out(prob, parts):
-    return expand(parts, [prob.Bl, prob.Br], [0, None, None, 1])
+    return expand(parts, [prob.Bl, prob.Br], [None, None, 0, 1])
