@@ -96,19 +96,21 @@ void k_fill_rows(Ctx* ctx, i32* p, i64 n, int w, const i64* row) {
 }
 
 // ---------------------------------------------------------------- preparation
-// One thread per group: the group's M / N boxes (max over its contributing
-// pairs), every pair's K extents (min of the two operands, widened where an
-// operand holds a non-finite value), the new C box, whether the old C box
-// reaches beyond it, and the number of tiles.
-__global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ seg, const i32* __restrict__ arow,
-                                  const i32* __restrict__ brow, const i32* __restrict__ abox,
-                                  const i32* __restrict__ aflag, const i32* __restrict__ bbox,
-                                  const i32* __restrict__ bflag, i32* __restrict__ gbox, i32* __restrict__ kbox,
-                                  i32* __restrict__ kc, i32* __restrict__ cbox, i32* __restrict__ cflag,
-                                  i32* __restrict__ cnew, i32* __restrict__ clear, i32* __restrict__ tiles, int BM,
-                                  int BN) {
-  i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+// One thread per group (taken in the locality order gorder): the group's M / N
+// boxes (max over its contributing pairs), every pair's K extents (min of the
+// two operands, widened where an operand holds a non-finite value), the new C
+// box, whether the old C box reaches beyond it (then the GEMM kernel clears the
+// difference), and the group's tile count at its position in the order.
+__global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ gorder, const i32* __restrict__ seg,
+                                  const i32* __restrict__ arow, const i32* __restrict__ brow,
+                                  const i32* __restrict__ abox, const i32* __restrict__ aflag,
+                                  const i32* __restrict__ bbox, const i32* __restrict__ bflag, i32* __restrict__ gbox,
+                                  i32* __restrict__ kbox, i32* __restrict__ kc, i32* __restrict__ cbox,
+                                  i32* __restrict__ cflag, i32* __restrict__ cnew, i32* __restrict__ clear,
+                                  i32* __restrict__ tiles, int BM, int BN) {
+  i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng) return;
+  const i64 g = gorder ? gorder[i] : i;
   int mb[MAXAX], nb[MAXAX];
 #pragma unroll
   for (int a = 0; a < MAXAX; ++a) mb[a] = nb[a] = 0;
@@ -184,53 +186,14 @@ __global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ seg
       if (!need) cbox[g * d.NC2 + a] = cb[a];
     }
   cflag[g] = 0;
-  tiles[g] = (Mc && Nc) ? ((Mc + BM - 1) / BM) * ((Nc + BN - 1) / BN) : 0;
+  tiles[i] = (Mc && Nc) ? ((Mc + BM - 1) / BM) * ((Nc + BN - 1) / BN) : 0;
 }
 
-__global__ void tile_groups_kernel(const i32* __restrict__ toff, i64 ng, i32* __restrict__ tg) {
-  i64 g = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
-  for (i32 t = toff[g]; t < toff[g + 1]; ++t) tg[t] = (i32)g;
-}
-__global__ void first_zero_kernel(i32* p) { p[0] = 0; }
-
-// zero the entries of C block g inside its old box and outside its new one,
-// then install the new box
-__global__ __launch_bounds__(256) void box_clear_kernel(BoxDesc d, i64 ng, const i32* __restrict__ clear,
-                                                        const i32* __restrict__ cnew, i32* __restrict__ cbox,
-                                                        const i64* __restrict__ coff, double* __restrict__ C) {
-  const int tid = threadIdx.x;
-  for (i64 g = blockIdx.x; g < ng; g += gridDim.x) {
-    if (!clear[g]) continue;
-    int ob[MAXAX], nbx[MAXAX];
-    i64 vol = 1;
-#pragma unroll
-    for (int a = 0; a < MAXAX; ++a)
-      if (a < d.NC2) {
-        ob[a] = cbox[g * d.NC2 + a];
-        nbx[a] = cnew[g * d.NC2 + a];
-        vol *= ob[a];
-      }
-    double* base = C + coff[g];
-    for (i64 e = tid; e < vol; e += 256) {
-      i64 x = e, off = 0, st = 1;
-      bool inside = true;
-#pragma unroll
-      for (int a = 0; a < MAXAX; ++a)
-        if (a < d.NC2) {
-          i64 q = x / ob[a];
-          int r = (int)(x - q * ob[a]);
-          inside = inside && r < nbx[a];
-          off += r * st;
-          st *= d.c_ext[a];
-          x = q;
-        }
-      if (!inside) base[off] = 0.0;
-    }
-    __syncthreads();
-    if (tid < d.NC2) cbox[g * d.NC2 + tid] = nbx[tid];
-    __syncthreads();
-  }
+// work list: tile t of position i -> tpos[t] = i (positions in locality order)
+__global__ void tile_groups_kernel(const i32* __restrict__ toff, i64 ng, i32* __restrict__ tpos) {
+  i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng) return;
+  for (i32 t = toff[i]; t < toff[i + 1]; ++t) tpos[t] = (i32)i;
 }
 
 size_t box_scan_tmp_bytes(Ctx* ctx, i64 ng) {
@@ -242,15 +205,14 @@ size_t box_scan_tmp_bytes(Ctx* ctx, i64 ng) {
 
 void box_prepare(Ctx* ctx, const BoxGemmArgs& g, const i32* abox, const i32* aflag, const i32* bbox,
                  const i32* bflag, i32* cbox, i32* cflag, double* C) {
+  (void)C;
   if (g.ng == 0) return;
   const BoxTile t = box_tile(g.cfg);
-  hipLaunchKernelGGL(box_groups_kernel, dim3(nb256(g.ng)), dim3(256), 0, ctx->stream, g.d, g.ng, g.seg, g.arow,
-                     g.brow, abox, aflag, bbox, bflag, g.gbox, g.kbox, g.kc, cbox, cflag, g.cnew, g.clear, g.tiles,
-                     t.BM, t.BN);
+  hipLaunchKernelGGL(box_groups_kernel, dim3(nb256(g.ng)), dim3(256), 0, ctx->stream, g.d, g.ng, g.gorder, g.seg,
+                     g.arow, g.brow, abox, aflag, bbox, bflag, g.gbox, g.kbox, g.kc, cbox, cflag, g.cnew, g.clear,
+                     g.tiles, t.BM, t.BN);
   HIP_CHECK(hipGetLastError());
-  if (g.tile_group) {  // the work list (absent while the tile shape is being chosen)
-    hipLaunchKernelGGL(first_zero_kernel, dim3(1), dim3(1), 0, ctx->stream, g.toff);
-    HIP_CHECK(hipGetLastError());
+  if (g.tile_group) {  // the work list (absent while the tile shape is being chosen); toff[0] == 0 from planning
     size_t tb = g.scan_tmp_bytes;
     HIP_CHECK(rocprim::inclusive_scan(g.scan_tmp, tb, g.tiles, g.toff + 1, (size_t)g.ng, rocprim::plus<i32>(),
                                       ctx->stream));
@@ -258,10 +220,6 @@ void box_prepare(Ctx* ctx, const BoxGemmArgs& g, const i32* abox, const i32* afl
                        g.tile_group);
     HIP_CHECK(hipGetLastError());
   }
-  unsigned cg = (unsigned)std::min<i64>(g.ng, 2048);
-  hipLaunchKernelGGL(box_clear_kernel, dim3(cg), dim3(256), 0, ctx->stream, g.d, g.ng, g.clear, g.cnew, cbox, g.coff,
-                     C);
-  HIP_CHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- block products
@@ -296,22 +254,29 @@ __device__ inline void split3_b(float x, unsigned short& h, unsigned short& m, u
   l = bf16_rne_b(r2);
 }
 
+constexpr int NXCD = 8;  // XCDs of the MI355X: consecutive workgroups go round-robin over them
+
 // One workgroup (4 waves) per (group, tile) of the work list, persistent over
-// the list.  The K dimension of a tile is the concatenation of the K boxes of
-// the group's pairs in the reference's pair order; a table in LDS maps each K
-// position to its absolute A and B element offsets.  PREC as grouped_gemm.
+// the list.  The list is cut into NXCD contiguous ranges, one per XCD, so the
+// tiles of neighbouring groups (which share operand blocks) meet in one L2.
+// The K dimension of a tile is the concatenation of the K boxes of the
+// group's pairs in the reference's pair order; a table in LDS maps each K
+// position to its absolute A and B element offsets.  Operand tiles for the
+// next K chunk are loaded into registers while the current one multiplies.
+// Before the tiles, the workgroups clear the C entries that fell out of a
+// shrunken box (disjoint from every tile's writes).  PREC as grouped_gemm.
 template <int BM, int BN, int WM, int WN, int PREC>
-__global__ __launch_bounds__(256) void box_gemm_kernel(BoxDesc d, i64 ng, const i32* __restrict__ seg,
-                                                       const i64* __restrict__ aoff, const i64* __restrict__ boff,
-                                                       const i64* __restrict__ coff, const i32* __restrict__ gbox,
-                                                       const i32* __restrict__ kbox, const i32* __restrict__ kc,
-                                                       const i32* __restrict__ toff, const i32* __restrict__ tgrp,
-                                                       const double* __restrict__ A, const double* __restrict__ B,
-                                                       double* __restrict__ C, i32* __restrict__ cflag, int amf,
-                                                       int bnf) {
+__global__ __launch_bounds__(256) void box_gemm_kernel(
+    BoxDesc d, i64 ng, const i32* __restrict__ gorder, const i32* __restrict__ seg, const i64* __restrict__ aoff,
+    const i64* __restrict__ boff, const i64* __restrict__ coff, const i32* __restrict__ gbox,
+    const i32* __restrict__ kbox, const i32* __restrict__ kc, const i32* __restrict__ toff,
+    const i32* __restrict__ tpos, const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
+    i32* __restrict__ cflag, const i32* __restrict__ clear, const i32* __restrict__ cnew, i32* __restrict__ cbox,
+    int amf, int bnf) {
   constexpr int BK = 16, KT = 512;
   constexpr int FM = BM / (16 * WM), FN = BN / (16 * WN);
-  static_assert(FM >= 1 && FN >= 1 && WM * WN == 4, "tile");
+  constexpr int AE = BM * BK / 256, BE = BN * BK / 256;
+  static_assert(FM >= 1 && FN >= 1 && WM * WN == 4 && AE >= 1 && BE >= 1, "tile");
   using T = typename std::conditional<PREC == 0, double, float>::type;
   __shared__ T As[BK][BM + 2];
   __shared__ T Bs[BK][BN + 2];
@@ -320,10 +285,48 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(BoxDesc d, i64 ng, const 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, wn = wave / WM;
   const int nmn = d.nm + d.nn;
+
+  // ---- stale entries of shrunken C boxes
+  for (i64 g = blockIdx.x; g < ng; g += gridDim.x) {
+    if (!clear[g]) continue;
+    int ob[MAXAX], nbx[MAXAX];
+    i64 vol = 1;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a)
+      if (a < d.NC2) {
+        ob[a] = cbox[g * d.NC2 + a];
+        nbx[a] = cnew[g * d.NC2 + a];
+        vol *= ob[a];
+      }
+    double* base = C + coff[g];
+    for (i64 e = tid; e < vol; e += 256) {
+      i64 x = e, off = 0, st = 1;
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < MAXAX; ++a)
+        if (a < d.NC2) {
+          i64 q = x / ob[a];
+          int r = (int)(x - q * ob[a]);
+          inside = inside && r < nbx[a];
+          off += r * st;
+          st *= d.c_ext[a];
+          x = q;
+        }
+      if (!inside) base[off] = 0.0;
+    }
+    __syncthreads();  // every thread has read the old box before it is replaced
+    if (tid < d.NC2) cbox[g * d.NC2 + tid] = nbx[tid];
+  }
+
+  // ---- tiles
   const i32 total = toff[ng];
-  for (i32 w = blockIdx.x; w < total; w += gridDim.x) {
-    const i32 g = tgrp[w];
-    const i32 lt = w - toff[g];
+  const int xcd = blockIdx.x % NXCD, slot = blockIdx.x / NXCD, nper = gridDim.x / NXCD;
+  const i32 chunk = (total + NXCD - 1) / NXCD;
+  const i32 wlo = xcd * chunk, whi = min(total, wlo + chunk);
+  for (i32 w = wlo + slot; w < whi; w += nper) {
+    const i32 pos = tpos[w];
+    const i32 g = gorder ? gorder[pos] : pos;
+    const i32 lt = w - toff[pos];
     int mb[MAXAX], nbx[MAXAX];
     int Mc = 1, Nc = 1;
 #pragma unroll
@@ -387,37 +390,40 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(BoxDesc d, i64 ng, const 
         }
       }
       __syncthreads();
-      for (int k0 = 0; k0 < fill; k0 += BK) {
-        if (k0) __syncthreads();
-        for (int idx = tid; idx < BM * BK; idx += 256) {
-          int m, k;
-          if (amf) {
-            m = idx % BM;
-            k = idx / BM;
-          } else {
-            k = idx % BK;
-            m = idx / BK;
-          }
-          double v = 0.0;
-          const int ra = rowA[m];
-          if (ra >= 0 && k0 + k < fill) v = A[tabA[k0 + k] + ra];
-          As[k][m] = (T)v;
+      double ra[AE], rb[BE];
+      auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+          const int idx = tid + e * 256;
+          const int m = amf ? idx % BM : idx / BK, k = amf ? idx / BM : idx % BK;
+          const int r = rowA[m];
+          ra[e] = (r >= 0 && k0 + k < fill) ? A[tabA[k0 + k] + r] : 0.0;
         }
-        for (int idx = tid; idx < BK * BN; idx += 256) {
-          int n, k;
-          if (bnf) {
-            n = idx % BN;
-            k = idx / BN;
-          } else {
-            k = idx % BK;
-            n = idx / BK;
-          }
-          double v = 0.0;
-          const int cb = colB[n];
-          if (cb >= 0 && k0 + k < fill) v = B[tabB[k0 + k] + cb];
-          Bs[k][n] = (T)v;
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+          const int idx = tid + e * 256;
+          const int n = bnf ? idx % BN : idx / BK, k = bnf ? idx / BN : idx % BK;
+          const int c = colB[n];
+          rb[e] = (c >= 0 && k0 + k < fill) ? B[tabB[k0 + k] + c] : 0.0;
+        }
+      };
+      load(0);
+      for (int k0 = 0; k0 < fill; k0 += BK) {
+        if (k0) __syncthreads();  // the previous chunk's products are done with As / Bs
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+          const int idx = tid + e * 256;
+          const int m = amf ? idx % BM : idx / BK, k = amf ? idx / BM : idx % BK;
+          As[k][m] = (T)ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+          const int idx = tid + e * 256;
+          const int n = bnf ? idx % BN : idx / BK, k = bnf ? idx / BN : idx % BK;
+          Bs[k][n] = (T)rb[e];
         }
         __syncthreads();
+        if (k0 + BK < fill) load(k0 + BK);  // in flight while this chunk multiplies
         if constexpr (PREC == 0) {
 #pragma unroll
           for (int kk = 0; kk < BK / 4; ++kk) {
@@ -519,35 +525,41 @@ BoxTile box_tile(int cfg) {
 }
 int box_tile_configs() { return kNumTiles; }
 
+template <int BM, int BN, int WM, int WN, int PREC>
+static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
+  auto kern = box_gemm_kernel<BM, BN, WM, WN, PREC>;
+  static int per_cu = 0;  // resident workgroups per CU (same for every device of this build)
+  if (!per_cu) {
+    int nb = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0));
+    per_cu = std::max(1, nb);
+  }
+  i64 nwg = (i64)ctx->num_cu * per_cu;
+  nwg = std::max<i64>(NXCD, nwg / NXCD * NXCD);  // a multiple of the XCD count
+  hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.gorder, g.seg, g.aoff,
+                     g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.tile_group, A, B, C, cflag, g.clear, g.cnew,
+                     g.cbox, g.amf, g.bnf);
+  HIP_CHECK(hipGetLastError());
+}
+
 template <int PREC>
 static void launch_box(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
-  const BoxTile t = kTiles[g.cfg];
-  i64 per_cu = 4;
-  i64 nwg = std::min<i64>(g.max_tiles, (i64)ctx->num_cu * per_cu);
-  if (nwg <= 0) return;
-  dim3 grid((unsigned)nwg), blk(256);
-#define SB_BOX_LAUNCH(BM_, BN_, WM_, WN_)                                                                        \
-  hipLaunchKernelGGL((box_gemm_kernel<BM_, BN_, WM_, WN_, PREC>), grid, blk, 0, ctx->stream, g.d, g.ng, g.seg, \
-                     g.aoff, g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.tile_group, A, B, C, cflag, g.amf,  \
-                     g.bnf)
   switch (g.cfg) {
-    case 0: SB_BOX_LAUNCH(16, 128, 1, 4); break;
-    case 1: SB_BOX_LAUNCH(128, 16, 4, 1); break;
-    case 2: SB_BOX_LAUNCH(32, 32, 2, 2); break;
-    case 3: SB_BOX_LAUNCH(64, 32, 2, 2); break;
-    case 4: SB_BOX_LAUNCH(32, 64, 2, 2); break;
-    case 5: SB_BOX_LAUNCH(64, 64, 2, 2); break;
-    case 6: SB_BOX_LAUNCH(16, 64, 1, 4); break;
-    case 7: SB_BOX_LAUNCH(64, 16, 4, 1); break;
+    case 0: launch_one<16, 128, 1, 4, PREC>(ctx, g, A, B, C, cflag); break;
+    case 1: launch_one<128, 16, 4, 1, PREC>(ctx, g, A, B, C, cflag); break;
+    case 2: launch_one<32, 32, 2, 2, PREC>(ctx, g, A, B, C, cflag); break;
+    case 3: launch_one<64, 32, 2, 2, PREC>(ctx, g, A, B, C, cflag); break;
+    case 4: launch_one<32, 64, 2, 2, PREC>(ctx, g, A, B, C, cflag); break;
+    case 5: launch_one<64, 64, 2, 2, PREC>(ctx, g, A, B, C, cflag); break;
+    case 6: launch_one<16, 64, 1, 4, PREC>(ctx, g, A, B, C, cflag); break;
+    case 7: launch_one<64, 16, 4, 1, PREC>(ctx, g, A, B, C, cflag); break;
     default: fail(SB_E_ASSERT, "box tile configuration");
   }
-#undef SB_BOX_LAUNCH
-  HIP_CHECK(hipGetLastError());
 }
 
 void box_gemm(Ctx* ctx, unsigned prec, const BoxGemmArgs& g, const double* A, const double* B, double* C,
               i32* cflag) {
-  if (g.ng == 0 || g.max_tiles == 0) return;
+  if (g.ng == 0) return;
   switch (prec & SB_PREC_MASK) {
     case SB_PREC_FP64: launch_box<0>(ctx, g, A, B, C, cflag); break;
     case SB_PREC_FP32: launch_box<1>(ctx, g, A, B, C, cflag); break;

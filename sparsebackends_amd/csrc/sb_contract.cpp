@@ -536,6 +536,28 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
   g.clear = pl.keep(DBuf<i32>(ctx, ng));
   g.tiles = pl.keep(DBuf<i32>(ctx, ng));
   g.toff = pl.keep(DBuf<i32>(ctx, ng + 1));
+  {
+    const i64 zero = 0;
+    k_fill_rows(ctx, g.toff, 1, 1, &zero);  // toff[0]; the scan writes toff[1..ng] every execution
+  }
+  g.cbox = C.box.get();
+  // locality order of the groups: by the C key with its A-sourced axes most
+  // significant, so groups reading the same A blocks sit next to each other
+  if (PC > 0 && ng > 1) {
+    std::vector<int> sel;
+    std::vector<i64> pd;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int j = 0; j < PC; ++j)
+        if ((src[j] > 0) == (pass == 1)) {
+          sel.push_back(j + 1);
+          pd.push_back(C.dims[j]);
+        }
+    DBuf<i32> tk(ctx, (size_t)ng * PC), idx(ctx, ng);
+    DBuf<u64> lin(ctx, ng);
+    k_remap_keys(ctx, C.keys.get(), ng, PC, sel.data(), PC, pd.data(), tk.get(), lin.get(), idx.get());
+    sort_pairs(ctx, lin.get(), idx.get(), ng, bits_for((u64)prod(pd, 0, PC) - 1));
+    g.gorder = pl.keep(std::move(idx));
+  }
   g.scan_tmp_bytes = box_scan_tmp_bytes(ctx, ng);
   g.scan_tmp = pl.keep(DBuf<unsigned char>(ctx, g.scan_tmp_bytes));
   std::vector<i64> cfull(C.dims.begin() + PC, C.dims.end());

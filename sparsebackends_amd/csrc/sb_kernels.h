@@ -139,9 +139,11 @@ struct BoxGemmArgs {
   const i32* arow = nullptr;  // per pair: box row of the A block
   const i32* brow = nullptr;
   const i64* coff = nullptr;  // per group: element offset of the C block
+  const i32* gorder = nullptr;  // groups in locality order (null: key order)
   // per execution
   i32 *gbox = nullptr, *kbox = nullptr, *kc = nullptr, *cnew = nullptr, *clear = nullptr;
-  i32 *tiles = nullptr, *toff = nullptr, *tile_group = nullptr;
+  i32 *tiles = nullptr, *toff = nullptr, *tile_group = nullptr;  // per position; toff[0] = 0 from planning
+  i32* cbox = nullptr;  // C's box rows (old until the GEMM kernel installs the new ones)
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
 };
@@ -155,7 +157,8 @@ size_t box_scan_tmp_bytes(Ctx*, i64 ng);
 // when tile_group is set, the (group, tile) work list
 void box_prepare(Ctx*, const BoxGemmArgs&, const i32* abox, const i32* aflag, const i32* bbox, const i32* bflag,
                  i32* cbox, i32* cflag, double* C);
-// C_g[box] = sum over pairs and K boxes of A_p * B_p (flags non-finite outputs in cflag)
+// C_g[box] = sum over pairs and K boxes of A_p * B_p (flags non-finite outputs in cflag); first
+// clears the entries of C that fell outside shrunken boxes and installs those boxes
 void box_gemm(Ctx*, unsigned prec, const BoxGemmArgs&, const double* A, const double* B, double* C, i32* cflag);
 
 // element offsets of blocks: off[i] = (ids[i]-1)*bs  (ids == null: i*bs)
