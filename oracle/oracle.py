@@ -21,10 +21,18 @@ part is pinned (tests/test_oracle_*.py):
   pinned by mathematics (brute-force dense einsum on random inputs,
   invariants: permute/to_dense commutation, round trips, closed forms)
     every contraction family, every permute, fusion of axes
+  AliasedBlockSparse storage and kernels (no reference test covers them):
+    logical content pinned by mathematics (to_dense of inputs and outputs
+    against a dense einsum; to_blocksparse / getindex / permutedims against
+    to_dense); the aliasing rules checked structurally (a key fed by one
+    combined template stays aliased with the summed scalar, two different
+    templates give a concrete block with scalar 1, template counts)
   parity unpinned (only an import of the Julia reference could pin them, and
   Julia is absent): the `ids` allocation order of each kernel, the tie order of
-  Julia's stable sortperm in contract_prefix_outer_bd!, and the Kahan
-  summation order there.  They are restated by reading the reference.
+  Julia's stable sortperm in contract_prefix_outer_bd!, the Kahan summation
+  order there, and the template ids of accumulated aliased blocks (Julia Dict
+  order; here column-major key order).  They are restated by reading the
+  reference.
 """
 from __future__ import annotations
 
@@ -196,6 +204,72 @@ class BS(Handle):
         out = C.c_void_p()
         _chk(lib().orc_bs_fuse_two_axes(self.h, C.c_int64(ax1), C.c_int64(ax2), C.byref(out)))
         return BS(out)
+
+
+class AL(Handle):
+    """Oracle AliasedBlockSparse: block i = scalars[i] * templates[alias_ids[i]]."""
+
+    @staticmethod
+    def new(dims, N2):
+        out = C.c_void_p()
+        d = _i64(dims)
+        _chk(lib().orc_al_new(C.c_int(len(dims)), C.c_int(N2), _p(d, C.c_int64), C.byref(out)))
+        return AL(out)
+
+    @staticmethod
+    def from_parts(dims, N2, templates, ntemplates, keys, alias_ids, scalars):
+        d, t = _i64(dims), _f64(templates).reshape(-1)
+        k, a, sc = _i64(keys).reshape(-1), _i64(alias_ids), _f64(scalars)
+        out = C.c_void_p()
+        _chk(lib().orc_al_from_parts(C.c_int(len(dims)), C.c_int(N2), _p(d, C.c_int64), C.c_int64(ntemplates),
+                                     _p(t, C.c_double), C.c_int64(len(a)), _p(k, C.c_int64), _p(a, C.c_int64),
+                                     _p(sc, C.c_double), C.byref(out)))
+        return AL(out)
+
+    def info(self):
+        N, N2, nb, bs, nt = C.c_int(), C.c_int(), C.c_int64(), C.c_int64(), C.c_int64()
+        dims = np.zeros(16, dtype=np.int64)
+        _chk(lib().orc_al_info(self.h, C.byref(N), C.byref(N2), _p(dims, C.c_int64), C.byref(nb), C.byref(bs),
+                               C.byref(nt)))
+        return tuple(int(x) for x in dims[:N.value]), N2.value, nb.value, bs.value, nt.value
+
+    def parts(self):
+        dims, N2, nb, bs, nt = self.info()
+        P = len(dims) - N2
+        keys = np.zeros((nb, P), dtype=np.int64)
+        ids = np.zeros(nb, dtype=np.int64)
+        sc = np.zeros(nb, dtype=np.float64)
+        tm = np.zeros(nt * bs, dtype=np.float64)
+        _chk(lib().orc_al_get(self.h, _p(keys, C.c_int64), _p(ids, C.c_int64), _p(sc, C.c_double),
+                              _p(tm, C.c_double)))
+        return dict(dims=dims, N2=N2, blksize=bs, ntemplates=nt, keys=keys, alias_ids=ids, scalars=sc, templates=tm)
+
+    def getindex(self, *I):
+        v = C.c_double()
+        a = _i64(I)
+        _chk(lib().orc_al_getindex(self.h, _p(a, C.c_int64), C.byref(v)))
+        return v.value
+
+    def permutedims(self, perm):
+        p = _i64(perm)
+        out = C.c_void_p()
+        _chk(lib().orc_al_permutedims(self.h, _p(p, C.c_int64), C.byref(out)))
+        return AL(out)
+
+    def to_blocksparse(self):
+        out = C.c_void_p()
+        _chk(lib().orc_al_to_blocksparse(self.h, C.byref(out)))
+        return BS(out)
+
+    def to_dense(self, init=0.0):
+        out = C.c_void_p()
+        _chk(lib().orc_al_to_dense(self.h, C.c_double(init), C.byref(out)))
+        return Dense(out).array()
+
+    def compression_ratio(self):
+        v = C.c_double()
+        _chk(lib().orc_al_compression_ratio(self.h, C.byref(v)))
+        return v.value
 
 
 class COO(Handle):

@@ -1739,6 +1739,900 @@ static void contract_coo_dense(Tensor& C, const Labels& labelsC, Oper& A, Oper& 
   }
 }
 
+// ================================================================ AliasedBlockSparse
+// Block i of an AliasedBlockSparse is scalars[i] * templates[alias_ids[i]]
+// (aliased/storage.jl:46-59).  Tensor fields: keys (n x P, sorted by the
+// column-major prefix lin), ids = alias_ids (1-based template ids), scalars,
+// data = templates (ntemplates x blksize).
+
+// AliasedBlockSparse{T,N,N2}(dims) (aliased/storage.jl:122-132)
+static Tensor al_new(const std::vector<i64>& dims, int N2) {
+  int N = (int)dims.size();
+  OASSERT(0 < N2 && N2 <= N, "0 < N2 <= N");
+  for (auto d : dims) OASSERT(d >= 1, "all(dims .>= 1)");
+  Tensor t;
+  t.kind = ALIASED;
+  t.dims = dims;
+  t.N2 = N2;
+  t.blksize = prod(dims, N - N2, N);
+  t.ntemplates = 0;
+  return t;
+}
+
+static void al_clear(Tensor& C) {  // empty!(templates / keys / alias_ids / scalars), n_templates = 0
+  C.data.clear();
+  C.ntemplates = 0;
+  C.keys.clear();
+  C.ids.clear();
+  C.scalars.clear();
+}
+
+static const double* al_template(const Tensor& A, i64 t) { return A.data.data() + (t - 1) * A.blksize; }
+
+// getindex (aliased/storage.jl:138-147) with _find_aliased_key (:97-111)
+static double al_getindex(const Tensor& A, const i64* I) {
+  check_index(A, I);
+  int P = A.P();
+  auto [pos, found] = find_key(A, I);
+  if (!found) return 0.0;
+  i64 tid = A.ids[pos];
+  i64 bi = block_index(A, I);
+  return A.scalars[pos] * A.data[(tid - 1) * A.blksize + bi - 1];
+  (void)P;
+}
+
+// compression_ratio (aliased/storage.jl:163-164)
+static double al_compression_ratio(const Tensor& A) {
+  return A.ntemplates == 0 ? 1.0 : (double)A.nblocks() / (double)A.ntemplates;
+}
+
+// Base.permutedims(::AliasedBlockSparse, perm) (aliased/storage.jl:172-239):
+// only the templates are transposed; keys are remapped and stably sorted.
+static Tensor al_permutedims(const Tensor& A, const std::vector<i64>& perm) {
+  int N = A.N(), P = A.P(), N2 = A.N2;
+  check_no_cross_perm(perm, N, P);
+  Tensor B;
+  B.kind = ALIASED;
+  B.N2 = N2;
+  B.dims.resize(N);
+  for (int j = 0; j < N; ++j) B.dims[j] = A.dims[perm[j] - 1];
+  std::vector<i64> perm_block(N2);
+  for (int i = 0; i < N2; ++i) perm_block[i] = perm[P + i] - P;
+  B.blksize = prod(B.dims, P, N);
+  OASSERT(B.blksize == A.blksize, "permutation within tail preserves block size");
+  bool ident = true;
+  for (int i = 0; i < N2; ++i) ident = ident && perm_block[i] == i + 1;
+  B.ntemplates = A.ntemplates;
+  if (ident) {
+    B.data = A.data;
+  } else {
+    std::vector<i64> tailA(A.dims.begin() + P, A.dims.end());
+    std::vector<i64> map_old = tail_map(tailA, perm_block);
+    B.data.resize((size_t)A.ntemplates * B.blksize);
+    for (i64 t = 0; t < A.ntemplates; ++t)
+      for (i64 nl = 0; nl < B.blksize; ++nl) B.data[t * B.blksize + nl] = A.data[t * A.blksize + map_old[nl] - 1];
+  }
+  i64 n = A.nblocks();
+  std::vector<i64> newkeys((size_t)n * P);
+  for (i64 i = 0; i < n; ++i)
+    for (int j = 0; j < P; ++j) newkeys[i * P + j] = A.keys[i * P + perm[j] - 1];
+  std::vector<i64> p = sortperm_lin(newkeys, n, P, B.dims.data());
+  B.keys.resize((size_t)n * P);
+  B.ids.resize(n);
+  B.scalars.resize(n);
+  for (i64 o = 0; o < n; ++o) {
+    for (int j = 0; j < P; ++j) B.keys[o * P + j] = newkeys[p[o] * P + j];
+    B.ids[o] = A.ids[p[o]];
+    B.scalars[o] = A.scalars[p[o]];
+  }
+  return B;
+}
+
+// to_blocksparse (aliased/conversions.jl:18-32)
+static Tensor al_to_blocksparse(const Tensor& A) {
+  Tensor B;
+  B.kind = BS;
+  B.dims = A.dims;
+  B.N2 = A.N2;
+  B.blksize = A.blksize;
+  B.keys = A.keys;
+  i64 n = A.nblocks();
+  B.ids.resize(n);
+  B.data.resize((size_t)n * A.blksize);
+  for (i64 i = 0; i < n; ++i) {
+    B.ids[i] = i + 1;
+    const double* t = al_template(A, A.ids[i]);
+    for (i64 j = 0; j < A.blksize; ++j) B.data[i * A.blksize + j] = A.scalars[i] * t[j];
+  }
+  return B;
+}
+
+// to_dense (aliased/conversions.jl:45-64)
+static Tensor al_to_dense(const Tensor& A, double init) {
+  Tensor D;
+  D.kind = DENSE;
+  D.dims = A.dims;
+  int N = A.N(), P = A.P();
+  i64 tot = prod(A.dims, 0, N);
+  D.data.assign(tot, init);
+  i64 pstride = 1;
+  std::vector<i64> ps(P);
+  for (int d = 0; d < P; ++d) {
+    ps[d] = pstride;
+    pstride *= A.dims[d];
+  }
+  for (i64 i = 0; i < A.nblocks(); ++i) {
+    i64 base = 0;
+    for (int d = 0; d < P; ++d) base += (A.keys[i * P + d] - 1) * ps[d];
+    const double* t = al_template(A, A.ids[i]);
+    for (i64 j = 0; j < A.blksize; ++j) D.data[base + j * pstride] = A.scalars[i] * t[j];
+  }
+  return D;
+}
+
+// _aliased_contribute! / _commit_aliased_dicts! (contract_aliased.jl:75-146):
+// a C key fed by contributions of one combined template stays aliased (its
+// scalars summed in order); a contribution of a different template demotes it
+// to a concrete accumulated block.
+struct AliasState {
+  std::map<std::vector<i64>, std::pair<i64, double>> alias;
+  std::map<std::vector<i64>, std::vector<double>> accum;
+};
+static void al_contribute(AliasState& S, const std::vector<double>& Ct, const std::vector<i64>& ckey, i64 tid,
+                          double a, i64 bs) {
+  auto it = S.accum.find(ckey);
+  if (it != S.accum.end()) {
+    const double* t = Ct.data() + (tid - 1) * bs;
+    for (i64 j = 0; j < bs; ++j) it->second[j] += a * t[j];
+    return;
+  }
+  auto ia = S.alias.find(ckey);
+  if (ia != S.alias.end()) {
+    i64 tprev = ia->second.first;
+    double aprev = ia->second.second;
+    if (tprev == tid) {
+      ia->second.second = aprev + a;
+    } else {
+      std::vector<double> acc(bs);
+      const double *t0 = Ct.data() + (tprev - 1) * bs, *t1 = Ct.data() + (tid - 1) * bs;
+      for (i64 j = 0; j < bs; ++j) acc[j] = aprev * t0[j] + a * t1[j];
+      S.alias.erase(ia);
+      S.accum.emplace(ckey, std::move(acc));
+    }
+    return;
+  }
+  S.alias.emplace(ckey, std::make_pair(tid, a));
+}
+// Julia appends the accumulated blocks in Dict iteration order; here they are
+// appended in column-major key order (their template ids are not pinned).
+static void al_commit(Tensor& C, AliasState& S) {
+  int PC = C.P();
+  for (auto& kv : S.alias) {
+    C.keys.insert(C.keys.end(), kv.first.begin(), kv.first.end());
+    C.ids.push_back(kv.second.first);
+    C.scalars.push_back(kv.second.second);
+  }
+  std::vector<const std::vector<i64>*> ak;
+  for (auto& kv : S.accum) ak.push_back(&kv.first);
+  std::stable_sort(ak.begin(), ak.end(), [&](const std::vector<i64>* x, const std::vector<i64>* y) {
+    return prefix_lin(x->data(), C.dims.data(), PC) < prefix_lin(y->data(), C.dims.data(), PC);
+  });
+  for (auto* k : ak) {
+    const auto& acc = S.accum[*k];
+    C.ntemplates += 1;
+    C.data.insert(C.data.end(), acc.begin(), acc.end());
+    C.keys.insert(C.keys.end(), k->begin(), k->end());
+    C.ids.push_back(C.ntemplates);
+    C.scalars.push_back(1.0);
+  }
+  i64 n = (i64)C.ids.size();
+  if (n == 0) return;
+  std::vector<i64> p = sortperm_lin(C.keys, n, PC, C.dims.data());
+  std::vector<i64> k2((size_t)n * PC), i2(n);
+  std::vector<double> s2(n);
+  for (i64 o = 0; o < n; ++o) {
+    for (int j = 0; j < PC; ++j) k2[o * PC + j] = C.keys[p[o] * PC + j];
+    i2[o] = C.ids[p[o]];
+    s2[o] = C.scalars[p[o]];
+  }
+  C.keys.swap(k2);
+  C.ids.swap(i2);
+  C.scalars.swap(s2);
+}
+
+// combined-template registry: first-seen order of the combination key
+struct Combined {
+  std::map<std::pair<i64, i64>, i64> id;
+  i64 get(const std::pair<i64, i64>& k) const {
+    auto it = id.find(k);
+    return it == id.end() ? 0 : it->second;
+  }
+};
+
+static void al_apply_perm(Oper& X, const std::vector<i64>& perm) {
+  X.own = al_permutedims(*X.t, perm);
+  X.set_own();
+  Labels l2(perm.size());
+  for (size_t i = 0; i < perm.size(); ++i) l2[i] = X.labels[perm[i] - 1];
+  X.labels = l2;
+  X.map = make_map(l2);
+}
+// _aliased_r_to_last_prefix / _aliased_r_to_last_dense (contract_aliased.jl:33-65)
+static void al_r_to_last_prefix(Oper& X, const Label& rlab) {
+  int axisr = at(X.map, rlab), P = X.t->P(), N = X.t->N();
+  OASSERT(axisr <= P, "rlab must be in sparse prefix");
+  if (axisr == P) return;
+  std::vector<i64> perm;
+  for (int a = 1; a <= P; ++a)
+    if (a != axisr) perm.push_back(a);
+  perm.push_back(axisr);
+  for (int a = P + 1; a <= N; ++a) perm.push_back(a);
+  al_apply_perm(X, perm);
+}
+static void al_r_to_last_dense(Oper& X, const Label& rlab) {
+  int axisr = at(X.map, rlab), P = X.t->P(), N = X.t->N(), N2 = X.t->N2;
+  OASSERT(axisr > P, "rlab must be in dense tail");
+  if (axisr == N) return;
+  int rl = axisr - P;
+  std::vector<i64> perm;
+  for (int a = 1; a <= P; ++a) perm.push_back(a);
+  for (int d = 1; d <= N2; ++d)
+    if (d != rl) perm.push_back(P + d);
+  perm.push_back(P + rl);
+  al_apply_perm(X, perm);
+}
+
+// dense B with rlab moved last (contract_aliased.jl:168-177)
+static void dense_r_last(Oper& B, const Label& rlab) {
+  int NB = B.t->N(), ax = at(B.map, rlab);
+  if (ax == NB) return;
+  std::vector<i64> perm;
+  for (int d = 1; d <= NB; ++d)
+    if (d != ax) perm.push_back(d);
+  perm.push_back(ax);
+  B.own = dense_permutedims(*B.t, perm);
+  B.set_own();
+  Labels l2(NB);
+  for (int i = 0; i < NB; ++i) l2[i] = B.labels[perm[i] - 1];
+  B.labels = l2;
+  B.map = make_map(l2);
+}
+
+// outer(a (m), b (n)) into a zeroed template: t[i + j*m] = 0 + a[i]*b[j] (_outer_add!, contract_bs_bs.jl:7-19)
+static void outer_into(std::vector<double>& Ct, i64 off, const double* a, i64 m, const double* b, i64 n) {
+  for (i64 j = 0; j < n; ++j)
+    for (i64 i = 0; i < m; ++i) Ct[off + i + j * m] += a[i] * b[j];
+}
+
+// _contract_aliased_prefix_outer_ad! (contract_aliased.jl:151-233)
+static void al_prefix_outer_ad(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  int PA = A.t->P(), NA = A.t->N(), PC = C.P(), NC = C.N();
+  OASSERT(at(A.map, rlab) == PA, "rlab must be the last prefix axis of A");
+  OASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  i64 R = B.t->dims[at(B.map, rlab) - 1];
+  dense_r_last(B, rlab);
+  int NB = B.t->N();
+  i64 chunkB = prod(B.t->dims, 0, NB - 1);
+  OASSERT(C.blksize == A.t->blksize * chunkB, "C.blksize must equal A.blksize * chunkB");
+  Labels Ad = sub(A.labels, PA, NA), Bwo = sub(B.labels, 0, NB - 1), Cd = sub(labelsC, PC, NC);
+  int order;
+  if (Cd == cat(Ad, Bwo))
+    order = 1;
+  else if (Cd == cat(Bwo, Ad))
+    order = 2;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense...,B_wo_r...] or [B_wo_r...,A_dense...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    int ax = at(A.map, labelsC[j]);
+    OASSERT(ax <= PA && labelsC[j] != rlab, "C prefix label must come from A sparse prefix excluding rlab");
+    src[j] = ax;
+  }
+  const Tensor& TA = *A.t;
+  const double* Bv = B.t->data.data();
+  Combined cm;
+  AliasState S;
+  std::vector<i64> ck(PC);
+  for (i64 i = 0; i < TA.nblocks(); ++i) {
+    const i64* ak = TA.keys.data() + i * PA;
+    i64 rv = ak[PA - 1];
+    if (!(1 <= rv && rv <= R)) continue;
+    i64 tidA = TA.ids[i];
+    double al = TA.scalars[i];
+    for (int j = 0; j < PC; ++j) ck[j] = ak[src[j] - 1];
+    i64 ct = cm.get({tidA, rv});
+    if (ct == 0) {
+      C.ntemplates += 1;
+      ct = C.ntemplates;
+      cm.id[{tidA, rv}] = ct;
+      i64 off = (i64)C.data.size();
+      C.data.resize(off + C.blksize, 0.0);
+      const double* ta = al_template(TA, tidA);
+      const double* bs = Bv + (rv - 1) * chunkB;
+      if (order == 1)
+        outer_into(C.data, off, ta, TA.blksize, bs, chunkB);
+      else
+        outer_into(C.data, off, bs, chunkB, ta, TA.blksize);
+    }
+    al_contribute(S, C.data, ck, ct, al, C.blksize);
+  }
+  al_commit(C, S);
+}
+
+// _contract_aliased_dense_ad! (contract_aliased.jl:238-365)
+static void al_dense_ad(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  int PA = A.t->P(), NA = A.t->N(), PC = C.P(), NC = C.N();
+  OASSERT(at(A.map, rlab) == NA, "rlab must be the last axis of A");
+  OASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  OASSERT(PC == PA, "C sparse prefix must equal A sparse prefix for dense-tail reduction");
+  dense_r_last(B, rlab);
+  int NB = B.t->N();
+  i64 R = B.t->dims[NB - 1];
+  i64 chunkB = prod(B.t->dims, 0, NB - 1);
+  OASSERT(A.t->dims[NA - 1] == R, "Reduction size R mismatch between A dense tail and B");
+  i64 chunkA = A.t->blksize / R;
+  OASSERT(C.blksize == chunkA * chunkB, "C.blksize == chunkA * chunkB");
+  Labels Awo = sub(A.labels, PA, NA - 1), Bwo = sub(B.labels, 0, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool AB;
+  if (Cd == cat(Awo, Bwo))
+    AB = true;
+  else if (Cd == cat(Bwo, Awo))
+    AB = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense_wo_r...,B_wo_r...] or [B_wo_r...,A_dense_wo_r...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    int ax = at(A.map, labelsC[j]);
+    OASSERT(ax <= PA, "C prefix label must come from A sparse prefix");
+    src[j] = ax;
+  }
+  const Tensor& TA = *A.t;
+  const double* Bm = B.t->data.data();  // (chunkB, R)
+  std::vector<i64> ctid(TA.ntemplates);
+  for (i64 t = 1; t <= TA.ntemplates; ++t) {
+    C.ntemplates += 1;
+    ctid[t - 1] = C.ntemplates;
+    i64 off = (i64)C.data.size();
+    C.data.resize(off + C.blksize, 0.0);
+    const double* Am = al_template(TA, t);  // (chunkA, R)
+    if (AB)
+      gemm_acc('N', 'T', chunkA, chunkB, R, Am, chunkA, Bm, chunkB, C.data.data() + off, chunkA);
+    else
+      gemm_acc('N', 'T', chunkB, chunkA, R, Bm, chunkB, Am, chunkA, C.data.data() + off, chunkB);
+  }
+  i64 n = TA.nblocks();
+  C.keys.resize((size_t)n * PC);
+  C.ids.resize(n);
+  C.scalars.resize(n);
+  for (i64 i = 0; i < n; ++i) {
+    for (int j = 0; j < PC; ++j) C.keys[i * PC + j] = TA.keys[i * PA + src[j] - 1];
+    C.ids[i] = ctid[TA.ids[i] - 1];
+    C.scalars[i] = TA.scalars[i];
+  }
+  if (n) {
+    std::vector<i64> p = sortperm_lin(C.keys, n, PC, C.dims.data());
+    std::vector<i64> k2((size_t)n * PC), i2(n);
+    std::vector<double> s2(n);
+    for (i64 o = 0; o < n; ++o) {
+      for (int j = 0; j < PC; ++j) k2[o * PC + j] = C.keys[p[o] * PC + j];
+      i2[o] = C.ids[p[o]];
+      s2[o] = C.scalars[p[o]];
+    }
+    C.keys.swap(k2);
+    C.ids.swap(i2);
+    C.scalars.swap(s2);
+  }
+}
+
+// contract_aliased! Aliased x Dense (contract_aliased.jl:383-408)
+static void contract_aliased_ad(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  OASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  al_clear(C);
+  if (at(A.map, rlab) <= A.t->P()) {
+    al_r_to_last_prefix(A, rlab);
+    al_prefix_outer_ad(C, labelsC, A, B, rlab);
+  } else {
+    al_r_to_last_dense(A, rlab);
+    al_dense_ad(C, labelsC, A, B, rlab);
+  }
+}
+
+// _contract_aliased_prefix_outer_aa! (contract_aliased.jl:415-507)
+static void al_prefix_outer_aa(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  int PA = A.t->P(), NA = A.t->N(), PB = B.t->P(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  OASSERT(at(A.map, rlab) == PA && at(B.map, rlab) == PB, "rlab is the last prefix axis of A and B");
+  OASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  OASSERT(C.blksize == A.t->blksize * B.t->blksize, "C.blksize == A.blksize * B.blksize");
+  Labels Ad = sub(A.labels, PA, NA), Bd = sub(B.labels, PB, NB), Cd = sub(labelsC, PC, NC);
+  int order;
+  if (Cd == cat(Ad, Bd))
+    order = 1;
+  else if (Cd == cat(Bd, Ad))
+    order = 2;
+  else
+    fail(SB_E_ERROR, "C dense must be [A_dense...,B_dense...] or [B_dense...,A_dense...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l) && at(A.map, l) != at(A.map, rlab)) {
+      int ax = at(A.map, l);
+      OASSERT(ax <= PA, "C prefix label from A prefix");
+      src[j] = ax;
+    } else {
+      OASSERT(has(B.map, l), "C prefix label not found in A or B");
+      int ax = at(B.map, l);
+      OASSERT(ax <= PB && l != rlab, "C prefix label from B prefix");
+      src[j] = -ax;
+    }
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  Combined cm;
+  AliasState S;
+  std::vector<i64> ck(PC);
+  i64 iA = 0, nA = TA.nblocks(), iB = 0, nB = TB.nblocks();
+  while (iA < nA && iB < nB) {
+    i64 rvA = TA.keys[iA * PA + PA - 1], rvB = TB.keys[iB * PB + PB - 1];
+    if (rvA < rvB) {
+      while (iA < nA && TA.keys[iA * PA + PA - 1] == rvA) ++iA;
+      continue;
+    } else if (rvB < rvA) {
+      while (iB < nB && TB.keys[iB * PB + PB - 1] == rvB) ++iB;
+      continue;
+    }
+    i64 blo = iB;
+    while (iB < nB && TB.keys[iB * PB + PB - 1] == rvA) ++iB;
+    i64 bhi = iB;
+    while (iA < nA && TA.keys[iA * PA + PA - 1] == rvA) {
+      const i64* ak = TA.keys.data() + iA * PA;
+      i64 tidA = TA.ids[iA];
+      double aA = TA.scalars[iA];
+      for (i64 j2 = blo; j2 < bhi; ++j2) {
+        const i64* bk = TB.keys.data() + j2 * PB;
+        i64 tidB = TB.ids[j2];
+        double aC = aA * TB.scalars[j2];
+        i64 ct = cm.get({tidA, tidB});
+        if (ct == 0) {
+          C.ntemplates += 1;
+          ct = C.ntemplates;
+          cm.id[{tidA, tidB}] = ct;
+          i64 off = (i64)C.data.size();
+          C.data.resize(off + C.blksize, 0.0);
+          const double *ta = al_template(TA, tidA), *tb = al_template(TB, tidB);
+          if (order == 1)
+            outer_into(C.data, off, ta, TA.blksize, tb, TB.blksize);
+          else
+            outer_into(C.data, off, tb, TB.blksize, ta, TA.blksize);
+        }
+        for (int j = 0; j < PC; ++j) ck[j] = src[j] > 0 ? ak[src[j] - 1] : bk[-src[j] - 1];
+        al_contribute(S, C.data, ck, ct, aC, C.blksize);
+      }
+      ++iA;
+    }
+  }
+  al_commit(C, S);
+}
+
+// _contract_aliased_dense_aa! (contract_aliased.jl:512-643)
+static void al_dense_aa(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  int PA = A.t->P(), NA = A.t->N(), PB = B.t->P(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  OASSERT(at(A.map, rlab) == NA && at(B.map, rlab) == NB, "rlab is the last axis of A and B");
+  OASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  i64 R = A.t->dims[NA - 1];
+  OASSERT(B.t->dims[NB - 1] == R, "Reduction size mismatch");
+  i64 chunkA = A.t->blksize / R, chunkB = B.t->blksize / R;
+  OASSERT(C.blksize == chunkA * chunkB, "C.blksize == chunkA * chunkB");
+  Labels Awo = sub(A.labels, PA, NA - 1), Bwo = sub(B.labels, PB, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool AB;
+  if (Cd == cat(Awo, Bwo))
+    AB = true;
+  else if (Cd == cat(Bwo, Awo))
+    AB = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense_wo_r...,B_dense_wo_r...] or swapped");
+  std::vector<int> srcA(PC, 0), srcB(PC, 0);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l)) {
+      int ax = at(A.map, l);
+      OASSERT(ax <= PA, "C prefix label from A prefix");
+      srcA[j] = ax;
+    } else {
+      OASSERT(has(B.map, l), "C prefix label from B");
+      int ax = at(B.map, l);
+      OASSERT(ax <= PB, "C prefix label from B prefix");
+      srcB[j] = ax;
+    }
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  Combined cm;
+  AliasState S;
+  std::vector<i64> ck(PC);
+  for (i64 ai = 0; ai < TA.nblocks(); ++ai) {
+    const i64* ak = TA.keys.data() + ai * PA;
+    i64 tidA = TA.ids[ai];
+    double aA = TA.scalars[ai];
+    for (i64 bi = 0; bi < TB.nblocks(); ++bi) {
+      const i64* bk = TB.keys.data() + bi * PB;
+      i64 tidB = TB.ids[bi];
+      double aC = aA * TB.scalars[bi];
+      i64 ct = cm.get({tidA, tidB});
+      if (ct == 0) {
+        C.ntemplates += 1;
+        ct = C.ntemplates;
+        cm.id[{tidA, tidB}] = ct;
+        i64 off = (i64)C.data.size();
+        C.data.resize(off + C.blksize, 0.0);
+        const double *Am = al_template(TA, tidA), *Bm = al_template(TB, tidB);
+        if (AB)
+          gemm_acc('N', 'T', chunkA, chunkB, R, Am, chunkA, Bm, chunkB, C.data.data() + off, chunkA);
+        else
+          gemm_acc('N', 'T', chunkB, chunkA, R, Bm, chunkB, Am, chunkA, C.data.data() + off, chunkB);
+      }
+      for (int j = 0; j < PC; ++j) ck[j] = srcA[j] ? ak[srcA[j] - 1] : bk[srcB[j] - 1];
+      al_contribute(S, C.data, ck, ct, aC, C.blksize);
+    }
+  }
+  al_commit(C, S);
+}
+
+// contract_aliased! Aliased x Aliased (contract_aliased.jl:659-690)
+static void contract_aliased_aa(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  OASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  al_clear(C);
+  if (at(B.map, rlab) <= B.t->P()) {
+    al_r_to_last_prefix(A, rlab);
+    al_r_to_last_prefix(B, rlab);
+    al_prefix_outer_aa(C, labelsC, A, B, rlab);
+  } else {
+    OASSERT(at(A.map, rlab) > A.t->P(), "rlab must be in A dense tail when it is in B dense tail");
+    al_r_to_last_dense(A, rlab);
+    al_r_to_last_dense(B, rlab);
+    al_dense_aa(C, labelsC, A, B, rlab);
+  }
+}
+
+// _advance_run (contract_bs_bs.jl:455-464): end (exclusive) of the run of rows
+// equal to row i0 on the given key columns (1-based)
+static i64 advance_run(const Tensor& T, i64 i0, const std::vector<int>& pos) {
+  int P = T.P();
+  i64 n = T.nblocks();
+  if (pos.empty()) return n;
+  i64 i = i0 + 1;
+  while (i < n) {
+    bool eq = true;
+    for (int p : pos) eq = eq && T.keys[i0 * P + p - 1] == T.keys[i * P + p - 1];
+    if (!eq) break;
+    ++i;
+  }
+  return i;
+}
+
+// contract_shared! Aliased x Dense (contract_aliased_shared.jl:103-302)
+static void shared_al_dense(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared) {
+  for (auto& l : shared) OASSERT(!contains(labelsC, l), "shared label must be reduced (not in labelsC)");
+  al_clear(C);
+  int PA = A.t->P(), NA = A.t->N(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    OASSERT(has(A.map, l), "shared label must exist in A");
+    OASSERT(has(B.map, l), "shared label must exist in B");
+    (at(A.map, l) <= PA ? sp : sd).push_back(l);
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : B.labels)
+    if (!contains(shared, l)) keepB0.push_back(l);
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; unsupported");
+  OASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA");
+  OASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB");
+  // A: prefix [keep_pref..., shared_pref...], dense [keepA..., red...]
+  {
+    std::vector<i64> spax, perm;
+    for (auto& l : sp) spax.push_back(at(A.map, l));
+    for (int a = 1; a <= PA; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) perm.push_back(a);
+    for (auto a : spax) perm.push_back(a);
+    std::vector<i64> ka, ra;
+    for (auto& l : dA) ka.push_back(at(A.map, l) - PA);
+    for (auto& l : red) ra.push_back(at(A.map, l) - PA);
+    OASSERT((int)(ka.size() + ra.size()) == A.t->N2, "perm covers dense axes");
+    for (auto a : ka) perm.push_back(PA + a);
+    for (auto a : ra) perm.push_back(PA + a);
+    if (!is_identity(perm)) al_apply_perm(A, perm);
+  }
+  // B: (shared_prefix..., red_dense..., keepB...)
+  std::vector<i64> permB;
+  for (auto& l : sp) permB.push_back(at(B.map, l));
+  for (auto& l : red) permB.push_back(at(B.map, l));
+  for (auto& l : dB) permB.push_back(at(B.map, l));
+  OASSERT((int)permB.size() == NB, "B perm length mismatch; labelsB must match B ndims");
+  if (!is_identity(permB)) {
+    B.own = dense_permutedims(*B.t, permB);
+    B.set_own();
+  }
+  int nsp = (int)sp.size(), nred = (int)red.size(), nkA = (int)dA.size(), nkB = (int)dB.size();
+  const Tensor& TA = *A.t;
+  const std::vector<i64>& dBp = B.t->dims;
+  i64 M = nkA == 0 ? 1 : prod(TA.dims, PA, PA + nkA);
+  i64 K = nred == 0 ? 1 : prod(TA.dims, PA + nkA, NA);
+  i64 Nn = nkB == 0 ? 1 : prod(dBp, nsp + nred, NB);
+  if (nred > 0) OASSERT(K == prod(dBp, nsp, nsp + nred), "Reduction extent mismatch between A dense tail and B");
+  OASSERT(C.blksize == M * Nn, "C.blksize must equal M*N");
+  std::vector<int> csrc(PC);
+  LabelMap mA = make_map(A.labels);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    OASSERT(has(mA, l), "C prefix label must exist in A");
+    int ap = at(mA, l);
+    OASSERT(ap <= PA, "C prefix label must come from A sparse prefix");
+    OASSERT(!contains(sp, l), "C prefix label cannot be a reduced shared-prefix label");
+    csrc[j] = ap;
+  }
+  std::vector<int> jA;
+  for (auto& l : sp) jA.push_back(at(mA, l));
+  i64 SP = prod(dBp, 0, nsp);
+  std::vector<i64> spstr(nsp);
+  {
+    i64 s = 1;
+    for (int t = 0; t < nsp; ++t) {
+      spstr[t] = s;
+      s *= dBp[t];
+    }
+  }
+  const double* Bv = B.t->data.data();
+  Combined cm;
+  AliasState S;
+  std::vector<i64> ck(PC);
+  std::vector<double> Bmat((size_t)K * Nn);
+  i64 iA = 0, nA = TA.nblocks();
+  while (iA < nA) {
+    i64 iA2 = advance_run(TA, iA, jA);
+    const i64* a0 = TA.keys.data() + iA * PA;
+    i64 sp_lin = 1;
+    for (int t = 0; t < nsp; ++t) sp_lin += (a0[jA[t] - 1] - 1) * spstr[t];
+    // Bmat(k, n) = Bp[sp..., k, n]
+    for (i64 n = 0; n < Nn; ++n)
+      for (i64 k = 0; k < K; ++k) Bmat[k + n * K] = Bv[(sp_lin - 1) + SP * (k + K * n)];
+    for (i64 ii = iA; ii < iA2; ++ii) {
+      const i64* ak = TA.keys.data() + ii * PA;
+      i64 tidA = TA.ids[ii];
+      i64 ct = cm.get({tidA, sp_lin});
+      if (ct == 0) {
+        C.ntemplates += 1;
+        ct = C.ntemplates;
+        cm.id[{tidA, sp_lin}] = ct;
+        i64 off = (i64)C.data.size();
+        C.data.resize(off + C.blksize, 0.0);
+        const double* Am = al_template(TA, tidA);  // (M, K)
+        if (mode == AthenB)
+          gemm_acc('N', 'N', M, Nn, K, Am, M, Bmat.data(), K, C.data.data() + off, M);
+        else
+          gemm_acc('T', 'T', Nn, M, K, Bmat.data(), K, Am, M, C.data.data() + off, Nn);
+      }
+      for (int j = 0; j < PC; ++j) ck[j] = ak[csrc[j] - 1];
+      al_contribute(S, C.data, ck, ct, TA.scalars[ii], C.blksize);
+    }
+    iA = iA2;
+  }
+  al_commit(C, S);
+}
+
+// contract_shared! Aliased x Aliased (contract_aliased_shared.jl:325-523)
+static void shared_al_al(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared,
+                         bool join_exact) {
+  for (auto& l : shared) OASSERT(!contains(labelsC, l), "shared label must be reduced (not in labelsC)");
+  al_clear(C);
+  int PA = A.t->P(), NA = A.t->N(), PB = B.t->P(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    OASSERT(has(A.map, l) && has(B.map, l), "shared label must exist in both A and B");
+    bool ap = at(A.map, l) <= PA, bp = at(B.map, l) <= PB;
+    if (ap && bp)
+      sp.push_back(l);
+    else if (!ap && !bp)
+      sd.push_back(l);
+    else
+      fail(SB_E_ERROR, "shared label crosses prefix/dense boundary; not supported");
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), Bd0 = sub(B.labels, PB, NB), red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : Bd0)
+    if (!contains(sd, l)) keepB0.push_back(l);
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; unsupported");
+  OASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA");
+  OASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB");
+  auto build_perm = [&](Oper& X, bool red_first, const Labels& keep) {
+    int P = X.t->P();
+    std::vector<i64> spax, perm;
+    for (auto& l : sp) spax.push_back(at(X.map, l));
+    for (int a = 1; a <= P; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) perm.push_back(a);
+    for (auto a : spax) perm.push_back(a);
+    std::vector<i64> ka, ra;
+    for (auto& l : keep) ka.push_back(at(X.map, l) - P);
+    for (auto& l : red) ra.push_back(at(X.map, l) - P);
+    OASSERT((int)(ka.size() + ra.size()) == X.t->N2, "perm covers dense axes");
+    for (auto a : (red_first ? ra : ka)) perm.push_back(P + a);
+    for (auto a : (red_first ? ka : ra)) perm.push_back(P + a);
+    return perm;
+  };
+  auto pA = build_perm(A, false, dA);
+  auto pB = build_perm(B, true, dB);
+  al_apply_perm(A, pA);
+  al_apply_perm(B, pB);
+  const Tensor &TA = *A.t, &TB = *B.t;
+  int nkA = (int)dA.size(), nkB = (int)dB.size(), nred = (int)red.size();
+  i64 M = nkA == 0 ? 1 : prod(TA.dims, PA, PA + nkA);
+  i64 K = nred == 0 ? 1 : prod(TA.dims, PA + nkA, NA);
+  i64 Nn = nkB == 0 ? 1 : prod(TB.dims, PB + nred, NB);
+  if (nred > 0) OASSERT(K == prod(TB.dims, PB, PB + nred), "Reduction extent mismatch between A and B dense tails");
+  OASSERT(C.blksize == M * Nn, "C.blksize must equal M*N");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l) && at(A.map, l) <= PA)
+      src[j] = at(A.map, l);
+    else if (has(B.map, l) && at(B.map, l) <= PB)
+      src[j] = -at(B.map, l);
+    else
+      fail(SB_E_ERROR, "C prefix label must come from A or B sparse prefix (excluding shared_prefix)");
+  }
+  std::vector<int> jA, jB;
+  for (auto& l : sp) {
+    jA.push_back(at(A.map, l));
+    jB.push_back(at(B.map, l));
+  }
+  auto cmp = [&](const i64* ak, const i64* bk) {  // _cmp_join_tuple (contract_bs_bs.jl:430-442)
+    int n = (int)jA.size();
+    for (int t = 0; t < n; ++t) {
+      int u = join_exact ? n - 1 - t : t;
+      i64 av = ak[jA[u] - 1], bv = bk[jB[u] - 1];
+      if (av < bv) return -1;
+      if (av > bv) return 1;
+    }
+    return 0;
+  };
+  Combined cm;
+  AliasState S;
+  std::vector<i64> ck(PC);
+  i64 iA = 0, nA = TA.nblocks(), iB = 0, nB = TB.nblocks();
+  while (iA < nA && iB < nB) {
+    int c = jA.empty() ? 0 : cmp(TA.keys.data() + iA * PA, TB.keys.data() + iB * PB);
+    if (c < 0) {
+      iA = advance_run(TA, iA, jA);
+      continue;
+    } else if (c > 0) {
+      iB = advance_run(TB, iB, jB);
+      continue;
+    }
+    i64 iA2 = advance_run(TA, iA, jA), iB2 = advance_run(TB, iB, jB);
+    for (i64 ii = iA; ii < iA2; ++ii) {
+      const i64* ak = TA.keys.data() + ii * PA;
+      i64 tidA = TA.ids[ii];
+      double aA = TA.scalars[ii];
+      const double* Am = al_template(TA, tidA);  // (M, K)
+      for (i64 jj = iB; jj < iB2; ++jj) {
+        const i64* bk = TB.keys.data() + jj * PB;
+        i64 tidB = TB.ids[jj];
+        double aC = aA * TB.scalars[jj];
+        i64 ct = cm.get({tidA, tidB});
+        if (ct == 0) {
+          C.ntemplates += 1;
+          ct = C.ntemplates;
+          cm.id[{tidA, tidB}] = ct;
+          i64 off = (i64)C.data.size();
+          C.data.resize(off + C.blksize, 0.0);
+          const double* Bm = al_template(TB, tidB);  // (K, N)
+          if (mode == AthenB)
+            gemm_acc('N', 'N', M, Nn, K, Am, M, Bm, K, C.data.data() + off, M);
+          else
+            gemm_acc('T', 'T', Nn, M, K, Bm, K, Am, M, C.data.data() + off, Nn);
+        }
+        for (int j = 0; j < PC; ++j) ck[j] = src[j] > 0 ? ak[src[j] - 1] : bk[-src[j] - 1];
+        al_contribute(S, C.data, ck, ct, aC, C.blksize);
+      }
+    }
+    iA = iA2;
+    iB = iB2;
+  }
+  al_commit(C, S);
+}
+
+// contract_aliased! COO x Dense -> Aliased (contract_aliased_coo_dense.jl:34-164)
+static void coo_dense_aliased(Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  int PC = C.P(), NC = C.N(), NA = A.t->N(), NB = B.t->N();
+  {  // aligned_A_to_Cprefix (contract_coo_dense.jl:4-18)
+    Labels desired = sub(labelsC, 0, PC);
+    desired.push_back(rlab);
+    LabelMap m = make_map(A.labels);
+    OASSERT((int)desired.size() == NA, "permutation length");
+    std::vector<i64> perm(NA);
+    for (int j = 0; j < NA; ++j) perm[j] = at(m, desired[j]);
+    A.own = coo_permutedims(*A.t, perm);
+    coo_sort(A.own);
+    A.set_own();
+    A.labels = desired;
+    A.map = make_map(desired);
+  }
+  Labels dnr;
+  for (int j = PC; j < NC; ++j)
+    if (labelsC[j] != rlab) dnr.push_back(labelsC[j]);
+  int rpos = -1;
+  for (int i = 0; i < NB; ++i)
+    if (B.labels[i] == rlab) {
+      rpos = i + 1;
+      break;
+    }
+  OASSERT(rpos > 0, "rlab not found in labelsB");
+  std::vector<i64> perm(NB, 0);
+  for (size_t k = 0; k < dnr.size(); ++k) {
+    int pos = -1;
+    for (int i = 0; i < NB; ++i)
+      if (B.labels[i] == dnr[k]) {
+        pos = i + 1;
+        break;
+      }
+    OASSERT(pos > 0, "label not found in labelsB");
+    if ((int)k >= NB) fail(SB_E_BOUNDS, "BoundsError: perm");
+    perm[k] = pos;
+  }
+  perm[NB - 1] = rpos;
+  if (!is_identity(perm)) {
+    for (auto p : perm)
+      if (p < 1) fail(SB_E_ARGUMENT, "invalid permutation of B");
+    B.own = dense_permutedims(*B.t, perm);
+    B.set_own();
+  }
+  i64 R = B.t->dims[NB - 1];
+  i64 bs = prod(B.t->dims, 0, NB - 1);
+  OASSERT(C.blksize == bs, "C.blksize must equal prod(B dims without r)");
+  const double* Bv = B.t->data.data();
+  al_clear(C);
+  std::map<i64, i64> rv_tid;
+  AliasState S;
+  const Tensor& TA = *A.t;
+  for (i64 idx = 0; idx < (i64)TA.data.size(); ++idx) {
+    const i64* ac = TA.keys.data() + idx * NA;
+    double al = TA.data[idx];
+    i64 rv = ac[NA - 1];
+    if (!(1 <= rv && rv <= R)) continue;
+    std::vector<i64> ck(ac, ac + PC);
+    const double* bsl = Bv + (rv - 1) * bs;
+    auto it = S.accum.find(ck);
+    if (it != S.accum.end()) {
+      for (i64 j = 0; j < bs; ++j) it->second[j] += al * bsl[j];
+      continue;
+    }
+    auto ia = S.alias.find(ck);
+    if (ia != S.alias.end()) {  // a second contribution always demotes
+      i64 tprev = ia->second.first;
+      double aprev = ia->second.second;
+      std::vector<double> acc(bs);
+      const double* t0 = C.data.data() + (tprev - 1) * bs;
+      for (i64 j = 0; j < bs; ++j) acc[j] = aprev * t0[j] + al * bsl[j];
+      S.alias.erase(ia);
+      S.accum.emplace(ck, std::move(acc));
+      continue;
+    }
+    i64 tid;
+    auto rt = rv_tid.find(rv);
+    if (rt == rv_tid.end()) {
+      C.ntemplates += 1;
+      tid = C.ntemplates;
+      rv_tid[rv] = tid;
+      C.data.insert(C.data.end(), bsl, bsl + bs);
+    } else {
+      tid = rt->second;
+    }
+    S.alias.emplace(ck, std::make_pair(tid, al));
+  }
+  al_commit(C, S);
+}
+
 // ---------------------------------------------------------------- dispatcher
 // contract!(C, labelsC, A, labelsA, B, labelsB) (contract.jl:37-104)
 static void contract(Tensor& C, const Labels& labelsC, const Tensor& A, const Labels& labelsA, const Tensor& B,
@@ -1769,6 +2663,12 @@ static void contract(Tensor& C, const Labels& labelsC, const Tensor& A, const La
       contract_coo_bs(C, labelsC, Ao, Bo, r);
     else if (C.kind == BS && A.kind == COO && B.kind == DENSE)
       contract_coo_dense(C, labelsC, Ao, Bo, r, (flags & SB_FLAG_SKIP_ZERO_SLICES) != 0);
+    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE)  // contract_aliased_shared.jl:535-547
+      contract_aliased_ad(C, labelsC, Ao, Bo, r);
+    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED)  // contract_aliased_shared.jl:550-562
+      contract_aliased_aa(C, labelsC, Ao, Bo, r);
+    else if (C.kind == ALIASED && A.kind == COO && B.kind == DENSE)  // tensor_wrappers_aliased.jl:263-275
+      coo_dense_aliased(C, labelsC, Ao, Bo, r);
     else
       fail(SB_E_METHOD, "MethodError: no single-label contract! method for these storage kinds");
   } else {
@@ -1776,6 +2676,10 @@ static void contract(Tensor& C, const Labels& labelsC, const Tensor& A, const La
       shared_bs_bs(C, labelsC, Ao, Bo, shared, (flags & SB_FLAG_EXACT_JOIN) != 0);
     else if (C.kind == BS && A.kind == BS && B.kind == DENSE)
       shared_bs_dense(C, labelsC, Ao, Bo, shared);
+    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE)
+      shared_al_dense(C, labelsC, Ao, Bo, shared);
+    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED)
+      shared_al_al(C, labelsC, Ao, Bo, shared, (flags & SB_FLAG_EXACT_JOIN) != 0);
     else
       fail(SB_E_METHOD, "MethodError: no contract_shared! method for these storage kinds");
   }
@@ -1975,6 +2879,49 @@ int orc_coo_from_dense(void* dense, double atol, double rtol, void** out) {
 int orc_coo_fuse_axes(void* h, int64_t ax1, int64_t ax2, void** out) {
   ORC_TRY(*out = new Tensor(coo_fuse_axes(*T(h), ax1, ax2)))
 }
+
+int orc_al_new(int N, int N2, const int64_t* dims, void** out) { ORC_TRY(*out = new Tensor(al_new(vec(dims, N), N2))) }
+int orc_al_from_parts(int N, int N2, const int64_t* dims, int64_t ntemplates, const double* templates, int64_t nblocks,
+                      const int64_t* keys, const int64_t* alias_ids, const double* scalars, void** out) {
+  ORC_TRY({
+    Tensor t = al_new(vec(dims, N), N2);
+    int P = t.P();
+    t.ntemplates = ntemplates;
+    t.data.assign(templates, templates + ntemplates * t.blksize);
+    t.keys.assign(keys, keys + nblocks * P);
+    t.ids.assign(alias_ids, alias_ids + nblocks);
+    t.scalars.assign(scalars, scalars + nblocks);
+    for (auto a : t.ids) OASSERT(1 <= a && a <= ntemplates, "alias id out of range");
+    *out = new Tensor(std::move(t));
+  })
+}
+int orc_al_info(void* h, int* N, int* N2, int64_t* dims, int64_t* nblocks, int64_t* blksize, int64_t* ntemplates) {
+  ORC_TRY({
+    const Tensor& t = *T(h);
+    *N = t.N();
+    *N2 = t.N2;
+    std::copy(t.dims.begin(), t.dims.end(), dims);
+    *nblocks = t.nblocks();
+    *blksize = t.blksize;
+    *ntemplates = t.ntemplates;
+  })
+}
+int orc_al_get(void* h, int64_t* keys, int64_t* alias_ids, double* scalars, double* templates) {
+  ORC_TRY({
+    const Tensor& t = *T(h);
+    std::copy(t.keys.begin(), t.keys.end(), keys);
+    std::copy(t.ids.begin(), t.ids.end(), alias_ids);
+    std::copy(t.scalars.begin(), t.scalars.end(), scalars);
+    std::copy(t.data.begin(), t.data.end(), templates);
+  })
+}
+int orc_al_getindex(void* h, const int64_t* I, double* v) { ORC_TRY(*v = al_getindex(*T(h), I)) }
+int orc_al_permutedims(void* h, const int64_t* perm, void** out) {
+  ORC_TRY(*out = new Tensor(al_permutedims(*T(h), vec(perm, T(h)->N()))))
+}
+int orc_al_to_blocksparse(void* h, void** out) { ORC_TRY(*out = new Tensor(al_to_blocksparse(*T(h)))) }
+int orc_al_to_dense(void* h, double init, void** out) { ORC_TRY(*out = new Tensor(al_to_dense(*T(h), init))) }
+int orc_al_compression_ratio(void* h, double* out) { ORC_TRY(*out = al_compression_ratio(*T(h))) }
 
 int orc_contract(void* C, const uint64_t* labelsC, int nC, void* A, const uint64_t* labelsA, int nA, void* B,
                  const uint64_t* labelsB, int nB, uint32_t flags) {
