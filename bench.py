@@ -16,7 +16,7 @@ value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
-                       [--prec fp64|fp32|bf16x3] [--no-cpu] [--calibrate-copy MB]
+                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--calibrate-copy MB]
 """
 from __future__ import annotations
 
@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="hubbard-chi2048", choices=["hubbard-chi2048", "heisenberg-chi256"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--tile", type=int, default=0,
+                    help="force tile shape 1+c of the block products (0: chosen at plan time)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -68,7 +70,7 @@ def main():
         sys.exit(2)
     import sparsebackends_amd as sb
 
-    flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec]
+    flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec] | (a.tile << 16)
     t0 = time.perf_counter()
     prob = make_problem(a.workload)
     t_gen = time.perf_counter() - t0

@@ -52,7 +52,8 @@
 
 /* tile shape of the BS x BS block products (bits 16..19): 0 chooses from the
  * operands' block boxes at plan time; 1 + c forces shape c of
- * 16x128, 128x16, 32x32, 64x32, 32x64, 64x64, 16x64, 64x16 (speed only) */
+ * workgroup tiles 16x128, 128x16, 32x32, 64x32, 32x64, 64x64, 16x64, 64x16 (c = 0..7) or
+ * wave tiles 16x16, 32x16, 16x32, 32x32, 64x16, 16x64 (c = 8..13); speed only */
 #define SB_TILE_SHIFT   16
 #define SB_TILE_MASK    (0xFu << SB_TILE_SHIFT)
 

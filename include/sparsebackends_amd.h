@@ -123,9 +123,30 @@ int sb_dense_info(sb_tensor_t t, int* N, int64_t* dims);
 int sb_dense_to_host(sb_ctx_t ctx, sb_tensor_t t, double* data);
 int sb_dense_device_ptr(sb_tensor_t t, double** data);
 
+/* ------------------------------------------------------------ AliasedBlockSparse */
+/* AliasedBlockSparse{T,N,N2}(dims) (aliased/storage.jl:122-132): no blocks, no templates */
+int sb_al_new(sb_ctx_t ctx, int N, int N2, const int64_t* dims, sb_tensor_t* out);
+/* the struct of aliased/storage.jl:46-59 from host arrays: templates (ntemplates*blksize),
+ * keys (nblocks x P, sorted by the column-major prefix lin), alias_ids (1-based), scalars */
+int sb_al_from_host(sb_ctx_t ctx, int N, int N2, const int64_t* dims, int64_t ntemplates, const double* templates,
+                    int64_t nblocks, const int64_t* keys, const int64_t* alias_ids, const double* scalars,
+                    sb_tensor_t* out);
+int sb_al_info(sb_tensor_t t, int* N, int* N2, int64_t* dims, int64_t* nblocks, int64_t* blksize,
+               int64_t* ntemplates);
+int sb_al_to_host(sb_ctx_t ctx, sb_tensor_t t, int64_t* keys, int64_t* alias_ids, double* scalars, double* templates);
+/* getindex (aliased/storage.jl:138-147) */
+int sb_al_getindex(sb_ctx_t ctx, sb_tensor_t t, const int64_t* I, double* v);
+/* compression_ratio (aliased/storage.jl:163-164) */
+int sb_al_compression_ratio(sb_tensor_t t, double* out);
+/* to_blocksparse (aliased/conversions.jl:18-32) */
+int sb_al_to_blocksparse(sb_ctx_t ctx, sb_tensor_t t, sb_tensor_t* out);
+/* to_dense(A; init) (aliased/conversions.jl:45-64) */
+int sb_al_to_dense(sb_ctx_t ctx, sb_tensor_t t, double init, sb_tensor_t* out);
+
 /* ------------------------------------------------------------ permutes */
 /* Base.permutedims: NewBlockSparseSorted (storage.jl:188-264), COOTensor
- * (coo/storage.jl:153-166, keys only, marks dirty), dense arrays; perm 1-based */
+ * (coo/storage.jl:153-166, keys only, marks dirty), AliasedBlockSparse
+ * (aliased/storage.jl:172-239, templates only), dense arrays; perm 1-based */
 int sb_permutedims(sb_ctx_t ctx, sb_tensor_t t, const int64_t* perm, sb_tensor_t* out);
 /* permutedims! (storage.jl:271-392; coo/storage.jl:168-180) */
 int sb_permutedims_inplace(sb_ctx_t ctx, sb_tensor_t t, const int64_t* perm);
@@ -135,7 +156,10 @@ int sb_permutedims_inplace(sb_ctx_t ctx, sb_tensor_t t, const int64_t* perm);
  * C was made by sb_bs_new / sb_coo_new with its dims and N2 and is overwritten.
  * A and B are not modified.  Operand kinds follow the reference's methods:
  * BS x BS, BS x Dense, COO x COO, COO x BS, COO x Dense (sparse operand first,
- * as wrapped_contract arranges, tensor_wrappers.jl:561-573).  flags: sb_status.h. */
+ * as wrapped_contract arranges, tensor_wrappers.jl:561-573), and into an
+ * AliasedBlockSparse C: Aliased x Dense, Aliased x Aliased (contract_aliased.jl,
+ * contract_aliased_shared.jl), COO x Dense (contract_aliased_coo_dense.jl).
+ * flags: sb_status.h. */
 int sb_contract(sb_ctx_t ctx, sb_tensor_t C, const uint64_t* labelsC, int nC, sb_tensor_t A, const uint64_t* labelsA,
                 int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags);
 /* contract_bs_bs_to_dense! (contract_bs_bs.jl:736-754): P_C = 0 sink into a dense C */

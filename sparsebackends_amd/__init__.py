@@ -6,12 +6,12 @@ Storage types and contract!/permutedims mirror the reference's Julia API
 libsparsebackends_amd.so (HIP), reached through its C ABI.
 """
 from ._abi import SBError, load, LIB_PATH
-from .storage import (NewBlockSparseSorted, COOTensor, DenseTensor, blocksparse_from_dense, coo_from_dense,
+from .storage import (NewBlockSparseSorted, COOTensor, DenseTensor, AliasedBlockSparse, blocksparse_from_dense, coo_from_dense,
                       to_dense, context, synchronize, profile, profile_reset, profile_read)
 from .tensoralgebra import (contract_, contract_bs_bs_to_dense_, Plan, labels_array, EXACT_JOIN, SKIP_ZERO_SLICES,
                             PREC_FP64, PREC_FP32, PREC_BF16X3)
 
-__all__ = ["SBError", "load", "LIB_PATH", "NewBlockSparseSorted", "COOTensor", "DenseTensor",
+__all__ = ["SBError", "load", "LIB_PATH", "NewBlockSparseSorted", "COOTensor", "DenseTensor", "AliasedBlockSparse",
            "blocksparse_from_dense", "coo_from_dense", "to_dense", "context", "synchronize", "profile", "profile_reset", "profile_read", "contract_",
            "contract_bs_bs_to_dense_", "Plan", "labels_array", "EXACT_JOIN", "SKIP_ZERO_SLICES", "PREC_FP64",
            "PREC_FP32", "PREC_BF16X3"]

@@ -42,6 +42,14 @@ SIGNATURES = {
     "sb_bs_to_host": [P, P, I64P, I64P, F64P],
     "sb_bs_device_ptrs": [P, PP, PP, PP],
     "sb_bs_set_payload": [P, P, P, C.c_int],
+    "sb_al_new": [P, C.c_int, C.c_int, I64P, PP],
+    "sb_al_from_host": [P, C.c_int, C.c_int, I64P, C.c_int64, F64P, C.c_int64, I64P, I64P, F64P, PP],
+    "sb_al_info": [P, I32P, I32P, I64P, I64P, I64P, I64P],
+    "sb_al_to_host": [P, P, I64P, I64P, F64P, F64P],
+    "sb_al_getindex": [P, P, I64P, F64P],
+    "sb_al_compression_ratio": [P, F64P],
+    "sb_al_to_blocksparse": [P, P, PP],
+    "sb_al_to_dense": [P, P, C.c_double, PP],
     "sb_bs_getindex": [P, P, I64P, F64P],
     "sb_bs_setindex": [P, P, C.c_double, I64P],
     "sb_bs_sort_lex": [P, P],

@@ -257,6 +257,82 @@ int sb_bs_fuse_two_axes(sb_ctx_t c, sb_tensor_t h, int64_t ax1, int64_t ax2, sb_
   SB_TRY(*out = H_(bs_fuse_two_axes(C_(c), *T_(h, BS), ax1, ax2)))
 }
 
+// ---------------------------------------------------------------- AliasedBlockSparse
+int sb_al_new(sb_ctx_t c, int N, int N2, const int64_t* dims, sb_tensor_t* out) {
+  SB_TRY(*out = H_(al_new(C_(c), vec(dims, N), N2)))
+}
+int sb_al_from_host(sb_ctx_t c, int N, int N2, const int64_t* dims, int64_t ntemplates, const double* templates,
+                    int64_t nblocks, const int64_t* keys, const int64_t* alias_ids, const double* scalars,
+                    sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    TP t = al_new(ctx, vec(dims, N), N2);
+    int P = N - N2;
+    if (ntemplates < 0 || nblocks < 0) fail(SB_E_ARGUMENT, "negative count");
+    for (i64 i = 0; i < nblocks; ++i)
+      if (alias_ids[i] < 1 || alias_ids[i] > ntemplates) fail(SB_E_BOUNDS, "alias id outside 1:ntemplates");
+    t->n = nblocks;
+    t->ntemplates = ntemplates;
+    up_i32(ctx, t->keys, keys, nblocks * P);
+    up_i32(ctx, t->ids, alias_ids, nblocks);
+    t->scalars.alloc(ctx, nblocks ? nblocks : 1);
+    t->data.alloc(ctx, ntemplates * t->blksize);
+    if (nblocks)
+      HIP_CHECK(hipMemcpyAsync(t->scalars.get(), scalars, nblocks * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+    if (ntemplates * t->blksize)
+      HIP_CHECK(hipMemcpyAsync(t->data.get(), templates, ntemplates * t->blksize * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = H_(std::move(t));
+  })
+}
+int sb_al_info(sb_tensor_t h, int* N, int* N2, int64_t* dims, int64_t* nblocks, int64_t* blksize,
+               int64_t* ntemplates) {
+  SB_TRY({
+    Tensor* t = T_(h, ALIASED);
+    *N = t->N();
+    *N2 = t->N2;
+    std::copy(t->dims.begin(), t->dims.end(), dims);
+    *nblocks = t->n;
+    *blksize = t->blksize;
+    *ntemplates = t->ntemplates;
+  })
+}
+int sb_al_to_host(sb_ctx_t c, sb_tensor_t h, int64_t* keys, int64_t* alias_ids, double* scalars, double* templates) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    Tensor* t = T_(h, ALIASED);
+    if (keys) down_i32(ctx, keys, t->keys.get(), t->n * t->P());
+    if (alias_ids) down_i32(ctx, alias_ids, t->ids.get(), t->n);
+    if (scalars && t->n)
+      HIP_CHECK(hipMemcpyAsync(scalars, t->scalars.get(), t->n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (templates && t->ntemplates * t->blksize)
+      HIP_CHECK(hipMemcpyAsync(templates, t->data.get(), t->ntemplates * t->blksize * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  })
+}
+int sb_al_getindex(sb_ctx_t c, sb_tensor_t h, const int64_t* I, double* v) {
+  SB_TRY(*v = al_getindex(C_(c), *T_(h, ALIASED), I))
+}
+int sb_al_compression_ratio(sb_tensor_t h, double* out) {
+  SB_TRY({
+    Tensor* t = T_(h, ALIASED);
+    *out = t->ntemplates == 0 ? 1.0 : (double)t->n / (double)t->ntemplates;
+  })
+}
+int sb_al_to_blocksparse(sb_ctx_t c, sb_tensor_t h, sb_tensor_t* out) {
+  SB_TRY(*out = H_(al_to_blocksparse(C_(c), *T_(h, ALIASED))))
+}
+int sb_al_to_dense(sb_ctx_t c, sb_tensor_t h, double init, sb_tensor_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    TP b = al_to_blocksparse(ctx, *T_(h, ALIASED));
+    *out = H_(bs_to_dense(ctx, *b, init, {}));
+  })
+}
+
 // ---------------------------------------------------------------- COO
 int sb_coo_new(sb_ctx_t c, int N, const int64_t* dims, sb_tensor_t* out) { SB_TRY(*out = H_(coo_new(C_(c), vec(dims, N)))) }
 int sb_coo_from_host(sb_ctx_t c, int N, const int64_t* dims, int64_t nnz, const int64_t* keys, const double* vals,
@@ -378,6 +454,8 @@ int sb_permutedims(sb_ctx_t c, sb_tensor_t h, const int64_t* perm, sb_tensor_t* 
       *out = H_(bs_permute(ctx, *t, p));
     else if (t->kind == COO)
       *out = H_(coo_permute(ctx, *t, p));
+    else if (t->kind == ALIASED)
+      *out = H_(al_permute(ctx, *t, p));
     else if (t->kind == DENSE) {
       std::vector<i64> s(p);
       std::sort(s.begin(), s.end());

@@ -189,11 +189,14 @@ __global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ gor
   tiles[i] = (Mc && Nc) ? ((Mc + BM - 1) / BM) * ((Nc + BN - 1) / BN) : 0;
 }
 
-// work list: tile t of position i -> tpos[t] = i (positions in locality order)
-__global__ void tile_groups_kernel(const i32* __restrict__ toff, i64 ng, i32* __restrict__ tpos) {
+// work list from the device-wide scan of the per-position tile counts (toff[0] = 0,
+// toff[ng] = total): trec[t] = {group, tile index within the group}, positions in order
+__global__ void tile_records_kernel(const i32* __restrict__ toff, i64 ng, const i32* __restrict__ gorder,
+                                    int2* __restrict__ trec) {
   i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ng) return;
-  for (i32 t = toff[i]; t < toff[i + 1]; ++t) tpos[t] = (i32)i;
+  const i32 g = gorder ? gorder[i] : (i32)i;
+  for (i32 t = toff[i]; t < toff[i + 1]; ++t) trec[t] = make_int2(g, t - toff[i]);
 }
 
 size_t box_scan_tmp_bytes(Ctx* ctx, i64 ng) {
@@ -212,12 +215,12 @@ void box_prepare(Ctx* ctx, const BoxGemmArgs& g, const i32* abox, const i32* afl
                      g.arow, g.brow, abox, aflag, bbox, bflag, g.gbox, g.kbox, g.kc, cbox, cflag, g.cnew, g.clear,
                      g.tiles, t.BM, t.BN);
   HIP_CHECK(hipGetLastError());
-  if (g.tile_group) {  // the work list (absent while the tile shape is being chosen); toff[0] == 0 from planning
+  if (g.trec) {  // the work list (absent while the tile shape is being chosen); toff[0] == 0 from planning
     size_t tb = g.scan_tmp_bytes;
     HIP_CHECK(rocprim::inclusive_scan(g.scan_tmp, tb, g.tiles, g.toff + 1, (size_t)g.ng, rocprim::plus<i32>(),
                                       ctx->stream));
-    hipLaunchKernelGGL(tile_groups_kernel, dim3(nb256(g.ng)), dim3(256), 0, ctx->stream, g.toff, g.ng,
-                       g.tile_group);
+    hipLaunchKernelGGL(tile_records_kernel, dim3(nb256(g.ng)), dim3(256), 0, ctx->stream, g.toff, g.ng, g.gorder,
+                       g.trec);
     HIP_CHECK(hipGetLastError());
   }
 }
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
     BoxDesc d, i64 ng, const i32* __restrict__ gorder, const i32* __restrict__ seg, const i64* __restrict__ aoff,
     const i64* __restrict__ boff, const i64* __restrict__ coff, const i32* __restrict__ gbox,
     const i32* __restrict__ kbox, const i32* __restrict__ kc, const i32* __restrict__ toff,
-    const i32* __restrict__ tpos, const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
+    const int2* __restrict__ trec, const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
     i32* __restrict__ cflag, const i32* __restrict__ clear, const i32* __restrict__ cnew, i32* __restrict__ cbox,
     int amf, int bnf) {
   constexpr int BK = 16, KT = 512;
@@ -282,6 +285,9 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
   __shared__ T Bs[BK][BN + 2];
   __shared__ int rowA[BM], rowC[BM], colB[BN], colC[BN];
   __shared__ i64 tabA[KT], tabB[KT];
+  __shared__ int pkc[257];          // per pair of the batch: K count, then its exclusive scan
+  __shared__ i64 pao[256], pbo[256];  // per pair: A / B block offsets
+  __shared__ int pkb[MAXAX][256];   // per pair: K box
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, wn = wave / WM;
   const int nmn = d.nm + d.nn;
@@ -324,9 +330,8 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
   const i32 chunk = (total + NXCD - 1) / NXCD;
   const i32 wlo = xcd * chunk, whi = min(total, wlo + chunk);
   for (i32 w = wlo + slot; w < whi; w += nper) {
-    const i32 pos = tpos[w];
-    const i32 g = gorder ? gorder[pos] : pos;
-    const i32 lt = w - toff[pos];
+    const int2 rec = trec[w];
+    const i32 g = rec.x, lt = rec.y;
     int mb[MAXAX], nbx[MAXAX];
     int Mc = 1, Nc = 1;
 #pragma unroll
@@ -361,35 +366,50 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
         accd[i][j] = d4{0, 0, 0, 0};
         accf[i][j] = f4{0, 0, 0, 0};
       }
+    // pairs in batches of up to 256: their records load in parallel, a prefix
+    // scan of their K counts places every K position of the batch
     const i32 p1 = seg[g + 1];
-    i32 pp = seg[g];
-    int kin = 0;
-    while (pp < p1) {
-      __syncthreads();
-      int fill = 0;
-      while (pp < p1 && fill < KT) {
-        const int kcp = kc[pp];
-        const int take = min(kcp - kin, KT - fill);
-        if (take > 0) {
-          int kb[MAXAX];
+    for (i32 pb0 = seg[g]; pb0 < p1; pb0 += 256) {
+      const int nbp = min(256, p1 - pb0);
+      __syncthreads();  // the previous batch is done with the pair tables
+      if (tid < nbp) {
+        const i32 pp = pb0 + tid;
+        pkc[tid] = kc[pp];
+        pao[tid] = aoff[pp];
+        pbo[tid] = boff[pp];
 #pragma unroll
-          for (int a = 0; a < MAXAX; ++a) kb[a] = a < d.nk ? kbox[(i64)pp * d.nk + a] : 1;
-          const i64 ao = aoff[pp], bo = boff[pp];
-          for (int t = tid; t < take; t += 256) {
-            int oa, ob;
-            decode2(kin + t, d.nk, kb, d.k_sa, d.k_sb, oa, ob);
-            tabA[fill + t] = ao + oa;
-            tabB[fill + t] = bo + ob;
-          }
-        }
-        fill += take;
-        kin += take;
-        if (kin >= kcp) {
-          ++pp;
-          kin = 0;
-        }
+        for (int a = 0; a < MAXAX; ++a)
+          if (a < d.nk) pkb[a][tid] = kbox[(i64)pp * d.nk + a];
       }
       __syncthreads();
+      if (tid == 0) {  // exclusive scan of the K counts (a handful of pairs per group)
+        int acc = 0;
+        for (int j = 0; j < nbp; ++j) {
+          const int c = pkc[j];
+          pkc[j] = acc;
+          acc += c;
+        }
+        pkc[nbp] = acc;
+      }
+      __syncthreads();
+      const int ktot = pkc[nbp];
+      for (int kb0 = 0; kb0 < ktot; kb0 += KT) {
+        const int fill = min(KT, ktot - kb0);
+        if (kb0) __syncthreads();  // the previous window's products are done with the K table
+        for (int j = 0; j < nbp; ++j) {  // pair j covers K positions [pkc[j], pkc[j+1])
+          const int lo = max(pkc[j], kb0), hi = min(pkc[j + 1], kb0 + fill);
+          if (lo >= hi) continue;
+          int kbv[MAXAX];
+#pragma unroll
+          for (int a = 0; a < MAXAX; ++a) kbv[a] = a < d.nk ? pkb[a][j] : 1;
+          for (int kp = lo + tid; kp < hi; kp += 256) {
+            int oa, ob;
+            decode2(kp - pkc[j], d.nk, kbv, d.k_sa, d.k_sb, oa, ob);
+            tabA[kp - kb0] = pao[j] + oa;
+            tabB[kp - kb0] = pbo[j] + ob;
+          }
+        }
+        __syncthreads();
       double ra[AE], rb[BE];
       auto load = [&](int k0) {
 #pragma unroll
@@ -491,6 +511,7 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
             }
         }
       }
+      }
     }
     double* Cb = C + coff[g];
     int bad = 0;
@@ -515,8 +536,212 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
   }
 }
 
+// Wave tiles: every wave (64 lanes) owns one (group, tile) of FM x FN
+// blocks of 16 x 16 and works alone, with no LDS and no barriers: each MFMA
+// step's operands are loaded straight into the lanes that hold them (lane l
+// takes A[m = l % 16][k] and B[k][n = l % 16] with k = l / 16 for the
+// 16x16x4 forms, k = 4 (l / 16) + v for bf16), K boxes padded per pair to the
+// MFMA depth.  Two K steps are loaded before either multiplies.  Suits the
+// small blocks of symmetry sectors: many independent waves hide the latency.
+template <int FM, int FN, int PREC>
+__global__ __launch_bounds__(256) void box_wave_kernel(
+    BoxDesc d, i64 ng, const i32* __restrict__ seg, const i64* __restrict__ aoff, const i64* __restrict__ boff,
+    const i64* __restrict__ coff, const i32* __restrict__ gbox, const i32* __restrict__ kbox,
+    const i32* __restrict__ kc, const i32* __restrict__ toff, const int2* __restrict__ trec,
+    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C, i32* __restrict__ cflag,
+    const i32* __restrict__ clear, const i32* __restrict__ cnew, i32* __restrict__ cbox) {
+  constexpr int TM = 16 * FM, TN = 16 * FN;
+  constexpr int KS = PREC == 2 ? 16 : 4;  // K depth of one MFMA
+  constexpr int KV = PREC == 2 ? 4 : 1;   // K values per lane per MFMA
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: per-wave work goes through scalar loads
+  const int lr = lane & 15, lq = lane >> 4;
+  const int nmn = d.nm + d.nn;
+
+  // ---- stale entries of shrunken C boxes (whole workgroup)
+  for (i64 g = blockIdx.x; g < ng; g += gridDim.x) {
+    if (!clear[g]) continue;
+    int ob[MAXAX], nbx[MAXAX];
+    i64 vol = 1;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a)
+      if (a < d.NC2) {
+        ob[a] = cbox[g * d.NC2 + a];
+        nbx[a] = cnew[g * d.NC2 + a];
+        vol *= ob[a];
+      }
+    double* base = C + coff[g];
+    for (i64 e = tid; e < vol; e += 256) {
+      i64 x = e, off = 0, st = 1;
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < MAXAX; ++a)
+        if (a < d.NC2) {
+          i64 q = x / ob[a];
+          int r = (int)(x - q * ob[a]);
+          inside = inside && r < nbx[a];
+          off += r * st;
+          st *= d.c_ext[a];
+          x = q;
+        }
+      if (!inside) base[off] = 0.0;
+    }
+    __syncthreads();  // every thread has read the old box before it is replaced
+    if (tid < d.NC2) cbox[g * d.NC2 + tid] = nbx[tid];
+  }
+
+  // ---- tiles, one per wave
+  const i32 total = toff[ng];
+  const int xcd = blockIdx.x % NXCD;
+  const int slot = (blockIdx.x / NXCD) * 4 + wave, nper = (gridDim.x / NXCD) * 4;
+  const i32 chunk = (total + NXCD - 1) / NXCD;
+  const i32 wlo = xcd * chunk, whi = min(total, wlo + chunk);
+  for (i32 w = wlo + slot; w < whi; w += nper) {
+    const int2 rec = trec[w];
+    const i32 g = rec.x, lt = rec.y;
+    int mb[MAXAX], nbx[MAXAX];
+    int Mc = 1, Nc = 1;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a) {
+      mb[a] = a < d.nm ? gbox[(i64)g * nmn + a] : 1;
+      nbx[a] = a < d.nn ? gbox[(i64)g * nmn + d.nm + a] : 1;
+      Mc *= mb[a];
+      Nc *= nbx[a];
+    }
+    const int tiles_m = (Mc + TM - 1) / TM;
+    const int m0 = (lt % tiles_m) * TM, n0 = (lt / tiles_m) * TN;
+    int ra[FM], cb[FN], cc[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      int oa = -1, oc = 0;
+      if (m0 + 16 * i + lr < Mc) decode2(m0 + 16 * i + lr, d.nm, mb, d.m_sa, d.m_sc, oa, oc);
+      ra[i] = oa;
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      int ob = -1, oc = -1;
+      if (n0 + 16 * j + lr < Nc) decode2(n0 + 16 * j + lr, d.nn, nbx, d.n_sb, d.n_sc, ob, oc);
+      cb[j] = ob;
+      cc[j] = oc;
+    }
+    d4 accd[FM][FN];
+    f4 accf[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        accd[i][j] = d4{0, 0, 0, 0};
+        accf[i][j] = f4{0, 0, 0, 0};
+      }
+    const i32 p1 = seg[g + 1];
+    for (i32 pp = seg[g]; pp < p1; ++pp) {
+      const int kcp = kc[pp];
+      if (kcp == 0) continue;
+      int kb[MAXAX];
+#pragma unroll
+      for (int a = 0; a < MAXAX; ++a) kb[a] = a < d.nk ? kbox[(i64)pp * d.nk + a] : 1;
+      const double* Ab = A + aoff[pp];
+      const double* Bb = B + boff[pp];
+      const int nsteps = (kcp + KS - 1) / KS;
+      for (int s0 = 0; s0 < nsteps; s0 += 2) {
+        double av[2][FM][KV], bv[2][FN][KV];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < KV; ++v) {
+            const int k = (s0 + u) * KS + (PREC == 2 ? 4 * lq + v : lq);
+            int ka = 0, kbo = 0;
+            const bool ok = k < kcp;
+            if (ok) decode2(k, d.nk, kb, d.k_sa, d.k_sb, ka, kbo);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) av[u][i][v] = (ok && ra[i] >= 0) ? Ab[ka + ra[i]] : 0.0;
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bv[u][j][v] = (ok && cb[j] >= 0) ? Bb[kbo + cb[j]] : 0.0;
+          }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          if (s0 + u >= nsteps) break;
+          if constexpr (PREC == 0) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j)
+                accd[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][i][0], bv[u][j][0], accd[i][j], 0, 0, 0);
+          } else if constexpr (PREC == 1) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j)
+                accf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[u][i][0], (float)bv[u][j][0], accf[i][j],
+                                                                  0, 0, 0);
+          } else {
+            s4 ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                unsigned short h, m, l;
+                split3_b((float)av[u][i][v], h, m, l);
+                ah[i][v] = (short)h;
+                am[i][v] = (short)m;
+                al[i][v] = (short)l;
+              }
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                unsigned short h, m, l;
+                split3_b((float)bv[u][j][v], h, m, l);
+                bh[j][v] = (short)h;
+                bm[j][v] = (short)m;
+                bl[j][v] = (short)l;
+              }
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j) {
+                f4 c = accf[i][j];
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bm[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bm[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bh[j], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], c, 0, 0, 0);
+                accf[i][j] = c;
+              }
+          }
+        }
+      }
+    }
+    double* Cb = C + coff[g];
+    int bad = 0;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // output row of register v: f64 16x16x4 holds rows 4*v + lane/16;
+        // the f32 / bf16 16x16 forms hold rows 4*(lane/16) + v
+        const int m = m0 + 16 * i + (PREC == 0 ? 4 * v + lq : 4 * lq + v);
+        if (m >= Mc) continue;
+        int oa, rc;
+        decode2(m, d.nm, mb, d.m_sa, d.m_sc, oa, rc);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          if (cc[j] >= 0) {
+            const double val = PREC == 0 ? accd[i][j][v] : (double)accf[i][j][v];
+            Cb[rc + cc[j]] = val;
+            bad |= !isfinite(val);
+          }
+      }
+    if (bad) atomicOr(&cflag[g], 1);
+  }
+}
+
 static const BoxTile kTiles[] = {
+    // workgroup tiles (4 waves share LDS operand tiles)
     {16, 128}, {128, 16}, {32, 32}, {64, 32}, {32, 64}, {64, 64}, {16, 64}, {64, 16},
+    // wave tiles (one wave per tile)
+    {16, 16}, {32, 16}, {16, 32}, {32, 32}, {64, 16}, {16, 64},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 BoxTile box_tile(int cfg) {
@@ -524,21 +749,37 @@ BoxTile box_tile(int cfg) {
   return kTiles[cfg];
 }
 int box_tile_configs() { return kNumTiles; }
+bool box_tile_is_wave(int cfg) { return cfg >= 8; }
 
-template <int BM, int BN, int WM, int WN, int PREC>
-static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
-  auto kern = box_gemm_kernel<BM, BN, WM, WN, PREC>;
-  static int per_cu = 0;  // resident workgroups per CU (same for every device of this build)
-  if (!per_cu) {
+template <class K>
+static i64 grid_for(Ctx* ctx, K kern, int& per_cu) {
+  if (!per_cu) {  // resident workgroups per CU (the same on every device of this build)
     int nb = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0));
     per_cu = std::max(1, nb);
   }
   i64 nwg = (i64)ctx->num_cu * per_cu;
-  nwg = std::max<i64>(NXCD, nwg / NXCD * NXCD);  // a multiple of the XCD count
+  return std::max<i64>(NXCD, nwg / NXCD * NXCD);  // a multiple of the XCD count
+}
+
+template <int BM, int BN, int WM, int WN, int PREC>
+static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
+  auto kern = box_gemm_kernel<BM, BN, WM, WN, PREC>;
+  static int per_cu = 0;
+  i64 nwg = grid_for(ctx, kern, per_cu);
   hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.gorder, g.seg, g.aoff,
-                     g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.tile_group, A, B, C, cflag, g.clear, g.cnew,
-                     g.cbox, g.amf, g.bnf);
+                     g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox,
+                     g.amf, g.bnf);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <int FM, int FN, int PREC>
+static void launch_wave(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
+  auto kern = box_wave_kernel<FM, FN, PREC>;
+  static int per_cu = 0;
+  i64 nwg = grid_for(ctx, kern, per_cu);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.seg, g.aoff, g.boff,
+                     g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -553,6 +794,12 @@ static void launch_box(Ctx* ctx, const BoxGemmArgs& g, const double* A, const do
     case 5: launch_one<64, 64, 2, 2, PREC>(ctx, g, A, B, C, cflag); break;
     case 6: launch_one<16, 64, 1, 4, PREC>(ctx, g, A, B, C, cflag); break;
     case 7: launch_one<64, 16, 4, 1, PREC>(ctx, g, A, B, C, cflag); break;
+    case 8: launch_wave<1, 1, PREC>(ctx, g, A, B, C, cflag); break;
+    case 9: launch_wave<2, 1, PREC>(ctx, g, A, B, C, cflag); break;
+    case 10: launch_wave<1, 2, PREC>(ctx, g, A, B, C, cflag); break;
+    case 11: launch_wave<2, 2, PREC>(ctx, g, A, B, C, cflag); break;
+    case 12: launch_wave<4, 1, PREC>(ctx, g, A, B, C, cflag); break;
+    case 13: launch_wave<1, 4, PREC>(ctx, g, A, B, C, cflag); break;
     default: fail(SB_E_ASSERT, "box tile configuration");
   }
 }

@@ -582,13 +582,14 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
     double best = 1e300;
     for (int c = 0; c < box_tile_configs(); ++c) {
+      if (box_tile_is_wave(c)) continue;  // wave tiles only on request
       BoxTile t = box_tile(c);
       double cost = 0;
       for (i64 q = 0; q < ng; ++q) {
         i64 mc = 1, nc = 1;
         for (int a = 0; a < d.nm; ++a) mc *= hb[q * (d.nm + d.nn) + a];
         for (int a = 0; a < d.nn; ++a) nc *= hb[q * (d.nm + d.nn) + d.nm + a];
-        if (mc && nc) cost += (double)((mc + t.BM - 1) / t.BM) * ((nc + t.BN - 1) / t.BN) * (t.BM * t.BN + 1024);
+        if (mc && nc) cost += (double)((mc + t.BM - 1) / t.BM) * ((nc + t.BN - 1) / t.BN) * (t.BM * t.BN + 512);
       }
       if (cost < best) {
         best = cost;
@@ -599,7 +600,7 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
   BoxTile t = box_tile(g.cfg);
   g.max_tiles = ng * ((M + t.BM - 1) / t.BM) * ((N + t.BN - 1) / t.BN);
   SB_ASSERT(g.max_tiles < (1ll << 31), "box_gemm: work list too large");
-  g.tile_group = pl.keep(DBuf<i32>(ctx, (size_t)std::max<i64>(1, g.max_tiles)));
+  g.trec = pl.keep(DBuf<int2>(ctx, (size_t)std::max<i64>(1, g.max_tiles)));
   pl.steps.push_back({"box_scan", [ctx, pA, pB]() {
                         ensure_box(ctx, *pA);
                         ensure_box(ctx, *pB);
@@ -1561,6 +1562,838 @@ static void contract_coo_dense(Plan& pl, Tensor& C, const Labels& labelsC, CooOp
                       }});
 }
 
+// ================================================================ AliasedBlockSparse
+// permutedims(::AliasedBlockSparse) (aliased/storage.jl:172-239) split into
+// its symbolic part (keys remapped and stably sorted, alias ids gathered; run
+// now) and its numeric part (templates transposed, scalars gathered; plan steps)
+static const Tensor* plan_al_permute(Plan& pl, const Tensor& A, const std::vector<i64>& perm) {
+  Ctx* ctx = pl.ctx;
+  int N = A.N(), P = A.P(), N2 = A.N2;
+  check_no_cross_perm(perm, N, P);
+  std::vector<i64> dimsB(N);
+  for (int j = 0; j < N; ++j) dimsB[j] = A.dims[perm[j] - 1];
+  TP B = al_new(ctx, dimsB, N2);
+  SB_ASSERT(B->blksize == A.blksize, "permutation within tail preserves block size");
+  i64 n = A.n, nt = A.ntemplates;
+  B->n = n;
+  B->ntemplates = nt;
+  if (n > 0) {
+    B->keys.alloc(ctx, (size_t)n * (P ? P : 1));
+    B->ids.alloc(ctx, n);
+    B->scalars.alloc(ctx, n);
+    DBuf<i32> idx(ctx, n);
+    if (P > 0) {
+      std::vector<int> sel(P);
+      for (int j = 0; j < P; ++j) sel[j] = (int)perm[j];
+      DBuf<i32> nk(ctx, (size_t)n * P);
+      DBuf<u64> lin(ctx, n);
+      k_remap_keys(ctx, A.keys.get(), n, P, sel.data(), P, dimsB.data(), nk.get(), lin.get(), idx.get());
+      sort_pairs(ctx, lin.get(), idx.get(), n, bits_for((u64)prod(dimsB, 0, P) - 1));
+      k_gather_rows(ctx, nk.get(), P, idx.get(), n, B->keys.get());
+    } else {
+      k_iota(ctx, idx.get(), n, 0);
+    }
+    k_gather_rows(ctx, A.ids.get(), 1, idx.get(), n, B->ids.get());
+    const double* sa = A.scalars.get();
+    double* sb = B->scalars.get();
+    i32* ip = pl.keep(std::move(idx));
+    pl.steps.push_back({"al_scalars_gather", [=]() { k_take_f64(ctx, sa, ip, n, sb); }});
+  }
+  if (nt > 0) {
+    B->data.alloc(ctx, (size_t)nt * B->blksize);
+    std::vector<i64> tdims(A.dims.begin() + P, A.dims.end());
+    std::vector<int> pblk(N2);
+    for (int i = 0; i < N2; ++i) pblk[i] = (int)(perm[P + i] - P - 1);
+    const double* in = A.data.get();
+    double* out = B->data.get();
+    pl.steps.push_back({"permute_blocks", [ctx, in, nt, N2, tdims, pblk, out]() {
+                          permute_blocks(ctx, in, nullptr, nt, N2, tdims.data(), pblk.data(), out);
+                        }});
+    pl.perm_bytes += 2.0 * nt * B->blksize * sizeof(double);
+  }
+  const Tensor* r = B.get();
+  pl.keep_tensor(std::move(B));
+  return r;
+}
+TP al_permute(Ctx* ctx, const Tensor& A, const std::vector<i64>& perm) {
+  Plan pl;
+  pl.ctx = ctx;
+  const Tensor* t = plan_al_permute(pl, A, perm);
+  pl.run_pending();
+  for (auto& h : pl.hold)  // hand the permuted tensor over; the plan keeps only scratch
+    if (h.get() == (const void*)t) {
+      TP out(new Tensor(std::move(*const_cast<Tensor*>(t))));
+      return out;
+    }
+  fail(SB_E_ASSERT, "aliased permute lost its result");
+}
+static void al_oper_perm(Plan& pl, Oper& X, const std::vector<i64>& perm) {
+  X.t = plan_al_permute(pl, *X.t, perm);
+  X.labels = permute_labels(X.labels, perm);
+  X.map = make_map(X.labels);
+}
+// _aliased_r_to_last_prefix / _aliased_r_to_last_dense (contract_aliased.jl:33-65)
+static void al_r_to_last_prefix(Plan& pl, Oper& X, const Label& rlab) {
+  int axisr = at(X.map, rlab), P = X.t->P(), N = X.t->N();
+  SB_ASSERT(axisr <= P, "rlab must be in sparse prefix");
+  if (axisr == P) return;
+  std::vector<i64> perm;
+  for (int a = 1; a <= P; ++a)
+    if (a != axisr) perm.push_back(a);
+  perm.push_back(axisr);
+  for (int a = P + 1; a <= N; ++a) perm.push_back(a);
+  al_oper_perm(pl, X, perm);
+}
+static void al_r_to_last_dense(Plan& pl, Oper& X, const Label& rlab) {
+  int axisr = at(X.map, rlab), P = X.t->P(), N = X.t->N(), N2 = X.t->N2;
+  SB_ASSERT(axisr > P, "rlab must be in dense tail");
+  if (axisr == N) return;
+  std::vector<i64> perm;
+  for (int a = 1; a <= P; ++a) perm.push_back(a);
+  for (int d = 1; d <= N2; ++d)
+    if (d != axisr - P) perm.push_back(P + d);
+  perm.push_back(axisr);
+  al_oper_perm(pl, X, perm);
+}
+// dense operand with rlab moved last (contract_aliased.jl:168-177)
+static void dense_r_last(Plan& pl, Oper& B, const Label& rlab) {
+  int NB = B.t->N(), ax = at(B.map, rlab);
+  if (ax == NB) return;
+  std::vector<i64> perm;
+  for (int d = 1; d <= NB; ++d)
+    if (d != ax) perm.push_back(d);
+  perm.push_back(ax);
+  B.t = plan_dense_permute(pl, *B.t, perm);
+  B.labels = permute_labels(B.labels, perm);
+  B.map = make_map(B.labels);
+}
+
+// The contributions of an aliased kernel in the reference's traversal order
+// and what the output needs from them.
+struct AlContrib {
+  i64 np = 0;
+  DBuf<i32> pa, pb;       // operand rows (pb empty when B is dense)
+  DBuf<i32> ck;           // np x PC output keys
+  DBuf<i32> xy;           // np x 2 combination keys (1-based), over (nx, ny)
+  i64 nx = 1, ny = 1;
+  const double* sA = nullptr;  // per A row scalar (COO: entry value)
+  const double* sB = nullptr;  // per B row scalar, or null
+};
+// computes template t (0-based, first-seen order) from combination (ux[t], uy[t])
+using TemplateFn = std::function<void(Ctx*, i64 nu, const i32* ux_uy, double* out)>;
+
+// Output of an aliased kernel from its contributions (contract_aliased.jl:75-146):
+// combined templates deduplicated in first-seen order (their values by tfn),
+// output keys grouped and sorted, each key aliased (one combined template,
+// scalars summed in order) or accumulated into a template of its own (scalar
+// 1), accumulated templates appended in key order.  COO mode
+// (contract_aliased_coo_dense.jl:96-145): templates are the B slices of the
+// keys' first entries (first-seen order of rv), any second entry demotes;
+// coo_B: the B slices (coo_R of them), coo_keys / coo_w: the aligned COO keys.
+static void al_finish(Plan& pl, Tensor& C, AlContrib& c, const TemplateFn& tfn, const double* coo_B, i64 coo_R,
+                      const i32* coo_keys, int coo_w) {
+  Ctx* ctx = pl.ctx;
+  int PC = C.P();
+  i64 bs = C.blksize, np = c.np;
+  bool coo = coo_B != nullptr;
+  C.n = 0;
+  C.ntemplates = 0;
+  C.dirty = false;
+  if (np == 0) return;
+  // combined templates
+  i64 ncomb = 0;
+  DBuf<i32> ctid, byid;
+  if (!coo) {
+    DBuf<u64> lin(ctx, np);
+    k_al_xy_lin(ctx, c.xy.get(), np, c.nx, lin.get());
+    Alloc U;
+    DBuf<i32> xy2(ctx, (size_t)np * 2);
+    HIP_CHECK(hipMemcpyAsync(xy2.get(), c.xy.get(), np * 2 * sizeof(i32), hipMemcpyDeviceToDevice, ctx->stream));
+    allocate_blocks(ctx, np, 2, xy2, lin, (u64)(c.nx * c.ny - 1), U);
+    ncomb = U.nC;
+    ctid.alloc(ctx, np);
+    byid.alloc(ctx, (size_t)ncomb * 2);
+    k_al_seg_ids(ctx, U.nC, U.seg_start.get(), U.order.get(), U.ids.get(), U.keys.get(), 2, ctid.get(), byid.get());
+  }
+  // output keys
+  Alloc G;
+  DBuf<u64> clin(ctx, np);
+  if (PC > 0) {
+    DBuf<i32> ck2(ctx, (size_t)np * PC);
+    std::vector<int> sel(PC);
+    std::iota(sel.begin(), sel.end(), 1);
+    DBuf<i32> dummy(ctx, np);
+    k_remap_keys(ctx, c.ck.get(), np, PC, sel.data(), PC, C.dims.data(), ck2.get(), clin.get(), dummy.get());
+    allocate_blocks(ctx, np, PC, ck2, clin, (u64)prod(C.dims, 0, PC) - 1, G);
+  } else {
+    k_zero(ctx, clin.get(), np * sizeof(u64));
+    DBuf<i32> ck2(ctx, 1);
+    allocate_blocks(ctx, np, 0, ck2, clin, 0, G);
+  }
+  i64 ng = G.nC;
+  DBuf<i32> is_acc(ctx, ng), fdiff(ctx, ng), aid(ctx, ng);
+  k_al_groups(ctx, ng, G.seg_start.get(), G.order.get(), coo ? nullptr : ctid.get(), coo ? 1 : 0, is_acc.get(),
+              fdiff.get(), aid.get());
+  i64 nbase = ncomb;
+  DBuf<i32> rvtid;  // COO: per template, its rv
+  if (coo) {
+    // first entry of every key, in traversal order; templates by first-seen rv
+    DBuf<i32> fpos(ctx, ng), gidx(ctx, ng);
+    k_gather_rows(ctx, G.order.get(), 1, G.seg_start.get(), ng, fpos.get());  // order[seg[g]]
+    DBuf<unsigned> fkey(ctx, ng);
+    k_i32_to_u32(ctx, fpos.get(), ng, fkey.get());
+    k_iota(ctx, gidx.get(), ng, 0);
+    sort_pairs32(ctx, fkey.get(), gidx.get(), ng, bits_for((u64)np));
+    DBuf<i32> fsorted(ctx, ng), ent(ctx, ng), rvk(ctx, ng);
+    k_gather_rows(ctx, fpos.get(), 1, gidx.get(), ng, fsorted.get());        // traversal index, increasing
+    k_gather_rows(ctx, c.pa.get(), 1, fsorted.get(), ng, ent.get());         // entry rows
+    k_take_col(ctx, coo_keys, coo_w, coo_w - 1, ent.get(), ng, rvk.get());  // rv of each
+    DBuf<u64> rlin(ctx, ng);
+    k_rv_lin(ctx, rvk.get(), ng, rlin.get());
+    DBuf<i32> rk2(ctx, ng);
+    HIP_CHECK(hipMemcpyAsync(rk2.get(), rvk.get(), ng * sizeof(i32), hipMemcpyDeviceToDevice, ctx->stream));
+    Alloc U;
+    allocate_blocks(ctx, ng, 1, rk2, rlin, (u64)std::max<i64>(1, coo_R - 1), U);
+    nbase = U.nC;
+    DBuf<i32> tsorted(ctx, ng);
+    rvtid.alloc(ctx, U.nC);
+    k_al_seg_ids(ctx, U.nC, U.seg_start.get(), U.order.get(), U.ids.get(), U.keys.get(), 1, tsorted.get(),
+                 rvtid.get());
+    k_scatter_i32(ctx, gidx.get(), tsorted.get(), ng, aid.get());  // aid[g] = template of g's first entry
+  }
+  DBuf<i64> acc64(ctx, ng), rank(ctx, ng);
+  k_i32_to_i64(ctx, is_acc.get(), ng, acc64.get());
+  i64 nacc = exclusive_scan_total(ctx, acc64.get(), rank.get(), ng);
+  // install the structure in C
+  C.n = ng;
+  C.keys = std::move(G.keys);
+  C.ids.alloc(ctx, ng);
+  k_al_final_ids(ctx, ng, is_acc.get(), rank.get(), aid.get(), nbase, C.ids.get());
+  C.scalars.alloc(ctx, ng);
+  C.ntemplates = nbase + nacc;
+  C.data.alloc(ctx, (size_t)C.ntemplates * bs);
+  pl.nblocks_out = ng;
+  pl.npairs = np;
+  // numeric steps
+  i32* pa = pl.keep(std::move(c.pa));
+  i32* pb = c.pb.n ? pl.keep(std::move(c.pb)) : nullptr;
+  double* alpha = pl.keep(DBuf<double>(ctx, np));
+  const double *sA = c.sA, *sB = c.sB;
+  i32* seg = pl.keep(std::move(G.seg_start));
+  i32* order = pl.keep(std::move(G.order));
+  i32* isacc = pl.keep(std::move(is_acc));
+  i32* fd = pl.keep(std::move(fdiff));
+  double* tm = C.data.get();
+  double* sc = C.scalars.get();
+  pl.steps.push_back({"al_alpha", [=]() { k_al_alpha(ctx, pa, pb, np, sA, sB, alpha); }});
+  i64* toff = pl.keep(DBuf<i64>(ctx, np));
+  const double* base;
+  if (!coo) {
+    i32* ub = pl.keep(std::move(byid));
+    k_al_ctid_off(ctx, ctid.get(), np, bs, toff);
+    if (ncomb) pl.steps.push_back({"al_templates", [=]() { tfn(ctx, ncomb, ub, tm); }});
+    base = tm;
+  } else {
+    // toff[p] = (rv_p - 1) * bs into B; templates = the first-seen B slices
+    k_rv_offsets(ctx, coo_keys, coo_w, coo_w, pa, nullptr, nullptr, np, bs, toff);
+    i32* rv = pl.keep(std::move(rvtid));
+    i64 nt = nbase;
+    i64* so = pl.keep(DBuf<i64>(ctx, nt));
+    k_al_offsets(ctx, rv, 1, 0, nt, bs, 0, so);
+    pl.steps.push_back({"al_templates", [=]() { gather_blocks(ctx, coo_B, so, nt, bs, tm); }});
+    base = coo_B;
+  }
+  pl.steps.push_back({"al_scalars", [=]() { k_al_scalars(ctx, ng, seg, order, alpha, isacc, sc); }});
+  if (nacc) {
+    i32* list = pl.keep(DBuf<i32>(ctx, nacc));
+    k_al_acc_list(ctx, ng, isacc, rank.get(), list);
+    double* out = tm + nbase * bs;
+    pl.steps.push_back({"al_accumulate", [=]() {
+                          k_al_accumulate(ctx, nacc, bs, list, seg, order, fd, alpha, base, toff, out);
+                        }});
+  }
+  pl.keep(std::move(rank));
+}
+
+// template builders: outer products (exact) and block products
+static TemplateFn outer_templates(const double* A, i64 blkA, const double* B, i64 blkB, bool ab) {
+  return [=](Ctx* ctx, i64 nu, const i32* u, double* out) {
+    DBuf<i64> ao(ctx, nu), bo(ctx, nu), co(ctx, nu);
+    DBuf<i32> seq(ctx, nu + 1);
+    k_al_offsets(ctx, u, 2, 0, nu, blkA, 0, ao.get());
+    k_al_offsets(ctx, u, 2, 1, nu, blkB, 0, bo.get());
+    k_block_offsets(ctx, nullptr, nu, blkA * blkB, co.get());
+    k_iota(ctx, seq.get(), nu + 1, 0);
+    if (ab)  // t[i + j*blkA] = a[i] * b[j]
+      outer_groups(ctx, blkA, blkB, A, ao.get(), B, bo.get(), out, co.get(), seq.get(), seq.get(), nu);
+    else  // t[i + j*blkB] = b[i] * a[j]
+      outer_groups(ctx, blkB, blkA, B, bo.get(), A, ao.get(), out, co.get(), seq.get(), seq.get(), nu);
+  };
+}
+// out_t = A_x (M x K, sam/sak) * B_y (K x N, boff = (y-1)*bscale + badd, sbk/sbn), C strides scm/scn
+static TemplateFn gemm_templates(unsigned prec, const double* A, i64 blkA, const double* B, i64 bscale, i64 badd,
+                                 const GemmDesc& d, i64 bsC) {
+  return [=](Ctx* ctx, i64 nu, const i32* u, double* out) {
+    DBuf<i64> ao(ctx, nu), bo(ctx, nu), co(ctx, nu);
+    DBuf<i32> seq(ctx, nu + 1);
+    k_al_offsets(ctx, u, 2, 0, nu, blkA, 0, ao.get());
+    k_al_offsets(ctx, u, 2, 1, nu, bscale, badd, bo.get());
+    k_block_offsets(ctx, nullptr, nu, bsC, co.get());
+    k_iota(ctx, seq.get(), nu + 1, 0);
+    grouped_gemm(ctx, prec, d, A, ao.get(), B, bo.get(), out, co.get(), seq.get(), seq.get(), nu, nullptr);
+  };
+}
+
+// _contract_aliased_prefix_outer_ad! (contract_aliased.jl:151-233)
+static void al_prefix_outer_ad(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "aliased_dense_prefix";
+  int PA = A.t->P(), NA = A.t->N(), PC = C.P(), NC = C.N();
+  SB_ASSERT(at(A.map, rlab) == PA, "rlab must be the last prefix axis of A");
+  SB_ASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  i64 R = B.t->dims[at(B.map, rlab) - 1];
+  dense_r_last(pl, B, rlab);
+  int NB = B.t->N();
+  i64 chunkB = prod(B.t->dims, 0, NB - 1);
+  SB_ASSERT(C.blksize == A.t->blksize * chunkB, "C.blksize must equal A.blksize * chunkB");
+  Labels Ad = sub(A.labels, PA, NA), Bwo = sub(B.labels, 0, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool ab;
+  if (Cd == cat(Ad, Bwo))
+    ab = true;
+  else if (Cd == cat(Bwo, Ad))
+    ab = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense...,B_wo_r...] or [B_wo_r...,A_dense...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    int ax = at(A.map, labelsC[j]);
+    SB_ASSERT(ax <= PA && labelsC[j] != rlab, "C prefix label must come from A sparse prefix excluding rlab");
+    src[j] = ax;
+  }
+  const Tensor& TA = *A.t;
+  AlContrib c;
+  i64 nA = TA.n;
+  if (nA) {
+    c.pa.alloc(ctx, nA);
+    k_iota(ctx, c.pa.get(), nA, 0);
+    DBuf<unsigned char> keep(ctx, nA);
+    k_rv_inrange(ctx, TA.keys.get(), PA, PA, nA, R, keep.get());
+    DBuf<i32> none;
+    c.np = compact_pairs(ctx, keep.get(), c.pa, none, nA);
+  }
+  if (c.np) {
+    c.ck.alloc(ctx, (size_t)c.np * std::max(1, PC));
+    DBuf<u64> lin(ctx, c.np);
+    if (PC) k_pair_keys(ctx, c.pa.get(), c.pa.get(), c.np, TA.keys.get(), PA, nullptr, 0, src.data(), PC,
+                        C.dims.data(), c.ck.get(), lin.get());
+    c.xy.alloc(ctx, (size_t)c.np * 2);
+    k_al_xy(ctx, 0, c.pa.get(), nullptr, c.np, TA.ids.get(), TA.keys.get(), PA, PA - 1, nullptr, nullptr, nullptr,
+            0, c.xy.get());
+  }
+  c.nx = std::max<i64>(1, TA.ntemplates);
+  c.ny = std::max<i64>(1, R);
+  c.sA = TA.scalars.get();
+  al_finish(pl, C, c, outer_templates(TA.data.get(), TA.blksize, B.t->dptr(), chunkB, ab), nullptr, 0, nullptr, 0);
+}
+
+// _contract_aliased_dense_ad! (contract_aliased.jl:238-365): one block product
+// per A template; keys, alias ids and scalars carried over
+static void al_dense_ad(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab,
+                        unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "aliased_dense_tail";
+  int PA = A.t->P(), NA = A.t->N(), PC = C.P(), NC = C.N();
+  SB_ASSERT(at(A.map, rlab) == NA, "rlab must be the last axis of A");
+  SB_ASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  SB_ASSERT(PC == PA, "C sparse prefix must equal A sparse prefix for dense-tail reduction");
+  dense_r_last(pl, B, rlab);
+  int NB = B.t->N();
+  i64 R = B.t->dims[NB - 1];
+  i64 chunkB = prod(B.t->dims, 0, NB - 1);
+  SB_ASSERT(A.t->dims[NA - 1] == R, "Reduction size R mismatch between A dense tail and B");
+  i64 chunkA = A.t->blksize / R;
+  SB_ASSERT(C.blksize == chunkA * chunkB, "C.blksize == chunkA * chunkB");
+  Labels Awo = sub(A.labels, PA, NA - 1), Bwo = sub(B.labels, 0, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool ab;
+  if (Cd == cat(Awo, Bwo))
+    ab = true;
+  else if (Cd == cat(Bwo, Awo))
+    ab = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense_wo_r...,B_wo_r...] or [B_wo_r...,A_dense_wo_r...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    int ax = at(A.map, labelsC[j]);
+    SB_ASSERT(ax <= PA, "C prefix label must come from A sparse prefix");
+    src[j] = ax;
+  }
+  const Tensor& TA = *A.t;
+  i64 n = TA.n, nt = TA.ntemplates;
+  C.n = n;
+  C.ntemplates = nt;
+  C.dirty = false;
+  C.data.alloc(ctx, (size_t)nt * C.blksize);
+  C.keys.alloc(ctx, (size_t)n * std::max(1, PC));
+  C.ids.alloc(ctx, n);
+  C.scalars.alloc(ctx, n);
+  pl.nblocks_out = n;
+  pl.npairs = nt;
+  if (n) {
+    DBuf<i32> nk(ctx, (size_t)n * std::max(1, PC)), idx(ctx, n);
+    DBuf<u64> lin(ctx, n);
+    if (PC) {
+      k_remap_keys(ctx, TA.keys.get(), n, PA, src.data(), PC, C.dims.data(), nk.get(), lin.get(), idx.get());
+      sort_pairs(ctx, lin.get(), idx.get(), n, bits_for((u64)prod(C.dims, 0, PC) - 1));
+      k_gather_rows(ctx, nk.get(), PC, idx.get(), n, C.keys.get());
+    } else {
+      k_iota(ctx, idx.get(), n, 0);
+    }
+    k_gather_rows(ctx, TA.ids.get(), 1, idx.get(), n, C.ids.get());
+    const double* sa = TA.scalars.get();
+    double* sc = C.scalars.get();
+    i32* ip = pl.keep(std::move(idx));
+    pl.steps.push_back({"al_scalars_gather", [=]() { k_take_f64(ctx, sa, ip, n, sc); }});
+  }
+  if (nt) {
+    // template t: (chunkA x R) * (chunkB x R)^T, AB -> (chunkA x chunkB), BA -> (chunkB x chunkA)
+    GemmDesc d{chunkA, chunkB, R, 1, chunkA, chunkB, 1, ab ? 1 : chunkB, ab ? chunkA : 1};
+    DBuf<i64> ao(ctx, nt), bo(ctx, nt), co(ctx, nt);
+    DBuf<i32> seq(ctx, nt + 1);
+    k_block_offsets(ctx, nullptr, nt, TA.blksize, ao.get());
+    k_zero(ctx, bo.get(), nt * sizeof(i64));
+    k_block_offsets(ctx, nullptr, nt, C.blksize, co.get());
+    k_iota(ctx, seq.get(), nt + 1, 0);
+    i64 *aop = pl.keep(std::move(ao)), *bop = pl.keep(std::move(bo)), *cop = pl.keep(std::move(co));
+    i32* sq = pl.keep(std::move(seq));
+    add_gemm_step(pl, flags, d, TA.data.get(), aop, B.t->dptr(), bop, C.data.get(), cop, sq, sq, nt, nullptr);
+    pl.flops += 2.0 * chunkA * chunkB * R * nt;
+  }
+}
+
+// _contract_aliased_prefix_outer_aa! (contract_aliased.jl:415-507)
+static void al_prefix_outer_aa(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "aliased_aliased_prefix";
+  int PA = A.t->P(), NA = A.t->N(), PB = B.t->P(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  SB_ASSERT(at(A.map, rlab) == PA && at(B.map, rlab) == PB, "rlab is the last prefix axis of A and B");
+  SB_ASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  SB_ASSERT(C.blksize == A.t->blksize * B.t->blksize, "C.blksize == A.blksize * B.blksize");
+  Labels Ad = sub(A.labels, PA, NA), Bd = sub(B.labels, PB, NB), Cd = sub(labelsC, PC, NC);
+  bool ab;
+  if (Cd == cat(Ad, Bd))
+    ab = true;
+  else if (Cd == cat(Bd, Ad))
+    ab = false;
+  else
+    fail(SB_E_ERROR, "C dense must be [A_dense...,B_dense...] or [B_dense...,A_dense...]");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l) && at(A.map, l) != at(A.map, rlab)) {
+      int ax = at(A.map, l);
+      SB_ASSERT(ax <= PA, "C prefix label from A prefix");
+      src[j] = ax;
+    } else {
+      SB_ASSERT(has(B.map, l), "C prefix label not found in A or B");
+      int ax = at(B.map, l);
+      SB_ASSERT(ax <= PB && l != rlab, "C prefix label from B prefix");
+      src[j] = -ax;
+    }
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 nA = TA.n, nB = TB.n;
+  AlContrib c;
+  if (nA && nB) {
+    DBuf<i32> lo(ctx, nA), hi(ctx, nA);
+    int ca = PA, cb = PB;
+    k_walk_ranges(ctx, 0, TA.keys.get(), PA, nA, TB.keys.get(), PB, nB, &ca, &cb, 1, lo.get(), hi.get());
+    Pairs pr = pairs_from_ranges(ctx, lo, hi, nA);
+    c.np = pr.n;
+    c.pa = std::move(pr.a);
+    c.pb = std::move(pr.b);
+  }
+  if (c.np) {
+    c.ck.alloc(ctx, (size_t)c.np * std::max(1, PC));
+    DBuf<u64> lin(ctx, c.np);
+    if (PC) k_pair_keys(ctx, c.pa.get(), c.pb.get(), c.np, TA.keys.get(), PA, TB.keys.get(), PB, src.data(), PC,
+                        C.dims.data(), c.ck.get(), lin.get());
+    c.xy.alloc(ctx, (size_t)c.np * 2);
+    k_al_xy(ctx, 1, c.pa.get(), c.pb.get(), c.np, TA.ids.get(), TA.keys.get(), PA, 0, TB.ids.get(), nullptr, nullptr,
+            0, c.xy.get());
+  }
+  c.nx = std::max<i64>(1, TA.ntemplates);
+  c.ny = std::max<i64>(1, TB.ntemplates);
+  c.sA = TA.scalars.get();
+  c.sB = TB.scalars.get();
+  al_finish(pl, C, c, outer_templates(TA.data.get(), TA.blksize, TB.data.get(), TB.blksize, ab), nullptr, 0, nullptr,
+            0);
+}
+
+// _contract_aliased_dense_aa! (contract_aliased.jl:512-643): every (A, B) block pair
+static void al_dense_aa(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab,
+                        unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "aliased_aliased_tail";
+  int PA = A.t->P(), NA = A.t->N(), PB = B.t->P(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  SB_ASSERT(at(A.map, rlab) == NA && at(B.map, rlab) == NB, "rlab is the last axis of A and B");
+  SB_ASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  i64 R = A.t->dims[NA - 1];
+  SB_ASSERT(B.t->dims[NB - 1] == R, "Reduction size mismatch");
+  i64 chunkA = A.t->blksize / R, chunkB = B.t->blksize / R;
+  SB_ASSERT(C.blksize == chunkA * chunkB, "C.blksize == chunkA * chunkB");
+  Labels Awo = sub(A.labels, PA, NA - 1), Bwo = sub(B.labels, PB, NB - 1), Cd = sub(labelsC, PC, NC);
+  bool ab;
+  if (Cd == cat(Awo, Bwo))
+    ab = true;
+  else if (Cd == cat(Bwo, Awo))
+    ab = false;
+  else
+    fail(SB_E_ERROR, "C dense tail must be [A_dense_wo_r...,B_dense_wo_r...] or swapped");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l)) {
+      int ax = at(A.map, l);
+      SB_ASSERT(ax <= PA, "C prefix label from A prefix");
+      src[j] = ax;
+    } else {
+      SB_ASSERT(has(B.map, l), "C prefix label from B");
+      int ax = at(B.map, l);
+      SB_ASSERT(ax <= PB, "C prefix label from B prefix");
+      src[j] = -ax;
+    }
+  }
+  const Tensor &TA = *A.t, &TB = *B.t;
+  i64 nA = TA.n, nB = TB.n;
+  AlContrib c;
+  if (nA && nB) {
+    if (nA * nB >= (1ll << 31)) fail(SB_E_LIMIT, "more than 2^31 block pairs in one contraction");
+    c.np = nA * nB;
+    c.pa.alloc(ctx, c.np);
+    c.pb.alloc(ctx, c.np);
+    k_cross_pairs(ctx, nA, nB, c.pa.get(), c.pb.get());
+    c.ck.alloc(ctx, (size_t)c.np * std::max(1, PC));
+    DBuf<u64> lin(ctx, c.np);
+    if (PC) k_pair_keys(ctx, c.pa.get(), c.pb.get(), c.np, TA.keys.get(), PA, TB.keys.get(), PB, src.data(), PC,
+                        C.dims.data(), c.ck.get(), lin.get());
+    c.xy.alloc(ctx, (size_t)c.np * 2);
+    k_al_xy(ctx, 1, c.pa.get(), c.pb.get(), c.np, TA.ids.get(), TA.keys.get(), PA, 0, TB.ids.get(), nullptr, nullptr,
+            0, c.xy.get());
+  }
+  c.nx = std::max<i64>(1, TA.ntemplates);
+  c.ny = std::max<i64>(1, TB.ntemplates);
+  c.sA = TA.scalars.get();
+  c.sB = TB.scalars.get();
+  GemmDesc d{chunkA, chunkB, R, 1, chunkA, chunkB, 1, ab ? 1 : chunkB, ab ? chunkA : 1};
+  al_finish(pl, C, c, gemm_templates(flags, TA.data.get(), TA.blksize, TB.data.get(), TB.blksize, 0, d, C.blksize),
+            nullptr, 0, nullptr, 0);
+}
+
+// contract_aliased! (contract_aliased.jl:383-408, 659-690) and the 9-argument
+// shims (contract_aliased_shared.jl:535-562)
+static void contract_aliased_ad(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab,
+                                unsigned flags) {
+  SB_ASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  if (at(A.map, rlab) <= A.t->P()) {
+    al_r_to_last_prefix(pl, A, rlab);
+    al_prefix_outer_ad(pl, C, labelsC, A, B, rlab);
+  } else {
+    al_r_to_last_dense(pl, A, rlab);
+    al_dense_ad(pl, C, labelsC, A, B, rlab, flags);
+  }
+}
+static void contract_aliased_aa(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Label& rlab,
+                                unsigned flags) {
+  SB_ASSERT(!contains(labelsC, rlab), "rlab not in labelsC");
+  if (at(B.map, rlab) <= B.t->P()) {
+    al_r_to_last_prefix(pl, A, rlab);
+    al_r_to_last_prefix(pl, B, rlab);
+    al_prefix_outer_aa(pl, C, labelsC, A, B, rlab);
+  } else {
+    SB_ASSERT(at(A.map, rlab) > A.t->P(), "rlab must be in A dense tail when it is in B dense tail");
+    al_r_to_last_dense(pl, A, rlab);
+    al_r_to_last_dense(pl, B, rlab);
+    al_dense_aa(pl, C, labelsC, A, B, rlab, flags);
+  }
+}
+
+// contract_shared! Aliased x Dense (contract_aliased_shared.jl:103-302)
+static void shared_al_dense(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared,
+                            unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "aliased_dense_shared";
+  for (auto& l : shared) SB_ASSERT(!contains(labelsC, l), "shared label must be reduced (not in labelsC)");
+  int PA = A.t->P(), NA = A.t->N(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    SB_ASSERT(has(A.map, l), "shared label must exist in A");
+    SB_ASSERT(has(B.map, l), "shared label must exist in B");
+    (at(A.map, l) <= PA ? sp : sd).push_back(l);
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : B.labels)
+    if (!contains(shared, l)) keepB0.push_back(l);
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; unsupported");
+  SB_ASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA covers keepA");
+  SB_ASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB covers keepB");
+  {
+    std::vector<i64> spax, perm;
+    for (auto& l : sp) spax.push_back(at(A.map, l));
+    for (int a = 1; a <= PA; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) perm.push_back(a);
+    for (auto a : spax) perm.push_back(a);
+    std::vector<i64> ka, ra;
+    for (auto& l : dA) ka.push_back(at(A.map, l) - PA);
+    for (auto& l : red) ra.push_back(at(A.map, l) - PA);
+    SB_ASSERT((int)(ka.size() + ra.size()) == A.t->N2, "perm covers dense axes");
+    for (auto a : ka) perm.push_back(PA + a);
+    for (auto a : ra) perm.push_back(PA + a);
+    if (!is_identity(perm)) al_oper_perm(pl, A, perm);
+  }
+  std::vector<i64> permB;
+  for (auto& l : sp) permB.push_back(at(B.map, l));
+  for (auto& l : red) permB.push_back(at(B.map, l));
+  for (auto& l : dB) permB.push_back(at(B.map, l));
+  SB_ASSERT((int)permB.size() == NB, "B perm length mismatch; labelsB must match B ndims");
+  const Tensor* Bt = B.t;
+  if (!is_identity(permB)) Bt = plan_dense_permute(pl, *Bt, permB);
+  int nsp = (int)sp.size(), nred = (int)red.size(), nkA = (int)dA.size(), nkB = (int)dB.size();
+  const Tensor& TA = *A.t;
+  const std::vector<i64>& dBp = Bt->dims;
+  i64 M = nkA == 0 ? 1 : prod(TA.dims, PA, PA + nkA);
+  i64 K = nred == 0 ? 1 : prod(TA.dims, PA + nkA, NA);
+  i64 Nn = nkB == 0 ? 1 : prod(dBp, nsp + nred, NB);
+  if (nred > 0) SB_ASSERT(K == prod(dBp, nsp, nsp + nred), "Reduction extent mismatch between A dense tail and B");
+  SB_ASSERT(C.blksize == M * Nn, "C.blksize must equal M*N");
+  std::vector<int> csrc(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    SB_ASSERT(has(A.map, l), "C prefix label must exist in A");
+    int ap = at(A.map, l);
+    SB_ASSERT(ap <= PA, "C prefix label must come from A sparse prefix");
+    SB_ASSERT(!contains(sp, l), "C prefix label cannot be a reduced shared-prefix label");
+    csrc[j] = ap;
+  }
+  std::vector<int> jA;
+  std::vector<i64> spstr;
+  i64 SP = 1;
+  for (int t = 0; t < nsp; ++t) {
+    jA.push_back(at(A.map, sp[t]) - 1);
+    spstr.push_back(SP);
+    SP *= dBp[t];
+  }
+  AlContrib c;
+  i64 nA = TA.n;
+  if (nA) {  // every A block, in key order (the runs of equal shared-prefix values keep that order)
+    c.np = nA;
+    c.pa.alloc(ctx, nA);
+    k_iota(ctx, c.pa.get(), nA, 0);
+    c.ck.alloc(ctx, (size_t)nA * std::max(1, PC));
+    DBuf<u64> lin(ctx, nA);
+    if (PC) k_pair_keys(ctx, c.pa.get(), c.pa.get(), nA, TA.keys.get(), PA, nullptr, 0, csrc.data(), PC,
+                        C.dims.data(), c.ck.get(), lin.get());
+    c.xy.alloc(ctx, (size_t)nA * 2);
+    k_al_xy(ctx, 2, c.pa.get(), nullptr, nA, TA.ids.get(), TA.keys.get(), PA, 0, nullptr, jA.data(), spstr.data(),
+            nsp, c.xy.get());
+  }
+  c.nx = std::max<i64>(1, TA.ntemplates);
+  c.ny = SP;
+  c.sA = TA.scalars.get();
+  // template (tidA, sp_lin): A_t (M x K) * Bp[sp_lin, :, :] (K x N, element (k, n) at sp_lin-1 + SP*(k + K*n))
+  GemmDesc d{M, Nn, K, 1, M, SP, SP * K, mode == AthenB ? 1 : Nn, mode == AthenB ? M : 1};
+  al_finish(pl, C, c, gemm_templates(flags, TA.data.get(), TA.blksize, Bt->dptr(), 1, 0, d, C.blksize), nullptr, 0,
+            nullptr, 0);
+}
+
+// contract_shared! Aliased x Aliased (contract_aliased_shared.jl:325-523)
+static void shared_al_al(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& B, const Labels& shared,
+                         unsigned flags) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "aliased_aliased_shared";
+  for (auto& l : shared) SB_ASSERT(!contains(labelsC, l), "shared label must be reduced (not in labelsC)");
+  int PA = A.t->P(), NA = A.t->N(), PB = B.t->P(), NB = B.t->N(), PC = C.P(), NC = C.N();
+  Labels sp, sd;
+  for (auto& l : shared) {
+    SB_ASSERT(has(A.map, l) && has(B.map, l), "shared label must exist in both A and B");
+    bool ap = at(A.map, l) <= PA, bp = at(B.map, l) <= PB;
+    if (ap && bp)
+      sp.push_back(l);
+    else if (!ap && !bp)
+      sd.push_back(l);
+    else
+      fail(SB_E_ERROR, "shared label crosses prefix/dense boundary; not supported");
+  }
+  Labels Ad0 = sub(A.labels, PA, NA), Bd0 = sub(B.labels, PB, NB), red, keepA0, keepB0;
+  for (auto& l : Ad0) (contains(sd, l) ? red : keepA0).push_back(l);
+  for (auto& l : Bd0)
+    if (!contains(sd, l)) keepB0.push_back(l);
+  Labels Cd = sub(labelsC, PC, NC), dA, dB;
+  Mode mode = cdense_grouping(Cd, keepA0, keepB0, dA, dB);
+  if (mode == Interleaved) fail(SB_E_ERROR, "Cdense interleaves A/B kept dims; unsupported");
+  SB_ASSERT(dA.size() == keepA0.size() && same_set(dA, keepA0), "desired_keepA covers keepA");
+  SB_ASSERT(dB.size() == keepB0.size() && same_set(dB, keepB0), "desired_keepB covers keepB");
+  auto build_perm = [&](Oper& X, bool red_first, const Labels& keep) {
+    int P = X.t->P();
+    std::vector<i64> spax, perm;
+    for (auto& l : sp) spax.push_back(at(X.map, l));
+    for (int a = 1; a <= P; ++a)
+      if (std::find(spax.begin(), spax.end(), a) == spax.end()) perm.push_back(a);
+    for (auto a : spax) perm.push_back(a);
+    std::vector<i64> ka, ra;
+    for (auto& l : keep) ka.push_back(at(X.map, l) - P);
+    for (auto& l : red) ra.push_back(at(X.map, l) - P);
+    SB_ASSERT((int)(ka.size() + ra.size()) == X.t->N2, "perm covers dense axes");
+    for (auto a : (red_first ? ra : ka)) perm.push_back(P + a);
+    for (auto a : (red_first ? ka : ra)) perm.push_back(P + a);
+    return perm;
+  };
+  auto pA = build_perm(A, false, dA);
+  auto pB = build_perm(B, true, dB);
+  if (!is_identity(pA)) al_oper_perm(pl, A, pA);
+  if (!is_identity(pB)) al_oper_perm(pl, B, pB);
+  const Tensor &TA = *A.t, &TB = *B.t;
+  int nkA = (int)dA.size(), nkB = (int)dB.size(), nred = (int)red.size();
+  i64 M = nkA == 0 ? 1 : prod(TA.dims, PA, PA + nkA);
+  i64 K = nred == 0 ? 1 : prod(TA.dims, PA + nkA, NA);
+  i64 Nn = nkB == 0 ? 1 : prod(TB.dims, PB + nred, NB);
+  if (nred > 0) SB_ASSERT(K == prod(TB.dims, PB, PB + nred), "Reduction extent mismatch between A and B dense tails");
+  SB_ASSERT(C.blksize == M * Nn, "C.blksize must equal M*N");
+  std::vector<int> src(PC);
+  for (int j = 0; j < PC; ++j) {
+    const Label& l = labelsC[j];
+    if (has(A.map, l) && at(A.map, l) <= PA)
+      src[j] = at(A.map, l);
+    else if (has(B.map, l) && at(B.map, l) <= PB)
+      src[j] = -at(B.map, l);
+    else
+      fail(SB_E_ERROR, "C prefix label must come from A or B sparse prefix (excluding shared_prefix)");
+  }
+  std::vector<int> jA, jB;
+  std::vector<i64> spd;
+  for (auto& l : sp) {
+    jA.push_back(at(A.map, l));
+    jB.push_back(at(B.map, l));
+    spd.push_back(TA.dims[at(A.map, l) - 1]);
+  }
+  i64 nA = TA.n, nB = TB.n;
+  int nsp = (int)sp.size();
+  AlContrib c;
+  if (nA && nB) {
+    DBuf<i32> lo(ctx, nA), hi(ctx, nA);
+    if (nsp == 0) {
+      DBuf<u64> z(ctx, nA), zb(ctx, nB);
+      k_zero(ctx, z.get(), nA * sizeof(u64));
+      k_zero(ctx, zb.get(), nB * sizeof(u64));
+      k_equal_ranges(ctx, z.get(), nA, zb.get(), nB, lo.get(), hi.get());
+    } else if (nsp >= 2 && !(flags & SB_FLAG_EXACT_JOIN)) {
+      k_walk_ranges(ctx, 2, TA.keys.get(), PA, nA, TB.keys.get(), PB, nB, jA.data(), jB.data(), nsp, lo.get(),
+                    hi.get());
+    } else {
+      DBuf<u64> ja(ctx, nA), jb(ctx, nB);
+      k_cols_lin(ctx, TA.keys.get(), nA, PA, jA.data(), nsp, spd.data(), ja.get());
+      k_cols_lin(ctx, TB.keys.get(), nB, PB, jB.data(), nsp, spd.data(), jb.get());
+      k_equal_ranges(ctx, ja.get(), nA, jb.get(), nB, lo.get(), hi.get());
+    }
+    Pairs pr = pairs_from_ranges(ctx, lo, hi, nA);
+    c.np = pr.n;
+    c.pa = std::move(pr.a);
+    c.pb = std::move(pr.b);
+  }
+  if (c.np) {
+    c.ck.alloc(ctx, (size_t)c.np * std::max(1, PC));
+    DBuf<u64> lin(ctx, c.np);
+    if (PC) k_pair_keys(ctx, c.pa.get(), c.pb.get(), c.np, TA.keys.get(), PA, TB.keys.get(), PB, src.data(), PC,
+                        C.dims.data(), c.ck.get(), lin.get());
+    c.xy.alloc(ctx, (size_t)c.np * 2);
+    k_al_xy(ctx, 1, c.pa.get(), c.pb.get(), c.np, TA.ids.get(), TA.keys.get(), PA, 0, TB.ids.get(), nullptr, nullptr,
+            0, c.xy.get());
+  }
+  c.nx = std::max<i64>(1, TA.ntemplates);
+  c.ny = std::max<i64>(1, TB.ntemplates);
+  c.sA = TA.scalars.get();
+  c.sB = TB.scalars.get();
+  // template (tidA, tidB): A_t (M x K) * B_t (K x N)
+  GemmDesc d{M, Nn, K, 1, M, 1, K, mode == AthenB ? 1 : Nn, mode == AthenB ? M : 1};
+  al_finish(pl, C, c, gemm_templates(flags, TA.data.get(), TA.blksize, TB.data.get(), TB.blksize, 0, d, C.blksize),
+            nullptr, 0, nullptr, 0);
+}
+
+// contract_aliased! COO x Dense -> Aliased (contract_aliased_coo_dense.jl:34-164)
+static void coo_dense_aliased(Plan& pl, Tensor& C, const Labels& labelsC, CooOper& A, Oper& B, const Label& rlab) {
+  Ctx* ctx = pl.ctx;
+  pl.family = "coo_dense_aliased";
+  int PC = C.P(), NC = C.N(), NA = A.t->N(), NB = B.t->N();
+  {  // aligned_A_to_Cprefix (contract_coo_dense.jl:4-18)
+    Labels desired = sub(labelsC, 0, PC);
+    desired.push_back(rlab);
+    LabelMap m = make_map(A.labels);
+    SB_ASSERT((int)desired.size() == NA, "length(desired) == NA");
+    std::vector<i64> perm(NA);
+    for (int j = 0; j < NA; ++j) perm[j] = at(m, desired[j]);
+    A.t = plan_coo_permute_sort(pl, *A.t, perm);
+    A.labels = desired;
+    A.map = make_map(desired);
+  }
+  Labels dnr;
+  for (int j = PC; j < NC; ++j)
+    if (labelsC[j] != rlab) dnr.push_back(labelsC[j]);
+  int rpos = -1;
+  for (int i = 0; i < NB; ++i)
+    if (B.labels[i] == rlab) {
+      rpos = i + 1;
+      break;
+    }
+  SB_ASSERT(rpos > 0, "rlab not found in labelsB");
+  std::vector<i64> perm(NB, 0);
+  for (size_t k = 0; k < dnr.size(); ++k) {
+    int pos = -1;
+    for (int i = 0; i < NB; ++i)
+      if (B.labels[i] == dnr[k]) {
+        pos = i + 1;
+        break;
+      }
+    SB_ASSERT(pos > 0, "label not found in labelsB");
+    if ((int)k >= NB) fail(SB_E_BOUNDS, "BoundsError: perm");
+    perm[k] = pos;
+  }
+  perm[NB - 1] = rpos;
+  const Tensor* Bt = B.t;
+  if (!is_identity(perm)) {
+    std::vector<i64> s(perm);
+    std::sort(s.begin(), s.end());
+    for (int i = 0; i < NB; ++i)
+      if (s[i] != i + 1) fail(SB_E_ARGUMENT, "invalid permutation of B");
+    Bt = plan_dense_permute(pl, *Bt, perm);
+  }
+  i64 R = Bt->dims[NB - 1];
+  i64 bs = prod(Bt->dims, 0, NB - 1);
+  SB_ASSERT(C.blksize == bs, "C.blksize must equal prod(B dims without r)");
+  const Tensor& TA = *A.t;
+  AlContrib c;
+  i64 nA = TA.n;
+  if (nA) {
+    c.pa.alloc(ctx, nA);
+    k_iota(ctx, c.pa.get(), nA, 0);
+    DBuf<unsigned char> keep(ctx, nA);
+    k_rv_inrange(ctx, TA.keys.get(), NA, NA, nA, R, keep.get());
+    DBuf<i32> none;
+    c.np = compact_pairs(ctx, keep.get(), c.pa, none, nA);
+  }
+  if (c.np) {
+    std::vector<int> src(PC);
+    std::iota(src.begin(), src.end(), 1);
+    c.ck.alloc(ctx, (size_t)c.np * std::max(1, PC));
+    DBuf<u64> lin(ctx, c.np);
+    if (PC) k_pair_keys(ctx, c.pa.get(), c.pa.get(), c.np, TA.keys.get(), NA, nullptr, 0, src.data(), PC,
+                        C.dims.data(), c.ck.get(), lin.get());
+  }
+  c.sA = TA.data.get();  // the aligned COO values (gathered by an earlier plan step)
+  al_finish(pl, C, c, TemplateFn(), Bt->dptr(), R, TA.keys.get(), NA);
+}
+
 // ================================================================ dispatcher
 std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, const Tensor& A,
                                     const Labels& labelsA, const Tensor& B, const Labels& labelsB, unsigned flags) {
@@ -1620,6 +2453,13 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
     } else if (C.kind == BS && A.kind == COO && B.kind == DENSE) {
       CooOper Ac{&A, labelsA, mapA};
       contract_coo_dense(*pl, C, labelsC, Ac, Bo, r, flags);
+    } else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE) {
+      contract_aliased_ad(*pl, C, labelsC, Ao, Bo, r, flags);  // contract_aliased_shared.jl:535-547
+    } else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED) {
+      contract_aliased_aa(*pl, C, labelsC, Ao, Bo, r, flags);  // contract_aliased_shared.jl:550-562
+    } else if (C.kind == ALIASED && A.kind == COO && B.kind == DENSE) {
+      CooOper Ac{&A, labelsA, mapA};
+      coo_dense_aliased(*pl, C, labelsC, Ac, Bo, r);  // tensor_wrappers_aliased.jl:263-275
     } else {
       fail(SB_E_METHOD, "MethodError: no single-label contract! method for these storage kinds");
     }
@@ -1628,6 +2468,10 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
       shared_bs_bs(*pl, C, labelsC, Ao, Bo, shared, flags);
     else if (C.kind == BS && A.kind == BS && B.kind == DENSE)
       shared_bs_dense(*pl, C, labelsC, Ao, Bo, shared, flags);
+    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE)
+      shared_al_dense(*pl, C, labelsC, Ao, Bo, shared, flags);
+    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED)
+      shared_al_al(*pl, C, labelsC, Ao, Bo, shared, flags);
     else
       fail(SB_E_METHOD, "MethodError: no contract_shared! method for these storage kinds");
   }

@@ -77,6 +77,9 @@ struct Plan {
   }
 };
 
+// permutedims(::AliasedBlockSparse, perm) (aliased/storage.jl:172-239)
+TP al_permute(Ctx* ctx, const Tensor& A, const std::vector<i64>& perm);
+
 // contract!(C, labelsC, A, labelsA, B, labelsB) (SparseBackends/src/tensoralgebra/contract.jl:37-104).
 // C is overwritten: its keys/ids are set at plan time, its payload by every
 // plan execution.  The plan keeps pointers to A's and B's payloads.

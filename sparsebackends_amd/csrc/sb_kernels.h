@@ -128,6 +128,7 @@ struct BoxTile {
 };
 BoxTile box_tile(int cfg);
 int box_tile_configs();
+bool box_tile_is_wave(int cfg);
 struct BoxGemmArgs {
   BoxDesc d;
   i64 ng = 0, np = 0, max_tiles = 0;
@@ -142,7 +143,8 @@ struct BoxGemmArgs {
   const i32* gorder = nullptr;  // groups in locality order (null: key order)
   // per execution
   i32 *gbox = nullptr, *kbox = nullptr, *kc = nullptr, *cnew = nullptr, *clear = nullptr;
-  i32 *tiles = nullptr, *toff = nullptr, *tile_group = nullptr;  // per position; toff[0] = 0 from planning
+  i32 *tiles = nullptr, *toff = nullptr;  // tile counts per position; toff[0] = 0 from planning
+  int2* trec = nullptr;                   // work list: {group, tile within the group}
   i32* cbox = nullptr;  // C's box rows (old until the GEMM kernel installs the new ones)
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
@@ -154,7 +156,7 @@ void box_scan(Ctx*, const double* data, const i32* ids, i64 n, i64 bs, int N2, c
 void k_fill_rows(Ctx*, i32* p, i64 n, int w, const i64* row);
 size_t box_scan_tmp_bytes(Ctx*, i64 ng);
 // group boxes, pair K extents, C boxes (cbox old -> new, entries outside a shrunken box zeroed) and,
-// when tile_group is set, the (group, tile) work list
+// when trec is set, the (group, tile) work list
 void box_prepare(Ctx*, const BoxGemmArgs&, const i32* abox, const i32* aflag, const i32* bbox, const i32* bflag,
                  i32* cbox, i32* cflag, double* C);
 // C_g[box] = sum over pairs and K boxes of A_p * B_p (flags non-finite outputs in cflag); first
@@ -201,4 +203,27 @@ void k_coo_dense_nonzero(Ctx*, const double* alpha, const i32* keys, int w, int 
 // is the column non-decreasing over rows? (synchronises)
 bool column_sorted(Ctx*, const i32* keys, int w, int col, i64 n);
 // COO output keys: ck[p, t] = src[t] > 0 ? A[pa[p], src[t]-1] : B[pb[p], -src[t]-1]   (same as k_pair_keys w/o lin)
+// ---- aliased block-sparse (sb_aliased.hip) ---------------------------------
+void k_al_expand(Ctx*, i64 n, i64 bs, const i32* ids, const double* sc, const double* tm, double* out);
+void k_al_xy(Ctx*, int mode, const i32* pa, const i32* pb, i64 np, const i32* idsA, const i32* keysA, int wA,
+             int col, const i32* idsB, const int* cols, const i64* str, int ncols, i32* xy);
+void k_al_xy_lin(Ctx*, const i32* xy, i64 np, i64 nx, u64* lin);
+void k_al_seg_ids(Ctx*, i64 ns, const i32* seg, const i32* order, const i32* ids, const i32* ukeys, int w, i32* val,
+                  i32* byid);
+void k_al_offsets(Ctx*, const i32* u, int w, int c, i64 n, i64 scale, i64 add, i64* off);
+void k_al_groups(Ctx*, i64 ng, const i32* seg, const i32* order, const i32* ctid, int demote_any, i32* is_acc,
+                 i32* first_diff, i32* aid);
+void k_al_final_ids(Ctx*, i64 ng, const i32* is_acc, const i64* rank, const i32* aid, i64 base, i32* out);
+void k_al_acc_list(Ctx*, i64 ng, const i32* is_acc, const i64* rank, i32* list);
+void k_al_alpha(Ctx*, const i32* pa, const i32* pb, i64 np, const double* sA, const double* sB, double* alpha);
+void k_al_scalars(Ctx*, i64 ng, const i32* seg, const i32* order, const double* alpha, const i32* is_acc,
+                  double* sc);
+void k_al_accumulate(Ctx*, i64 nacc, i64 bs, const i32* list, const i32* seg, const i32* order,
+                     const i32* first_diff, const double* alpha, const double* base, const i64* toff, double* out);
+void k_al_ctid_off(Ctx*, const i32* ctid, i64 np, i64 bs, i64* toff);
+void k_i32_to_u32(Ctx*, const i32* a, i64 n, unsigned* b);
+void k_i32_to_i64(Ctx*, const i32* a, i64 n, i64* b);
+void k_take_col(Ctx*, const i32* keys, int w, int col0, const i32* idx, i64 n, i32* out);
+void k_rv_lin(Ctx*, const i32* v, i64 n, u64* lin);
+void k_scatter_i32(Ctx*, const i32* idx, const i32* vals, i64 n, i32* out);
 }  // namespace sb

@@ -12,6 +12,11 @@ using TP = std::unique_ptr<Tensor>;
 TP bs_new(Ctx*, const std::vector<i64>& dims, int N2);   // public constructor (0 < N2)
 TP bs_make(Ctx*, const std::vector<i64>& dims, int N2);  // inner constructor (0 <= N2)
 TP dense_new(Ctx*, const std::vector<i64>& dims);
+TP al_new(Ctx*, const std::vector<i64>& dims, int N2);    // AliasedBlockSparse (0 < N2)
+double al_getindex(Ctx*, const Tensor& A, const i64* I);
+TP al_to_blocksparse(Ctx*, const Tensor& A);
+// _check_no_cross_perm (storage.jl:126-132)
+void check_no_cross_perm(const std::vector<i64>& perm, int N, int P);
 TP coo_new(Ctx*, const std::vector<i64>& dims);
 // permutedims(::NewBlockSparseSorted, perm) (blocksparse/storage.jl:188-264); perm 1-based
 TP bs_permute(Ctx*, const Tensor& A, const std::vector<i64>& perm);

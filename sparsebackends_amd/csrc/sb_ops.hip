@@ -53,6 +53,23 @@ TP bs_new(Ctx* ctx, const std::vector<i64>& dims, int N2) {
   return bs_make(ctx, dims, N2);
 }
 
+// AliasedBlockSparse{T,N,N2}(dims) (aliased/storage.jl:122-132)
+TP al_new(Ctx* ctx, const std::vector<i64>& dims, int N2) {
+  int N = (int)dims.size();
+  SB_ASSERT(0 < N2 && N2 <= N, "0 < N2 <= N");
+  for (auto d : dims) SB_ASSERT(d >= 1, "all(dims .>= 1)");
+  if (N - N2 > MAXR || N2 > MAXR) fail(SB_E_LIMIT, "prefix or dense-tail rank above 8");
+  TP t(new Tensor);
+  t->ctx = ctx;
+  t->kind = ALIASED;
+  t->dims = dims;
+  t->N2 = N2;
+  t->blksize = prod(dims, N - N2, N);
+  t->n = 0;
+  t->ntemplates = 0;
+  return t;
+}
+
 TP dense_new(Ctx* ctx, const std::vector<i64>& dims) {
   check_dims(dims);
   TP t(new Tensor);
@@ -85,7 +102,7 @@ DBuf<i64> block_offsets(Ctx* ctx, const Tensor& A) {
 }
 
 // ---------------------------------------------------------------- permutes
-static void check_no_cross_perm(const std::vector<i64>& perm, int N, int P) {
+void check_no_cross_perm(const std::vector<i64>& perm, int N, int P) {
   SB_ASSERT((int)perm.size() == N, "length(perm) == N");
   std::vector<i64> s(perm);
   std::sort(s.begin(), s.end());
@@ -714,6 +731,41 @@ double bs_getindex(Ctx* ctx, const Tensor& A, const i64* I) {
   }
   i32 id = read1i(ctx, A.ids.get() + pos);
   return read1(ctx, A.data.get() + (i64)(id - 1) * A.blksize + tail_index(A, I));
+}
+
+// getindex(::AliasedBlockSparse) (aliased/storage.jl:138-147)
+double al_getindex(Ctx* ctx, const Tensor& A, const i64* I) {
+  check_index(A, I);
+  int P = A.P();
+  if (A.n == 0) return 0.0;
+  i64 pos = 0;
+  if (P > 0) {
+    auto [p, found] = dev_search(ctx, A.keys.get(), A.n, P, I, A.dims.data());
+    if (!found) return 0.0;
+    pos = p;
+  }
+  i32 tid = read1i(ctx, A.ids.get() + pos);
+  double a = read1(ctx, A.scalars.get() + pos);
+  return a * read1(ctx, A.data.get() + (i64)(tid - 1) * A.blksize + tail_index(A, I));
+}
+
+// to_blocksparse (aliased/conversions.jl:18-32): ids 1:n, payload scalar * template
+TP al_to_blocksparse(Ctx* ctx, const Tensor& A) {
+  TP B = bs_make(ctx, A.dims, A.N2);
+  i64 n = A.n;
+  int P = A.P();
+  B->n = n;
+  B->keys.alloc(ctx, (size_t)n * (P ? P : 1));
+  B->ids.alloc(ctx, n ? n : 1);
+  B->data.alloc(ctx, (size_t)n * A.blksize);
+  if (n) {
+    if (P)
+      HIP_CHECK(hipMemcpyAsync(B->keys.get(), A.keys.get(), (size_t)n * P * sizeof(i32), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    k_iota(ctx, B->ids.get(), n, 1);
+    k_al_expand(ctx, n, A.blksize, A.ids.get(), A.scalars.get(), A.data.get(), B->data.get());
+  }
+  return B;
 }
 
 __global__ void set1_kernel(double* p, double v) { *p = v; }

@@ -267,6 +267,76 @@ class NewBlockSparseSorted(_Tensor):
         return NewBlockSparseSorted(out, self.device)
 
 
+class AliasedBlockSparse(_Tensor):
+    """Block-sparse tensor whose block i is scalars[i] * templates[alias_ids[i]]
+    (aliased/storage.jl:46-59); keys sorted by the column-major prefix lin."""
+    kind = 4
+
+    @staticmethod
+    def new(dims, N2, device=0):
+        """AliasedBlockSparse{T,N,N2}(dims): no blocks (aliased/storage.jl:122-132)."""
+        d = _i64(dims)
+        out = C.c_void_p()
+        check(load().sb_al_new(context(device), len(dims), N2, _ptr(d, C.c_int64), C.byref(out)))
+        return AliasedBlockSparse(out, device)
+
+    @staticmethod
+    def from_parts(dims, N2, templates, ntemplates, keys, alias_ids, scalars, device=0):
+        d = _i64(dims)
+        t = np.ascontiguousarray(np.asarray(templates, dtype=np.float64).reshape(-1))
+        k = _i64(keys).reshape(-1)
+        a = _i64(alias_ids)
+        sc = np.ascontiguousarray(np.asarray(scalars, dtype=np.float64).reshape(-1))
+        out = C.c_void_p()
+        check(load().sb_al_from_host(context(device), len(dims), N2, _ptr(d, C.c_int64), int(ntemplates),
+                                     _ptr(t, C.c_double), len(a), _ptr(k, C.c_int64), _ptr(a, C.c_int64),
+                                     _ptr(sc, C.c_double), C.byref(out)))
+        return AliasedBlockSparse(out, device)
+
+    def info(self):
+        N, N2 = C.c_int(), C.c_int()
+        nb, bs, nt = C.c_int64(), C.c_int64(), C.c_int64()
+        d = np.zeros(16, dtype=np.int64)
+        check(load().sb_al_info(self._h, C.byref(N), C.byref(N2), _ptr(d, C.c_int64), C.byref(nb), C.byref(bs),
+                                C.byref(nt)))
+        return tuple(int(x) for x in d[:N.value]), N2.value, nb.value, bs.value, nt.value
+
+    def parts(self):
+        dims, N2, nb, bs, nt = self.info()
+        P = len(dims) - N2
+        keys = np.zeros((nb, P), dtype=np.int64)
+        ids = np.zeros(nb, dtype=np.int64)
+        sc = np.zeros(nb, dtype=np.float64)
+        tm = np.zeros(nt * bs, dtype=np.float64)
+        check(load().sb_al_to_host(self.ctx, self._h, _ptr(keys, C.c_int64), _ptr(ids, C.c_int64),
+                                   _ptr(sc, C.c_double), _ptr(tm, C.c_double)))
+        return dict(dims=dims, N2=N2, blksize=bs, ntemplates=nt, keys=keys, alias_ids=ids, scalars=sc, templates=tm)
+
+    def __getitem__(self, I):
+        a = _i64(I)
+        v = C.c_double()
+        check(load().sb_al_getindex(self.ctx, self._h, _ptr(a, C.c_int64), C.byref(v)))
+        return v.value
+
+    def compression_ratio(self):
+        """n_blocks / n_templates (aliased/storage.jl:163-164)."""
+        v = C.c_double()
+        check(load().sb_al_compression_ratio(self._h, C.byref(v)))
+        return v.value
+
+    def to_blocksparse(self):
+        """to_blocksparse (aliased/conversions.jl:18-32)."""
+        out = C.c_void_p()
+        check(load().sb_al_to_blocksparse(self.ctx, self._h, C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
+    def to_dense(self, init=0.0):
+        """to_dense(A; init) (aliased/conversions.jl:45-64) -> numpy."""
+        out = C.c_void_p()
+        check(load().sb_al_to_dense(self.ctx, self._h, float(init), C.byref(out)))
+        return DenseTensor(out, self.device).numpy()
+
+
 class COOTensor(_Tensor):
     """Element-sparse tensor: keys (nnz x N), vals, dirty flag (coo/storage.jl:1-6)."""
     kind = 2

@@ -21,7 +21,7 @@ from both import compare_bs
 
 pytestmark = pytest.mark.gpu
 TILE_SHIFT = 16
-NTILES = 8
+NTILES = 14
 
 
 def dev(p):
@@ -59,7 +59,7 @@ def test_every_tile_shape(tile):
 
 
 @pytest.mark.parametrize("prec", [sb.PREC_FP32, sb.PREC_BF16X3])
-@pytest.mark.parametrize("tile", [0, 1, 2, 6])
+@pytest.mark.parametrize("tile", [0, 1, 2, 6, 9, 12])
 def test_precision_forms_padded(prec, tile):
     rng = np.random.default_rng(5 + tile)
     c = gen_padded_case(rng, big=True, vals=gauss_vals)
@@ -82,7 +82,7 @@ def chain_case(rng, K1=24, K2=30, M=40, N=36, vals=gauss_vals):
     return dict(A=A, la=la, B=B, lb=lb, lc=lc, cdims=(3, 3, M, N), N2C=2)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 6])
+@pytest.mark.parametrize("tile", [0, 1, 2, 6, 9, 12, 14])
 def test_long_k_many_pairs(tile):
     rng = np.random.default_rng(31)
     print(f"[box long K tile {tile}]", flush=True)
