@@ -16,6 +16,7 @@
 // Per execution: box_prepare (group boxes, per-pair K extents, tile counts,
 // clearing of C entries that fall outside a shrunken box) then box_gemm, a
 // persistent kernel that walks the (group, tile) work list.
+#include <cstdlib>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -285,9 +286,6 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
   __shared__ T Bs[BK][BN + 2];
   __shared__ int rowA[BM], rowC[BM], colB[BN], colC[BN];
   __shared__ i64 tabA[KT], tabB[KT];
-  __shared__ int pkc[257];          // per pair of the batch: K count, then its exclusive scan
-  __shared__ i64 pao[256], pbo[256];  // per pair: A / B block offsets
-  __shared__ int pkb[MAXAX][256];   // per pair: K box
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave % WM, wn = wave / WM;
   const int nmn = d.nm + d.nn;
@@ -366,50 +364,35 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
         accd[i][j] = d4{0, 0, 0, 0};
         accf[i][j] = f4{0, 0, 0, 0};
       }
-    // pairs in batches of up to 256: their records load in parallel, a prefix
-    // scan of their K counts places every K position of the batch
     const i32 p1 = seg[g + 1];
-    for (i32 pb0 = seg[g]; pb0 < p1; pb0 += 256) {
-      const int nbp = min(256, p1 - pb0);
-      __syncthreads();  // the previous batch is done with the pair tables
-      if (tid < nbp) {
-        const i32 pp = pb0 + tid;
-        pkc[tid] = kc[pp];
-        pao[tid] = aoff[pp];
-        pbo[tid] = boff[pp];
-#pragma unroll
-        for (int a = 0; a < MAXAX; ++a)
-          if (a < d.nk) pkb[a][tid] = kbox[(i64)pp * d.nk + a];
-      }
+    i32 pp = seg[g];
+    int kin = 0;
+    while (pp < p1) {
       __syncthreads();
-      if (tid == 0) {  // exclusive scan of the K counts (a handful of pairs per group)
-        int acc = 0;
-        for (int j = 0; j < nbp; ++j) {
-          const int c = pkc[j];
-          pkc[j] = acc;
-          acc += c;
-        }
-        pkc[nbp] = acc;
-      }
-      __syncthreads();
-      const int ktot = pkc[nbp];
-      for (int kb0 = 0; kb0 < ktot; kb0 += KT) {
-        const int fill = min(KT, ktot - kb0);
-        if (kb0) __syncthreads();  // the previous window's products are done with the K table
-        for (int j = 0; j < nbp; ++j) {  // pair j covers K positions [pkc[j], pkc[j+1])
-          const int lo = max(pkc[j], kb0), hi = min(pkc[j + 1], kb0 + fill);
-          if (lo >= hi) continue;
-          int kbv[MAXAX];
+      int fill = 0;
+      while (pp < p1 && fill < KT) {
+        const int kcp = kc[pp];
+        const int take = min(kcp - kin, KT - fill);
+        if (take > 0) {
+          int kb[MAXAX];
 #pragma unroll
-          for (int a = 0; a < MAXAX; ++a) kbv[a] = a < d.nk ? pkb[a][j] : 1;
-          for (int kp = lo + tid; kp < hi; kp += 256) {
+          for (int a = 0; a < MAXAX; ++a) kb[a] = a < d.nk ? kbox[(i64)pp * d.nk + a] : 1;
+          const i64 ao = aoff[pp], bo = boff[pp];
+          for (int t = tid; t < take; t += 256) {
             int oa, ob;
-            decode2(kp - pkc[j], d.nk, kbv, d.k_sa, d.k_sb, oa, ob);
-            tabA[kp - kb0] = pao[j] + oa;
-            tabB[kp - kb0] = pbo[j] + ob;
+            decode2(kin + t, d.nk, kb, d.k_sa, d.k_sb, oa, ob);
+            tabA[fill + t] = ao + oa;
+            tabB[fill + t] = bo + ob;
           }
         }
-        __syncthreads();
+        fill += take;
+        kin += take;
+        if (kin >= kcp) {
+          ++pp;
+          kin = 0;
+        }
+      }
+      __syncthreads();
       double ra[AE], rb[BE];
       auto load = [&](int k0) {
 #pragma unroll
@@ -475,13 +458,15 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
         } else {
           // bf16x3: both operands split in three bf16 terms; the six products of
           // order <= 2 accumulate in FP32
+#pragma unroll
+          for (int kk = 0; kk < BK / 16; ++kk) {
           s4 ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
 #pragma unroll
           for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               unsigned short h, m, l;
-              split3_b(As[4 * (lane >> 4) + v][(wm * FM + i) * 16 + (lane & 15)], h, m, l);
+              split3_b(As[kk * 16 + 4 * (lane >> 4) + v][(wm * FM + i) * 16 + (lane & 15)], h, m, l);
               ah[i][v] = (short)h;
               am[i][v] = (short)m;
               al[i][v] = (short)l;
@@ -491,7 +476,7 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               unsigned short h, m, l;
-              split3_b(Bs[4 * (lane >> 4) + v][(wn * FN + j) * 16 + (lane & 15)], h, m, l);
+              split3_b(Bs[kk * 16 + 4 * (lane >> 4) + v][(wn * FN + j) * 16 + (lane & 15)], h, m, l);
               bh[j][v] = (short)h;
               bm[j][v] = (short)m;
               bl[j][v] = (short)l;
@@ -509,8 +494,8 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
               c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], c, 0, 0, 0);
               accf[i][j] = c;
             }
+          }
         }
-      }
       }
     }
     double* Cb = C + coff[g];
@@ -757,6 +742,7 @@ static i64 grid_for(Ctx* ctx, K kern, int& per_cu) {
     int nb = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0));
     per_cu = std::max(1, nb);
+    if (const char* e = std::getenv("SB_BOX_WG_PER_CU")) per_cu = std::max(1, std::atoi(e));  // tuning override
   }
   i64 nwg = (i64)ctx->num_cu * per_cu;
   return std::max<i64>(NXCD, nwg / NXCD * NXCD);  // a multiple of the XCD count

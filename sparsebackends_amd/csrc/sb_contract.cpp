@@ -589,7 +589,7 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
         i64 mc = 1, nc = 1;
         for (int a = 0; a < d.nm; ++a) mc *= hb[q * (d.nm + d.nn) + a];
         for (int a = 0; a < d.nn; ++a) nc *= hb[q * (d.nm + d.nn) + d.nm + a];
-        if (mc && nc) cost += (double)((mc + t.BM - 1) / t.BM) * ((nc + t.BN - 1) / t.BN) * (t.BM * t.BN + 512);
+        if (mc && nc) cost += (double)((mc + t.BM - 1) / t.BM) * ((nc + t.BN - 1) / t.BN) * (t.BM * t.BN + 1024);
       }
       if (cost < best) {
         best = cost;
