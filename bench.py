@@ -76,6 +76,9 @@ def main():
     t_gen = time.perf_counter() - t0
     alg = prob.algorithmic_flops()["total"]
     pad = prob.padded_flops()["total"]
+    prob_desc = (("two-site H_eff*psi, Hubbard t=1 U=4, d=4, U(1)xU(1), MPO w=6, chi=2048 in "
+                  f"{prob.Bl.nsec}+{prob.Br.nsec} sectors (2D Gaussian sigma=7)") if a.workload == "hubbard-chi2048" else
+                 "two-site H_eff*psi, Heisenberg J=1, U(1) Sz, MPO w=5, chi=256 in 17 sectors")
     prob.upload().outputs()
     sb.synchronize()
     t0 = time.perf_counter()
@@ -126,6 +129,15 @@ def main():
     roof["kernel"] = dom_name
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload (kept beside the profile summary); used when it covers this run
+    tf = os.path.join(ROOT, "profiles", "box_gemm_heff_hubbard_chi2048.traffic.json")
+    if dom_name == "grouped_gemm" and a.prec == "fp64" and a.tile == 0 and os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        if t.get("workload") == prob_desc:
+            roof["traffic"] = (t["fetch_bytes_per_step"] + t["write_bytes_per_step"]) / t["launches_per_step"]
+            roof["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/)"
     roof["avg_launch_ms"] = avg_ms
 
     out = {
@@ -142,9 +154,7 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32 (f64 storage)", "bf16x3": "bf16x3->f32 (f64 storage)"}[a.prec],
         "data": "synthetic: random psi/L/R on the charge-allowed entries (seed 0), exact model MPO",
         "config": {
-            "workload": ("two-site H_eff*psi, Hubbard t=1 U=4, d=4, U(1)xU(1), MPO w=6, chi=2048 in "
-                         f"{prob.Bl.nsec}+{prob.Br.nsec} sectors (2D Gaussian sigma=7)") if a.workload == "hubbard-chi2048" else
-            "two-site H_eff*psi, Heisenberg J=1, U(1) Sz, MPO w=5, chi=256 in 17 sectors",
+            "workload": prob_desc,
             "format": "sector + in-sector axes, dense tail padded to the largest sector",
             "psi_blocks": int(len(prob.psi_parts["ids"])),
             "algorithmic_gflop_per_step": alg / 1e9,
