@@ -187,6 +187,51 @@ int sb_plan_stats(sb_plan_t plan, double* flops, double* gemm_bytes, double* per
 int sb_plan_operand_bytes(sb_plan_t plan, double* a_bytes, double* b_bytes, double* c_bytes);
 int sb_plan_free(sb_plan_t plan);
 
+/* ------------------------------------------------------------ Krylov vector operations
+ * The storage hooks a Krylov solver reaches through ITensors' ExternalStorage
+ * (SparseBackends/src/tensor_wrappers.jl:792-923), on block-sparse tensors. */
+/* norm(w::WrappedBlockSparse) = norm(data) (tensor_wrappers.jl:793) */
+int sb_bs_norm(sb_ctx_t ctx, sb_tensor_t A, double* out);
+/* copy(::WrappedBlockSparse) (tensor_wrappers.jl:817-823): a new tensor */
+int sb_bs_copy(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t* out);
+/* _external_similar -> _zero_similar (tensor_wrappers.jl:898-904,915-923): same keys and ids, zero data */
+int sb_bs_similar(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t* out);
+/* alpha * wbs (tensor_wrappers.jl:801-807): a new tensor with data alpha .* data */
+int sb_bs_scale(sb_ctx_t ctx, sb_tensor_t A, double alpha, sb_tensor_t* out);
+/* _external_fill! (tensor_wrappers.jl:850-853): fill!(data, v) */
+int sb_bs_fill(sb_ctx_t ctx, sb_tensor_t A, double v);
+/* _external_map_storage! -> _apply_elementwise! (tensor_wrappers.jl:856-862,877-886)
+ * with f(r, a) = alpha*a + beta*r, position by position over the payloads;
+ * SB_E_ERROR when the payload lengths differ (the reference's only check).
+ * Covers scale! (A == R), axpy!, axpby! and copy! (alpha 1, beta 0). */
+int sb_bs_axpby(sb_ctx_t ctx, double alpha, sb_tensor_t A, double beta, sb_tensor_t R);
+/* wA + wB (tensor_wrappers.jl:826-840) and ITensors._add (:842-847): copy of A,
+ * every block of B added by key, missing keys created by _ensure_block! */
+int sb_bs_add(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out);
+/* inner product over blocks of equal key (the scalar ITensors' inner gets
+ * from the P_C = 0 contraction, tensor_wrappers.jl:646-652) */
+int sb_bs_dot(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, double* out);
+
+/* ------------------------------------------------------------ eigensolver
+ * Replaces KrylovKit's eigsolve(A, x0, 1, :SR; ishermitian=true, krylovdim,
+ * maxiter, tol) as two-site DMRG calls it on H_eff (call sites
+ * test_product_dmrg_state.jl:221-226, test_product.jl:235-238; KrylovKit 0.10.2
+ * is not part of the reference's sources).  FP64 Lanczos with full
+ * reorthogonalisation (classical Gram-Schmidt twice) and thick restarts.
+ * The operator is the chain of plans: plans[0] reads x, plans[nplans-1] writes
+ * y, y has x's dims and its blocks are matched to x's by key: blocks of x
+ * absent from y count as zero, blocks of y absent from x are left out (the
+ * solve works in x's block space; for a charge-conserving H_eff those blocks
+ * are structural zeros of the contraction).  The
+ * plans' precision flags select the GEMM form; every vector operation is FP64.
+ * krylovdim in 2..64 vectors per factorisation, maxiter factorisations at
+ * most, converged when the residual norm of the lowest Ritz pair is <= tol.
+ * On return x holds the normalised Ritz vector; ritz (capacity krylovdim, may
+ * be NULL) the Ritz values of the last factorisation, ascending, nritz many. */
+int sb_eigsolve_lanczos(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_tensor_t x, sb_tensor_t y,
+                        int krylovdim, int maxiter, double tol, double* value, double* ritz, int* nritz,
+                        double* normres, int64_t* numops, int* numiter, int* converged);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,16 @@ SIGNATURES = {
     "sb_plan_stats": [P, F64P, F64P, F64P, I64P, I64P],
     "sb_plan_operand_bytes": [P, F64P, F64P, F64P],
     "sb_plan_free": [P],
+    "sb_bs_norm": [P, P, F64P],
+    "sb_bs_copy": [P, P, PP],
+    "sb_bs_similar": [P, P, PP],
+    "sb_bs_scale": [P, P, C.c_double, PP],
+    "sb_bs_fill": [P, P, C.c_double],
+    "sb_bs_axpby": [P, C.c_double, P, C.c_double, P],
+    "sb_bs_add": [P, P, P, PP],
+    "sb_bs_dot": [P, P, P, F64P],
+    "sb_eigsolve_lanczos": [P, PP, C.c_int, P, P, C.c_int, C.c_int, C.c_double, F64P, F64P, I32P, F64P, I64P,
+                            I32P, I32P],
 }
 
 _lib = None

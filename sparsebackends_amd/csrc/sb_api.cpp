@@ -5,7 +5,7 @@
 #include <set>
 
 #include "../../include/sparsebackends_amd.h"
-#include "sb_contract.h"
+#include "sb_krylov.h"
 
 using namespace sb;
 
@@ -557,5 +557,42 @@ int sb_plan_operand_bytes(sb_plan_t p, double* a_bytes, double* b_bytes, double*
   })
 }
 int sb_plan_free(sb_plan_t p) { SB_TRY(delete reinterpret_cast<Plan*>(p)) }
+
+int sb_bs_norm(sb_ctx_t c, sb_tensor_t A, double* out) { SB_TRY(*out = bs_norm(C_(c), *T_(A, BS))) }
+int sb_bs_copy(sb_ctx_t c, sb_tensor_t A, sb_tensor_t* out) { SB_TRY(*out = H_(bs_copy(C_(c), *T_(A, BS)))) }
+int sb_bs_similar(sb_ctx_t c, sb_tensor_t A, sb_tensor_t* out) { SB_TRY(*out = H_(bs_similar(C_(c), *T_(A, BS)))) }
+int sb_bs_scale(sb_ctx_t c, sb_tensor_t A, double alpha, sb_tensor_t* out) {
+  SB_TRY(*out = H_(bs_scale(C_(c), *T_(A, BS), alpha)))
+}
+int sb_bs_fill(sb_ctx_t c, sb_tensor_t A, double v) { SB_TRY(bs_fill(C_(c), *T_(A, BS), v)) }
+int sb_bs_axpby(sb_ctx_t c, double alpha, sb_tensor_t A, double beta, sb_tensor_t R) {
+  SB_TRY(bs_axpby(C_(c), alpha, *T_(A, BS), beta, *T_(R, BS)))
+}
+int sb_bs_add(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out) {
+  SB_TRY(*out = H_(bs_add(C_(c), *T_(A, BS), *T_(B, BS))))
+}
+int sb_bs_dot(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, double* out) {
+  SB_TRY(*out = bs_dot(C_(c), *T_(A, BS), *T_(B, BS)))
+}
+
+int sb_eigsolve_lanczos(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_tensor_t x, sb_tensor_t y, int krylovdim,
+                        int maxiter, double tol, double* value, double* ritz, int* nritz, double* normres,
+                        int64_t* numops, int* numiter, int* converged) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (nplans < 1 || !plans) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve needs at least one plan");
+    std::vector<const Plan*> pl(nplans);
+    for (int i = 0; i < nplans; ++i) pl[i] = reinterpret_cast<const Plan*>(plans[i]);
+    EigResult r = eigsolve_lanczos(ctx, pl, *T_(x, BS), *T_(y, BS), krylovdim, maxiter, tol);
+    if (value) *value = r.value;
+    if (ritz)
+      for (size_t i = 0; i < r.ritz.size(); ++i) ritz[i] = r.ritz[i];
+    if (nritz) *nritz = (int)r.ritz.size();
+    if (normres) *normres = r.normres;
+    if (numops) *numops = r.numops;
+    if (numiter) *numiter = r.numiter;
+    if (converged) *converged = r.converged ? 1 : 0;
+  })
+}
 
 }  // extern "C"

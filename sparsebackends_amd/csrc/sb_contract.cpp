@@ -2404,6 +2404,8 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
   pl->ctx = ctx;
   C.touch();  // C's block list and payload are replaced
   pl->out = &C;
+  pl->in_a = &A;
+  pl->in_b = &B;
   LabelMap mapA = make_map(labelsA), mapB = make_map(labelsB);
   Labels shared, reduced;
   for (auto& l : labelsA)

@@ -48,6 +48,8 @@ struct Plan {
   // last step leaves valid block boxes in it say so in out_box
   Tensor* out = nullptr;
   bool out_box = false;
+  const Tensor* in_a = nullptr;  // the operands the plan reads at every execution
+  const Tensor* in_b = nullptr;
   void finish() const {
     if (!out) return;
     out->touch();

@@ -1,0 +1,40 @@
+// Krylov vector operations on block-sparse tensors and the Lanczos
+// eigensolver over a chain of contraction plans (sb_krylov.cpp).
+#pragma once
+
+#include <vector>
+
+#include "sb_contract.h"
+
+namespace sb {
+
+double bs_norm(Ctx*, const Tensor& A);
+TP bs_copy(Ctx*, const Tensor& A);
+TP bs_similar(Ctx*, const Tensor& A);
+TP bs_scale(Ctx*, const Tensor& A, double alpha);
+void bs_fill(Ctx*, Tensor& A, double v);
+void bs_axpby(Ctx*, double alpha, const Tensor& A, double beta, Tensor& R);
+TP bs_add(Ctx*, const Tensor& A, const Tensor& B);
+double bs_dot(Ctx*, const Tensor& A, const Tensor& B);
+
+// eigenvalues (ascending) and eigenvectors (columns) of a symmetric
+// column-major m x m matrix, cyclic Jacobi
+void sym_eig(int m, const std::vector<double>& A, std::vector<double>& w, std::vector<double>& U);
+
+struct EigResult {
+  double value = 0;           // lowest Ritz value
+  std::vector<double> ritz;   // Ritz values of the last projected matrix, ascending
+  double normres = 0;         // residual norm of the returned Ritz pair
+  i64 numops = 0;             // operator applications
+  int numiter = 0;            // Krylov factorizations (1 + restarts)
+  bool converged = false;
+};
+
+// Lowest eigenpair of the Hermitian operator y = plans(x): the first plan
+// reads x, the last writes y; y's blocks are matched to x's by key and those
+// whose key x lacks are left out (the solve stays in x's block space).
+// On return x holds the normalised Ritz vector.
+EigResult eigsolve_lanczos(Ctx*, const std::vector<const Plan*>& plans, Tensor& x, Tensor& y, int krylovdim,
+                           int maxiter, double tol);
+
+}  // namespace sb

@@ -1,0 +1,221 @@
+// Vector kernels of the Krylov eigensolver and of the Krylov vector
+// operations on block-sparse payloads.  All reductions use a fixed blocking
+// and a fixed tree, so a solve repeats bit for bit on the same inputs.
+#include "sb_kernels.h"
+
+namespace sb {
+
+constexpr int NB = 1024;  // partial sums per reduction
+constexpr int RT = 256;   // threads per block
+constexpr int MCH = 8;    // vectors handled per multi-dot / combine pass
+
+int linalg_nb() { return NB; }
+
+__device__ inline double block_sum(double a, double* s) {
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = RT / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  double r = s[0];
+  __syncthreads();
+  return r;
+}
+
+// part[i*NB + b] = sum over block b's range of V_i * w  (i < k), and, with
+// norm, part[k*NB + b] = sum of w*w.  Block b covers [b*per, (b+1)*per).
+template <int K>
+__global__ __launch_bounds__(RT) void mdot_kernel(const double* __restrict__ V, i64 ldv, int k,
+                                                  const double* __restrict__ w, i64 n, int norm,
+                                                  double* __restrict__ part) {
+  __shared__ double s[RT];
+  const i64 per = (n + NB - 1) / NB;
+  const i64 lo = (i64)blockIdx.x * per, hi = min(n, lo + per);
+  double acc[K + 1];
+#pragma unroll
+  for (int i = 0; i <= K; ++i) acc[i] = 0.0;
+  for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
+    const double we = w[e];
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+      if (i < k) acc[i] = fma(V[(i64)i * ldv + e], we, acc[i]);
+    acc[K] = fma(we, we, acc[K]);
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+    if (i < k) {
+      double t = block_sum(acc[i], s);
+      if (threadIdx.x == 0) part[(i64)i * NB + blockIdx.x] = t;
+    }
+  if (norm) {
+    double t = block_sum(acc[K], s);
+    if (threadIdx.x == 0) part[(i64)k * NB + blockIdx.x] = t;
+  }
+}
+
+// out[i] = sum of the NB partials of row i (one block per row)
+__global__ __launch_bounds__(RT) void reduce_rows_kernel(const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double s[RT];
+  const double* p = part + (i64)blockIdx.x * NB;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < NB; i += RT) a += p[i];
+  double t = block_sum(a, s);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+void k_mdot(Ctx* ctx, const double* V, i64 ldv, int k, const double* w, i64 n, bool norm, double* part, double* out) {
+  // rows: the k dot products in chunks of MCH vectors, then the norm row
+  for (int c = 0; c < k || (c == 0 && norm); c += MCH) {
+    const int kc = std::min(MCH, k - c);
+    const bool last = c + MCH >= k;
+    hipLaunchKernelGGL(mdot_kernel<MCH>, dim3(NB), dim3(RT), 0, ctx->stream, V + (i64)c * ldv, ldv, std::max(kc, 0),
+                       w, n, (norm && last) ? 1 : 0, part + (i64)c * NB);
+    HIP_CHECK(hipGetLastError());
+    if (last) break;
+  }
+  const int rows = k + (norm ? 1 : 0);
+  if (rows) {
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(rows), dim3(RT), 0, ctx->stream, part, out);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// w -= sum_i V_i h[i] (i ascending, one fma each); with norm, the block
+// partials of |w|^2 after the update go to part[b]
+__global__ __launch_bounds__(RT) void orth_update_kernel(const double* __restrict__ V, i64 ldv, int k,
+                                                         const double* __restrict__ h, double* __restrict__ w, i64 n,
+                                                         int norm, double* __restrict__ part) {
+  __shared__ double s[RT];
+  const i64 per = (n + NB - 1) / NB;
+  const i64 lo = (i64)blockIdx.x * per, hi = min(n, lo + per);
+  double a = 0.0;
+  for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
+    double we = w[e];
+    for (int i = 0; i < k; ++i) we = fma(-V[(i64)i * ldv + e], h[i], we);
+    w[e] = we;
+    a = fma(we, we, a);
+  }
+  if (norm) {
+    double t = block_sum(a, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+void k_orth_update(Ctx* ctx, const double* V, i64 ldv, int k, const double* h, double* w, i64 n, double* part,
+                   double* norm_out) {
+  hipLaunchKernelGGL(orth_update_kernel, dim3(NB), dim3(RT), 0, ctx->stream, V, ldv, k, h, w, n,
+                     norm_out ? 1 : 0, part);
+  HIP_CHECK(hipGetLastError());
+  if (norm_out) {
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(RT), 0, ctx->stream, part, norm_out);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// out_j = sum_l V_l U[l + j*kin]  (j < kout, l ascending); out must not overlap V
+__global__ __launch_bounds__(RT) void combine_kernel(const double* __restrict__ V, i64 ldv, int kin,
+                                                     const double* __restrict__ U, int j0, int kc,
+                                                     double* __restrict__ out, i64 ldo, i64 n) {
+  const i64 e = (i64)blockIdx.x * RT + threadIdx.x;
+  if (e >= n) return;
+  double acc[MCH];
+#pragma unroll
+  for (int j = 0; j < MCH; ++j) acc[j] = 0.0;
+  for (int l = 0; l < kin; ++l) {
+    const double v = V[(i64)l * ldv + e];
+#pragma unroll
+    for (int j = 0; j < MCH; ++j)
+      if (j < kc) acc[j] = fma(v, U[l + (i64)(j0 + j) * kin], acc[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < MCH; ++j)
+    if (j < kc) out[(i64)(j0 + j) * ldo + e] = acc[j];
+}
+void k_combine(Ctx* ctx, const double* V, i64 ldv, int kin, const double* U, int kout, double* out, i64 ldo, i64 n) {
+  if (!n) return;
+  for (int j0 = 0; j0 < kout; j0 += MCH) {
+    hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((n + RT - 1) / RT)), dim3(RT), 0, ctx->stream, V, ldv, kin, U,
+                       j0, std::min(MCH, kout - j0), out, ldo, n);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// y = a*x + b*y (b == 0: y = a*x, y not read)
+__global__ void axpby_kernel(i64 n, double a, const double* x, double b, double* y) {  // x may be y
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = b == 0.0 ? a * x[i] : fma(a, x[i], b * y[i]);
+}
+void k_axpby(Ctx* ctx, i64 n, double a, const double* x, double b, double* y) {
+  if (!n) return;
+  hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((n + RT - 1) / RT)), dim3(RT), 0, ctx->stream, n, a, x, b, y);
+  HIP_CHECK(hipGetLastError());
+}
+
+// out block (xids[i]-1) = y's block at offset off[i] (off < 0: zeros); bs values per block
+__global__ void gather_blocks_by_key_kernel(i64 n, i64 bs, const i32* __restrict__ xids, const i64* __restrict__ off,
+                                            const double* __restrict__ y, double* __restrict__ out) {
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * bs) return;
+  const i64 i = t / bs, e = t - i * bs;
+  const i64 o = off[i];
+  out[(i64)(xids[i] - 1) * bs + e] = o >= 0 ? y[o + e] : 0.0;
+}
+void k_gather_blocks_by_key(Ctx* ctx, i64 n, i64 bs, const i32* xids, const i64* off, const double* y, double* out) {
+  if (!n || !bs) return;
+  hipLaunchKernelGGL(gather_blocks_by_key_kernel, dim3((unsigned)((n * bs + RT - 1) / RT)), dim3(RT), 0, ctx->stream,
+                     n, bs, xids, off, y, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+// dst[doff[b] + e] += src[soff[b] + e] for every block b (destinations distinct)
+__global__ void add_blocks_kernel(i64 n, i64 bs, const i64* __restrict__ soff, const double* __restrict__ src,
+                                  const i64* __restrict__ doff, double* __restrict__ dst) {
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * bs) return;
+  const i64 b = t / bs, e = t - b * bs;
+  dst[doff[b] + e] += src[soff[b] + e];
+}
+void k_add_blocks(Ctx* ctx, i64 n, i64 bs, const i64* soff, const double* src, const i64* doff, double* dst) {
+  if (!n || !bs) return;
+  hipLaunchKernelGGL(add_blocks_kernel, dim3((unsigned)((n * bs + RT - 1) / RT)), dim3(RT), 0, ctx->stream, n, bs,
+                     soff, src, doff, dst);
+  HIP_CHECK(hipGetLastError());
+}
+
+// off[i] = (yids[j]-1)*bs for the row j of y whose key equals row i of x
+// (both key lists sorted by the column-major lin), -1 where y has none
+__global__ void match_keys_kernel(const i32* __restrict__ xk, i64 nx, const i32* __restrict__ yk,
+                                  const i32* __restrict__ yids, i64 ny, int P, IArr pd, i64 bs, i64* __restrict__ off) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  auto lin = [&](const i32* k) {
+    u64 l = 0, s = 1;
+    for (int d = 0; d < P; ++d) {
+      l += (u64)(k[d] - 1) * s;
+      s *= (u64)pd.v[d];
+    }
+    return l;
+  };
+  const u64 v = lin(xk + i * P);
+  i64 a = 0, b = ny;
+  while (a < b) {
+    const i64 m = (a + b) >> 1;
+    if (lin(yk + m * P) < v)
+      a = m + 1;
+    else
+      b = m;
+  }
+  off[i] = (a < ny && lin(yk + a * P) == v) ? (i64)(yids[a] - 1) * bs : -1;
+}
+void k_match_keys(Ctx* ctx, const i32* xk, i64 nx, const i32* yk, const i32* yids, i64 ny, int P, const i64* pdims,
+                  i64 bs, i64* off) {
+  if (!nx) return;
+  IArr pd{};
+  for (int d = 0; d < P; ++d) pd.v[d] = pdims[d];
+  hipLaunchKernelGGL(match_keys_kernel, dim3((unsigned)((nx + RT - 1) / RT)), dim3(RT), 0, ctx->stream, xk, nx, yk,
+                     yids, ny, P, pd, bs, off);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace sb

@@ -81,13 +81,17 @@ class SiteModel:
     the charge Qlink[w] of every MPO link state, with W nonzero only where
     q(s') - q(s) = Qlink[wl] - Qlink[wr]."""
 
-    def __init__(self, name, charges, W, Qlink):
+    def __init__(self, name, charges, W, Qlink, link_conj):
         self.name = name
         self.charges = [tuple(c) for c in charges]
         self.W = np.asarray(W, dtype=np.float64)
         self.Qlink = [tuple(q) for q in Qlink]
         self.d = len(charges)
         self.w = self.W.shape[0]
+        # link involution with W[a, :, :, b].T == W[conj[a], :, :, conj[b]]: the
+        # Hamiltonian is symmetric, and environments with L[conj[w]] = L[w].T keep
+        # H_eff symmetric
+        self.link_conj = list(link_conj)
 
 
 def heisenberg_model(J=1.0, Jz=None):
@@ -106,7 +110,7 @@ def heisenberg_model(J=1.0, Jz=None):
     W[4, :, :, 2] = 0.5 * J * Sp
     W[4, :, :, 3] = Jz * Sz
     W[4, :, :, 4] = I
-    return SiteModel("heisenberg", [(1,), (-1,)], W, [(0,), (2,), (-2,), (0,), (0,)])
+    return SiteModel("heisenberg", [(1,), (-1,)], W, [(0,), (2,), (-2,), (0,), (0,)], [0, 2, 1, 3, 4])
 
 
 def j1j2_model(J1=1.0, J2=0.0):
@@ -131,7 +135,8 @@ def j1j2_model(J1=1.0, J2=0.0):
     W[7, :, :, 5] = 0.5 * J2 * Sp
     W[7, :, :, 6] = J2 * Sz
     W[7, :, :, 7] = I
-    return SiteModel("j1j2", [(1,), (-1,)], W, [(0,), (2,), (-2,), (0,), (2,), (-2,), (0,), (0,)])
+    return SiteModel("j1j2", [(1,), (-1,)], W, [(0,), (2,), (-2,), (0,), (2,), (-2,), (0,), (0,)],
+                     [0, 2, 1, 3, 5, 4, 6, 7])
 
 
 def hubbard_model(t=1.0, U=4.0):
@@ -164,7 +169,7 @@ def hubbard_model(t=1.0, U=4.0):
     W[4, :, :, 0] = cd.T
     charges = [(0, 0), (1, 1), (1, -1), (2, 0)]
     Qlink = [(0, 0), (-1, -1), (-1, 1), (1, 1), (1, -1), (0, 0)]
-    return SiteModel("hubbard", charges, W, Qlink)
+    return SiteModel("hubbard", charges, W, Qlink, [0, 3, 4, 1, 2, 5])
 
 
 # ------------------------------------------------------------------ sector layouts of the configs
@@ -201,9 +206,14 @@ def _lab(name):
 class HeffProblem:
     """A two-site H_eff*psi at one bond: left bond sectors Bl, right Br,
     site model m.  Random psi / L / R of the implied block structure (random
-    values on the charge-allowed entries, zero elsewhere and in the padding)."""
+    values on the charge-allowed entries, zero elsewhere and in the padding).
 
-    def __init__(self, model, Bl, Br, seed=0, device=0):
+    hermitian: environments symmetrised so that H_eff is symmetric
+    (L[w] <- (L[w] + L[conj w]^T) / 2 with the model's link involution, same for R).
+    envs: (Lfun, Rfun) giving exact environment blocks instead of random ones:
+    Lfun(ip, i) -> array [w, Bl.dims[ip-1], Bl.dims[i-1]] (sector numbers 1-based)."""
+
+    def __init__(self, model, Bl, Br, seed=0, device=0, hermitian=False, envs=None):
         self.m, self.Bl, self.Br = model, Bl, Br
         self.device = device
         rng = np.random.default_rng(seed)
@@ -241,6 +251,12 @@ class HeffProblem:
         self.R_parts = self._parts([Br.nsec, Br.nsec], (w, Dr, Dr), Rb,
                                    lambda k: (w, Br.dims[k[0] - 1], Br.dims[k[1] - 1]),
                                    lambda k, blk: blk * Rb[k][:, None, None], rng)
+        if envs is not None:
+            self._set_env(self.L_parts, (w, Dl, Dl), envs[0])
+            self._set_env(self.R_parts, (w, Dr, Dr), envs[1])
+        if hermitian:
+            self._hermitize(self.L_parts, (w, Dl, Dl), model.link_conj)
+            self._hermitize(self.R_parts, (w, Dr, Dr), model.link_conj)
         # W [ | wl, s', s, wr] : one dense block
         self.W_parts = dict(dims=(w, d, d, w), N2=4, keys=np.zeros((1, 0), dtype=np.int64),
                             ids=np.array([1]), data=np.asfortranarray(model.W).reshape(-1, order="F"))
@@ -259,6 +275,29 @@ class HeffProblem:
         self.dims_T2 = (nl, nr, d, w, d, Dr, Dl)
         self.dims_T3 = (nl, nr, d, w, d, Dr, Dl)
         self.dims_out = (nl, nr, d, d, Dl, Dr)
+
+    @staticmethod
+    def _block(parts, i, tdims):
+        bs = int(np.prod(tdims))
+        return parts["data"][i * bs:(i + 1) * bs].reshape(tdims, order="F")  # a view
+
+    def _set_env(self, parts, tdims, fun):
+        for i, (a, b) in enumerate(parts["keys"]):
+            blk = self._block(parts, i, tdims)
+            v = np.asarray(fun(int(a), int(b)), dtype=np.float64)
+            blk[...] = 0.0
+            blk[:, :v.shape[1], :v.shape[2]] = v
+
+    def _hermitize(self, parts, tdims, conj):
+        index = {tuple(map(int, k)): i for i, k in enumerate(parts["keys"])}
+        old = parts["data"].copy()
+        src = dict(parts, data=old)
+        for (a, b), i in index.items():
+            j = index[(b, a)]  # present: the charge rule maps to itself under the involution
+            A, B = self._block(src, i, tdims), self._block(src, j, tdims)
+            out = self._block(parts, i, tdims)
+            for x in range(tdims[0]):
+                out[x] = 0.5 * (A[x] + B[conj[x]].T)
 
     @staticmethod
     def _parts(pdims, tdims, blocks, ext, mask, rng):
@@ -394,3 +433,69 @@ def hubbard_chi2048_problem(seed=0, sigma=7.0, device=0):
     Bl = hubbard_bond(2048, n0=0, sigma=sigma)
     Br = hubbard_bond(2048, n0=2, sigma=sigma)
     return HeffProblem(hubbard_model(1.0, 4.0), Bl, Br, seed=seed, device=device)
+
+
+# ------------------------------------------------------------------ exact chains
+
+
+def product_bond(model, nsites, total=None):
+    """The product basis of nsites sites (site 1 most significant, the order of
+    np.kron) grouped into charge sectors.  total=None: a left block, bond charge
+    = the block's charge; else a right block, bond charge = total - the block's
+    charge.  Returns (Bond, sector of each state, position in its sector)."""
+    d = model.d
+    n = d ** nsites
+    q = []
+    for idx in range(n):
+        digits = np.unravel_index(idx, (d,) * nsites) if nsites else ()
+        c = tuple(0 for _ in model.charges[0])
+        for s_ in digits:
+            c = qadd(c, model.charges[s_])
+        q.append(c if total is None else qsub(tuple(total), c))
+    charges = sorted(set(q))
+    sec = {c: i for i, c in enumerate(charges)}
+    pos, counts = [], [0] * len(charges)
+    for c in q:
+        pos.append(counts[sec[c]])
+        counts[sec[c]] += 1
+    return Bond(charges, counts), [sec[c] for c in q], pos
+
+
+def block_mpo_left(model, nsites):
+    """M[w] = sum W[start, :, :, a1] x W[a1, :, :, a2] x ... x W[., :, :, w]
+    over the product basis of the block (start = last link state)"""
+    W = model.W
+    M = [W[model.w - 1, :, :, a] for a in range(model.w)]
+    for _ in range(nsites - 1):
+        M = [sum(np.kron(M[a], W[a, :, :, b]) for a in range(model.w)) for b in range(model.w)]
+    return M
+
+
+def block_mpo_right(model, nsites):
+    """N[w] = sum W[w, :, :, a] x ... x W[., :, :, done] (done = link state 0)"""
+    W = model.W
+    N = [W[a, :, :, 0] for a in range(model.w)]
+    for _ in range(nsites - 1):
+        N = [sum(np.kron(W[b, :, :, a], N[a]) for a in range(model.w)) for b in range(model.w)]
+    return N
+
+
+def chain_problem(model, nl, nr, total, seed=0, device=0):
+    """H_eff of sites nl+1, nl+2 of an (nl+2+nr)-site open chain whose left and
+    right bonds are the exact product bases of the outer sites: H_eff is then
+    the whole chain's Hamiltonian in the sector of total charge `total`."""
+    Bl, secl, posl = product_bond(model, nl)
+    Br, secr, posr = product_bond(model, nr, total)
+    Ml, Nr = block_mpo_left(model, nl), block_mpo_right(model, nr)
+    st_l = [[i for i in range(len(secl)) if secl[i] == k] for k in range(Bl.nsec)]
+    st_r = [[i for i in range(len(secr)) if secr[i] == k] for k in range(Br.nsec)]
+
+    def Lfun(a, b):
+        ra, rb = st_l[a - 1], st_l[b - 1]
+        return np.stack([Ml[x][np.ix_(ra, rb)] for x in range(model.w)])
+
+    def Rfun(a, b):
+        ra, rb = st_r[a - 1], st_r[b - 1]
+        return np.stack([Nr[x][np.ix_(ra, rb)] for x in range(model.w)])
+
+    return HeffProblem(model, Bl, Br, seed=seed, device=device, envs=(Lfun, Rfun))

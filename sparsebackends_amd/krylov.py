@@ -1,0 +1,71 @@
+"""eigsolve for two-site DMRG: the lowest eigenpair of H_eff given as a chain
+of contraction plans, solved by the library's FP64 Lanczos.
+
+The call shape follows KrylovKit's eigsolve(A, x0, howmany, which; ishermitian,
+krylovdim, maxiter, tol) as ITensorMPS-based DMRG uses it on H_eff
+(test_product_dmrg_state.jl:221-226 with krylovdim=3, maxiter=1, tol=1e-14;
+test_product.jl:235-238 with krylovdim=8, tol=1e-12).  Defaults are KrylovKit's
+(krylovdim=30, maxiter=100, tol=1e-12).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._abi import check, load
+
+
+@dataclass
+class ConvergenceInfo:
+    """converged: 1 if the lowest Ritz pair met tol; normres: its residual norm;
+    numiter: Krylov factorisations used; numops: operator applications;
+    ritz: Ritz values of the last factorisation, ascending."""
+    converged: int
+    normres: float
+    numiter: int
+    numops: int
+    ritz: np.ndarray = field(repr=False)
+
+
+def eigsolve(plans, x0, howmany=1, which="SR", *, y=None, ishermitian=True, krylovdim=30, maxiter=100, tol=1e-12):
+    """Lowest eigenpair of the Hermitian operator y = plans(x0).
+
+    plans: Plan objects applied in order; the first reads x0, the last writes
+    y (by default the last plan's output).  x0 (a NewBlockSparseSorted) is the
+    start vector and is overwritten with the normalised Ritz vector.
+    Returns (vals, vecs, info) like KrylovKit: vals = [lowest Ritz value],
+    vecs = [x0], info = ConvergenceInfo."""
+    if howmany != 1 or which != "SR" or not ishermitian:
+        raise ValueError("eigsolve supports howmany=1, which='SR', ishermitian=True (the DMRG use)")
+    plans = list(plans)
+    if y is None:
+        y = plans[-1]._ops[0]
+    k = max(int(krylovdim), 2)
+    arr = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
+    val, nres = C.c_double(), C.c_double()
+    ritz = np.zeros(k, dtype=np.float64)
+    nritz, numiter, conv = C.c_int(), C.c_int(), C.c_int()
+    numops = C.c_int64()
+    check(load().sb_eigsolve_lanczos(x0.ctx, C.cast(arr, C.POINTER(C.c_void_p)), len(plans), x0.handle, y.handle, k,
+                                     int(maxiter), float(tol), C.byref(val),
+                                     ritz.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nritz), C.byref(nres),
+                                     C.byref(numops), C.byref(numiter), C.byref(conv)))
+    info = ConvergenceInfo(conv.value, nres.value, numiter.value, numops.value, ritz[:nritz.value].copy())
+    return [val.value], [x0], info
+
+
+def eigsolve_refined(plans_lo, plans_hi, x_lo, x_hi, *, krylovdim=30, maxiter=100, tol_lo=1e-7, tol=1e-12,
+                     polish_krylovdim=8, polish_maxiter=10):
+    """Mixed-precision solve: Lanczos on the low-precision operator plans_lo
+    (FP32 or bf16x3 block products) down to tol_lo, then FP64 Lanczos on
+    plans_hi from that Ritz vector down to tol.  The energy of the second
+    stage is a Rayleigh-Ritz value of the FP64 operator, so its error is
+    quadratic in the first stage's eigenvector error.  x_lo and x_hi are the
+    vectors the two chains read (same block structure); x_hi ends with the
+    Ritz vector.  Returns (vals, vecs, info_lo, info_hi)."""
+    _, _, info_lo = eigsolve(plans_lo, x_lo, krylovdim=krylovdim, maxiter=maxiter, tol=tol_lo)
+    x_hi.axpby_(1.0, x_lo, 0.0)
+    vals, vecs, info_hi = eigsolve(plans_hi, x_hi, krylovdim=polish_krylovdim, maxiter=polish_maxiter, tol=tol)
+    return vals, vecs, info_lo, info_hi

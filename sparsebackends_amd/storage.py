@@ -240,6 +240,54 @@ class NewBlockSparseSorted(_Tensor):
         a = _i64(I)
         check(load().sb_bs_setindex(self.ctx, self._h, float(v), _ptr(a, C.c_int64)))
 
+    # ---- Krylov vector operations (SparseBackends/src/tensor_wrappers.jl:792-923)
+    def norm(self):
+        """norm(data) (tensor_wrappers.jl:793)."""
+        v = C.c_double()
+        check(load().sb_bs_norm(self.ctx, self._h, C.byref(v)))
+        return v.value
+
+    def copy(self):
+        """copy: keys, ids and data (tensor_wrappers.jl:817-823)."""
+        out = C.c_void_p()
+        check(load().sb_bs_copy(self.ctx, self._h, C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
+    def similar(self):
+        """_zero_similar: same keys and ids, zero data (tensor_wrappers.jl:898-904)."""
+        out = C.c_void_p()
+        check(load().sb_bs_similar(self.ctx, self._h, C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
+    def __rmul__(self, alpha):
+        """alpha * A: a new tensor (tensor_wrappers.jl:801-807)."""
+        out = C.c_void_p()
+        check(load().sb_bs_scale(self.ctx, self._h, float(alpha), C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
+    def fill_(self, v):
+        """fill!(data, v) (tensor_wrappers.jl:850-853)."""
+        check(load().sb_bs_fill(self.ctx, self._h, float(v)))
+        return self
+
+    def axpby_(self, alpha, A, beta):
+        """self.data = alpha*A.data + beta*self.data position by position
+        (_apply_elementwise!, tensor_wrappers.jl:856-862)."""
+        check(load().sb_bs_axpby(self.ctx, float(alpha), A.handle, float(beta), self._h))
+        return self
+
+    def __add__(self, other):
+        """block-wise addition over the union of keys (tensor_wrappers.jl:826-840)."""
+        out = C.c_void_p()
+        check(load().sb_bs_add(self.ctx, self._h, other.handle, C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
+    def dot(self, other):
+        """inner product over blocks of equal key."""
+        v = C.c_double()
+        check(load().sb_bs_dot(self.ctx, self._h, other.handle, C.byref(v)))
+        return v.value
+
     def sort_(self):
         """sort! (storage.jl:403-410): lexicographic keys, ids follow, data stays."""
         check(load().sb_bs_sort_lex(self.ctx, self._h))
