@@ -8,15 +8,21 @@ H_eff*psi = four contract! calls on NewBlockSparseSorted tensors
 charge-allowed entries, seed 0; the MPO is the Hubbard MPO.
 
 A step is one H_eff*psi: every step recomputes all four contractions
-(operand permutes + grouped block products) from the inputs resident in
-HBM.  The block structure is planned once (sb_plan_create), as a Krylov
-solve keeps it fixed; the planning time is reported separately.
+(box-aware grouped block products, no operand permutes) from the inputs
+resident in HBM.  The block structure is planned once (sb_plan_create), as a
+Krylov solve keeps it fixed; the planning time is reported separately.
+
+The environments are symmetrised so that H_eff is symmetric; "eigsolve" then
+reports the ground energy by FP64 Lanczos, the mixed-precision solve's
+energy difference and time-to-energy against it, and solves/s with p50/p99
+latency of the DMRG-step solve (krylovdim 3, maxiter 1).
 
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
-                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--calibrate-copy MB]
+                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--no-eig]
+                       [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--calibrate-copy MB]
 """
 from __future__ import annotations
 
@@ -47,6 +53,12 @@ def parse():
                     help="force tile shape 1+c of the block products (0: chosen at plan time)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-eig", action="store_true", help="skip the eigensolver measurements")
+    ap.add_argument("--krylovdim", type=int, default=30)
+    ap.add_argument("--eig-tol", type=float, default=1e-8, help="residual norm of the FP64 solves")
+    ap.add_argument("--eig-tol-lo", type=float, default=1e-6,
+                    help="residual norm of the low-precision stage, relative to |E|")
+    ap.add_argument("--solves", type=int, default=30, help="DMRG-step solves timed for solves/s and p99")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -54,12 +66,57 @@ def parse():
 
 def make_problem(cfg):
     from sparsebackends_amd import heff as H
-    return H.hubbard_chi2048_problem(0) if cfg == "hubbard-chi2048" else H.heisenberg_chi256_problem(0)
+    return H.hubbard_chi2048_problem(0, hermitian=True) if cfg == "hubbard-chi2048" else \
+        H.heisenberg_chi256_problem(0, hermitian=True)
 
 
 def rel_err(x, y):
     n = np.linalg.norm(y)
     return float(np.linalg.norm(x - y) / (n if n else 1.0))
+
+
+def eig_section(sb, a, prob, plans):
+    """Ground state of the (symmetric) H_eff: FP64 Lanczos against the
+    mixed-precision solve (FP32 or bf16x3 block products, then an FP64
+    second stage from its Ritz vector), and the DMRG-step solve (krylovdim 3,
+    maxiter 1, the reference DMRG's call) as solves/s with its latency."""
+    x0 = prob.psi_parts["data"]
+    res = {}
+
+    def timed(fn):
+        sb.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        sb.synchronize()
+        return r, time.perf_counter() - t0
+
+    prob.psi.set_data_(x0)
+    x0d = prob.psi.copy()  # the start vector, kept on the device
+    (vals, _, info), t64 = timed(lambda: sb.eigsolve(plans, prob.psi, krylovdim=a.krylovdim, maxiter=50, tol=a.eig_tol))
+    e64 = vals[0]
+    res["tol"] = a.eig_tol
+    res["krylovdim"] = a.krylovdim
+    res["fp64"] = dict(energy=e64, seconds=t64, numops=info.numops, converged=info.converged, normres=info.normres)
+    for name, fl in (("fp32", sb.PREC_FP32), ("bf16x3", sb.PREC_BF16X3)):
+        pm = copy.copy(prob).upload().outputs()
+        plans_lo = pm.make_plans(fl)
+        pm.psi.axpby_(1.0, x0d, 0.0)
+        tol_lo = a.eig_tol_lo * max(1.0, abs(e64))
+        (r, t) = timed(lambda: sb.eigsolve_refined(plans_lo, plans, pm.psi, prob.psi, krylovdim=a.krylovdim,
+                                                   maxiter=50, tol_lo=tol_lo, tol=a.eig_tol))
+        vals, _, ilo, ihi = r
+        res[name] = dict(energy=vals[0], abs_dE_vs_fp64=abs(vals[0] - e64), seconds=t, time_to_energy_vs_fp64=t64 / t,
+                         numops_low=ilo.numops, numops_fp64=ihi.numops, converged=ihi.converged, tol_low=tol_lo)
+        del pm, plans_lo
+    lat = []
+    for _ in range(a.solves):
+        prob.psi.axpby_(1.0, x0d, 0.0)
+        _, t = timed(lambda: sb.eigsolve(plans, prob.psi, krylovdim=3, maxiter=1, tol=1e-14))
+        lat.append(t)
+    lat = np.array(lat)
+    res["dmrg_step_solve"] = dict(krylovdim=3, maxiter=1, solves=a.solves, solves_per_s=len(lat) / lat.sum(),
+                                  p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3))
+    return res
 
 
 def main():
@@ -152,7 +209,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 (f64 storage)", "bf16x3": "bf16x3->f32 (f64 storage)"}[a.prec],
-        "data": "synthetic: random psi/L/R on the charge-allowed entries (seed 0), exact model MPO",
+        "data": "synthetic: random psi and environments on the charge-allowed entries (seed 0), environments "
+                "symmetrised with the MPO link involution (H_eff symmetric), exact model MPO",
         "config": {
             "workload": prob_desc,
             "format": "sector + in-sector axes, dense tail padded to the largest sector",
@@ -208,6 +266,10 @@ def main():
         else:
             cpu["gpu_vs_cpu_rel_err"] = "block lists differ"
         out["cpu_baseline"] = cpu
+
+    if not a.no_eig and a.prec == "fp64":
+        # last: the solves overwrite psi and Hpsi
+        out["eigsolve"] = eig_section(sb, a, prob, plans)
 
     if a.calibrate_copy > 0:
         n = int(a.calibrate_copy * 1e6 / 8)

@@ -323,8 +323,8 @@ EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tens
   auto vec = [&](int i) { return V.get() + (i64)i * n; };
 
   // y's blocks in x's payload layout
+  // y holds the block list the last plan fixed when it was created
   DBuf<i64> off;
-  for (auto* p : plans) p->run();  // the structure of y is the plans'; one run makes it current
   match_blocks(ctx, x, y, off);  // blocks of y whose key x lacks are left out: the solve stays in x's block space
 
   EigResult R;

@@ -222,28 +222,20 @@ class HeffProblem:
         self.Dl, self.Dr = Dl, Dr
         qs, Ql = model.charges, model.Qlink
         # psi [Ql, Qr | s1, s2, l, r] : q(r) = q(l) + q(s1) + q(s2)
-        psi = {}
-        for il, ql in enumerate(Bl.charges):
-            for ir, qr in enumerate(Br.charges):
-                m = np.zeros((d, d))
-                for s1 in range(d):
-                    for s2 in range(d):
-                        m[s1, s2] = qadd(qadd(ql, qs[s1]), qs[s2]) == qr
-                if m.any():
-                    psi[(il + 1, ir + 1)] = m
+        cl, cr, cs = np.array(Bl.charges), np.array(Br.charges), np.array(qs)
+        allowed = np.all(cl[:, None, None, None, :] + cs[None, None, :, None, :] + cs[None, None, None, :, :]
+                         == cr[None, :, None, None, :], axis=-1)  # [il, ir, s1, s2]
+        psi = {(int(il) + 1, int(ir) + 1): allowed[il, ir].astype(np.float64)
+               for il, ir in zip(*np.nonzero(allowed.any(axis=(2, 3))))}
         self.psi_parts = self._parts([Bl.nsec, Br.nsec], (d, d, Dl, Dr), psi,
                                      lambda k: (d, d, Bl.dims[k[0] - 1], Br.dims[k[1] - 1]),
                                      lambda k, blk: blk * psi[k][:, :, None, None], rng)
 
         # environments: X[w, x', x] nonzero where q(x') - q(x) = -Qlink(w)
         def env(B):
-            blocks = {}
-            for ip, qp in enumerate(B.charges):
-                for i, q in enumerate(B.charges):
-                    m = np.array([qsub(qp, q) == tuple(-c for c in Ql[x]) for x in range(w)], dtype=np.float64)
-                    if m.any():
-                        blocks[(ip + 1, i + 1)] = m
-            return blocks
+            c, cw = np.array(B.charges), np.array(Ql)
+            m = np.all(c[:, None, None, :] - c[None, :, None, :] == -cw[None, None, :, :], axis=-1)  # [ip, i, w]
+            return {(int(ip) + 1, int(i) + 1): m[ip, i].astype(np.float64) for ip, i in zip(*np.nonzero(m.any(axis=2)))}
         Lb, Rb = env(Bl), env(Br)
         self.L_parts = self._parts([Bl.nsec, Bl.nsec], (w, Dl, Dl), Lb,
                                    lambda k: (w, Bl.dims[k[0] - 1], Bl.dims[k[1] - 1]),
@@ -295,9 +287,7 @@ class HeffProblem:
         for (a, b), i in index.items():
             j = index[(b, a)]  # present: the charge rule maps to itself under the involution
             A, B = self._block(src, i, tdims), self._block(src, j, tdims)
-            out = self._block(parts, i, tdims)
-            for x in range(tdims[0]):
-                out[x] = 0.5 * (A[x] + B[conj[x]].T)
+            self._block(parts, i, tdims)[...] = 0.5 * (A + B[conj].swapaxes(1, 2))
 
     @staticmethod
     def _parts(pdims, tdims, blocks, ext, mask, rng):
@@ -422,17 +412,17 @@ class HeffProblem:
         return dict(total=sum(steps), steps=steps)
 
 
-def heisenberg_chi256_problem(seed=0, device=0):
+def heisenberg_chi256_problem(seed=0, device=0, hermitian=False):
     """Heisenberg S=1/2, J=1, chi=256 in Sz sectors q in {-8..8}, dims ~ exp(-q^2/32)."""
     B = heisenberg_bond(256)
-    return HeffProblem(heisenberg_model(1.0), B, B, seed=seed, device=device)
+    return HeffProblem(heisenberg_model(1.0), B, B, seed=seed, device=device, hermitian=hermitian)
 
 
-def hubbard_chi2048_problem(seed=0, sigma=7.0, device=0):
+def hubbard_chi2048_problem(seed=0, sigma=7.0, device=0, hermitian=False):
     """Hubbard t=1, U=4, chi=2048 in (n, 2Sz) sectors with a 2D Gaussian law of width sigma."""
     Bl = hubbard_bond(2048, n0=0, sigma=sigma)
     Br = hubbard_bond(2048, n0=2, sigma=sigma)
-    return HeffProblem(hubbard_model(1.0, 4.0), Bl, Br, seed=seed, device=device)
+    return HeffProblem(hubbard_model(1.0, 4.0), Bl, Br, seed=seed, device=device, hermitian=hermitian)
 
 
 # ------------------------------------------------------------------ exact chains
