@@ -232,6 +232,21 @@ int sb_eigsolve_lanczos(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_ten
                         int krylovdim, int maxiter, double tol, double* value, double* ritz, int* nritz,
                         double* normres, int64_t* numops, int* numiter, int* converged);
 
+/* ------------------------------------------------------------ two-site SVD
+ * blocksparse_svd (SparseBackends/src/ops_factorize.jl:50-148; ITensor-level
+ * wrapper itensor_blocksparse_svd, tensor_wrappers.jl:975-1005) of a
+ * NewBlockSparseSorted{T,4,2,2} A with prefix (L, R) and dense tail (s1, s2):
+ * one SVD per L value of its d_s1 x (n_R * d_s2) block row, the singular
+ * values sorted descending across all L, the kept count min(maxdim, all)
+ * lowered while above mindim and below cutoff * max (maxdim < 0: no limit).
+ * U: dims (n_L, k, d_s1), keys (L, nb); SV: dims (k, n_R, d_s2), keys (nb, R);
+ * the singular value goes into SV (ortho_right = 0, "left") or U (1, "right").
+ * svs receives min(svs_cap, k) kept singular values (may be NULL); truncerr =
+ * sum of the discarded singular values squared.  d_s1 at most 32.  k = 0
+ * (nothing kept) fails like the reference's constructor (dims .>= 1). */
+int sb_bs_svd(sb_ctx_t ctx, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_t mindim, double cutoff,
+              sb_tensor_t* U, sb_tensor_t* SV, double* svs, int64_t svs_cap, int64_t* newdim, double* truncerr);
+
 #ifdef __cplusplus
 }
 #endif

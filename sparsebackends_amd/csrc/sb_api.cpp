@@ -5,6 +5,7 @@
 #include <set>
 
 #include "../../include/sparsebackends_amd.h"
+#include "sb_factorize.h"
 #include "sb_krylov.h"
 
 using namespace sb;
@@ -592,6 +593,20 @@ int sb_eigsolve_lanczos(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_tenso
     if (numops) *numops = r.numops;
     if (numiter) *numiter = r.numiter;
     if (converged) *converged = r.converged ? 1 : 0;
+  })
+}
+
+int sb_bs_svd(sb_ctx_t c, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_t mindim, double cutoff,
+              sb_tensor_t* U, sb_tensor_t* SV, double* svs, int64_t svs_cap, int64_t* newdim, double* truncerr) {
+  SB_TRY({
+    if (!U || !SV) fail(SB_E_ARGUMENT, "ArgumentError: null output handle");
+    SvdResult r = bs_svd(C_(c), *T_(A, BS), ortho_right != 0, maxdim, mindim, cutoff);
+    if (svs)
+      for (int64_t i = 0; i < std::min<int64_t>(svs_cap, (int64_t)r.svs.size()); ++i) svs[i] = r.svs[i];
+    if (newdim) *newdim = (int64_t)r.svs.size();
+    if (truncerr) *truncerr = r.truncerr;
+    *U = H_(std::move(r.U));
+    *SV = H_(std::move(r.SV));
   })
 }
 

@@ -1,0 +1,119 @@
+"""Two-site DMRG on symmetry-sector block-sparse tensors, driven through the
+library's C ABI: H_eff*psi as four contract! plans, the Lanczos eigensolver,
+the sector SVD and the environment updates makeL!/makeR! as three contract!
+calls each (ITensorMPS's L = L * psi[j] * H[j] * dag(prime(psi[j])), the
+order the reference's tests decompose at test_product_dmrg_state.jl:313-378).
+
+Tensors use the sector format of heff.py: every bond index is a sector axis
+in the key prefix plus an in-sector axis in the dense tail, padded to the
+bond's largest sector; bond charges accumulate from the left
+(q(right bond) = q(left bond) + q(s)).
+
+    site A_j  [Ql, Qr | s, l, r]
+    L_j       [Ql', Ql | w, l', l]      (left environment of site j+1)
+    R_j       [Qr', Qr | w, r', r]      (right environment of site j-1)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .storage import NewBlockSparseSorted
+from .tensoralgebra import Plan, contract_
+
+
+def lab(name, plev=0):
+    """a contraction label (id, plev) from a short name (at most 8 bytes)"""
+    return (int.from_bytes(("" + name).encode()[:8].ljust(8, b"\0"), "little") | (1 << 63), plev)
+
+
+# labels of the environment updates
+_A = [lab("Ql"), lab("Qr"), lab("s"), lab("l"), lab("r")]
+_Ap = [lab("Ql", 1), lab("Qr", 1), lab("s", 1), lab("l", 1), lab("r", 1)]  # the bra copy (primed)
+
+
+def make_L(L, A, W, flags=0):
+    """L_new[Qr', Qr | wr, r', r] = sum L[Ql', Ql | wl, l', l] A[Ql, Qr | s, l, r]
+    W[wl, s', s, wr] A[Ql', Qr' | s', l', r'] as three contract! calls:
+    X1 = L*A, X2 = X1*W, L_new = X2*A' (makeL!'s order).  Returns (L_new, the
+    three plans, the intermediates)."""
+    lL = [lab("Ql", 1), lab("Ql"), lab("wl"), lab("l", 1), lab("l")]
+    lW = [lab("wl"), lab("s", 1), lab("s"), lab("wr")]
+    dL, dA, dW = L.dims, A.dims, W.dims
+    l1 = [lab("Ql", 1), lab("Qr"), lab("wl"), lab("l", 1), lab("s"), lab("r")]
+    X1 = NewBlockSparseSorted.new((dL[0], dA[1], dL[2], dL[3], dA[2], dA[4]), 4, device=A.device)
+    p1 = Plan(X1, l1, L, lL, A, _A, flags)
+    l2 = [lab("Ql", 1), lab("Qr"), lab("l", 1), lab("r"), lab("s", 1), lab("wr")]
+    X2 = NewBlockSparseSorted.new((dL[0], dA[1], dL[3], dA[4], dW[1], dW[3]), 4, device=A.device)
+    p2 = Plan(X2, l2, X1, l1, W, lW, flags)
+    # the bra index comes from the second operand and sits between the first
+    # operand's w and ket index in the environment layout: contract to
+    # [r', wr, r] (second operand's group first) and permute the tail
+    l3 = [lab("Qr", 1), lab("Qr"), lab("r", 1), lab("wr"), lab("r")]
+    X3 = NewBlockSparseSorted.new((dA[1], dA[1], dA[4], dW[3], dA[4]), 3, device=A.device)
+    p3 = Plan(X3, l3, X2, l2, A, _Ap, flags)
+    return X3.permutedims([1, 2, 4, 3, 5]), [p1, p2, p3], (X1, X2, X3)
+
+
+def make_R(R, B, W, flags=0):
+    """R_new[Ql', Ql | wl, l', l] = sum R[Qr', Qr | wr, r', r] B[Ql, Qr | s, l, r]
+    W[wl, s', s, wr] B[Ql', Qr' | s', l', r'] (makeR!'s order: R*B, *W, *B')."""
+    lR = [lab("Qr", 1), lab("Qr"), lab("wr"), lab("r", 1), lab("r")]
+    lW = [lab("wl"), lab("s", 1), lab("s"), lab("wr")]
+    dR, dB, dW = R.dims, B.dims, W.dims
+    l1 = [lab("Qr", 1), lab("Ql"), lab("wr"), lab("r", 1), lab("s"), lab("l")]
+    X1 = NewBlockSparseSorted.new((dR[0], dB[0], dR[2], dR[3], dB[2], dB[3]), 4, device=B.device)
+    p1 = Plan(X1, l1, R, lR, B, _A, flags)
+    l2 = [lab("Qr", 1), lab("Ql"), lab("r", 1), lab("l"), lab("s", 1), lab("wl")]
+    X2 = NewBlockSparseSorted.new((dR[0], dB[0], dR[3], dB[3], dW[1], dW[0]), 4, device=B.device)
+    p2 = Plan(X2, l2, X1, l1, W, lW, flags)
+    l3 = [lab("Ql", 1), lab("Ql"), lab("l", 1), lab("wl"), lab("l")]
+    X3 = NewBlockSparseSorted.new((dB[0], dB[0], dB[3], dW[0], dB[3]), 3, device=B.device)
+    p3 = Plan(X3, l3, X2, l2, B, _Ap, flags)
+    return X3.permutedims([1, 2, 4, 3, 5]), [p1, p2, p3], (X1, X2, X3)
+
+
+# ---------------------------------------------------------------- host-side parts
+
+
+def site_parts(model, Bl, Br, rng=None, fill=None):
+    """A [Ql, Qr | s, l, r] host parts: blocks where q(r) = q(l) + q(s) for some s;
+    values random (rng) or from fill(ql_index, qr_index, s) -> (dl, dr) array,
+    zero on the charge-forbidden s and in the padding"""
+    d = model.d
+    cl, cr, cs = np.array(Bl.charges), np.array(Br.charges), np.array(model.charges)
+    ok = np.all(cl[:, None, None, :] + cs[None, None, :, :] == cr[None, :, None, :], axis=-1)  # [il, ir, s]
+    keys = [(int(il) + 1, int(ir) + 1) for ir in range(Br.nsec) for il in range(Bl.nsec) if ok[il, ir].any()]
+    tdims = (d, Bl.dmax, Br.dmax)
+    bs = int(np.prod(tdims))
+    data = np.zeros(len(keys) * bs)
+    for i, (a, b) in enumerate(keys):
+        blk = np.zeros(tdims, order="F")
+        dl, dr = Bl.dims[a - 1], Br.dims[b - 1]
+        for s_ in range(d):
+            if ok[a - 1, b - 1, s_]:
+                blk[s_, :dl, :dr] = fill(a, b, s_) if fill is not None else rng.standard_normal((dl, dr))
+        data[i * bs:(i + 1) * bs] = blk.reshape(-1, order="F")
+    return dict(dims=(Bl.nsec, Br.nsec) + tdims, N2=3, keys=np.array(keys, dtype=np.int64).reshape(-1, 2),
+                ids=np.arange(1, len(keys) + 1), data=data)
+
+
+def env_parts(model, B, rng):
+    """random environment [Q', Q | w, x', x] on the blocks the charge rule allows
+    (q(x') - q(x) = -Qlink(w)), zero elsewhere"""
+    from .heff import HeffProblem
+    c, cw = np.array(B.charges), np.array(model.Qlink)
+    m = np.all(c[:, None, None, :] - c[None, :, None, :] == -cw[None, None, :, :], axis=-1)
+    blocks = {(int(ip) + 1, int(i) + 1): m[ip, i].astype(np.float64) for ip, i in zip(*np.nonzero(m.any(axis=2)))}
+    w = model.w
+    return HeffProblem._parts([B.nsec, B.nsec], (w, B.dmax, B.dmax), blocks,
+                              lambda k: (w, B.dims[k[0] - 1], B.dims[k[1] - 1]),
+                              lambda k, blk: blk * blocks[k][:, None, None], rng)
+
+
+def mpo_parts(model):
+    return dict(dims=model.W.shape, N2=4, keys=np.zeros((1, 0), dtype=np.int64), ids=np.array([1]),
+                data=np.asfortranarray(model.W).reshape(-1, order="F"))
+
+
+def upload(p, device=0):
+    return NewBlockSparseSorted.from_parts(p["dims"], p["N2"], p["keys"], p["ids"], p["data"], device=device)
