@@ -247,6 +247,24 @@ int sb_eigsolve_lanczos(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_ten
 int sb_bs_svd(sb_ctx_t ctx, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_t mindim, double cutoff,
               sb_tensor_t* U, sb_tensor_t* SV, double* svs, int64_t svs_cap, int64_t* newdim, double* truncerr);
 
+/* New (the sector-format counterpart of blocksparse_svd; the reference holds
+ * psi only element-sparse): block-diagonal SVD of a two-site tensor
+ * psi [Ql, Qr | s1, s2, l, r] (N = 6, N2 = 4; l, r padded in-sector axes) by
+ * fused charge sector.  rowsec[il + nl*s1] (colsec[ir + nr*s2]) is the sector,
+ * 0..nsec-1 or -1, of left sector il with site state s1 (right sector ir with
+ * s2); dl[nl] and dr[nr] the actual extents of the bond sectors.  Sector m's
+ * matrix has rows (Ql, s1, l) and columns (Qr, s2, r), both ascending; its
+ * SVD runs in one workgroup (one-sided Jacobi in LDS; a sector matrix must
+ * fit 160 KB).  Truncation as sb_bs_svd.  New bond: the sectors with kept
+ * values in ascending order, padded to the largest kept count Dm.
+ * A [Ql, Qm | s1, l, m] and B [Qm, Qr | s2, m, r] (ids in key order); the
+ * singular values go into B (ortho_right = 0) or A (1).  kept[nsec]: kept
+ * count per sector; svs: min(svs_cap, total) kept values, sector by sector. */
+int sb_bs_svd_sectors(sb_ctx_t ctx, sb_tensor_t psi, int64_t nsec, const int64_t* rowsec, const int64_t* colsec,
+                      const int64_t* dl, const int64_t* dr, int ortho_right, int64_t maxdim, int64_t mindim,
+                      double cutoff, sb_tensor_t* A, sb_tensor_t* B, int64_t* kept, double* svs, int64_t svs_cap,
+                      int64_t* newdim, double* truncerr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,12 @@ exact reconstruction without truncation, orthonormal U within each group,
 the per-group singular values of numpy.linalg.svd, the truncation rules on
 hand-built spectra, and a literal key/id case worked by hand from the loop
 order of the reference.
+
+svd_sectors: the same factorisation for the sector format (psi
+[Ql, Qr | s1, s2, l, r], blocks by fused charge), which the reference does not
+have; pinned (tests/test_sector_svd.py) by exact reconstruction over the full
+bonds, isometric factors, and its singular values equalling numpy's SVD of the
+whole (s1, l) x (s2, r) matrix.
 """
 from __future__ import annotations
 
@@ -99,3 +105,103 @@ def reconstruct(Up, SVp):
     for (nb, R), i in zip(SVp["keys"], SVp["ids"]):
         S[nb - 1, R - 1] = SVp["data"][(i - 1) * d2:i * d2]
     return np.einsum("lka,krb->lrab", U, S)
+
+
+def svd_sectors(parts, rowsec, colsec, dl, dr, nsec, ortho="left", maxdim=None, mindim=1, cutoff=0.0):
+    """Block-diagonal SVD of psi [Ql, Qr | s1, s2, l, r] by fused sector (the
+    sector-format counterpart of blocksparse_svd, with its truncation rules):
+    sector m's matrix has rows (Ql, s1, l) and columns (Qr, s2, r) ascending;
+    numpy's svd per sector; singular values sorted descending across sectors
+    by a stable sort (sector, then index order for ties); new bond = sectors
+    with kept values, ascending.  Returns (A parts, B parts, svs, kept, truncerr)."""
+    nl, nr, d1, d2, Dl, Dr = (int(x) for x in parts["dims"])
+    rowsec = np.asarray(rowsec).reshape(nl, d1)
+    colsec = np.asarray(colsec).reshape(nr, d2)
+    bs = d1 * d2 * Dl * Dr
+    rows = [[] for _ in range(nsec)]
+    cols = [[] for _ in range(nsec)]
+    for il in range(nl):
+        for s1 in range(d1):
+            if rowsec[il, s1] >= 0:
+                rows[rowsec[il, s1]].append((il, s1))
+    for ir in range(nr):
+        for s2 in range(d2):
+            if colsec[ir, s2] >= 0:
+                cols[colsec[ir, s2]].append((ir, s2))
+    roff, coff = {}, {}
+    R, Cn = [0] * nsec, [0] * nsec
+    for m in range(nsec):
+        for (il, s1) in rows[m]:
+            roff[(il, s1)] = R[m]
+            R[m] += int(dl[il])
+        for (ir, s2) in cols[m]:
+            coff[(ir, s2)] = Cn[m]
+            Cn[m] += int(dr[ir])
+    M = [np.zeros((R[m], Cn[m])) for m in range(nsec)]
+    for (a, b), i in zip(parts["keys"], parts["ids"]):
+        il, ir = int(a) - 1, int(b) - 1
+        blk = np.asarray(parts["data"][(i - 1) * bs:i * bs]).reshape((d1, d2, Dl, Dr), order="F")
+        for s1 in range(d1):
+            m = rowsec[il, s1]
+            if m < 0:
+                continue
+            for s2 in range(d2):
+                if colsec[ir, s2] != m:
+                    continue
+                ro, co = roff[(il, s1)], coff[(ir, s2)]
+                M[m][ro:ro + dl[il], co:co + dr[ir]] = blk[s1, s2, :dl[il], :dr[ir]]
+    fac = {}
+    all_svs = []
+    for m in range(nsec):
+        if R[m] and Cn[m]:
+            U, S, Vt = np.linalg.svd(M[m], full_matrices=False)
+            fac[m] = (U, S, Vt)
+            all_svs += [(float(s), m) for s in S]
+    all_svs.sort(key=lambda x: -x[0])
+    max_sv = all_svs[0][0] if all_svs else 1.0
+    keep = len(all_svs) if maxdim is None else min(int(maxdim), len(all_svs))
+    while keep > mindim and all_svs[keep - 1][0] < cutoff * max_sv:
+        keep -= 1
+    truncerr = 0.0
+    for x in all_svs[keep:]:
+        truncerr += x[0] ** 2
+    kept = [0] * nsec
+    for x in all_svs[:keep]:
+        kept[x[1]] += 1
+    live = [m for m in range(nsec) if kept[m] > 0]
+    qm = {m: i for i, m in enumerate(live)}
+    Dm = max(kept) if live else 0
+    svs = np.concatenate([fac[m][1][:kept[m]] for m in live]) if live else np.zeros(0)
+    akeys, adata, bkeys, bdata = [], [], [], []
+    for m in live:  # A keys (Ql, Qm): Qm major
+        U, S, Vt = fac[m]
+        k = kept[m]
+        for il in range(nl):
+            ss = [s1 for s1 in range(d1) if rowsec[il, s1] == m]
+            if not ss or not dl[il]:
+                continue
+            blk = np.zeros((d1, Dl, Dm))
+            for s1 in ss:
+                ro = roff[(il, s1)]
+                blk[s1, :dl[il], :k] = U[ro:ro + dl[il], :k] * (S[:k] if ortho == "right" else 1.0)
+            akeys.append((il + 1, qm[m] + 1))
+            adata.append(blk.reshape(-1, order="F"))
+    for ir in range(nr):  # B keys (Qm, Qr): Qr major
+        for m in live:
+            U, S, Vt = fac[m]
+            k = kept[m]
+            ss = [s2 for s2 in range(d2) if colsec[ir, s2] == m]
+            if not ss or not dr[ir]:
+                continue
+            blk = np.zeros((d2, Dm, Dr))
+            for s2 in ss:
+                co = coff[(ir, s2)]
+                v = Vt[:k, co:co + dr[ir]]
+                blk[s2, :k, :dr[ir]] = v * (S[:k, None] if ortho == "left" else 1.0)
+            bkeys.append((qm[m] + 1, ir + 1))
+            bdata.append(blk.reshape(-1, order="F"))
+    A = dict(dims=(nl, len(live), d1, Dl, Dm), N2=3, keys=np.array(akeys, dtype=np.int64).reshape(-1, 2),
+             ids=np.arange(1, len(akeys) + 1), data=np.concatenate(adata) if adata else np.zeros(0))
+    B = dict(dims=(len(live), nr, d2, Dm, Dr), N2=3, keys=np.array(bkeys, dtype=np.int64).reshape(-1, 2),
+             ids=np.arange(1, len(bkeys) + 1), data=np.concatenate(bdata) if bdata else np.zeros(0))
+    return A, B, svs, np.array(kept), truncerr

@@ -610,4 +610,25 @@ int sb_bs_svd(sb_ctx_t c, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_
   })
 }
 
+int sb_bs_svd_sectors(sb_ctx_t c, sb_tensor_t psi, int64_t nsec, const int64_t* rowsec, const int64_t* colsec,
+                      const int64_t* dl, const int64_t* dr, int ortho_right, int64_t maxdim, int64_t mindim,
+                      double cutoff, sb_tensor_t* A, sb_tensor_t* B, int64_t* kept, double* svs, int64_t svs_cap,
+                      int64_t* newdim, double* truncerr) {
+  SB_TRY({
+    if (!A || !B) fail(SB_E_ARGUMENT, "ArgumentError: null output handle");
+    Tensor* t = T_(psi, BS);
+    if (t->N() != 6) fail(SB_E_METHOD, "MethodError: the sector SVD needs psi [Ql, Qr | s1, s2, l, r]");
+    if (!rowsec || !colsec || !dl || !dr) fail(SB_E_ARGUMENT, "ArgumentError: null sector table");
+    SectSvdResult r = bs_svd_sectors(C_(c), *t, nsec, rowsec, colsec, dl, dr, ortho_right != 0, maxdim, mindim, cutoff);
+    if (kept)
+      for (int64_t i = 0; i < nsec; ++i) kept[i] = r.kept[i];
+    if (svs)
+      for (int64_t i = 0; i < std::min<int64_t>(svs_cap, (int64_t)r.svs.size()); ++i) svs[i] = r.svs[i];
+    if (newdim) *newdim = (int64_t)r.svs.size();
+    if (truncerr) *truncerr = r.truncerr;
+    *A = H_(std::move(r.A));
+    *B = H_(std::move(r.B));
+  })
+}
+
 }  // extern "C"

@@ -16,6 +16,16 @@ void k_svd_emit(Ctx*, i64 k, int d1, const i32* kg, const i32* kc, const i64* so
                 const i64* woff, const double* W, const double* sig, const double* Uout, const int* perm, bool right,
                 double* Ud, double* SVd);
 
+void k_sect_gather(Ctx*, const double* psi, int d1, int d2, i64 Dl, i64 ntask, const i64* task, const i64* xoff,
+                   const int* R, const int* Cc, const int* trans, double* X);
+size_t sect_svd_lds(int n, int len);
+void k_sect_svd(Ctx*, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
+                const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
+                double* SVo);
+void k_sect_emit(Ctx*, bool isA, i64 nblk, int d, i64 D1, i64 D2, i64 nside, const i64* blk, const i64* sec,
+                 const i64* off, const i64* dims, const i64* kept, const int* RC, const i64* uoff_or_soff,
+                 const i64* sgoff, const double* sig, const double* F, bool right, double* out);
+
 namespace {
 template <class T>
 void up(Ctx* ctx, DBuf<T>& d, const std::vector<T>& h) {
@@ -150,6 +160,198 @@ SvdResult bs_svd(Ctx* ctx, const Tensor& A, bool right, i64 maxdim, i64 mindim, 
              Uo.get(), perm.get(), right, R.U->data.get(), R.SV->data.get());
   HIP_CHECK(hipStreamSynchronize(ctx->stream));  // host staging vectors go out of scope
   return R;
+}
+
+// ================================================================ sector SVD
+
+SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* rowsec, const i64* colsec,
+                             const i64* dl, const i64* dr, bool right, i64 maxdim, i64 mindim, double cutoff) {
+  if (psi.kind != BS || psi.N() != 6 || psi.N2 != 4)
+    fail(SB_E_METHOD, "MethodError: the sector SVD needs psi [Ql, Qr | s1, s2, l, r] (N = 6, N2 = 4)");
+  const i64 nl = psi.dims[0], nr = psi.dims[1], Dl = psi.dims[4], Dr = psi.dims[5];
+  const int d1 = (int)psi.dims[2], d2 = (int)psi.dims[3];
+  if (nsec < 0) fail(SB_E_ARGUMENT, "ArgumentError: negative sector count");
+  for (i64 i = 0; i < nl * d1; ++i)
+    if (rowsec[i] < -1 || rowsec[i] >= nsec) fail(SB_E_ARGUMENT, "ArgumentError: row sector out of range");
+  for (i64 i = 0; i < nr * d2; ++i)
+    if (colsec[i] < -1 || colsec[i] >= nsec) fail(SB_E_ARGUMENT, "ArgumentError: column sector out of range");
+  for (i64 i = 0; i < nl; ++i)
+    if (dl[i] < 0 || dl[i] > Dl) fail(SB_E_ARGUMENT, "ArgumentError: left sector extent outside 0..Dl");
+  for (i64 i = 0; i < nr; ++i)
+    if (dr[i] < 0 || dr[i] > Dr) fail(SB_E_ARGUMENT, "ArgumentError: right sector extent outside 0..Dr");
+  // rows (Ql, s1, l) and columns (Qr, s2, r) of every sector, in that order
+  std::vector<i64> R(nsec, 0), Cn(nsec, 0), rowoff(nl * d1, -1), coloff(nr * d2, -1);
+  for (i64 il = 0; il < nl; ++il)
+    for (int s1 = 0; s1 < d1; ++s1) {
+      const i64 m = rowsec[il + nl * s1];
+      if (m >= 0) {
+        rowoff[il + nl * s1] = R[m];
+        R[m] += dl[il];
+      }
+    }
+  for (i64 ir = 0; ir < nr; ++ir)
+    for (int s2 = 0; s2 < d2; ++s2) {
+      const i64 m = colsec[ir + nr * s2];
+      if (m >= 0) {
+        coloff[ir + nr * s2] = Cn[m];
+        Cn[m] += dr[ir];
+      }
+    }
+  // active sectors (a nonempty matrix), compact index g
+  std::vector<i64> gof(nsec, -1), act;
+  for (i64 m = 0; m < nsec; ++m)
+    if (R[m] > 0 && Cn[m] > 0) {
+      gof[m] = (i64)act.size();
+      act.push_back(m);
+    }
+  const i64 ng = (i64)act.size();
+  std::vector<int> hR(ng), hC(ng), htr(ng);
+  std::vector<i64> xoff(ng + 1, 0), uoff(ng + 1, 0), soff(ng + 1, 0), sgoff(ng + 1, 0);
+  size_t lds = 0;
+  for (i64 g = 0; g < ng; ++g) {
+    const i64 m = act[g];
+    if (R[m] > INT32_MAX || Cn[m] > INT32_MAX) fail(SB_E_LIMIT, "sector matrix dimension beyond int32");
+    hR[g] = (int)R[m];
+    hC[g] = (int)Cn[m];
+    htr[g] = R[m] > Cn[m];
+    const int n = (int)std::min(R[m], Cn[m]), len = (int)std::max(R[m], Cn[m]);
+    lds = std::max(lds, sect_svd_lds(n, len));
+    xoff[g + 1] = xoff[g] + R[m] * Cn[m];
+    uoff[g + 1] = uoff[g] + R[m] * n;
+    soff[g + 1] = soff[g] + (i64)n * Cn[m];
+    sgoff[g + 1] = sgoff[g] + n;
+  }
+  if (lds > 160 * 1024) fail(SB_E_LIMIT, "sector SVD: a sector matrix does not fit the 160 KB LDS of a workgroup");
+  // gather tasks from psi's blocks
+  std::vector<i32> hk, hi;
+  down(ctx, hk, psi.keys.get(), (size_t)psi.n * 2);
+  down(ctx, hi, psi.ids.get(), (size_t)psi.n);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  std::vector<i64> task;
+  for (i64 b = 0; b < psi.n; ++b) {
+    const i64 il = hk[2 * b] - 1, ir = hk[2 * b + 1] - 1;
+    if (!dl[il] || !dr[ir]) continue;
+    for (int s1 = 0; s1 < d1; ++s1) {
+      const i64 m = rowsec[il + nl * s1];
+      if (m < 0 || gof[m] < 0) continue;
+      for (int s2 = 0; s2 < d2; ++s2) {
+        if (colsec[ir + nr * s2] != m) continue;
+        const i64 t[8] = {(i64)(hi[b] - 1) * psi.blksize, s1, s2, dl[il], dr[ir], gof[m], rowoff[il + nl * s1],
+                          coloff[ir + nr * s2]};
+        task.insert(task.end(), t, t + 8);
+      }
+    }
+  }
+  const i64 ntask = (i64)task.size() / 8;
+  DBuf<i64> d_task, d_xoff, d_uoff, d_soff, d_sgoff;
+  DBuf<int> d_R, d_C, d_tr;
+  up(ctx, d_task, task);
+  up(ctx, d_xoff, xoff);
+  up(ctx, d_uoff, uoff);
+  up(ctx, d_soff, soff);
+  up(ctx, d_sgoff, sgoff);
+  up(ctx, d_R, hR);
+  up(ctx, d_C, hC);
+  up(ctx, d_tr, htr);
+  DBuf<double> X(ctx, xoff[ng] ? xoff[ng] : 1), Uo(ctx, uoff[ng] ? uoff[ng] : 1), SVo(ctx, soff[ng] ? soff[ng] : 1),
+      sig(ctx, sgoff[ng] ? sgoff[ng] : 1);
+  k_zero(ctx, X.get(), (size_t)(xoff[ng] ? xoff[ng] : 1) * sizeof(double));
+  k_sect_gather(ctx, psi.dptr(), d1, d2, Dl, ntask, d_task.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(),
+                X.get());
+  k_sect_svd(ctx, ng, lds, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(), d_soff.get(),
+             d_sgoff.get(), sig.get(), Uo.get(), SVo.get());
+  std::vector<double> hs;
+  down(ctx, hs, sig.get(), (size_t)sgoff[ng]);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+  // global order and truncation, the rules of blocksparse_svd (ops_factorize.jl:86-104)
+  struct SV {
+    double s;
+    i64 g;
+  };
+  std::vector<SV> all;
+  for (i64 g = 0; g < ng; ++g)
+    for (i64 c = sgoff[g]; c < sgoff[g + 1]; ++c) all.push_back({hs[c], g});
+  std::stable_sort(all.begin(), all.end(), [](const SV& a, const SV& b) { return a.s > b.s; });
+  const double max_sv = all.empty() ? 1.0 : all[0].s;
+  i64 keep = (maxdim < 0) ? (i64)all.size() : std::min<i64>(maxdim, (i64)all.size());
+  while (keep > mindim && all[keep - 1].s < cutoff * max_sv) --keep;
+  SectSvdResult out;
+  for (size_t i = keep; i < all.size(); ++i) out.truncerr += all[i].s * all[i].s;
+  std::vector<i64> kg(ng, 0);
+  for (i64 i = 0; i < keep; ++i) ++kg[all[i].g];
+  out.kept.assign(nsec, 0);
+  std::vector<i64> qm(ng, -1);  // new-bond sector number of each compact sector
+  i64 nm = 0, Dm = 0;
+  for (i64 g = 0; g < ng; ++g) {
+    out.kept[act[g]] = kg[g];
+    if (kg[g] > 0) {
+      qm[g] = nm++;
+      Dm = std::max(Dm, kg[g]);
+      for (i64 c = 0; c < kg[g]; ++c) out.svs.push_back(hs[sgoff[g] + c]);
+    }
+  }
+  // factors; blocks in key order with ids 1..n
+  out.A = bs_make(ctx, {nl, nm, (i64)d1, Dl, Dm}, 3);
+  out.B = bs_make(ctx, {nm, nr, (i64)d2, Dm, Dr}, 3);
+  std::vector<i64> rsec_c(nl * d1, -1), csec_c(nr * d2, -1);
+  for (i64 i = 0; i < nl * d1; ++i)
+    if (rowsec[i] >= 0 && gof[rowsec[i]] >= 0) rsec_c[i] = gof[rowsec[i]];
+  for (i64 i = 0; i < nr * d2; ++i)
+    if (colsec[i] >= 0 && gof[colsec[i]] >= 0) csec_c[i] = gof[colsec[i]];
+  std::vector<i64> ablk, bblk;
+  std::vector<i32> akeys, bkeys;
+  for (i64 g = 0; g < ng; ++g) {
+    if (qm[g] < 0) continue;
+    for (i64 il = 0; il < nl; ++il) {
+      bool any = false;
+      for (int s1 = 0; s1 < d1 && !any; ++s1) any = rsec_c[il + nl * s1] == g;
+      if (!any || !dl[il]) continue;
+      ablk.push_back(il);
+      ablk.push_back(g);
+      akeys.push_back((i32)(il + 1));
+      akeys.push_back((i32)(qm[g] + 1));
+    }
+  }
+  for (i64 ir = 0; ir < nr; ++ir)
+    for (i64 g = 0; g < ng; ++g) {
+      if (qm[g] < 0 || !dr[ir]) continue;
+      bool any = false;
+      for (int s2 = 0; s2 < d2 && !any; ++s2) any = csec_c[ir + nr * s2] == g;
+      if (!any) continue;
+      bblk.push_back(g);
+      bblk.push_back(ir);
+      bkeys.push_back((i32)(qm[g] + 1));
+      bkeys.push_back((i32)(ir + 1));
+    }
+  const i64 nA = (i64)ablk.size() / 2, nB = (i64)bblk.size() / 2;
+  std::vector<i32> aid(nA), bid(nB);
+  for (i64 i = 0; i < nA; ++i) aid[i] = (i32)(i + 1);
+  for (i64 i = 0; i < nB; ++i) bid[i] = (i32)(i + 1);
+  out.A->n = nA;
+  out.B->n = nB;
+  up(ctx, out.A->keys, akeys);
+  up(ctx, out.A->ids, aid);
+  up(ctx, out.B->keys, bkeys);
+  up(ctx, out.B->ids, bid);
+  out.A->data.alloc(ctx, (size_t)(nA * out.A->blksize));
+  out.B->data.alloc(ctx, (size_t)(nB * out.B->blksize));
+  DBuf<i64> d_ablk, d_bblk, d_rsec, d_csec, d_roff, d_coff, d_dl, d_dr, d_kept;
+  up(ctx, d_ablk, ablk);
+  up(ctx, d_bblk, bblk);
+  up(ctx, d_rsec, rsec_c);
+  up(ctx, d_csec, csec_c);
+  up(ctx, d_roff, rowoff);
+  up(ctx, d_coff, coloff);
+  up(ctx, d_dl, std::vector<i64>(dl, dl + nl));
+  up(ctx, d_dr, std::vector<i64>(dr, dr + nr));
+  up(ctx, d_kept, kg);
+  k_sect_emit(ctx, true, nA, d1, Dl, Dm, nl, d_ablk.get(), d_rsec.get(), d_roff.get(), d_dl.get(), d_kept.get(),
+              d_R.get(), d_uoff.get(), d_sgoff.get(), sig.get(), Uo.get(), right, out.A->data.get());
+  k_sect_emit(ctx, false, nB, d2, Dm, Dr, nr, d_bblk.get(), d_csec.get(), d_coff.get(), d_dr.get(), d_kept.get(),
+              d_C.get(), d_soff.get(), d_sgoff.get(), sig.get(), SVo.get(), right, out.B->data.get());
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return out;
 }
 
 }  // namespace sb

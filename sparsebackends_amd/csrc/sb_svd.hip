@@ -141,4 +141,231 @@ void k_svd_emit(Ctx* ctx, i64 k, int d1, const i32* kg, const i32* kc, const i64
   HIP_CHECK(hipGetLastError());
 }
 
+// ================================================================ sector SVD
+// Block-diagonal SVD of a two-site tensor psi [Ql, Qr | s1, s2, l, r] by fused
+// charge sector m: rows (Ql, s1, l) with (Ql, s1) in m, columns (Qr, s2, r).
+// Each sector's matrix X (M, or M^T when M has more rows than columns, so X
+// has the fewer rows) is factorised by one workgroup with one-sided Jacobi
+// in LDS, pairs of rows rotated in parallel in round-robin order.
+
+constexpr int SS_THREADS = 512;
+
+// copy psi sub-blocks into the sector matrices: task t = (psi block offset,
+// s1, s2, dl, dr, sector g, row offset, column offset)
+__global__ __launch_bounds__(64) void sect_gather_kernel(const double* __restrict__ psi, int d1, int d2, i64 Dl,
+                                                        const i64* __restrict__ task, const i64* __restrict__ xoff,
+                                                        const int* __restrict__ R, const int* __restrict__ Cc,
+                                                        const int* __restrict__ trans, double* __restrict__ X) {
+  const i64* t = task + (i64)blockIdx.x * 8;
+  const i64 base = t[0];
+  const int s1 = (int)t[1], s2 = (int)t[2];
+  const i64 dl = t[3], dr = t[4];
+  const int g = (int)t[5];
+  const i64 ro = t[6], co = t[7];
+  const i64 Rg = R[g], Cg = Cc[g];
+  const bool tr = trans[g] != 0;
+  double* Xg = X + xoff[g];
+  for (i64 e = threadIdx.x; e < dl * dr; e += 64) {
+    const i64 l = e % dl, r = e / dl;
+    const double v = psi[base + s1 + (i64)d1 * s2 + (i64)d1 * d2 * (l + Dl * r)];
+    const i64 row = ro + l, col = co + r;
+    Xg[tr ? col * Rg + row : row * Cg + col] = v;
+  }
+}
+
+void k_sect_gather(Ctx* ctx, const double* psi, int d1, int d2, i64 Dl, i64 ntask, const i64* task, const i64* xoff,
+                   const int* R, const int* Cc, const int* trans, double* X) {
+  if (!ntask) return;
+  hipLaunchKernelGGL(sect_gather_kernel, dim3((unsigned)ntask), dim3(64), 0, ctx->stream, psi, d1, d2, Dl, task, xoff,
+                     R, Cc, trans, X);
+  HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __restrict__ X, const i64* __restrict__ xoff,
+                                                             const int* __restrict__ R, const int* __restrict__ Cc,
+                                                             const int* __restrict__ trans,
+                                                             const i64* __restrict__ uoff, const i64* __restrict__ soff,
+                                                             const i64* __restrict__ sgoff, double* __restrict__ sig,
+                                                             double* __restrict__ Uo, double* __restrict__ SVo) {
+  extern __shared__ double lds[];
+  const int g = blockIdx.x;
+  const bool tr = trans[g] != 0;
+  const int Rg = R[g], Cg = Cc[g];
+  const int n = tr ? Cg : Rg, len = tr ? Rg : Cg;  // rows of X, row length
+  double* Xs = lds;
+  double* Q = Xs + (i64)n * len;  // Q[r + n*c]
+  double* nrm = Q + (i64)n * n;
+  int* ord = reinterpret_cast<int*>(nrm + n);
+  int* flag = ord + n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
+  const double* Xg = X + xoff[g];
+  for (i64 e = tid; e < (i64)n * len; e += SS_THREADS) Xs[e] = Xg[e];
+  for (i64 e = tid; e < (i64)n * n; e += SS_THREADS) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
+  __syncthreads();
+  const int n2 = n + (n & 1), m = n2 - 1;
+  for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int rd = 0; rd < m; ++rd) {
+      for (int pi = wave; pi < n2 / 2; pi += nwave) {
+        const int p = pi == 0 ? 0 : 1 + (rd + pi) % m;
+        const int q = pi == 0 ? 1 + rd : 1 + (rd - pi + m) % m;
+        if (p >= n || q >= n) continue;
+        double* xp = Xs + (i64)p * len;
+        double* xq = Xs + (i64)q * len;
+        double a = 0, b = 0, c = 0;
+        for (int j = lane; j < len; j += 64) {
+          const double x = xp[j], y = xq[j];
+          a = fma(x, x, a);
+          b = fma(y, y, b);
+          c = fma(x, y, c);
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        c = wave_sum(c);
+        if (c == 0.0 || fabs(c) <= 1e-15 * sqrt(a) * sqrt(b)) continue;
+        const double zeta = (b - a) / (2.0 * c);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        for (int j = lane; j < len; j += 64) {
+          const double x = xp[j], y = xq[j];
+          xp[j] = cs * x - sn * y;
+          xq[j] = sn * x + cs * y;
+        }
+        for (int r = lane; r < n; r += 64) {
+          const double up = Q[r + (i64)n * p], uq = Q[r + (i64)n * q];
+          Q[r + (i64)n * p] = cs * up - sn * uq;
+          Q[r + (i64)n * q] = sn * up + cs * uq;
+        }
+        if (lane == 0) *flag = 1;
+      }
+      __syncthreads();
+    }
+    const int f = *flag;
+    __syncthreads();
+    if (!f) break;
+  }
+  for (int i = wave; i < n; i += nwave) {
+    const double* xi = Xs + (i64)i * len;
+    double a = 0;
+    for (int j = lane; j < len; j += 64) a = fma(xi[j], xi[j], a);
+    a = wave_sum(a);
+    if (lane == 0) nrm[i] = sqrt(a);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    for (int i = 1; i < n; ++i) {  // stable insertion sort, descending
+      const int v = ord[i];
+      int k = i - 1;
+      while (k >= 0 && nrm[ord[k]] < nrm[v]) {
+        ord[k + 1] = ord[k];
+        --k;
+      }
+      ord[k + 1] = v;
+    }
+  }
+  __syncthreads();
+  // k = n triplets: U (Rg x n, column-major), sigma * V^T (n x Cg, row-major), sigma
+  for (int c = tid; c < n; c += SS_THREADS) sig[sgoff[g] + c] = nrm[ord[c]];
+  double* Ug = Uo + uoff[g];
+  double* Sg = SVo + soff[g];
+  if (!tr) {  // X = M: U = Q, sigma V^T = rows of X
+    for (i64 e = tid; e < (i64)Rg * n; e += SS_THREADS) {
+      const int r = (int)(e % Rg), c = (int)(e / Rg);
+      Ug[e] = Q[r + (i64)n * ord[c]];
+    }
+    for (i64 e = tid; e < (i64)n * Cg; e += SS_THREADS) {
+      const int c = (int)(e / Cg), j = (int)(e % Cg);
+      Sg[e] = Xs[(i64)ord[c] * len + j];
+    }
+  } else {  // X = M^T: U columns = rows of X / sigma, sigma V^T row c = sigma * column ord[c] of Q
+    for (i64 e = tid; e < (i64)Rg * n; e += SS_THREADS) {
+      const int r = (int)(e % Rg), c = (int)(e / Rg);
+      const double s = nrm[ord[c]];
+      Ug[e] = s > 0 ? Xs[(i64)ord[c] * len + r] / s : 0.0;
+    }
+    for (i64 e = tid; e < (i64)n * Cg; e += SS_THREADS) {
+      const int c = (int)(e / Cg), j = (int)(e % Cg);
+      Sg[e] = nrm[ord[c]] * Q[j + (i64)n * ord[c]];
+    }
+  }
+}
+
+size_t sect_svd_lds(int n, int len) {
+  return ((size_t)n * len + (size_t)n * n + n) * sizeof(double) + (size_t)(n + 1) * sizeof(int);
+}
+
+void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
+                const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
+                double* SVo) {
+  if (!ng) return;
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_svd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(sect_svd_kernel, dim3((unsigned)ng), dim3(SS_THREADS), lds, ctx->stream, X, xoff, R, Cc, trans,
+                     uoff, soff, sgoff, sig, Uo, SVo);
+  HIP_CHECK(hipGetLastError());
+}
+
+// factors of the kept triplets.  A block b: (il, sector g) -> [s1, l, mm]
+// (d1 x Dl x Dm); B block b: (g, ir) -> [s2, mm, r] (d2 x Dm x Dr).
+__global__ void sect_emit_A_kernel(i64 nblk, int d1, i64 Dl, i64 Dm, i64 nl, const i64* __restrict__ blk,
+                                   const i64* __restrict__ rowsec, const i64* __restrict__ rowoff,
+                                   const i64* __restrict__ dl, const i64* __restrict__ kept, const int* __restrict__ R,
+                                   const i64* __restrict__ uoff, const i64* __restrict__ sgoff,
+                                   const double* __restrict__ sig, const double* __restrict__ Uo, int right,
+                                   double* __restrict__ out) {
+  const i64 bsz = (i64)d1 * Dl * Dm;
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * bsz) return;
+  const i64 b = t / bsz, e = t - b * bsz;
+  const i64 il = blk[2 * b], g = blk[2 * b + 1];
+  const int s1 = (int)(e % d1);
+  const i64 l = (e / d1) % Dl, mm = e / (d1 * Dl);
+  double v = 0.0;
+  if (rowsec[il + nl * s1] == g && l < dl[il] && mm < kept[g]) {
+    v = Uo[uoff[g] + rowoff[il + nl * s1] + l + (i64)R[g] * mm];
+    if (right) v *= sig[sgoff[g] + mm];
+  }
+  out[t] = v;
+}
+
+__global__ void sect_emit_B_kernel(i64 nblk, int d2, i64 Dm, i64 Dr, i64 nr, const i64* __restrict__ blk,
+                                   const i64* __restrict__ colsec, const i64* __restrict__ coloff,
+                                   const i64* __restrict__ dr, const i64* __restrict__ kept, const int* __restrict__ Cc,
+                                   const i64* __restrict__ soff, const i64* __restrict__ sgoff,
+                                   const double* __restrict__ sig, const double* __restrict__ SVo, int right,
+                                   double* __restrict__ out) {
+  const i64 bsz = (i64)d2 * Dm * Dr;
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * bsz) return;
+  const i64 b = t / bsz, e = t - b * bsz;
+  const i64 g = blk[2 * b], ir = blk[2 * b + 1];
+  const int s2 = (int)(e % d2);
+  const i64 mm = (e / d2) % Dm, r = e / (d2 * Dm);
+  double v = 0.0;
+  if (colsec[ir + nr * s2] == g && r < dr[ir] && mm < kept[g]) {
+    v = SVo[soff[g] + mm * Cc[g] + coloff[ir + nr * s2] + r];
+    if (right) {
+      const double s = sig[sgoff[g] + mm];
+      v = s > 0 ? v / s : 0.0;
+    }
+  }
+  out[t] = v;
+}
+
+void k_sect_emit(Ctx* ctx, bool isA, i64 nblk, int d, i64 D1, i64 D2, i64 nside, const i64* blk, const i64* sec,
+                 const i64* off, const i64* dims, const i64* kept, const int* RC, const i64* uoff_or_soff,
+                 const i64* sgoff, const double* sig, const double* F, bool right, double* out) {
+  const i64 tot = nblk * d * D1 * D2;
+  if (!tot) return;
+  const unsigned grid = (unsigned)((tot + 255) / 256);
+  if (isA)
+    hipLaunchKernelGGL(sect_emit_A_kernel, dim3(grid), dim3(256), 0, ctx->stream, nblk, d, D1, D2, nside, blk, sec,
+                       off, dims, kept, RC, uoff_or_soff, sgoff, sig, F, right ? 1 : 0, out);
+  else
+    hipLaunchKernelGGL(sect_emit_B_kernel, dim3(grid), dim3(256), 0, ctx->stream, nblk, d, D1, D2, nside, blk, sec,
+                       off, dims, kept, RC, uoff_or_soff, sgoff, sig, F, right ? 1 : 0, out);
+  HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace sb

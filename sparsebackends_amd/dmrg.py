@@ -117,3 +117,119 @@ def mpo_parts(model):
 
 def upload(p, device=0):
     return NewBlockSparseSorted.from_parts(p["dims"], p["N2"], p["keys"], p["ids"], p["data"], device=device)
+
+
+def fused_sectors(model, Bl, Br):
+    """sector tables of a two-site psi [Ql, Qr | s1, s2, l, r] for its SVD:
+    rows (Ql, s1) carry q(Ql) + q(s1), columns (Qr, s2) carry q(Qr) - q(s2);
+    the sectors are the charges present on both sides, ascending.  Returns
+    (rowsec (nl, d), colsec (nr, d), charges)."""
+    from .heff import qadd, qsub
+    d = model.d
+    rq = {(il, s): qadd(Bl.charges[il], model.charges[s]) for il in range(Bl.nsec) for s in range(d)}
+    cq = {(ir, s): qsub(Br.charges[ir], model.charges[s]) for ir in range(Br.nsec) for s in range(d)}
+    charges = sorted(set(rq.values()) & set(cq.values()))
+    idx = {q: i for i, q in enumerate(charges)}
+    rowsec = np.full((Bl.nsec, d), -1, dtype=np.int64)
+    colsec = np.full((Br.nsec, d), -1, dtype=np.int64)
+    for (il, s), q in rq.items():
+        rowsec[il, s] = idx.get(q, -1)
+    for (ir, s), q in cq.items():
+        colsec[ir, s] = idx.get(q, -1)
+    return rowsec, colsec, charges
+
+
+# ---------------------------------------------------------------- two-site DMRG
+
+_PSI_AB = [lab("Ql"), lab("Qr"), lab("s1"), lab("l"), lab("s2"), lab("r")]
+
+
+def two_site(A, B):
+    """psi [Ql, Qr | s1, s2, l, r] = A [Ql, Qm | s1, l, m] * B [Qm, Qr | s2, m, r]
+    (contracted to [s1, l, s2, r], each operand's group contiguous, then the
+    tail permuted)"""
+    lA = [lab("Ql"), lab("Qm"), lab("s1"), lab("l"), lab("m")]
+    lB = [lab("Qm"), lab("Qr"), lab("s2"), lab("m"), lab("r")]
+    nl, _, d1, Dl, _ = A.dims
+    _, nr, d2, _, Dr = B.dims
+    X = NewBlockSparseSorted.new((nl, nr, d1, Dl, d2, Dr), 4, device=A.device)
+    contract_(X, _PSI_AB, A, lA, B, lB)
+    return X.permutedims([1, 2, 3, 5, 4, 6])
+
+
+class DMRG:
+    """Two-site DMRG of an open chain with a uniform site model, in a fixed
+    total-charge sector, from a product state.
+
+    Tensors, environments and bonds stay on the device between steps; the
+    host keeps the sector structure (charges and extents of every bond)."""
+
+    def __init__(self, model, states, device=0, flags=0):
+        from .heff import Bond
+        self.m, self.N, self.device, self.flags = model, len(states), device, flags
+        d, w = model.d, model.w
+        zero = tuple(0 for _ in model.charges[0])
+        # bonds 0..N: bond j sits between sites j and j+1; charges accumulate from the left
+        charges = [zero]
+        for s_ in states:
+            charges.append(tuple(a + b for a, b in zip(charges[-1], model.charges[s_])))
+        self.total = charges[-1]
+        self.bonds = [Bond([q], [1]) for q in charges]
+        self.A = []
+        for j, s_ in enumerate(states):
+            blk = np.zeros((d, 1, 1))
+            blk[s_, 0, 0] = 1.0
+            p = dict(dims=(1, 1, d, 1, 1), N2=3, keys=np.array([[1, 1]]), ids=np.array([1]),
+                     data=blk.reshape(-1, order="F"))
+            self.A.append(upload(p, device))
+        self.W = upload(mpo_parts(model), device)
+        # boundary environments: L0 selects the start link state, R_{N+1} the done state
+        e = np.zeros((w, 1, 1))
+        e[w - 1, 0, 0] = 1.0
+        self.L = {0: upload(dict(dims=(1, 1, w, 1, 1), N2=3, keys=np.array([[1, 1]]), ids=np.array([1]),
+                                 data=e.reshape(-1, order="F")), device)}
+        e = np.zeros((w, 1, 1))
+        e[0, 0, 0] = 1.0
+        self.R = {self.N + 1: upload(dict(dims=(1, 1, w, 1, 1), N2=3, keys=np.array([[1, 1]]), ids=np.array([1]),
+                                          data=e.reshape(-1, order="F")), device)}
+        for j in range(self.N, 2, -1):  # R_j from R_{j+1} and site j (1-based)
+            self.R[j] = make_R(self.R[j + 1], self.A[j - 1], self.W, flags)[0]
+        self.energies = []
+        self.truncerr = 0.0
+
+    def step(self, b, direction, maxdim, cutoff, krylovdim, maxiter, tol, mindim=1):
+        """optimise sites b, b+1 (1-based): eigsolve on H_eff, sector SVD, environment update"""
+        from .heff import Bond, heff_plans
+        from .krylov import eigsolve
+        from .factorize import svd_sectors
+        psi = two_site(self.A[b - 1], self.A[b])
+        plans, Hp = heff_plans(psi, self.L[b - 1], self.W, self.W, self.R[b + 2], self.flags)
+        vals, _, info = eigsolve(plans, psi, krylovdim=krylovdim, maxiter=maxiter, tol=tol)
+        Bl, Br = self.bonds[b - 1], self.bonds[b + 1]
+        rowsec, colsec, charges = fused_sectors(self.m, Bl, Br)
+        A, B, svs, kept, spec = svd_sectors(psi, rowsec, colsec, Bl.dims, Br.dims, len(charges),
+                                            ortho="left" if direction > 0 else "right", maxdim=maxdim,
+                                            mindim=mindim, cutoff=cutoff)
+        live = [i for i in range(len(charges)) if kept[i] > 0]
+        self.bonds[b] = Bond([charges[i] for i in live], [int(kept[i]) for i in live])
+        self.A[b - 1], self.A[b] = A, B
+        if direction > 0:
+            self.L[b] = make_L(self.L[b - 1], A, self.W, self.flags)[0]
+        else:
+            self.R[b + 1] = make_R(self.R[b + 2], B, self.W, self.flags)[0]
+        self.truncerr = max(self.truncerr, spec.truncerr)
+        return vals[0], info
+
+    def sweep(self, maxdim=None, cutoff=0.0, krylovdim=3, maxiter=1, tol=1e-14):
+        """one left-to-right and one right-to-left half sweep; returns the last energy"""
+        self.truncerr = 0.0
+        e = None
+        for b in range(1, self.N):
+            e, _ = self.step(b, +1, maxdim, cutoff, krylovdim, maxiter, tol)
+        for b in range(self.N - 1, 0, -1):
+            e, _ = self.step(b, -1, maxdim, cutoff, krylovdim, maxiter, tol)
+        self.energies.append(e)
+        return e
+
+    def maxlinkdim(self):
+        return max(bd.dim for bd in self.bonds)

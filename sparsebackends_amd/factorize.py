@@ -32,3 +32,30 @@ def blocksparse_svd(A, ortho="left", maxdim=None, mindim=1, cutoff=0.0):
                            svs.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(k), C.byref(terr)))
     s = svs[:k.value].copy()
     return (NewBlockSparseSorted(U, A.device), NewBlockSparseSorted(SV, A.device), s, Spectrum(s ** 2, terr.value))
+
+
+def svd_sectors(psi, rowsec, colsec, dl, dr, nsec, ortho="left", maxdim=None, mindim=1, cutoff=0.0):
+    """Block-diagonal SVD of psi [Ql, Qr | s1, s2, l, r] by fused charge sector
+    (sb_bs_svd_sectors).  rowsec[il, s1] / colsec[ir, s2]: sector number or -1
+    (arrays of shape (nl, d1) / (nr, d2)); dl, dr: actual bond sector extents.
+    Returns (A [Ql, Qm | s1, l, m], B [Qm, Qr | s2, m, r], svs, kept per sector, spec)."""
+    if ortho not in ("left", "right"):
+        raise ValueError("ortho must be 'left' or 'right'")
+    i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64))  # noqa: E731
+    rs = i64(np.asarray(rowsec).reshape(-1, order="F"))
+    cs = i64(np.asarray(colsec).reshape(-1, order="F"))
+    dl_, dr_ = i64(dl), i64(dr)
+    kept = np.zeros(max(1, nsec), dtype=np.int64)
+    cap = int(sum(int(x) for x in dl_)) * int(psi.dims[2]) + 1
+    svs = np.zeros(cap)
+    A, B = C.c_void_p(), C.c_void_p()
+    k, terr = C.c_int64(), C.c_double()
+    p64 = C.POINTER(C.c_int64)
+    check(load().sb_bs_svd_sectors(psi.ctx, psi.handle, int(nsec), rs.ctypes.data_as(p64), cs.ctypes.data_as(p64),
+                                   dl_.ctypes.data_as(p64), dr_.ctypes.data_as(p64), 1 if ortho == "right" else 0,
+                                   -1 if maxdim is None else int(maxdim), int(mindim), float(cutoff), C.byref(A),
+                                   C.byref(B), kept.ctypes.data_as(p64), svs.ctypes.data_as(C.POINTER(C.c_double)),
+                                   cap, C.byref(k), C.byref(terr)))
+    s = svs[:k.value].copy()
+    return (NewBlockSparseSorted(A, psi.device), NewBlockSparseSorted(B, psi.device), s, kept[:nsec].copy(),
+            Spectrum(s ** 2, terr.value))

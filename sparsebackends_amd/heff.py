@@ -203,6 +203,35 @@ def _lab(name):
     return (int.from_bytes(("h:" + name).encode()[:8].ljust(8, b"\0"), "little"), 0)
 
 
+# labels of the four H_eff*psi contractions (layouts in the module docstring)
+L_PSI = [_lab("Ql"), _lab("Qr"), _lab("s1"), _lab("s2"), _lab("l"), _lab("r")]
+L_L = [_lab("Qlp"), _lab("Ql"), _lab("wl"), _lab("lp"), _lab("l")]
+L_W1 = [_lab("wl"), _lab("s1p"), _lab("s1"), _lab("wm")]
+L_W2 = [_lab("wm"), _lab("s2p"), _lab("s2"), _lab("wr")]
+L_R = [_lab("Qrp"), _lab("Qr"), _lab("wr"), _lab("rp"), _lab("r")]
+L_T1 = [_lab("Qlp"), _lab("Qr"), _lab("s1"), _lab("s2"), _lab("r"), _lab("wl"), _lab("lp")]
+L_T2 = [_lab("Qlp"), _lab("Qr"), _lab("s1p"), _lab("wm"), _lab("s2"), _lab("r"), _lab("lp")]
+L_T3 = [_lab("Qlp"), _lab("Qr"), _lab("s2p"), _lab("wr"), _lab("s1p"), _lab("r"), _lab("lp")]
+L_OUT = [_lab("Qlp"), _lab("Qrp"), _lab("s1p"), _lab("s2p"), _lab("lp"), _lab("rp")]
+
+
+def heff_plans(psi, L, W1, W2, R, flags=0):
+    """the four H_eff*psi contractions of psi [Ql, Qr | s1, s2, l, r] with
+    L [Ql', Ql | wl, l', l], W [wl, s', s, wr] and R [Qr', Qr | wr, r', r] as
+    plans with fresh outputs; returns (plans, Hpsi)"""
+    nl, nr, d1, d2, Dl, Dr = psi.dims
+    w, wr = W1.dims[0], W2.dims[3]
+    wm = W1.dims[3]
+    dev = psi.device
+    T1 = NewBlockSparseSorted.new((nl, nr, d1, d2, Dr, w, Dl), 5, device=dev)
+    T2 = NewBlockSparseSorted.new((nl, nr, d1, wm, d2, Dr, Dl), 5, device=dev)
+    T3 = NewBlockSparseSorted.new((nl, nr, d2, wr, d1, Dr, Dl), 5, device=dev)
+    Hp = NewBlockSparseSorted.new((nl, nr, d1, d2, Dl, Dr), 4, device=dev)
+    plans = [Plan(T1, L_T1, psi, L_PSI, L, L_L, flags), Plan(T2, L_T2, T1, L_T1, W1, L_W1, flags),
+             Plan(T3, L_T3, T2, L_T2, W2, L_W2, flags), Plan(Hp, L_OUT, T3, L_T3, R, L_R, flags)]
+    return plans, Hp
+
+
 class HeffProblem:
     """A two-site H_eff*psi at one bond: left bond sectors Bl, right Br,
     site model m.  Random psi / L / R of the implied block structure (random
@@ -252,16 +281,8 @@ class HeffProblem:
         # W [ | wl, s', s, wr] : one dense block
         self.W_parts = dict(dims=(w, d, d, w), N2=4, keys=np.zeros((1, 0), dtype=np.int64),
                             ids=np.array([1]), data=np.asfortranarray(model.W).reshape(-1, order="F"))
-        L_ = _lab
-        self.l_psi = [L_("Ql"), L_("Qr"), L_("s1"), L_("s2"), L_("l"), L_("r")]
-        self.l_L = [L_("Qlp"), L_("Ql"), L_("wl"), L_("lp"), L_("l")]
-        self.l_W1 = [L_("wl"), L_("s1p"), L_("s1"), L_("wm")]
-        self.l_W2 = [L_("wm"), L_("s2p"), L_("s2"), L_("wr")]
-        self.l_R = [L_("Qrp"), L_("Qr"), L_("wr"), L_("rp"), L_("r")]
-        self.l_T1 = [L_("Qlp"), L_("Qr"), L_("s1"), L_("s2"), L_("r"), L_("wl"), L_("lp")]
-        self.l_T2 = [L_("Qlp"), L_("Qr"), L_("s1p"), L_("wm"), L_("s2"), L_("r"), L_("lp")]
-        self.l_T3 = [L_("Qlp"), L_("Qr"), L_("s2p"), L_("wr"), L_("s1p"), L_("r"), L_("lp")]
-        self.l_out = [L_("Qlp"), L_("Qrp"), L_("s1p"), L_("s2p"), L_("lp"), L_("rp")]
+        (self.l_psi, self.l_L, self.l_W1, self.l_W2, self.l_R, self.l_T1, self.l_T2, self.l_T3,
+         self.l_out) = (L_PSI, L_L, L_W1, L_W2, L_R, L_T1, L_T2, L_T3, L_OUT)
         nl, nr = Bl.nsec, Br.nsec
         self.dims_T1 = (nl, nr, d, d, Dr, w, Dl)
         self.dims_T2 = (nl, nr, d, w, d, Dr, Dl)
