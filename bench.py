@@ -15,14 +15,16 @@ Krylov solve keeps it fixed; the planning time is reported separately.
 The environments are symmetrised so that H_eff is symmetric; "eigsolve" then
 reports the ground energy by FP64 Lanczos, the mixed-precision solve's
 energy difference and time-to-energy against it, and solves/s with p50/p99
-latency of the DMRG-step solve (krylovdim 3, maxiter 1).
+latency of the DMRG-step solve (krylovdim 3, maxiter 1); "dmrg_solve" times
+whole two-site DMRG ground-state solves of XXZ chains (solves/s, p50/p99).
 
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
                        [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--no-eig]
-                       [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--calibrate-copy MB]
+                       [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--dmrg-solves N]
+                       [--dmrg-sites N] [--dmrg-chi X] [--calibrate-copy MB]
 """
 from __future__ import annotations
 
@@ -59,6 +61,9 @@ def parse():
     ap.add_argument("--eig-tol-lo", type=float, default=1e-6,
                     help="residual norm of the low-precision stage, relative to |E|")
     ap.add_argument("--solves", type=int, default=30, help="DMRG-step solves timed for solves/s and p99")
+    ap.add_argument("--dmrg-solves", type=int, default=8, help="full DMRG solves (0: skip)")
+    ap.add_argument("--dmrg-sites", type=int, default=16)
+    ap.add_argument("--dmrg-chi", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -117,6 +122,40 @@ def eig_section(sb, a, prob, plans):
     res["dmrg_step_solve"] = dict(krylovdim=3, maxiter=1, solves=a.solves, solves_per_s=len(lat) / lat.sum(),
                                   p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3))
     return res
+
+
+def dmrg_section(sb, a):
+    """Full two-site DMRG ground-state solves of XXZ chains (the serving
+    configuration's model at its smallest length): N sites from the Neel state,
+    bond dimension chi, 4 sweeps of (krylovdim 3, maxiter 1) steps, one solve
+    per anisotropy Delta in [0.5, 1.5], run back to back.  Reported: solves/s
+    and p50/p99 latency, and each energy against the same chain solved at bond
+    dimension 256 (exact for 16 sites) with converged eigensolves."""
+    from sparsebackends_amd import heff as H
+    from sparsebackends_amd import dmrg as D
+    N, chi, sweeps = a.dmrg_sites, a.dmrg_chi, 4
+    deltas = np.linspace(0.5, 1.5, a.dmrg_solves)
+    lat, es = [], []
+    for dlt in deltas:
+        sb.synchronize()
+        t0 = time.perf_counter()
+        dm = D.DMRG(H.heisenberg_model(1.0, float(dlt)), [0, 1] * (N // 2))
+        for _ in range(sweeps):
+            e = dm.sweep(maxdim=chi, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
+        sb.synchronize()
+        lat.append(time.perf_counter() - t0)
+        es.append(e)
+    errs = []
+    for dlt, e in zip(deltas, es):
+        ref = D.DMRG(H.heisenberg_model(1.0, float(dlt)), [0, 1] * (N // 2))
+        for _ in range(5):
+            er = ref.sweep(maxdim=256, cutoff=0.0, krylovdim=8, maxiter=3, tol=1e-13)
+        errs.append(e - er)
+    lat = np.array(lat)
+    return dict(model="XXZ J=1", sites=N, chi=chi, sweeps=sweeps, krylovdim=3, maxiter=1, solves=len(lat),
+                solves_per_s=len(lat) / lat.sum(), p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), max_energy_error_vs_chi256=float(np.max(np.abs(errs))),
+                note="sequential solves, each recomputing its plans at every step")
 
 
 def main():
@@ -270,6 +309,8 @@ def main():
     if not a.no_eig and a.prec == "fp64":
         # last: the solves overwrite psi and Hpsi
         out["eigsolve"] = eig_section(sb, a, prob, plans)
+    if a.dmrg_solves > 0 and a.prec == "fp64" and not a.no_eig:
+        out["dmrg_solve"] = dmrg_section(sb, a)
 
     if a.calibrate_copy > 0:
         n = int(a.calibrate_copy * 1e6 / 8)
