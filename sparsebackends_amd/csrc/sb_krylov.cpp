@@ -22,6 +22,11 @@ void k_gather_blocks_by_key(Ctx*, i64 n, i64 bs, const i32* xids, const i64* off
 void k_match_keys(Ctx*, const i32* xk, i64 nx, const i32* yk, const i32* yids, i64 ny, int P, const i64* pdims,
                   i64 bs, i64* off);
 void k_add_blocks(Ctx*, i64 n, i64 bs, const i64* soff, const double* src, const i64* doff, double* dst);
+void k_match_rows(Ctx*, const i32* xk, i64 nx, const i32* yk, i64 ny, int P, const i64* pdims, i32* row);
+void k_box_exceeds(Ctx*, i64 n, int N2, const i32* S, const i32* ybox, const i32* row, i32* flag);
+void k_support(Ctx*, i64 n, int N2, const i32* xbox, const i32* ybox, const i32* row, i32* S);
+void k_compact(Ctx*, int mode, i64 nrows, int N2, const i64* tdims, const i32* S, const i64* coff, i64 total,
+               const i64* boff, const double* src, double* dst);
 
 namespace {
 
@@ -311,39 +316,123 @@ EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tens
   if (maxiter < 1) fail(SB_E_ARGUMENT, "ArgumentError: maxiter must be positive");
   if (!(tol >= 0)) fail(SB_E_ARGUMENT, "ArgumentError: tol must be non-negative");
 
-  const i64 nb = x.n, bs = x.blksize, n = nb * bs;
-  if (n == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve of an empty vector");
+  const i64 nb = x.n, bs = x.blksize, nfull = nb * bs;
+  if (nfull == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve of an empty vector");
+  const int N2 = x.N2;
+  std::vector<i64> td(x.dims.begin() + x.P(), x.dims.end());
+  EigResult R;
+
+  // ---- v0 = x / |x| in x's payload
+  const double b0 = norm_of(ctx, x.dptr(), nfull);
+  if (!(b0 > 0) || !std::isfinite(b0)) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: initial vector has zero or non-finite norm");
+  // x's own nonzero extents per block (box)
+  DBuf<i32> xb0(ctx, (size_t)(nb * N2 ? nb * N2 : 1)), xf0(ctx, (size_t)(nb ? nb : 1));
+  box_scan(ctx, x.dptr(), x.ids.get(), nb, bs, N2, td.data(), xb0.get(), xf0.get());
+  k_axpby(ctx, nfull, 1.0 / b0, x.dptr(), 0.0, x.dptr());
+  x.touch();
+  for (auto* p : plans) p->run();  // H v0
+  ++R.numops;
+
+  // ---- the compact layout of the Krylov vectors: per block of x, a box S
+  // holding every nonzero any Krylov vector has: x0's extents joined with
+  // those of H v0, widened later if an application reaches beyond (the
+  // padding of symmetry sectors is left out).  When y's boxes are not known
+  // (an operator that does not end in a box product) S is the full block.
+  DBuf<i64> off;  // y's block offsets by x row (-1: y lacks the key)
+  match_blocks(ctx, x, y, off);  // blocks of y whose key x lacks are left out: the solve stays in x's block space
+  const bool track = y.box_trusted();
+  DBuf<i32> yrow(ctx, (size_t)(nb ? nb : 1)), S(ctx, (size_t)(nb * N2 ? nb * N2 : 1));
+  if (track) {
+    std::vector<i64> pd(x.dims.begin(), x.dims.begin() + x.P());
+    k_match_rows(ctx, x.keys.get(), nb, y.keys.get(), y.n, x.P(), pd.data(), yrow.get());
+    k_support(ctx, nb, N2, xb0.get(), y.box.get(), yrow.get(), S.get());
+  } else {
+    k_fill_rows(ctx, S.get(), nb, N2, td.data());
+  }
+  std::vector<i32> hid(nb);
+  HIP_CHECK(hipMemcpyAsync(hid.data(), x.ids.get(), nb * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<i64> hxoff(nb);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (i64 i = 0; i < nb; ++i) hxoff[i] = (i64)(hid[i] - 1) * bs;
+  DBuf<i64> coff(ctx, nb + 1), xoff(ctx, nb ? nb : 1);
+  HIP_CHECK(hipMemcpyAsync(xoff.get(), hxoff.data(), nb * sizeof(i64), hipMemcpyHostToDevice, ctx->stream));
+  i64 n = 0;  // Krylov vector length
+  auto layout = [&]() {  // coff from S; returns the length
+    std::vector<i32> hS((size_t)nb * N2);
+    if (!hS.empty())
+      HIP_CHECK(hipMemcpyAsync(hS.data(), S.get(), hS.size() * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<i64> hc(nb + 1, 0);
+    for (i64 i = 0; i < nb; ++i) {
+      i64 c = 1;
+      for (int a = 0; a < N2; ++a) c *= hS[i * N2 + a];
+      hc[i + 1] = hc[i] + c;
+    }
+    HIP_CHECK(hipMemcpyAsync(coff.get(), hc.data(), (nb + 1) * sizeof(i64), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return hc[nb];
+  };
+  n = layout();
+  if (n == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: the vector and its image are empty");
+
   const int NBP = linalg_nb();
   const int ld = kdim + 1;
   DBuf<double> V(ctx, (size_t)(kdim + 1) * n), part(ctx, (size_t)(kdim + 2) * NBP);
   DBuf<double> small(ctx, 3 * (size_t)ld + 3), dU(ctx, (size_t)kdim * kdim), tmp;
+  DBuf<i32> viol(ctx, 1);
   double* h1 = small.get();
   double* h2 = h1 + ld + 1;
   double* nr = h2 + ld + 1;
   auto vec = [&](int i) { return V.get() + (i64)i * n; };
+  // v0 and H v0 in the compact layout
+  k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), x.dptr(), vec(0));
+  k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, off.get(), y.dptr(), vec(1));
 
-  // y's blocks in x's payload layout
-  // y holds the block list the last plan fixed when it was created
-  DBuf<i64> off;
-  match_blocks(ctx, x, y, off);  // blocks of y whose key x lacks are left out: the solve stays in x's block space
+  // S widened to y's boxes where an application reached beyond it: the
+  // vectors 0..nv-1 move to the new layout (they are zero in the new entries)
+  auto widen = [&](int nv) {
+    DBuf<double> full(ctx, (size_t)nfull);
+    DBuf<i32> S2(ctx, (size_t)(nb * N2 ? nb * N2 : 1));
+    k_support(ctx, nb, N2, S.get(), y.box.get(), yrow.get(), S2.get());
+    DBuf<i64> coff_old(ctx, nb + 1);
+    HIP_CHECK(hipMemcpyAsync(coff_old.get(), coff.get(), (nb + 1) * sizeof(i64), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+    const i64 n_old = n;
+    DBuf<i32> S_old = std::move(S);
+    S = std::move(S2);
+    n = layout();
+    DBuf<double> V2(ctx, (size_t)(kdim + 1) * n);
+    for (int i = 0; i < nv; ++i) {
+      k_zero(ctx, full.get(), (size_t)nfull * sizeof(double));
+      k_compact(ctx, 1, nb, N2, td.data(), S_old.get(), coff_old.get(), n_old, xoff.get(), V.get() + (i64)i * n_old,
+                full.get());
+      k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), full.get(), V2.get() + (i64)i * n);
+    }
+    V = std::move(V2);
+    tmp.release();
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  };
 
-  EigResult R;
-  // v0 = x / |x|
-  HIP_CHECK(hipMemcpyAsync(vec(0), x.dptr(), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  const double b0 = norm_of(ctx, vec(0), n);
-  if (!(b0 > 0) || !std::isfinite(b0)) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: initial vector has zero or non-finite norm");
-  k_axpby(ctx, n, 1.0 / b0, vec(0), 0.0, vec(0));
-
+  // later applications: the basis vector into x's payload (entries outside S
+  // stay zero: x0 is zero there), the operator, y's blocks by key into the
+  // compact layout; y's boxes are checked against S
   auto matvec = [&](int src, int dst) {
-    step(ctx, "lanczos_load", [&] {
-      HIP_CHECK(hipMemcpyAsync(x.dptr(), vec(src), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    });
+    step(ctx, "lanczos_load",
+         [&] { k_compact(ctx, 1, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), vec(src), x.dptr()); });
     x.touch();
     for (auto* p : plans) p->run();
-    step(ctx, "lanczos_gather",
-         [&] { k_gather_blocks_by_key(ctx, nb, bs, x.ids.get(), off.get(), y.dptr(), vec(dst)); });
     ++R.numops;
+    if (track) {
+      k_box_exceeds(ctx, nb, N2, S.get(), y.box.get(), yrow.get(), viol.get());
+      i32 hv = 0;
+      HIP_CHECK(hipMemcpyAsync(&hv, viol.get(), sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      if (hv) widen(dst);
+    }
+    step(ctx, "lanczos_gather",
+         [&] { k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, off.get(), y.dptr(), vec(dst)); });
   };
+  bool have_first = true;  // H v0 is in vec(1) already
 
   std::vector<double> T((size_t)ld * ld, 0.0), hh(3 * (size_t)ld + 3), theta, U;
   int j = 0;
@@ -352,7 +441,10 @@ EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tens
   bool converged = false;
   int m = 0;
   for (;;) {
-    matvec(j, j + 1);
+    if (have_first)
+      have_first = false;
+    else
+      matvec(j, j + 1);
     double* w = vec(j + 1);
     // classical Gram-Schmidt, twice, against v_0 .. v_j
     step(ctx, "lanczos_orth", [&] {
@@ -407,12 +499,14 @@ EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tens
     ++cycles;
   }
   // Ritz vector of the lowest Ritz value into x, normalised
+  DBuf<double> ritzv(ctx, (size_t)n);
   step(ctx, "lanczos_ritz", [&] {
     HIP_CHECK(hipMemcpyAsync(dU.get(), U.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    k_combine(ctx, V.get(), n, m, dU.get(), 1, x.dptr(), n, n);
+    k_combine(ctx, V.get(), n, m, dU.get(), 1, ritzv.get(), n, n);
   });
-  const double xn = norm_of(ctx, x.dptr(), n);
-  if (xn > 0) k_axpby(ctx, n, 1.0 / xn, x.dptr(), 0.0, x.dptr());
+  const double xn = norm_of(ctx, ritzv.get(), n);
+  if (xn > 0) k_axpby(ctx, n, 1.0 / xn, ritzv.get(), 0.0, ritzv.get());
+  k_compact(ctx, 1, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), ritzv.get(), x.dptr());
   x.touch();
   HIP_CHECK(hipStreamSynchronize(ctx->stream));
   R.value = theta[0];

@@ -218,4 +218,121 @@ void k_match_keys(Ctx* ctx, const i32* xk, i64 nx, const i32* yk, const i32* yid
   HIP_CHECK(hipGetLastError());
 }
 
+
+// row of y holding the key of each row of x (-1 if none); keys sorted by the
+// column-major lin in both
+__global__ void match_rows_kernel(const i32* __restrict__ xk, i64 nx, const i32* __restrict__ yk, i64 ny, int P,
+                                  IArr pd, i32* __restrict__ row) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  auto lin = [&](const i32* k) {
+    u64 l = 0, s = 1;
+    for (int d = 0; d < P; ++d) {
+      l += (u64)(k[d] - 1) * s;
+      s *= (u64)pd.v[d];
+    }
+    return l;
+  };
+  const u64 v = lin(xk + i * P);
+  i64 a = 0, b = ny;
+  while (a < b) {
+    const i64 m = (a + b) >> 1;
+    if (lin(yk + m * P) < v)
+      a = m + 1;
+    else
+      b = m;
+  }
+  row[i] = (a < ny && lin(yk + a * P) == v) ? (i32)a : -1;
+}
+void k_match_rows(Ctx* ctx, const i32* xk, i64 nx, const i32* yk, i64 ny, int P, const i64* pdims, i32* row) {
+  if (!nx) return;
+  IArr pd{};
+  for (int d = 0; d < P; ++d) pd.v[d] = pdims[d];
+  hipLaunchKernelGGL(match_rows_kernel, dim3((unsigned)((nx + RT - 1) / RT)), dim3(RT), 0, ctx->stream, xk, nx, yk,
+                     ny, P, pd, row);
+  HIP_CHECK(hipGetLastError());
+}
+
+// S[i, a] = max(xbox[i, a], ybox[row[i], a]) (row[i] < 0: xbox only)
+__global__ void support_kernel(i64 n, int N2, const i32* __restrict__ xbox, const i32* __restrict__ ybox,
+                               const i32* __restrict__ row, i32* __restrict__ S) {
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * N2) return;
+  const i64 i = t / N2;
+  const int a = (int)(t - i * N2);
+  int v = xbox[t];
+  if (row[i] >= 0) v = max(v, ybox[(i64)row[i] * N2 + a]);
+  S[t] = v;
+}
+void k_support(Ctx* ctx, i64 n, int N2, const i32* xbox, const i32* ybox, const i32* row, i32* S) {
+  if (!n || !N2) return;
+  hipLaunchKernelGGL(support_kernel, dim3((unsigned)((n * N2 + RT - 1) / RT)), dim3(RT), 0, ctx->stream, n, N2, xbox,
+                     ybox, row, S);
+  HIP_CHECK(hipGetLastError());
+}
+
+// compact layout: row i's box S[i, :] (column-major, axis 0 fastest) packed at
+// coff[i]; element t of the compact vector <-> entry of the row's block at
+// boff[i] (boff < 0: absent, reads as zero).  mode 0: gather full -> compact;
+// mode 1: scatter compact -> full (only inside the boxes)
+__global__ void compact_kernel(int mode, i64 nrows, int N2, IArr td, const i32* __restrict__ S,
+                               const i64* __restrict__ coff, const i64* __restrict__ boff,
+                               const double* __restrict__ src, double* __restrict__ dst) {
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 total = coff[nrows];
+  if (t >= total) return;
+  i64 a = 0, b = nrows - 1;  // last row with coff <= t
+  while (a < b) {
+    const i64 m = (a + b + 1) >> 1;
+    if (coff[m] <= t)
+      a = m;
+    else
+      b = m - 1;
+  }
+  const i64 i = a;
+  i64 x = t - coff[i], off = 0, st = 1;
+  for (int q = 0; q < N2; ++q) {
+    const i64 e = S[i * N2 + q];
+    const i64 qq = x / e;
+    off += (x - qq * e) * st;
+    st *= td.v[q];
+    x = qq;
+  }
+  const i64 bo = boff[i];
+  if (mode == 0)
+    dst[t] = bo >= 0 ? src[bo + off] : 0.0;
+  else if (bo >= 0)
+    dst[bo + off] = src[t];
+}
+void k_compact(Ctx* ctx, int mode, i64 nrows, int N2, const i64* tdims, const i32* S, const i64* coff, i64 total,
+               const i64* boff, const double* src, double* dst) {
+  if (!total) return;
+  IArr td{};
+  for (int q = 0; q < N2; ++q) td.v[q] = tdims[q];
+  hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((total + RT - 1) / RT)), dim3(RT), 0, ctx->stream, mode, nrows,
+                     N2, td, S, coff, boff, src, dst);
+  HIP_CHECK(hipGetLastError());
+}
+
+
+// flag = 1 if some y box reaches beyond S (matched rows); written with a
+// plain store so it is cleared first by a one-thread kernel
+__global__ void box_exceeds_kernel(i64 n, int N2, const i32* __restrict__ S, const i32* __restrict__ ybox,
+                                   const i32* __restrict__ row, i32* __restrict__ flag) {
+  const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * N2) return;
+  const i64 i = t / N2;
+  const int a = (int)(t - i * N2);
+  if (row[i] >= 0 && ybox[(i64)row[i] * N2 + a] > S[t]) *flag = 1;
+}
+__global__ void clear_flag_kernel(i32* flag) { *flag = 0; }
+void k_box_exceeds(Ctx* ctx, i64 n, int N2, const i32* S, const i32* ybox, const i32* row, i32* flag) {
+  hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, ctx->stream, flag);
+  HIP_CHECK(hipGetLastError());
+  if (!n || !N2) return;
+  hipLaunchKernelGGL(box_exceeds_kernel, dim3((unsigned)((n * N2 + RT - 1) / RT)), dim3(RT), 0, ctx->stream, n, N2,
+                     S, ybox, row, flag);
+  HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace sb
