@@ -173,7 +173,10 @@ int sb_contract_bs_bs_to_dense(sb_ctx_t ctx, sb_tensor_t Cdense, const uint64_t*
  * values, which must keep their block structure (same keys, ids and payload
  * buffers).  For the families whose output blocks depend on values (rank-1 over a
  * dense-tail index, COO products, the zero-slice flag) the block list of the
- * first call is kept.  Plans of one context may be executed back to back and
+ * first call is kept; in the rank-1 families (contract_dense_bb!,
+ * contract_dense_bd!) the r slices that contribute follow each replay's values,
+ * in the COO families the contributing (entry, block) pairs are the first
+ * call's.  Plans of one context may be executed back to back and
  * captured into a hipGraph by the caller. */
 int sb_plan_create(sb_ctx_t ctx, sb_tensor_t C, const uint64_t* labelsC, int nC, sb_tensor_t A,
                    const uint64_t* labelsA, int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags,

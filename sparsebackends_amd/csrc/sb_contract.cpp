@@ -896,11 +896,27 @@ static void dense_bb(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& 
   i32* seg = pl.keep(std::move(al.seg_start));
   i32* order = pl.keep(std::move(al.order));
   unsigned char* km = pl.keep(std::move(kmask));
-  pl.keep(std::move(pr.a));
-  pl.keep(std::move(pr.b));
+  const i32* pa = pl.keep(std::move(pr.a));
+  const i32* pb = pl.keep(std::move(pr.b));
+  // existence was decided on the current values and replays keep that block
+  // list; which r slices contribute (both slices nonzero, :374-379) follows
+  // the values of every replay
+  {
+    const i64* oA = pl.keep(block_offsets(ctx, TA));
+    const i64* oB = pl.keep(block_offsets(ctx, TB));
+    unsigned char* fa = pl.keep(DBuf<unsigned char>(ctx, nA * R));
+    unsigned char* fb = pl.keep(DBuf<unsigned char>(ctx, nB * R));
+    unsigned char* kp = pl.keep(DBuf<unsigned char>(ctx, np));
+    const double* dA = TA.data.get();
+    const double* dB = TB.data.get();
+    pl.steps.push_back({"rank1_mask", [=]() {
+                          slice_nonzero(ctx, dA, oA, nA, R, chunkA, fa);
+                          slice_nonzero(ctx, dB, oB, nB, R, chunkB, fb);
+                          k_rank1_exist(ctx, fa, fb, pa, pb, np, R, kp, km);
+                        }});
+  }
   // A block (chunkA x R), B block (chunkB x R): C[i,j] = sum_r A[i,r] B[j,r]
   GemmDesc d{chunkA, chunkB, R, 1, chunkA, chunkB, 1, AB ? 1 : chunkB, AB ? chunkA : 1};
-  // existence was decided on the current values; replays keep that block list
   add_gemm_step(pl, flags, d, TA.data.get(), aoff, TB.data.get(), boff, C.data.get(), coff, seg, order, ng, km);
 }
 
@@ -1080,7 +1096,22 @@ static void dense_bd(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Oper& 
   i32* seg = pl.keep(std::move(al.seg_start));
   i32* order = pl.keep(std::move(al.order));
   unsigned char* km = pl.keep(std::move(kmask));
-  pl.keep(std::move(pr.a));
+  const i32* pa = pl.keep(std::move(pr.a));
+  // the contributing r slices follow the values of every replay (the block
+  // list stays the first call's)
+  {
+    const i64* oA = pl.keep(block_offsets(ctx, TA));
+    unsigned char* fa = pl.keep(DBuf<unsigned char>(ctx, nA * R));
+    unsigned char* fb = pl.keep(DBuf<unsigned char>(ctx, R));
+    unsigned char* kp = pl.keep(DBuf<unsigned char>(ctx, np));
+    const double* dA = TA.data.get();
+    const double* dB = Bt->dptr();
+    pl.steps.push_back({"rank1_mask", [=]() {
+                          slice_nonzero(ctx, dA, oA, nA, R, chunkA, fa);
+                          slice_nonzero(ctx, dB, nullptr, 1, R, chunkB, fb);
+                          k_rank1_exist(ctx, fa, fb, pa, nullptr, np, R, kp, km);
+                        }});
+  }
   GemmDesc d{chunkA, chunkB, R, 1, chunkA, chunkB, 1, AB ? 1 : chunkB, AB ? chunkA : 1};
   add_gemm_step(pl, flags, d, TA.data.get(), aoff, Bt->dptr(), boff, C.data.get(), coff, seg, order, ng, km);
 }

@@ -177,3 +177,55 @@ def test_errors():
     with pytest.raises(sb.SBError) as e:
         sb.eigsolve(plans, prob.psi)
     assert e.value.kind == "ArgumentError"
+
+
+def tridiag_operator(n):
+    Mat = np.diag(np.full(n, 2.0)) + np.diag(np.full(n - 1, -1.0), 1) + np.diag(np.full(n - 1, -1.0), -1)
+    Mat[3, 3] = 0.5  # break the uniform spectrum
+    return Mat
+
+
+LAB = lambda s: (int.from_bytes(("t:" + s).encode().ljust(8, b"\0"), "little"), 0)  # noqa: E731
+
+
+def test_compact_layout_widens():
+    """an operator whose reach grows one entry per application (a symmetric
+    tridiagonal matrix on a tail axis, applied as contract_shared! with the
+    block products of H_eff's psi*L step) from a start vector holding one
+    entry: the compact Krylov layout is widened at every step and the result
+    equals the matrix's lowest eigenpair"""
+    n = 12
+    Mat = tridiag_operator(n)
+    x0 = np.zeros(n)
+    x0[0] = 1.0
+    X = sb.NewBlockSparseSorted.from_parts((1, n), 1, np.array([[1]]), np.array([1]), x0)
+    Y = sb.NewBlockSparseSorted.new((1, n), 1)
+    Mt = sb.NewBlockSparseSorted.from_parts((1, 1, n, n), 2, np.array([[1, 1]]), np.array([1]),
+                                            Mat.reshape(-1, order="F"))
+    plan = sb.Plan(Y, [LAB("Qp"), LAB("j")], X, [LAB("Q"), LAB("i")], Mt, [LAB("Qp"), LAB("Q"), LAB("i"), LAB("j")])
+    print("[eigsolve widening]", flush=True)
+    vals, _, info = sb.eigsolve([plan], X, krylovdim=n, maxiter=3, tol=1e-12)
+    w, U = np.linalg.eigh(Mat)
+    assert info.converged == 1
+    assert abs(vals[0] - w[0]) <= 1e-12
+    x = X.parts()["data"]
+    assert abs(abs(x @ U[:, 0]) - 1.0) <= 1e-10
+
+
+def test_rank1_plan_replay_follows_values():
+    """a single-label BS x BS plan over a dense-tail index (contract_dense_bb!,
+    GER per r): replayed with a different zero pattern, every r slice that is
+    nonzero now contributes (the block list stays the first call's)"""
+    n = 12
+    Mat = tridiag_operator(n)
+    x0 = np.zeros(n)
+    x0[0] = 1.0
+    X = sb.NewBlockSparseSorted.from_parts((1, n), 1, np.array([[1]]), np.array([1]), x0)
+    Y = sb.NewBlockSparseSorted.new((1, n), 1)
+    Mt = sb.NewBlockSparseSorted.from_parts((n, n), 2, np.zeros((1, 0), dtype=np.int64), np.array([1]),
+                                            Mat.reshape(-1, order="F"))
+    plan = sb.Plan(Y, [LAB("Q"), LAB("j")], X, [LAB("Q"), LAB("i")], Mt, [LAB("i"), LAB("j")])
+    v = np.arange(1.0, n + 1.0)
+    X.set_data_(v)
+    plan.execute()
+    np.testing.assert_allclose(Y.parts()["data"], Mat.T @ v, rtol=1e-14, atol=1e-12)
