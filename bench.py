@@ -16,7 +16,9 @@ The environments are symmetrised so that H_eff is symmetric; "eigsolve" then
 reports the ground energy by FP64 Lanczos, the mixed-precision solve's
 energy difference and time-to-energy against it, and solves/s with p50/p99
 latency of the DMRG-step solve (krylovdim 3, maxiter 1); "dmrg_solve" times
-whole two-site DMRG ground-state solves of XXZ chains (solves/s, p50/p99).
+whole two-site DMRG ground-state solves of XXZ chains (solves/s, p50/p99);
+"batch" runs 64 independent J1-J2 H_eff*psi problems plan by plan and as one
+batch (one box product launch per contraction stage over all problems).
 
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
@@ -24,7 +26,8 @@ block pairs with the actual sector extents) / time per step, in TFLOP/s.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
                        [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--no-eig]
                        [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--dmrg-solves N]
-                       [--dmrg-sites N] [--dmrg-chi X] [--calibrate-copy MB]
+                       [--dmrg-sites N] [--dmrg-chi X] [--batch-problems N] [--batch-reps R]
+                       [--calibrate-copy MB]
 """
 from __future__ import annotations
 
@@ -64,6 +67,9 @@ def parse():
     ap.add_argument("--dmrg-solves", type=int, default=8, help="full DMRG solves (0: skip)")
     ap.add_argument("--dmrg-sites", type=int, default=16)
     ap.add_argument("--dmrg-chi", type=int, default=64)
+    ap.add_argument("--batch-problems", type=int, default=64,
+                    help="independent J1-J2 problems of the batching measurement (0: skip)")
+    ap.add_argument("--batch-reps", type=int, default=10)
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -156,6 +162,72 @@ def dmrg_section(sb, a):
                 solves_per_s=len(lat) / lat.sum(), p50_s=float(np.percentile(lat, 50)),
                 p99_s=float(np.percentile(lat, 99)), max_energy_error_vs_chi256=float(np.max(np.abs(errs))),
                 note="sequential solves, each recomputing its plans at every step")
+
+
+def batch_section(sb, a):
+    """Many independent problems at once (the batching configuration): the
+    two-site H_eff*psi of a J1-J2 chain's centre bond for each of
+    a.batch_problems seeds (J2 uniform in [0, 0.5], chi log-uniform in
+    [64, 1024] in 17 Sz sectors).  One matvec of the batch = the four
+    contractions of every problem; run plan by plan (sequential) and as a
+    PlanBatch (one box product launch per stage over all problems), FP64 and
+    FP32 forms; the two give identical bits."""
+    from sparsebackends_amd import heff as H
+    t0 = time.perf_counter()
+    probs = H.j1j2_batch_problems(a.batch_problems)
+    t_gen = time.perf_counter() - t0
+    alg = sum(p.algorithmic_flops()["total"] for p, _, _ in probs)
+    chis = [c for _, _, c in probs]
+    res = dict(problems=len(probs), model="S=1/2 J1-J2 chains, J1=1, J2~U[0,0.5], U(1) Sz, MPO w=8, centre bond",
+               chi_min=min(chis), chi_max=max(chis), algorithmic_gflop_per_matvec=alg / 1e9, generate_s=t_gen,
+               reps=a.batch_reps)
+
+    def timed(fn, reps):
+        fn()
+        sb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        sb.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    for p, _, _ in probs:
+        p.upload()
+    for name, fl in (("fp64", sb.PREC_FP64), ("fp32", sb.PREC_FP32)):
+        sb.synchronize()
+        t0 = time.perf_counter()
+        for p, _, _ in probs:
+            p.outputs().make_plans(fl)
+        batch = sb.PlanBatch([p.plans for p, _, _ in probs])
+        sb.synchronize()
+        t_plan = time.perf_counter() - t0
+
+        def seq():
+            for p, _, _ in probs:
+                p.execute()
+        t_seq = timed(seq, a.batch_reps)
+        ref = [p.Hpsi.parts()["data"].copy() for p, _, _ in probs]
+        sb.profile(True)
+        sb.profile_reset()
+        t_bat = timed(batch.execute, a.batch_reps)
+        prof = sb.profile_read()
+        sb.profile(False)
+        same = all(np.array_equal(p.Hpsi.parts()["data"], r) for (p, _, _), r in zip(probs, ref))
+        gemm_ms = prof.get("grouped_gemm", {}).get("total_ms", 0.0) / (a.batch_reps + 1)
+        r = dict(sequential_ms=t_seq * 1e3, batched_ms=t_bat * 1e3, sequential_tflops=alg / t_seq / 1e12,
+                 batched_tflops=alg / t_bat / 1e12, speedup=t_seq / t_bat, bit_identical=bool(same),
+                 fused_stages=batch.stats()["fused_stages"], plan_s=t_plan,
+                 kernels_ms_per_matvec={k: v["total_ms"] / (a.batch_reps + 1) for k, v in prof.items()})
+        if gemm_ms > 0:
+            peak = PEAK[name]
+            r["gemm_roofline"] = dict(bound="mfma", achieved=alg / (gemm_ms * 1e-3) / 1e12, peak=peak,
+                                      unit="TFLOP/s", frac=alg / (gemm_ms * 1e-3) / 1e12 / peak,
+                                      note="the four batched grouped_gemm launches of one matvec")
+        res[name] = r
+        del batch
+        for p, _, _ in probs:
+            p.plans = None
+    return res
 
 
 def main():
@@ -311,6 +383,9 @@ def main():
         out["eigsolve"] = eig_section(sb, a, prob, plans)
     if a.dmrg_solves > 0 and a.prec == "fp64" and not a.no_eig:
         out["dmrg_solve"] = dmrg_section(sb, a)
+
+    if a.batch_problems > 0 and a.prec == "fp64" and not a.no_eig:
+        out["batch"] = batch_section(sb, a)
 
     if a.calibrate_copy > 0:
         n = int(a.calibrate_copy * 1e6 / 8)

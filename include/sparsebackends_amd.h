@@ -36,6 +36,7 @@ extern "C" {
 typedef struct sb_ctx_s* sb_ctx_t;      /* device + stream + device-memory cache */
 typedef struct sb_tensor_s* sb_tensor_t; /* a device-resident tensor of one storage kind */
 typedef struct sb_plan_s* sb_plan_t;     /* a contraction with its block structure fixed */
+typedef struct sb_batch_s* sb_batch_t;   /* independent plan chains executed together */
 
 /* ------------------------------------------------------------ runtime */
 const char* sb_last_error(void);
@@ -189,6 +190,23 @@ int sb_plan_stats(sb_plan_t plan, double* flops, double* gemm_bytes, double* per
 /* distinct payload bytes of the block-product operands (A, B) and result C */
 int sb_plan_operand_bytes(sb_plan_t plan, double* a_bytes, double* b_bytes, double* c_bytes);
 int sb_plan_free(sb_plan_t plan);
+
+/* New (no counterpart in the reference, which contracts one problem per call;
+ * it is the batched schedule the many-small-problems workloads need): a batch
+ * of nchains independent plan chains of nstages plans each, plans[c*nstages+s]
+ * being stage s of chain c (e.g. the four contractions of one H_eff*psi per
+ * problem).  sb_batch_execute runs stage by stage; where every plan of a stage
+ * is one box product of one precision form (the strided BS x BS route), the
+ * stage is ONE device launch over the union of the chains' work lists, with
+ * bit-identical results to executing each plan alone.  Other stages run their
+ * plans one after the other.  active (ascending chain numbers; NULL or
+ * nactive = 0: all) selects the chains to execute.  The batch refers to the
+ * plans, which must outlive it. */
+int sb_batch_create(sb_ctx_t ctx, const sb_plan_t* plans, int nchains, int nstages, sb_batch_t* out);
+int sb_batch_execute(sb_ctx_t ctx, sb_batch_t batch, const int* active, int nactive);
+/* flops of one execution of every chain; number of fused stages */
+int sb_batch_stats(sb_batch_t batch, double* flops, int* fused_stages);
+int sb_batch_free(sb_batch_t batch);
 
 /* ------------------------------------------------------------ Krylov vector operations
  * The storage hooks a Krylov solver reaches through ITensors' ExternalStorage

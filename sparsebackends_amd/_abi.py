@@ -82,6 +82,10 @@ SIGNATURES = {
     "sb_plan_stats": [P, F64P, F64P, F64P, I64P, I64P],
     "sb_plan_operand_bytes": [P, F64P, F64P, F64P],
     "sb_plan_free": [P],
+    "sb_batch_create": [P, PP, C.c_int, C.c_int, PP],
+    "sb_batch_execute": [P, P, I32P, C.c_int],
+    "sb_batch_stats": [P, F64P, I32P],
+    "sb_batch_free": [P],
     "sb_bs_norm": [P, P, F64P],
     "sb_bs_copy": [P, P, PP],
     "sb_bs_similar": [P, P, PP],

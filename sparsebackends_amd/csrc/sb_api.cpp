@@ -5,6 +5,7 @@
 #include <set>
 
 #include "../../include/sparsebackends_amd.h"
+#include "sb_batch.h"
 #include "sb_factorize.h"
 #include "sb_krylov.h"
 
@@ -558,6 +559,40 @@ int sb_plan_operand_bytes(sb_plan_t p, double* a_bytes, double* b_bytes, double*
   })
 }
 int sb_plan_free(sb_plan_t p) { SB_TRY(delete reinterpret_cast<Plan*>(p)) }
+
+int sb_batch_create(sb_ctx_t c, const sb_plan_t* plans, int nchains, int nstages, sb_batch_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!plans || !out) fail(SB_E_ARGUMENT, "ArgumentError: null argument");
+    if (nchains < 1 || nstages < 1) fail(SB_E_ARGUMENT, "ArgumentError: a batch needs at least one chain and one stage");
+    std::vector<Plan*> v((size_t)nchains * nstages);
+    for (size_t i = 0; i < v.size(); ++i) v[i] = reinterpret_cast<Plan*>(plans[i]);
+    *out = reinterpret_cast<sb_batch_t>(make_batch(ctx, v, nchains, nstages).release());
+  })
+}
+int sb_batch_execute(sb_ctx_t c, sb_batch_t b, const int* active, int nactive) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!b) fail(SB_E_HANDLE, "null batch");
+    Batch* bt = reinterpret_cast<Batch*>(b);
+    if (bt->ctx != ctx) fail(SB_E_ARGUMENT, "ArgumentError: the batch belongs to another context");
+    if (nactive < 0 || (nactive > 0 && !active)) fail(SB_E_ARGUMENT, "ArgumentError: active chains");
+    bt->run(std::vector<int>(active, active + (active ? nactive : 0)));
+  })
+}
+int sb_batch_stats(sb_batch_t b, double* flops, int* fused_stages) {
+  SB_TRY({
+    if (!b) fail(SB_E_HANDLE, "null batch");
+    Batch* bt = reinterpret_cast<Batch*>(b);
+    if (flops) *flops = bt->flops;
+    if (fused_stages) {
+      int f = 0;
+      for (auto& s : bt->stages) f += s.fused;
+      *fused_stages = f;
+    }
+  })
+}
+int sb_batch_free(sb_batch_t b) { SB_TRY(delete reinterpret_cast<Batch*>(b)) }
 
 int sb_bs_norm(sb_ctx_t c, sb_tensor_t A, double* out) { SB_TRY(*out = bs_norm(C_(c), *T_(A, BS))) }
 int sb_bs_copy(sb_ctx_t c, sb_tensor_t A, sb_tensor_t* out) { SB_TRY(*out = H_(bs_copy(C_(c), *T_(A, BS)))) }

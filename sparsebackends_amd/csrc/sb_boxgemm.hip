@@ -102,16 +102,13 @@ void k_fill_rows(Ctx* ctx, i32* p, i64 n, int w, const i64* row) {
 // two operands, widened where an operand holds a non-finite value), the new C
 // box, whether the old C box reaches beyond it (then the GEMM kernel clears the
 // difference), and the group's tile count at its position in the order.
-__global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ gorder, const i32* __restrict__ seg,
-                                  const i32* __restrict__ arow, const i32* __restrict__ brow,
-                                  const i32* __restrict__ abox, const i32* __restrict__ aflag,
-                                  const i32* __restrict__ bbox, const i32* __restrict__ bflag, i32* __restrict__ gbox,
-                                  i32* __restrict__ kbox, i32* __restrict__ kc, i32* __restrict__ cbox,
-                                  i32* __restrict__ cflag, i32* __restrict__ cnew, i32* __restrict__ clear,
-                                  i32* __restrict__ tiles, int BM, int BN) {
-  i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ng) return;
-  const i64 g = gorder ? gorder[i] : i;
+__device__ __forceinline__ void box_group_one(const BoxDesc& d, const i64 g, const i32* __restrict__ seg,
+                                              const i32* __restrict__ arow, const i32* __restrict__ brow,
+                                              const i32* __restrict__ abox, const i32* __restrict__ aflag,
+                                              const i32* __restrict__ bbox, const i32* __restrict__ bflag,
+                                              i32* __restrict__ gbox, i32* __restrict__ kbox, i32* __restrict__ kc,
+                                              i32* __restrict__ cbox, i32* __restrict__ cflag, i32* __restrict__ cnew,
+                                              i32* __restrict__ clear, i32& tiles, int BM, int BN) {
   int mb[MAXAX], nb[MAXAX];
 #pragma unroll
   for (int a = 0; a < MAXAX; ++a) mb[a] = nb[a] = 0;
@@ -187,7 +184,21 @@ __global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ gor
       if (!need) cbox[g * d.NC2 + a] = cb[a];
     }
   cflag[g] = 0;
-  tiles[i] = (Mc && Nc) ? ((Mc + BM - 1) / BM) * ((Nc + BN - 1) / BN) : 0;
+  tiles = (Mc && Nc) ? ((Mc + BM - 1) / BM) * ((Nc + BN - 1) / BN) : 0;
+}
+
+__global__ void box_groups_kernel(BoxDesc d, i64 ng, const i32* __restrict__ gorder, const i32* __restrict__ seg,
+                                  const i32* __restrict__ arow, const i32* __restrict__ brow,
+                                  const i32* __restrict__ abox, const i32* __restrict__ aflag,
+                                  const i32* __restrict__ bbox, const i32* __restrict__ bflag, i32* __restrict__ gbox,
+                                  i32* __restrict__ kbox, i32* __restrict__ kc, i32* __restrict__ cbox,
+                                  i32* __restrict__ cflag, i32* __restrict__ cnew, i32* __restrict__ clear,
+                                  i32* __restrict__ tiles, int BM, int BN) {
+  i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng) return;
+  const i64 g = gorder ? gorder[i] : i;
+  box_group_one(d, g, seg, arow, brow, abox, aflag, bbox, bflag, gbox, kbox, kc, cbox, cflag, cnew, clear, tiles[i], BM,
+                BN);
 }
 
 // work list from the device-wide scan of the per-position tile counts (toff[0] = 0,
@@ -198,6 +209,59 @@ __global__ void tile_records_kernel(const i32* __restrict__ toff, i64 ng, const 
   if (i >= ng) return;
   const i32 g = gorder ? gorder[i] : (i32)i;
   for (i32 t = toff[i]; t < toff[i + 1]; ++t) trec[t] = make_int2(g, t - toff[i]);
+}
+
+// the active job owning batch position i: the last a with gbase[a] <= i
+__device__ __forceinline__ int job_of(const i64* __restrict__ gbase, int nact, i64 i) {
+  int lo = 0, hi = nact;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (gbase[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// box_groups_kernel over the positions of every active job of a batch
+__global__ void box_groups_batch_kernel(const BoxJob* __restrict__ jobs, const i32* __restrict__ act,
+                                        const i64* __restrict__ gbase, int nact, i64 npos, i32* __restrict__ tiles,
+                                        int BM, int BN) {
+  i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npos) return;
+  const int a = job_of(gbase, nact, i);
+  const BoxJob& J = jobs[act[a]];
+  const i64 pos = i - gbase[a];
+  const i64 g = J.gorder ? J.gorder[pos] : pos;
+  box_group_one(J.d, g, J.seg, J.arow, J.brow, J.abox, J.aflag, J.bbox, J.bflag, J.gbox, J.kbox, J.kc, J.cbox,
+                J.cflag, J.cnew, J.clear, tiles[i], BM, BN);
+}
+
+__global__ void tile_records_batch_kernel(const i32* __restrict__ toff, i64 npos, const BoxJob* __restrict__ jobs,
+                                          const i32* __restrict__ act, const i64* __restrict__ gbase, int nact,
+                                          int4* __restrict__ trec) {
+  i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npos) return;
+  const int a = job_of(gbase, nact, i);
+  const i32 j = act[a];
+  const i64 pos = i - gbase[a];
+  const i32 g = jobs[j].gorder ? jobs[j].gorder[pos] : (i32)pos;
+  for (i32 t = toff[i]; t < toff[i + 1]; ++t) trec[t] = make_int4(j, g, t - toff[i], 0);
+}
+
+void box_prepare_batch(Ctx* ctx, const BoxBatchArgs& b) {
+  if (b.npos == 0) return;
+  const BoxTile t = box_tile(b.cfg);
+  hipLaunchKernelGGL(box_groups_batch_kernel, dim3(nb256(b.npos)), dim3(256), 0, ctx->stream, b.jobs, b.act, b.gbase,
+                     b.nact, b.npos, b.tiles, t.BM, t.BN);
+  HIP_CHECK(hipGetLastError());
+  size_t tb = b.scan_tmp_bytes;
+  HIP_CHECK(rocprim::inclusive_scan(b.scan_tmp, tb, b.tiles, b.toff + 1, (size_t)b.npos, rocprim::plus<i32>(),
+                                    ctx->stream));
+  hipLaunchKernelGGL(tile_records_batch_kernel, dim3(nb256(b.npos)), dim3(256), 0, ctx->stream, b.toff, b.npos,
+                     b.jobs, b.act, b.gbase, b.nact, b.trec);
+  HIP_CHECK(hipGetLastError());
 }
 
 size_t box_scan_tmp_bytes(Ctx* ctx, i64 ng) {
@@ -260,6 +324,246 @@ __device__ inline void split3_b(float x, unsigned short& h, unsigned short& m, u
 
 constexpr int NXCD = 8;  // XCDs of the MI355X: consecutive workgroups go round-robin over them
 
+// zero the entries of C block g that fell outside its shrunken box, then
+// install the new box (one workgroup)
+__device__ __forceinline__ void box_clear_stale(const BoxDesc& d, const i64 g, const i64* __restrict__ coff,
+                                                const i32* __restrict__ cnew, i32* __restrict__ cbox,
+                                                double* __restrict__ C, const int tid) {
+  int ob[MAXAX], nbx[MAXAX];
+  i64 vol = 1;
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a)
+    if (a < d.NC2) {
+      ob[a] = cbox[g * d.NC2 + a];
+      nbx[a] = cnew[g * d.NC2 + a];
+      vol *= ob[a];
+    }
+  double* base = C + coff[g];
+  for (i64 e = tid; e < vol; e += 256) {
+    i64 x = e, off = 0, st = 1;
+    bool inside = true;
+#pragma unroll
+    for (int a = 0; a < MAXAX; ++a)
+      if (a < d.NC2) {
+        i64 q = x / ob[a];
+        int r = (int)(x - q * ob[a]);
+        inside = inside && r < nbx[a];
+        off += r * st;
+        st *= d.c_ext[a];
+        x = q;
+      }
+    if (!inside) base[off] = 0.0;
+  }
+  __syncthreads();  // every thread has read the old box before it is replaced
+  if (tid < d.NC2) cbox[g * d.NC2 + tid] = nbx[tid];
+}
+
+// one (group, tile) of the work list: C_g[tile] = sum over the group's pairs of A_p * B_p
+template <int BM, int BN, int WM, int WN, int PREC, class T>
+__device__ __forceinline__ void box_tile(const BoxDesc& d, const i32 g, const i32 lt, const i32* __restrict__ seg,
+                                         const i64* __restrict__ aoff, const i64* __restrict__ boff,
+                                         const i64* __restrict__ coff, const i32* __restrict__ gbox,
+                                         const i32* __restrict__ kbox, const i32* __restrict__ kc,
+                                         const double* __restrict__ A, const double* __restrict__ B,
+                                         double* __restrict__ C, i32* __restrict__ cflag, const int amf, const int bnf,
+                                         T (&As)[16][BM + 2], T (&Bs)[16][BN + 2], int (&rowA)[BM], int (&rowC)[BM],
+                                         int (&colB)[BN], int (&colC)[BN], i64 (&tabA)[512], i64 (&tabB)[512]) {
+  constexpr int BK = 16, KT = 512;
+  constexpr int FM = BM / (16 * WM), FN = BN / (16 * WN);
+  constexpr int AE = BM * BK / 256, BE = BN * BK / 256;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave % WM, wn = wave / WM;
+  const int nmn = d.nm + d.nn;
+  int mb[MAXAX], nbx[MAXAX];
+  int Mc = 1, Nc = 1;
+#pragma unroll
+  for (int a = 0; a < MAXAX; ++a) {
+    mb[a] = a < d.nm ? gbox[(i64)g * nmn + a] : 1;
+    nbx[a] = a < d.nn ? gbox[(i64)g * nmn + d.nm + a] : 1;
+    Mc *= mb[a];
+    Nc *= nbx[a];
+  }
+  const int tiles_m = (Mc + BM - 1) / BM;
+  const int tm = lt % tiles_m, tn = lt / tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  __syncthreads();  // the previous tile is done with the LDS tables
+  for (int r = tid; r < BM; r += 256) {
+    int oa = -1, oc = -1;
+    if (m0 + r < Mc) decode2(m0 + r, d.nm, mb, d.m_sa, d.m_sc, oa, oc);
+    rowA[r] = oa;
+    rowC[r] = oc;
+  }
+  for (int c = tid; c < BN; c += 256) {
+    int ob = -1, oc = -1;
+    if (n0 + c < Nc) decode2(n0 + c, d.nn, nbx, d.n_sb, d.n_sc, ob, oc);
+    colB[c] = ob;
+    colC[c] = oc;
+  }
+  d4 accd[FM][FN];
+  f4 accf[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      accd[i][j] = d4{0, 0, 0, 0};
+      accf[i][j] = f4{0, 0, 0, 0};
+    }
+  const i32 p1 = seg[g + 1];
+  i32 pp = seg[g];
+  int kin = 0;
+  while (pp < p1) {
+    __syncthreads();
+    int fill = 0;
+    while (pp < p1 && fill < KT) {
+      const int kcp = kc[pp];
+      const int take = min(kcp - kin, KT - fill);
+      if (take > 0) {
+        int kb[MAXAX];
+#pragma unroll
+        for (int a = 0; a < MAXAX; ++a) kb[a] = a < d.nk ? kbox[(i64)pp * d.nk + a] : 1;
+        const i64 ao = aoff[pp], bo = boff[pp];
+        for (int t = tid; t < take; t += 256) {
+          int oa, ob;
+          decode2(kin + t, d.nk, kb, d.k_sa, d.k_sb, oa, ob);
+          tabA[fill + t] = ao + oa;
+          tabB[fill + t] = bo + ob;
+        }
+      }
+      fill += take;
+      kin += take;
+      if (kin >= kcp) {
+        ++pp;
+        kin = 0;
+      }
+    }
+    __syncthreads();
+    double ra[AE], rb[BE];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int e = 0; e < AE; ++e) {
+        const int idx = tid + e * 256;
+        const int m = amf ? idx % BM : idx / BK, k = amf ? idx / BM : idx % BK;
+        const int r = rowA[m];
+        ra[e] = (r >= 0 && k0 + k < fill) ? A[tabA[k0 + k] + r] : 0.0;
+      }
+#pragma unroll
+      for (int e = 0; e < BE; ++e) {
+        const int idx = tid + e * 256;
+        const int n = bnf ? idx % BN : idx / BK, k = bnf ? idx / BN : idx % BK;
+        const int c = colB[n];
+        rb[e] = (c >= 0 && k0 + k < fill) ? B[tabB[k0 + k] + c] : 0.0;
+      }
+    };
+    load(0);
+    for (int k0 = 0; k0 < fill; k0 += BK) {
+      if (k0) __syncthreads();  // the previous chunk's products are done with As / Bs
+#pragma unroll
+      for (int e = 0; e < AE; ++e) {
+        const int idx = tid + e * 256;
+        const int m = amf ? idx % BM : idx / BK, k = amf ? idx / BM : idx % BK;
+        As[k][m] = (T)ra[e];
+      }
+#pragma unroll
+      for (int e = 0; e < BE; ++e) {
+        const int idx = tid + e * 256;
+        const int n = bnf ? idx % BN : idx / BK, k = bnf ? idx / BN : idx % BK;
+        Bs[k][n] = (T)rb[e];
+      }
+      __syncthreads();
+      if (k0 + BK < fill) load(k0 + BK);  // in flight while this chunk multiplies
+      if constexpr (PREC == 0) {
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+          double a[FM], b[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + (lane >> 4)][(wm * FM + i) * 16 + (lane & 15)];
+#pragma unroll
+          for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + (lane >> 4)][(wn * FN + j) * 16 + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              accd[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], accd[i][j], 0, 0, 0);
+        }
+      } else if constexpr (PREC == 1) {
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+          float a[FM], b[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + (lane >> 4)][(wm * FM + i) * 16 + (lane & 15)];
+#pragma unroll
+          for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + (lane >> 4)][(wn * FN + j) * 16 + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              accf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], accf[i][j], 0, 0, 0);
+        }
+      } else {
+        // bf16x3: both operands split in three bf16 terms; the six products of
+        // order <= 2 accumulate in FP32
+#pragma unroll
+        for (int kk = 0; kk < BK / 16; ++kk) {
+        s4 ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            unsigned short h, m, l;
+            split3_b(As[kk * 16 + 4 * (lane >> 4) + v][(wm * FM + i) * 16 + (lane & 15)], h, m, l);
+            ah[i][v] = (short)h;
+            am[i][v] = (short)m;
+            al[i][v] = (short)l;
+          }
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            unsigned short h, m, l;
+            split3_b(Bs[kk * 16 + 4 * (lane >> 4) + v][(wn * FN + j) * 16 + (lane & 15)], h, m, l);
+            bh[j][v] = (short)h;
+            bm[j][v] = (short)m;
+            bl[j][v] = (short)l;
+          }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            f4 c = accf[i][j];
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bm[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bm[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bh[j], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], c, 0, 0, 0);
+            accf[i][j] = c;
+          }
+        }
+      }
+    }
+  }
+  double* Cb = C + coff[g];
+  int bad = 0;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // output row of register v: f64 16x16x4 holds rows 4*v + lane/16;
+        // the f32 / bf16 16x16 forms hold rows 4*(lane/16) + v
+        const int ml = (wm * FM + i) * 16 + (PREC == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
+        const int nl = (wn * FN + j) * 16 + (lane & 15);
+        const int rc = rowC[ml], cc = colC[nl];
+        if (rc >= 0 && cc >= 0) {
+          const double val = PREC == 0 ? accd[i][j][v] : (double)accf[i][j][v];
+          Cb[rc + cc] = val;
+          bad |= !isfinite(val);
+        }
+      }
+  if (bad) atomicOr(&cflag[g], 1);
+}
+
 // One workgroup (4 waves) per (group, tile) of the work list, persistent over
 // the list.  The list is cut into NXCD contiguous ranges, one per XCD, so the
 // tiles of neighbouring groups (which share operand blocks) meet in one L2.
@@ -269,8 +573,10 @@ constexpr int NXCD = 8;  // XCDs of the MI355X: consecutive workgroups go round-
 // next K chunk are loaded into registers while the current one multiplies.
 // Before the tiles, the workgroups clear the C entries that fell out of a
 // shrunken box (disjoint from every tile's writes).  PREC as grouped_gemm.
+// The tiles are latency-bound (short K per tile), so the register budget is held
+// to five resident waves per SIMD, one more than the unconstrained allocation gives.
 template <int BM, int BN, int WM, int WN, int PREC>
-__global__ __launch_bounds__(256) void box_gemm_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void box_gemm_kernel(
     BoxDesc d, i64 ng, const i32* __restrict__ gorder, const i32* __restrict__ seg, const i64* __restrict__ aoff,
     const i64* __restrict__ boff, const i64* __restrict__ coff, const i32* __restrict__ gbox,
     const i32* __restrict__ kbox, const i32* __restrict__ kc, const i32* __restrict__ toff,
@@ -286,41 +592,11 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
   __shared__ T Bs[BK][BN + 2];
   __shared__ int rowA[BM], rowC[BM], colB[BN], colC[BN];
   __shared__ i64 tabA[KT], tabB[KT];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int wm = wave % WM, wn = wave / WM;
-  const int nmn = d.nm + d.nn;
+  const int tid = threadIdx.x;
 
   // ---- stale entries of shrunken C boxes
-  for (i64 g = blockIdx.x; g < ng; g += gridDim.x) {
-    if (!clear[g]) continue;
-    int ob[MAXAX], nbx[MAXAX];
-    i64 vol = 1;
-#pragma unroll
-    for (int a = 0; a < MAXAX; ++a)
-      if (a < d.NC2) {
-        ob[a] = cbox[g * d.NC2 + a];
-        nbx[a] = cnew[g * d.NC2 + a];
-        vol *= ob[a];
-      }
-    double* base = C + coff[g];
-    for (i64 e = tid; e < vol; e += 256) {
-      i64 x = e, off = 0, st = 1;
-      bool inside = true;
-#pragma unroll
-      for (int a = 0; a < MAXAX; ++a)
-        if (a < d.NC2) {
-          i64 q = x / ob[a];
-          int r = (int)(x - q * ob[a]);
-          inside = inside && r < nbx[a];
-          off += r * st;
-          st *= d.c_ext[a];
-          x = q;
-        }
-      if (!inside) base[off] = 0.0;
-    }
-    __syncthreads();  // every thread has read the old box before it is replaced
-    if (tid < d.NC2) cbox[g * d.NC2 + tid] = nbx[tid];
-  }
+  for (i64 g = blockIdx.x; g < ng; g += gridDim.x)
+    if (clear[g]) box_clear_stale(d, g, coff, cnew, cbox, C, tid);
 
   // ---- tiles
   const i32 total = toff[ng];
@@ -329,195 +605,42 @@ __global__ __launch_bounds__(256) void box_gemm_kernel(
   const i32 wlo = xcd * chunk, whi = min(total, wlo + chunk);
   for (i32 w = wlo + slot; w < whi; w += nper) {
     const int2 rec = trec[w];
-    const i32 g = rec.x, lt = rec.y;
-    int mb[MAXAX], nbx[MAXAX];
-    int Mc = 1, Nc = 1;
-#pragma unroll
-    for (int a = 0; a < MAXAX; ++a) {
-      mb[a] = a < d.nm ? gbox[(i64)g * nmn + a] : 1;
-      nbx[a] = a < d.nn ? gbox[(i64)g * nmn + d.nm + a] : 1;
-      Mc *= mb[a];
-      Nc *= nbx[a];
-    }
-    const int tiles_m = (Mc + BM - 1) / BM;
-    const int tm = lt % tiles_m, tn = lt / tiles_m;
-    const int m0 = tm * BM, n0 = tn * BN;
-    __syncthreads();  // the previous tile is done with the LDS tables
-    for (int r = tid; r < BM; r += 256) {
-      int oa = -1, oc = -1;
-      if (m0 + r < Mc) decode2(m0 + r, d.nm, mb, d.m_sa, d.m_sc, oa, oc);
-      rowA[r] = oa;
-      rowC[r] = oc;
-    }
-    for (int c = tid; c < BN; c += 256) {
-      int ob = -1, oc = -1;
-      if (n0 + c < Nc) decode2(n0 + c, d.nn, nbx, d.n_sb, d.n_sc, ob, oc);
-      colB[c] = ob;
-      colC[c] = oc;
-    }
-    d4 accd[FM][FN];
-    f4 accf[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        accd[i][j] = d4{0, 0, 0, 0};
-        accf[i][j] = f4{0, 0, 0, 0};
-      }
-    const i32 p1 = seg[g + 1];
-    i32 pp = seg[g];
-    int kin = 0;
-    while (pp < p1) {
-      __syncthreads();
-      int fill = 0;
-      while (pp < p1 && fill < KT) {
-        const int kcp = kc[pp];
-        const int take = min(kcp - kin, KT - fill);
-        if (take > 0) {
-          int kb[MAXAX];
-#pragma unroll
-          for (int a = 0; a < MAXAX; ++a) kb[a] = a < d.nk ? kbox[(i64)pp * d.nk + a] : 1;
-          const i64 ao = aoff[pp], bo = boff[pp];
-          for (int t = tid; t < take; t += 256) {
-            int oa, ob;
-            decode2(kin + t, d.nk, kb, d.k_sa, d.k_sb, oa, ob);
-            tabA[fill + t] = ao + oa;
-            tabB[fill + t] = bo + ob;
-          }
-        }
-        fill += take;
-        kin += take;
-        if (kin >= kcp) {
-          ++pp;
-          kin = 0;
-        }
-      }
-      __syncthreads();
-      double ra[AE], rb[BE];
-      auto load = [&](int k0) {
-#pragma unroll
-        for (int e = 0; e < AE; ++e) {
-          const int idx = tid + e * 256;
-          const int m = amf ? idx % BM : idx / BK, k = amf ? idx / BM : idx % BK;
-          const int r = rowA[m];
-          ra[e] = (r >= 0 && k0 + k < fill) ? A[tabA[k0 + k] + r] : 0.0;
-        }
-#pragma unroll
-        for (int e = 0; e < BE; ++e) {
-          const int idx = tid + e * 256;
-          const int n = bnf ? idx % BN : idx / BK, k = bnf ? idx / BN : idx % BK;
-          const int c = colB[n];
-          rb[e] = (c >= 0 && k0 + k < fill) ? B[tabB[k0 + k] + c] : 0.0;
-        }
-      };
-      load(0);
-      for (int k0 = 0; k0 < fill; k0 += BK) {
-        if (k0) __syncthreads();  // the previous chunk's products are done with As / Bs
-#pragma unroll
-        for (int e = 0; e < AE; ++e) {
-          const int idx = tid + e * 256;
-          const int m = amf ? idx % BM : idx / BK, k = amf ? idx / BM : idx % BK;
-          As[k][m] = (T)ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < BE; ++e) {
-          const int idx = tid + e * 256;
-          const int n = bnf ? idx % BN : idx / BK, k = bnf ? idx / BN : idx % BK;
-          Bs[k][n] = (T)rb[e];
-        }
-        __syncthreads();
-        if (k0 + BK < fill) load(k0 + BK);  // in flight while this chunk multiplies
-        if constexpr (PREC == 0) {
-#pragma unroll
-          for (int kk = 0; kk < BK / 4; ++kk) {
-            double a[FM], b[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + (lane >> 4)][(wm * FM + i) * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + (lane >> 4)][(wn * FN + j) * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-              for (int j = 0; j < FN; ++j)
-                accd[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], accd[i][j], 0, 0, 0);
-          }
-        } else if constexpr (PREC == 1) {
-#pragma unroll
-          for (int kk = 0; kk < BK / 4; ++kk) {
-            float a[FM], b[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) a[i] = As[kk * 4 + (lane >> 4)][(wm * FM + i) * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < FN; ++j) b[j] = Bs[kk * 4 + (lane >> 4)][(wn * FN + j) * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-              for (int j = 0; j < FN; ++j)
-                accf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], accf[i][j], 0, 0, 0);
-          }
-        } else {
-          // bf16x3: both operands split in three bf16 terms; the six products of
-          // order <= 2 accumulate in FP32
-#pragma unroll
-          for (int kk = 0; kk < BK / 16; ++kk) {
-          s4 ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
-#pragma unroll
-          for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              unsigned short h, m, l;
-              split3_b(As[kk * 16 + 4 * (lane >> 4) + v][(wm * FM + i) * 16 + (lane & 15)], h, m, l);
-              ah[i][v] = (short)h;
-              am[i][v] = (short)m;
-              al[i][v] = (short)l;
-            }
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              unsigned short h, m, l;
-              split3_b(Bs[kk * 16 + 4 * (lane >> 4) + v][(wn * FN + j) * 16 + (lane & 15)], h, m, l);
-              bh[j][v] = (short)h;
-              bm[j][v] = (short)m;
-              bl[j][v] = (short)l;
-            }
-#pragma unroll
-          for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-              f4 c = accf[i][j];
-              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bm[j], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bm[j], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[i], bh[j], c, 0, 0, 0);
-              c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], c, 0, 0, 0);
-              accf[i][j] = c;
-            }
-          }
-        }
-      }
-    }
-    double* Cb = C + coff[g];
-    int bad = 0;
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          // output row of register v: f64 16x16x4 holds rows 4*v + lane/16;
-          // the f32 / bf16 16x16 forms hold rows 4*(lane/16) + v
-          const int ml = (wm * FM + i) * 16 + (PREC == 0 ? 4 * v + (lane >> 4) : 4 * (lane >> 4) + v);
-          const int nl = (wn * FN + j) * 16 + (lane & 15);
-          const int rc = rowC[ml], cc = colC[nl];
-          if (rc >= 0 && cc >= 0) {
-            const double val = PREC == 0 ? accd[i][j][v] : (double)accf[i][j][v];
-            Cb[rc + cc] = val;
-            bad |= !isfinite(val);
-          }
-        }
-    if (bad) atomicOr(&cflag[g], 1);
+    box_tile<BM, BN, WM, WN, PREC>(d, rec.x, rec.y, seg, aoff, boff, coff, gbox, kbox, kc, A, B, C, cflag, amf, bnf, As,
+                                   Bs, rowA, rowC, colB, colC, tabA, tabB);
+  }
+}
+
+// The same over the work list of a batch of jobs: each tile reads its job's
+// descriptor (uniform per workgroup, so scalar loads), every job's stale C
+// entries are cleared first.
+template <int BM, int BN, int WM, int WN, int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void box_gemm_batch_kernel(
+    const BoxJob* __restrict__ jobs, const i32* __restrict__ act, const i64* __restrict__ gbase, int nact, i64 npos,
+    const i32* __restrict__ toff, const int4* __restrict__ trec) {
+  constexpr int BK = 16, KT = 512;
+  using T = typename std::conditional<PREC == 0, double, float>::type;
+  __shared__ T As[BK][BM + 2];
+  __shared__ T Bs[BK][BN + 2];
+  __shared__ int rowA[BM], rowC[BM], colB[BN], colC[BN];
+  __shared__ i64 tabA[KT], tabB[KT];
+  const int tid = threadIdx.x;
+
+  for (i64 i = blockIdx.x; i < npos; i += gridDim.x) {
+    const int a = job_of(gbase, nact, i);
+    const BoxJob& J = jobs[act[a]];
+    const i64 g = i - gbase[a];  // every group of the job once, in any order
+    if (J.clear[g]) box_clear_stale(J.d, g, J.coff, J.cnew, J.cbox, J.C, tid);
+  }
+
+  const i32 total = toff[npos];
+  const int xcd = blockIdx.x % NXCD, slot = blockIdx.x / NXCD, nper = gridDim.x / NXCD;
+  const i32 chunk = (total + NXCD - 1) / NXCD;
+  const i32 wlo = xcd * chunk, whi = min(total, wlo + chunk);
+  for (i32 w = wlo + slot; w < whi; w += nper) {
+    const int4 rec = trec[w];
+    const BoxJob& J = jobs[rec.x];
+    box_tile<BM, BN, WM, WN, PREC>(J.d, rec.y, rec.z, J.seg, J.aoff, J.boff, J.coff, J.gbox, J.kbox, J.kc, J.A, J.B,
+                                   J.C, J.cflag, J.amf, J.bnf, As, Bs, rowA, rowC, colB, colC, tabA, tabB);
   }
 }
 
@@ -757,6 +880,41 @@ static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const do
                      g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox,
                      g.amf, g.bnf);
   HIP_CHECK(hipGetLastError());
+}
+
+template <int BM, int BN, int WM, int WN, int PREC>
+static void launch_one_batch(Ctx* ctx, const BoxBatchArgs& b) {
+  auto kern = box_gemm_batch_kernel<BM, BN, WM, WN, PREC>;
+  static int per_cu = 0;
+  i64 nwg = grid_for(ctx, kern, per_cu);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, b.jobs, b.act, b.gbase, b.nact, b.npos,
+                     b.toff, b.trec);
+  HIP_CHECK(hipGetLastError());
+}
+
+template <int PREC>
+static void launch_box_batch(Ctx* ctx, const BoxBatchArgs& b) {
+  switch (b.cfg) {
+    case 0: launch_one_batch<16, 128, 1, 4, PREC>(ctx, b); break;
+    case 1: launch_one_batch<128, 16, 4, 1, PREC>(ctx, b); break;
+    case 2: launch_one_batch<32, 32, 2, 2, PREC>(ctx, b); break;
+    case 3: launch_one_batch<64, 32, 2, 2, PREC>(ctx, b); break;
+    case 4: launch_one_batch<32, 64, 2, 2, PREC>(ctx, b); break;
+    case 5: launch_one_batch<64, 64, 2, 2, PREC>(ctx, b); break;
+    case 6: launch_one_batch<16, 64, 1, 4, PREC>(ctx, b); break;
+    case 7: launch_one_batch<64, 16, 4, 1, PREC>(ctx, b); break;
+    default: fail(SB_E_ASSERT, "batched box GEMM takes workgroup tiles only");
+  }
+}
+
+void box_gemm_batch(Ctx* ctx, unsigned prec, const BoxBatchArgs& b) {
+  if (b.npos == 0) return;
+  switch (prec & SB_PREC_MASK) {
+    case SB_PREC_FP64: launch_box_batch<0>(ctx, b); break;
+    case SB_PREC_FP32: launch_box_batch<1>(ctx, b); break;
+    case SB_PREC_BF16X3: launch_box_batch<2>(ctx, b); break;
+    default: fail(SB_E_ARGUMENT, "unknown precision form");
+  }
 }
 
 template <int FM, int FN, int PREC>

@@ -615,6 +615,7 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
   i32* cflag = C.bflag.get();
   pl.steps.push_back({"grouped_gemm", [ctx, flags, g, Ad, Bd, Cdp, cflag]() { box_gemm(ctx, flags, g, Ad, Bd, Cdp, cflag); }});
   pl.out_box = true;
+  if (pl.steps.size() == 3) pl.box = std::make_shared<BoxPlan>(BoxPlan{g, pA, pB, pC, flags, cfull});
 }
 
 // contract_shared! BS x BS through permuted operand copies (the reference's

@@ -24,6 +24,18 @@ struct Step {
   void operator()() const { fn(); }
 };
 
+// The block product of a plan whose numeric phase is exactly box_scan,
+// box_prepare, grouped_gemm (the strided BS x BS product): what a batch needs
+// to run it inside a launch shared with other plans.
+struct BoxPlan {
+  BoxGemmArgs g;
+  const Tensor* A = nullptr;
+  const Tensor* B = nullptr;
+  Tensor* C = nullptr;
+  unsigned flags = 0;
+  std::vector<i64> cfull;  // C's full tail extents
+};
+
 struct Plan {
   Ctx* ctx = nullptr;
   std::vector<Step> steps;
@@ -50,6 +62,7 @@ struct Plan {
   bool out_box = false;
   const Tensor* in_a = nullptr;  // the operands the plan reads at every execution
   const Tensor* in_b = nullptr;
+  std::shared_ptr<BoxPlan> box;  // set when the steps are one box product
   void finish() const {
     if (!out) return;
     out->touch();

@@ -149,6 +149,36 @@ struct BoxGemmArgs {
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
 };
+// One box GEMM of a batch, as a device-resident descriptor: the fixed plan
+// arrays and per-execution buffers of a BoxGemmArgs with its operands.
+struct BoxJob {
+  BoxDesc d;
+  i64 ng;
+  int amf, bnf;
+  const i32 *gorder, *seg, *arow, *brow;
+  const i64 *aoff, *boff, *coff;
+  const i32 *abox, *aflag, *bbox, *bflag;
+  i32 *gbox, *kbox, *kc, *cnew, *clear, *cbox, *cflag;
+  const double* A;
+  const double* B;
+  double* C;
+};
+// A launch over the group positions of several jobs (the active ones, act[0..nact)):
+// job act[a] owns positions gbase[a] .. gbase[a+1]-1; one work list over all of them.
+struct BoxBatchArgs {
+  const BoxJob* jobs = nullptr;
+  const i32* act = nullptr;
+  const i64* gbase = nullptr;  // nact + 1 entries
+  int nact = 0, cfg = 0;
+  i64 npos = 0;
+  i32 *tiles = nullptr, *toff = nullptr;  // per position; toff[0] = 0
+  int4* trec = nullptr;                   // work list: {job, group, tile within the group, 0}
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+};
+void box_prepare_batch(Ctx*, const BoxBatchArgs&);
+void box_gemm_batch(Ctx*, unsigned prec, const BoxBatchArgs&);
+
 // boxes and flags of n blocks (block i at (ids[i]-1)*bs, tail extents tdims)
 void box_scan(Ctx*, const double* data, const i32* ids, i64 n, i64 bs, int N2, const i64* tdims, i32* box,
               i32* flag);

@@ -446,6 +446,23 @@ def hubbard_chi2048_problem(seed=0, sigma=7.0, device=0, hermitian=False):
     return HeffProblem(hubbard_model(1.0, 4.0), Bl, Br, seed=seed, device=device, hermitian=hermitian)
 
 
+def j1j2_batch_problems(count=64, chi_lo=64, chi_hi=1024, seed0=0, device=0, hermitian=True):
+    """Independent J1-J2 H_eff*psi problems at the centre bond of a long chain:
+    problem i (seed seed0 + i) draws J2 uniform in [0, 0.5] and chi log-uniform
+    in [chi_lo, chi_hi] from numpy's PCG64 with that seed; both bonds hold
+    chi in Sz sectors q in {-8..8} with dims ~ exp(-q^2/32).  Returns a list of
+    (HeffProblem, J2, chi)."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed0 + i)
+        J2 = float(rng.uniform(0.0, 0.5))
+        chi = int(round(float(np.exp(rng.uniform(np.log(chi_lo), np.log(chi_hi))))))
+        B = heisenberg_bond(chi)
+        out.append((HeffProblem(j1j2_model(1.0, J2), B, B, seed=seed0 + i, device=device, hermitian=hermitian), J2,
+                    chi))
+    return out
+
+
 # ------------------------------------------------------------------ exact chains
 
 

@@ -94,3 +94,47 @@ class Plan:
                 self._h = C.c_void_p()
         except Exception:
             pass
+
+
+class PlanBatch:
+    """Independent plan chains executed together: chains[c] is the list of
+    plans of problem c (every chain the same length).  Stage s of all chains
+    runs as one box product launch when each of its plans is one (the strided
+    BS x BS route), with results identical to executing the plans one by one;
+    other stages run plan by plan.  execute(active) runs the chains numbered in
+    active (default all)."""
+
+    def __init__(self, chains):
+        chains = [list(ch) for ch in chains]
+        if not chains or not chains[0]:
+            raise ValueError("a batch needs at least one chain of at least one plan")
+        ns = len(chains[0])
+        if any(len(ch) != ns for ch in chains):
+            raise ValueError("every chain of a batch must have the same number of plans")
+        self._chains = chains  # the batch refers to the plans
+        self.ctx = chains[0][0].ctx
+        arr = (C.c_void_p * (len(chains) * ns))(*[p._h.value for ch in chains for p in ch])
+        h = C.c_void_p()
+        check(load().sb_batch_create(self.ctx, C.cast(arr, C.POINTER(C.c_void_p)), len(chains), ns, C.byref(h)))
+        self._h = h
+        self.nchains, self.nstages = len(chains), ns
+
+    def execute(self, active=None):
+        if active is None:
+            check(load().sb_batch_execute(self.ctx, self._h, None, 0))
+        else:
+            a = np.ascontiguousarray(active, dtype=np.int32)
+            check(load().sb_batch_execute(self.ctx, self._h, a.ctypes.data_as(C.POINTER(C.c_int)), len(a)))
+
+    def stats(self):
+        f, n = C.c_double(), C.c_int()
+        check(load().sb_batch_stats(self._h, C.byref(f), C.byref(n)))
+        return dict(flops=f.value, fused_stages=n.value)
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().sb_batch_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
