@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--batch-problems", type=int, default=64,
                     help="independent J1-J2 problems of the batching measurement (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=10)
+    ap.add_argument("--batch-kdim", type=int, default=16)
+    ap.add_argument("--batch-eig-tol", type=float, default=1e-8, help="0: skip the batched ground-state solves")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -223,11 +225,43 @@ def batch_section(sb, a):
             r["gemm_roofline"] = dict(bound="mfma", achieved=alg / (gemm_ms * 1e-3) / 1e12, peak=peak,
                                       unit="TFLOP/s", frac=alg / (gemm_ms * 1e-3) / 1e12 / peak,
                                       note="the four batched grouped_gemm launches of one matvec")
+        if name == "fp64" and a.batch_eig_tol > 0:
+            r["eigsolve"] = batch_eig(sb, a, probs, batch)
         res[name] = r
         del batch
         for p, _, _ in probs:
             p.plans = None
     return res
+
+
+def batch_eig(sb, a, probs, batch):
+    """Ground states of all problems of the batch: FP64 Lanczos (krylovdim
+    a.batch_kdim, tol a.batch_eig_tol) one problem after the other, then all
+    of them co-batched (one batched operator application per iteration for
+    the solves still running); the energies agree bit for bit."""
+    x0 = [p.psi.copy() for p, _, _ in probs]
+    kw = dict(krylovdim=a.batch_kdim, maxiter=50, tol=a.batch_eig_tol)
+    sb.synchronize()
+    t0 = time.perf_counter()
+    e_seq, ops = [], 0
+    for (p, _, _), x in zip(probs, x0):
+        p.psi.axpby_(1.0, x, 0.0)
+        vals, _, info = sb.eigsolve(p.plans, p.psi, **kw)
+        e_seq.append(vals[0])
+        ops += info.numops
+    sb.synchronize()
+    t_seq = time.perf_counter() - t0
+    for (p, _, _), x in zip(probs, x0):
+        p.psi.axpby_(1.0, x, 0.0)
+    sb.synchronize()
+    t0 = time.perf_counter()
+    e_bat, infos = sb.eigsolve_batch(batch, [p.psi for p, _, _ in probs], **kw)
+    sb.synchronize()
+    t_bat = time.perf_counter() - t0
+    return dict(krylovdim=a.batch_kdim, tol=a.batch_eig_tol, solves=len(probs), operator_applications=ops,
+                converged=int(sum(i.converged for i in infos)), sequential_s=t_seq, batched_s=t_bat,
+                sequential_solves_per_s=len(probs) / t_seq, batched_solves_per_s=len(probs) / t_bat,
+                speedup=t_seq / t_bat, energies_identical=bool(e_seq == e_bat))
 
 
 def main():

@@ -253,6 +253,17 @@ int sb_eigsolve_lanczos(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_ten
                         int krylovdim, int maxiter, double tol, double* value, double* ritz, int* nritz,
                         double* normres, int64_t* numops, int* numiter, int* converged);
 
+/* New (no counterpart in the reference, whose eigsolve takes one operator):
+ * the same solve for every chain c of a batch, from xs[c] (first plan of
+ * chain c reads it) to ys[c] (its last plan writes it), in lock step: each
+ * iteration applies the chains still running as ONE sb_batch_execute and
+ * synchronises once for all of them.  Every chain's result and Ritz vector
+ * are bit for bit what sb_eigsolve_lanczos gives for it alone.  Output
+ * arrays (may be NULL) hold nchains entries. */
+int sb_eigsolve_lanczos_batch(sb_ctx_t ctx, sb_batch_t batch, const sb_tensor_t* xs, const sb_tensor_t* ys,
+                              int nchains, int krylovdim, int maxiter, double tol, double* values, double* normres,
+                              int64_t* numops, int* numiter, int* converged);
+
 /* ------------------------------------------------------------ two-site SVD
  * blocksparse_svd (SparseBackends/src/ops_factorize.jl:50-148; ITensor-level
  * wrapper itensor_blocksparse_svd, tensor_wrappers.jl:975-1005) of a

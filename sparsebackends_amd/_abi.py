@@ -97,6 +97,8 @@ SIGNATURES = {
     "sb_bs_svd": [P, P, C.c_int, C.c_int64, C.c_int64, C.c_double, PP, PP, F64P, C.c_int64, I64P, F64P],
     "sb_bs_svd_sectors": [P, P, C.c_int64, I64P, I64P, I64P, I64P, C.c_int, C.c_int64, C.c_int64, C.c_double, PP,
                           PP, I64P, F64P, C.c_int64, I64P, F64P],
+    "sb_eigsolve_lanczos_batch": [P, P, PP, PP, C.c_int, C.c_int, C.c_int, C.c_double, F64P, F64P, I64P, I32P,
+                                  I32P],
     "sb_eigsolve_lanczos": [P, PP, C.c_int, P, P, C.c_int, C.c_int, C.c_double, F64P, F64P, I32P, F64P, I64P,
                             I32P, I32P],
 }

@@ -631,6 +631,32 @@ int sb_eigsolve_lanczos(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_tenso
   })
 }
 
+int sb_eigsolve_lanczos_batch(sb_ctx_t c, sb_batch_t b, const sb_tensor_t* xs, const sb_tensor_t* ys, int nchains,
+                              int krylovdim, int maxiter, double tol, double* values, double* normres,
+                              int64_t* numops, int* numiter, int* converged) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!b) fail(SB_E_HANDLE, "null batch");
+    Batch* bt = reinterpret_cast<Batch*>(b);
+    if (bt->ctx != ctx) fail(SB_E_ARGUMENT, "ArgumentError: the batch belongs to another context");
+    if (!xs || !ys || nchains != bt->nchains)
+      fail(SB_E_ARGUMENT, "ArgumentError: one vector x and one output y per chain of the batch");
+    std::vector<Tensor*> vx(nchains), vy(nchains);
+    for (int i = 0; i < nchains; ++i) {
+      vx[i] = T_(xs[i], BS);
+      vy[i] = T_(ys[i], BS);
+    }
+    auto r = eigsolve_lanczos_batch(ctx, *bt, vx, vy, krylovdim, maxiter, tol);
+    for (int i = 0; i < nchains; ++i) {
+      if (values) values[i] = r[i].value;
+      if (normres) normres[i] = r[i].normres;
+      if (numops) numops[i] = r[i].numops;
+      if (numiter) numiter[i] = r[i].numiter;
+      if (converged) converged[i] = r[i].converged ? 1 : 0;
+    }
+  })
+}
+
 int sb_bs_svd(sb_ctx_t c, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_t mindim, double cutoff,
               sb_tensor_t* U, sb_tensor_t* SV, double* svs, int64_t svs_cap, int64_t* newdim, double* truncerr) {
   SB_TRY({

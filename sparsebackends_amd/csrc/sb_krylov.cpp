@@ -299,222 +299,358 @@ void sym_eig(int m, const std::vector<double>& A_in, std::vector<double>& w, std
 
 // ---------------------------------------------------------------- Lanczos
 
-EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tensor& x, Tensor& y, int kdim,
-                           int maxiter, double tol) {
-  need_bs(x, "eigsolve");
-  need_bs(y, "eigsolve");
-  if (plans.empty()) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve needs at least one plan");
-  for (auto* p : plans)
-    if (!p) fail(SB_E_HANDLE, "null plan");
-  if (plans.front()->in_a != &x && plans.front()->in_b != &x)
-    fail(SB_E_ARGUMENT, "ArgumentError: the first plan does not read the vector x");
-  if (plans.back()->out != &y) fail(SB_E_ARGUMENT, "ArgumentError: the last plan does not write y");
-  if (&x == &y) fail(SB_E_ARGUMENT, "ArgumentError: x and y must be distinct tensors");
-  if (x.dims != y.dims || x.N2 != y.N2)
-    fail(SB_E_ARGUMENT, "ArgumentError: the operator's output must have the vector's dims");
-  if (kdim < 2 || kdim > 64) fail(SB_E_ARGUMENT, "ArgumentError: krylovdim must be in 2..64");
-  if (maxiter < 1) fail(SB_E_ARGUMENT, "ArgumentError: maxiter must be positive");
-  if (!(tol >= 0)) fail(SB_E_ARGUMENT, "ArgumentError: tol must be non-negative");
+namespace {
 
-  const i64 nb = x.n, bs = x.blksize, nfull = nb * bs;
-  if (nfull == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve of an empty vector");
-  const int N2 = x.N2;
-  std::vector<i64> td(x.dims.begin() + x.P(), x.dims.end());
-  EigResult R;
-
-  // ---- v0 = x / |x| in x's payload
-  const double b0 = norm_of(ctx, x.dptr(), nfull);
-  if (!(b0 > 0) || !std::isfinite(b0)) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: initial vector has zero or non-finite norm");
-  // x's own nonzero extents per block (box)
-  DBuf<i32> xb0(ctx, (size_t)(nb * N2 ? nb * N2 : 1)), xf0(ctx, (size_t)(nb ? nb : 1));
-  box_scan(ctx, x.dptr(), x.ids.get(), nb, bs, N2, td.data(), xb0.get(), xf0.get());
-  k_axpby(ctx, nfull, 1.0 / b0, x.dptr(), 0.0, x.dptr());
-  x.touch();
-  for (auto* p : plans) p->run();  // H v0
-  ++R.numops;
-
-  // ---- the compact layout of the Krylov vectors: per block of x, a box S
-  // holding every nonzero any Krylov vector has: x0's extents joined with
-  // those of H v0, widened later if an application reaches beyond (the
-  // padding of symmetry sectors is left out).  When y's boxes are not known
-  // (an operator that does not end in a box product) S is the full block.
-  DBuf<i64> off;  // y's block offsets by x row (-1: y lacks the key)
-  match_blocks(ctx, x, y, off);  // blocks of y whose key x lacks are left out: the solve stays in x's block space
-  const bool track = y.box_trusted();
-  DBuf<i32> yrow(ctx, (size_t)(nb ? nb : 1)), S(ctx, (size_t)(nb * N2 ? nb * N2 : 1));
-  if (track) {
-    std::vector<i64> pd(x.dims.begin(), x.dims.begin() + x.P());
-    k_match_rows(ctx, x.keys.get(), nb, y.keys.get(), y.n, x.P(), pd.data(), yrow.get());
-    k_support(ctx, nb, N2, xb0.get(), y.box.get(), yrow.get(), S.get());
-  } else {
-    k_fill_rows(ctx, S.get(), nb, N2, td.data());
+// One Lanczos solve as a resumable state, so that several solves can share
+// each operator application (a batch of plan chains) and each host
+// synchronisation.  Per iteration: [load(); the operator; applied(); a sync
+// when tracking(); store()] unless apply_needed() says H v is already in
+// place, then orth(); a sync; update() (true when the solve is over).
+class Lanczos {
+ public:
+  Lanczos(Ctx* ctx, Tensor& x, Tensor& y, int kdim, int maxiter, double tol)
+      : ctx_(ctx), x_(x), y_(y), kdim_(kdim), maxiter_(maxiter), tol_(tol) {
+    need_bs(x, "eigsolve");
+    need_bs(y, "eigsolve");
+    if (&x == &y) fail(SB_E_ARGUMENT, "ArgumentError: x and y must be distinct tensors");
+    if (x.dims != y.dims || x.N2 != y.N2)
+      fail(SB_E_ARGUMENT, "ArgumentError: the operator's output must have the vector's dims");
+    if (kdim < 2 || kdim > 64) fail(SB_E_ARGUMENT, "ArgumentError: krylovdim must be in 2..64");
+    if (maxiter < 1) fail(SB_E_ARGUMENT, "ArgumentError: maxiter must be positive");
+    if (!(tol >= 0)) fail(SB_E_ARGUMENT, "ArgumentError: tol must be non-negative");
+    nb_ = x.n;
+    bs_ = x.blksize;
+    nfull_ = nb_ * bs_;
+    if (nfull_ == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve of an empty vector");
+    N2_ = x.N2;
+    td_.assign(x.dims.begin() + x.P(), x.dims.end());
+    ld_ = kdim + 1;
   }
-  std::vector<i32> hid(nb);
-  HIP_CHECK(hipMemcpyAsync(hid.data(), x.ids.get(), nb * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
-  std::vector<i64> hxoff(nb);
-  HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  for (i64 i = 0; i < nb; ++i) hxoff[i] = (i64)(hid[i] - 1) * bs;
-  DBuf<i64> coff(ctx, nb + 1), xoff(ctx, nb ? nb : 1);
-  HIP_CHECK(hipMemcpyAsync(xoff.get(), hxoff.data(), nb * sizeof(i64), hipMemcpyHostToDevice, ctx->stream));
-  i64 n = 0;  // Krylov vector length
-  auto layout = [&]() {  // coff from S; returns the length
-    std::vector<i32> hS((size_t)nb * N2);
+
+  // v0 = x / |x| in x's payload, and x's own nonzero extents per block
+  void begin() {
+    const double b0 = norm_of(ctx_, x_.dptr(), nfull_);
+    if (!(b0 > 0) || !std::isfinite(b0))
+      fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: initial vector has zero or non-finite norm");
+    xb0_.alloc(ctx_, (size_t)(nb_ * N2_ ? nb_ * N2_ : 1));
+    DBuf<i32> xf0(ctx_, (size_t)(nb_ ? nb_ : 1));
+    box_scan(ctx_, x_.dptr(), x_.ids.get(), nb_, bs_, N2_, td_.data(), xb0_.get(), xf0.get());
+    k_axpby(ctx_, nfull_, 1.0 / b0, x_.dptr(), 0.0, x_.dptr());
+    x_.touch();
+  }
+
+  // after the operator's first application (H v0 in y): the compact layout
+  // of the Krylov vectors, per block of x a box S holding every nonzero any
+  // Krylov vector has: x0's extents joined with those of H v0, widened later
+  // if an application reaches beyond (the padding of symmetry sectors is left
+  // out).  When y's boxes are not known (an operator that does not end in a
+  // box product) S is the full block.
+  void first() {
+    ++R_.numops;
+    match_blocks(ctx_, x_, y_, off_);  // blocks of y whose key x lacks are left out: the solve stays in x's block space
+    track_ = y_.box_trusted();
+    yrow_.alloc(ctx_, (size_t)(nb_ ? nb_ : 1));
+    S_.alloc(ctx_, (size_t)(nb_ * N2_ ? nb_ * N2_ : 1));
+    if (track_) {
+      std::vector<i64> pd(x_.dims.begin(), x_.dims.begin() + x_.P());
+      k_match_rows(ctx_, x_.keys.get(), nb_, y_.keys.get(), y_.n, x_.P(), pd.data(), yrow_.get());
+      k_support(ctx_, nb_, N2_, xb0_.get(), y_.box.get(), yrow_.get(), S_.get());
+    } else {
+      k_fill_rows(ctx_, S_.get(), nb_, N2_, td_.data());
+    }
+    std::vector<i32> hid(nb_);
+    HIP_CHECK(hipMemcpyAsync(hid.data(), x_.ids.get(), nb_ * sizeof(i32), hipMemcpyDeviceToHost, ctx_->stream));
+    std::vector<i64> hxoff(nb_);
+    HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    for (i64 i = 0; i < nb_; ++i) hxoff[i] = (i64)(hid[i] - 1) * bs_;
+    coff_.alloc(ctx_, nb_ + 1);
+    xoff_.alloc(ctx_, nb_ ? nb_ : 1);
+    HIP_CHECK(hipMemcpyAsync(xoff_.get(), hxoff.data(), nb_ * sizeof(i64), hipMemcpyHostToDevice, ctx_->stream));
+    n_ = layout();
+    if (n_ == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: the vector and its image are empty");
+    const int NBP = linalg_nb();
+    V_.alloc(ctx_, (size_t)(kdim_ + 1) * n_);
+    part_.alloc(ctx_, (size_t)(kdim_ + 2) * NBP);
+    small_.alloc(ctx_, 3 * (size_t)ld_ + 3);
+    dU_.alloc(ctx_, (size_t)kdim_ * kdim_);
+    viol_.alloc(ctx_, 1);
+    // v0 and H v0 in the compact layout
+    k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), x_.dptr(), vec(0));
+    k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(1));
+    have_first_ = true;  // H v0 is in vec(1) already
+    T_.assign((size_t)ld_ * ld_, 0.0);
+    hh_.assign(3 * (size_t)ld_ + 3, 0.0);
+  }
+
+  // whether this iteration applies the operator (the first one has H v0)
+  bool apply_needed() {
+    if (have_first_) {
+      have_first_ = false;
+      return false;
+    }
+    return true;
+  }
+  bool tracking() const { return track_; }
+
+  // the basis vector into x's payload (entries outside S stay zero: x0 is zero there)
+  void load() {
+    step(ctx_, "lanczos_load", [&] {
+      k_compact(ctx_, 1, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), vec(j_), x_.dptr());
+    });
+    x_.touch();
+  }
+  // after the operator: y's boxes checked against S (read on the host after a sync)
+  void applied() {
+    ++R_.numops;
+    if (track_) {
+      k_box_exceeds(ctx_, nb_, N2_, S_.get(), y_.box.get(), yrow_.get(), viol_.get());
+      HIP_CHECK(hipMemcpyAsync(&hv_, viol_.get(), sizeof(i32), hipMemcpyDeviceToHost, ctx_->stream));
+    }
+  }
+  // y's blocks by key into the compact layout, S widened first where needed
+  void store() {
+    if (track_ && hv_) widen(j_ + 1);
+    step(ctx_, "lanczos_gather", [&] {
+      k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(j_ + 1));
+    });
+  }
+
+  // classical Gram-Schmidt, twice, of vec(j+1) against v_0 .. v_j; the
+  // coefficients reach the host after the next sync
+  void orth() {
+    double* w = vec(j_ + 1);
+    double* h1 = small_.get();
+    double* h2 = h1 + ld_ + 1;
+    double* nr = h2 + ld_ + 1;
+    step(ctx_, "lanczos_orth", [&] {
+      k_mdot(ctx_, V_.get(), n_, j_ + 1, w, n_, true, part_.get(), h1);
+      k_orth_update(ctx_, V_.get(), n_, j_ + 1, h1, w, n_, part_.get(), nullptr);
+      k_mdot(ctx_, V_.get(), n_, j_ + 1, w, n_, false, part_.get(), h2);
+      k_orth_update(ctx_, V_.get(), n_, j_ + 1, h2, w, n_, part_.get(), nr);
+    });
+    HIP_CHECK(hipMemcpyAsync(hh_.data(), small_.get(), hh_.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx_->stream));
+  }
+
+  // the projected matrix, its lowest Ritz pair and the residual; then
+  // converged (or broken down), next vector, or thick restart.  true: done
+  bool update() {
+    double* w = vec(j_ + 1);
+    const double* H1 = hh_.data();
+    const double* H2 = H1 + ld_ + 1;
+    const double nAv = std::sqrt(H1[j_ + 1]);
+    beta_ = std::sqrt(hh_[2 * (ld_ + 1)]);
+    if (!std::isfinite(beta_) || !std::isfinite(nAv))
+      fail(SB_E_ERROR, "eigsolve: the operator produced non-finite values");
+    for (int i = 0; i <= j_; ++i) T_[i + (size_t)j_ * ld_] = T_[j_ + (size_t)i * ld_] = H1[i] + H2[i];
+    m_ = j_ + 1;
+    std::vector<double> Tm((size_t)m_ * m_);
+    for (int c = 0; c < m_; ++c)
+      for (int r = 0; r < m_; ++r) Tm[r + (size_t)c * m_] = T_[r + (size_t)c * ld_];
+    sym_eig(m_, Tm, theta_, U_);
+    const bool breakdown = beta_ <= 1e-13 * std::max(nAv, 1e-300);
+    resid_ = breakdown ? 0.0 : beta_ * std::fabs(U_[(m_ - 1) + 0 * (size_t)m_]);
+    if (!breakdown) k_axpby(ctx_, n_, 1.0 / beta_, w, 0.0, w);
+    if (resid_ <= tol_ || breakdown) {
+      converged_ = true;
+      return true;
+    }
+    if (m_ < kdim_) {
+      ++j_;
+      return false;
+    }
+    if (cycles_ == maxiter_) return true;
+    // thick restart: keep the lowest Ritz vectors, then the residual direction
+    const int keep = std::max(1, std::min(kdim_ - 1, (3 * kdim_) / 5));
+    step(ctx_, "lanczos_restart", [&] {
+      HIP_CHECK(hipMemcpyAsync(dU_.get(), U_.data(), (size_t)m_ * keep * sizeof(double), hipMemcpyHostToDevice,
+                               ctx_->stream));
+      if (!tmp_.get()) tmp_.alloc(ctx_, (size_t)keep * n_);
+      k_combine(ctx_, V_.get(), n_, m_, dU_.get(), keep, tmp_.get(), n_, n_);
+      HIP_CHECK(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)keep * n_ * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx_->stream));
+      HIP_CHECK(hipMemcpyAsync(vec(keep), vec(m_), n_ * sizeof(double), hipMemcpyDeviceToDevice, ctx_->stream));
+    });
+    std::fill(T_.begin(), T_.end(), 0.0);
+    for (int i = 0; i < keep; ++i) {
+      T_[i + (size_t)i * ld_] = theta_[i];
+      T_[keep + (size_t)i * ld_] = T_[i + (size_t)keep * ld_] = beta_ * U_[(m_ - 1) + (size_t)i * m_];
+    }
+    HIP_CHECK(hipStreamSynchronize(ctx_->stream));  // U is reused on the host
+    j_ = keep;
+    ++cycles_;
+    return false;
+  }
+
+  // the Ritz vector of the lowest Ritz value into x, normalised
+  EigResult finish() {
+    DBuf<double> ritzv(ctx_, (size_t)n_);
+    step(ctx_, "lanczos_ritz", [&] {
+      HIP_CHECK(
+          hipMemcpyAsync(dU_.get(), U_.data(), (size_t)m_ * sizeof(double), hipMemcpyHostToDevice, ctx_->stream));
+      k_combine(ctx_, V_.get(), n_, m_, dU_.get(), 1, ritzv.get(), n_, n_);
+    });
+    const double xn = norm_of(ctx_, ritzv.get(), n_);
+    if (xn > 0) k_axpby(ctx_, n_, 1.0 / xn, ritzv.get(), 0.0, ritzv.get());
+    k_compact(ctx_, 1, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), ritzv.get(), x_.dptr());
+    x_.touch();
+    HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    R_.value = theta_[0];
+    R_.ritz = theta_;
+    R_.normres = resid_;
+    R_.numiter = cycles_;
+    R_.converged = converged_;
+    return R_;
+  }
+
+ private:
+  double* vec(int i) { return V_.get() + (i64)i * n_; }
+
+  i64 layout() {  // coff from S; returns the length
+    std::vector<i32> hS((size_t)nb_ * N2_);
     if (!hS.empty())
-      HIP_CHECK(hipMemcpyAsync(hS.data(), S.get(), hS.size() * sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    std::vector<i64> hc(nb + 1, 0);
-    for (i64 i = 0; i < nb; ++i) {
+      HIP_CHECK(hipMemcpyAsync(hS.data(), S_.get(), hS.size() * sizeof(i32), hipMemcpyDeviceToHost, ctx_->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    std::vector<i64> hc(nb_ + 1, 0);
+    for (i64 i = 0; i < nb_; ++i) {
       i64 c = 1;
-      for (int a = 0; a < N2; ++a) c *= hS[i * N2 + a];
+      for (int a = 0; a < N2_; ++a) c *= hS[i * N2_ + a];
       hc[i + 1] = hc[i] + c;
     }
-    HIP_CHECK(hipMemcpyAsync(coff.get(), hc.data(), (nb + 1) * sizeof(i64), hipMemcpyHostToDevice, ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    return hc[nb];
-  };
-  n = layout();
-  if (n == 0) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve: the vector and its image are empty");
-
-  const int NBP = linalg_nb();
-  const int ld = kdim + 1;
-  DBuf<double> V(ctx, (size_t)(kdim + 1) * n), part(ctx, (size_t)(kdim + 2) * NBP);
-  DBuf<double> small(ctx, 3 * (size_t)ld + 3), dU(ctx, (size_t)kdim * kdim), tmp;
-  DBuf<i32> viol(ctx, 1);
-  double* h1 = small.get();
-  double* h2 = h1 + ld + 1;
-  double* nr = h2 + ld + 1;
-  auto vec = [&](int i) { return V.get() + (i64)i * n; };
-  // v0 and H v0 in the compact layout
-  k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), x.dptr(), vec(0));
-  k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, off.get(), y.dptr(), vec(1));
+    HIP_CHECK(hipMemcpyAsync(coff_.get(), hc.data(), (nb_ + 1) * sizeof(i64), hipMemcpyHostToDevice, ctx_->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    return hc[nb_];
+  }
 
   // S widened to y's boxes where an application reached beyond it: the
   // vectors 0..nv-1 move to the new layout (they are zero in the new entries)
-  auto widen = [&](int nv) {
-    DBuf<double> full(ctx, (size_t)nfull);
-    DBuf<i32> S2(ctx, (size_t)(nb * N2 ? nb * N2 : 1));
-    k_support(ctx, nb, N2, S.get(), y.box.get(), yrow.get(), S2.get());
-    DBuf<i64> coff_old(ctx, nb + 1);
-    HIP_CHECK(hipMemcpyAsync(coff_old.get(), coff.get(), (nb + 1) * sizeof(i64), hipMemcpyDeviceToDevice,
-                             ctx->stream));
-    const i64 n_old = n;
-    DBuf<i32> S_old = std::move(S);
-    S = std::move(S2);
-    n = layout();
-    DBuf<double> V2(ctx, (size_t)(kdim + 1) * n);
+  void widen(int nv) {
+    DBuf<double> full(ctx_, (size_t)nfull_);
+    DBuf<i32> S2(ctx_, (size_t)(nb_ * N2_ ? nb_ * N2_ : 1));
+    k_support(ctx_, nb_, N2_, S_.get(), y_.box.get(), yrow_.get(), S2.get());
+    DBuf<i64> coff_old(ctx_, nb_ + 1);
+    HIP_CHECK(hipMemcpyAsync(coff_old.get(), coff_.get(), (nb_ + 1) * sizeof(i64), hipMemcpyDeviceToDevice,
+                             ctx_->stream));
+    const i64 n_old = n_;
+    DBuf<i32> S_old = std::move(S_);
+    S_ = std::move(S2);
+    n_ = layout();
+    DBuf<double> V2(ctx_, (size_t)(kdim_ + 1) * n_);
     for (int i = 0; i < nv; ++i) {
-      k_zero(ctx, full.get(), (size_t)nfull * sizeof(double));
-      k_compact(ctx, 1, nb, N2, td.data(), S_old.get(), coff_old.get(), n_old, xoff.get(), V.get() + (i64)i * n_old,
-                full.get());
-      k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), full.get(), V2.get() + (i64)i * n);
+      k_zero(ctx_, full.get(), (size_t)nfull_ * sizeof(double));
+      k_compact(ctx_, 1, nb_, N2_, td_.data(), S_old.get(), coff_old.get(), n_old, xoff_.get(),
+                V_.get() + (i64)i * n_old, full.get());
+      k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), full.get(),
+                V2.get() + (i64)i * n_);
     }
-    V = std::move(V2);
-    tmp.release();
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  };
-
-  // later applications: the basis vector into x's payload (entries outside S
-  // stay zero: x0 is zero there), the operator, y's blocks by key into the
-  // compact layout; y's boxes are checked against S
-  auto matvec = [&](int src, int dst) {
-    step(ctx, "lanczos_load",
-         [&] { k_compact(ctx, 1, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), vec(src), x.dptr()); });
-    x.touch();
-    for (auto* p : plans) p->run();
-    ++R.numops;
-    if (track) {
-      k_box_exceeds(ctx, nb, N2, S.get(), y.box.get(), yrow.get(), viol.get());
-      i32 hv = 0;
-      HIP_CHECK(hipMemcpyAsync(&hv, viol.get(), sizeof(i32), hipMemcpyDeviceToHost, ctx->stream));
-      HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      if (hv) widen(dst);
-    }
-    step(ctx, "lanczos_gather",
-         [&] { k_compact(ctx, 0, nb, N2, td.data(), S.get(), coff.get(), n, off.get(), y.dptr(), vec(dst)); });
-  };
-  bool have_first = true;  // H v0 is in vec(1) already
-
-  std::vector<double> T((size_t)ld * ld, 0.0), hh(3 * (size_t)ld + 3), theta, U;
-  int j = 0;
-  int cycles = 1;
-  double beta = 0, resid = 0;
-  bool converged = false;
-  int m = 0;
-  for (;;) {
-    if (have_first)
-      have_first = false;
-    else
-      matvec(j, j + 1);
-    double* w = vec(j + 1);
-    // classical Gram-Schmidt, twice, against v_0 .. v_j
-    step(ctx, "lanczos_orth", [&] {
-      k_mdot(ctx, V.get(), n, j + 1, w, n, true, part.get(), h1);
-      k_orth_update(ctx, V.get(), n, j + 1, h1, w, n, part.get(), nullptr);
-      k_mdot(ctx, V.get(), n, j + 1, w, n, false, part.get(), h2);
-      k_orth_update(ctx, V.get(), n, j + 1, h2, w, n, part.get(), nr);
-    });
-    HIP_CHECK(hipMemcpyAsync(hh.data(), small.get(), hh.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    const double* H1 = hh.data();
-    const double* H2 = H1 + ld + 1;
-    const double nAv = std::sqrt(H1[j + 1]);
-    beta = std::sqrt(hh[2 * (ld + 1)]);
-    if (!std::isfinite(beta) || !std::isfinite(nAv)) fail(SB_E_ERROR, "eigsolve: the operator produced non-finite values");
-    for (int i = 0; i <= j; ++i) T[i + (size_t)j * ld] = T[j + (size_t)i * ld] = H1[i] + H2[i];
-    m = j + 1;
-    std::vector<double> Tm((size_t)m * m);
-    for (int c = 0; c < m; ++c)
-      for (int r = 0; r < m; ++r) Tm[r + (size_t)c * m] = T[r + (size_t)c * ld];
-    sym_eig(m, Tm, theta, U);
-    const bool breakdown = beta <= 1e-13 * std::max(nAv, 1e-300);
-    resid = breakdown ? 0.0 : beta * std::fabs(U[(m - 1) + 0 * (size_t)m]);
-    if (!breakdown) k_axpby(ctx, n, 1.0 / beta, w, 0.0, w);
-    if (resid <= tol || breakdown) {
-      converged = true;
-      break;
-    }
-    if (m < kdim) {
-      ++j;
-      continue;
-    }
-    if (cycles == maxiter) break;
-    // thick restart: keep the lowest Ritz vectors, then the residual direction
-    const int keep = std::max(1, std::min(kdim - 1, (3 * kdim) / 5));
-    step(ctx, "lanczos_restart", [&] {
-      HIP_CHECK(hipMemcpyAsync(dU.get(), U.data(), (size_t)m * keep * sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
-      if (!tmp.get()) tmp.alloc(ctx, (size_t)keep * n);
-      k_combine(ctx, V.get(), n, m, dU.get(), keep, tmp.get(), n, n);
-      HIP_CHECK(hipMemcpyAsync(V.get(), tmp.get(), (size_t)keep * n * sizeof(double), hipMemcpyDeviceToDevice,
-                               ctx->stream));
-      HIP_CHECK(hipMemcpyAsync(vec(keep), vec(m), n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    });
-    std::fill(T.begin(), T.end(), 0.0);
-    for (int i = 0; i < keep; ++i) {
-      T[i + (size_t)i * ld] = theta[i];
-      T[keep + (size_t)i * ld] = T[i + (size_t)keep * ld] = beta * U[(m - 1) + (size_t)i * m];
-    }
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));  // U is reused on the host
-    j = keep;
-    ++cycles;
+    V_ = std::move(V2);
+    tmp_.release();
+    HIP_CHECK(hipStreamSynchronize(ctx_->stream));
   }
-  // Ritz vector of the lowest Ritz value into x, normalised
-  DBuf<double> ritzv(ctx, (size_t)n);
-  step(ctx, "lanczos_ritz", [&] {
-    HIP_CHECK(hipMemcpyAsync(dU.get(), U.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    k_combine(ctx, V.get(), n, m, dU.get(), 1, ritzv.get(), n, n);
-  });
-  const double xn = norm_of(ctx, ritzv.get(), n);
-  if (xn > 0) k_axpby(ctx, n, 1.0 / xn, ritzv.get(), 0.0, ritzv.get());
-  k_compact(ctx, 1, nb, N2, td.data(), S.get(), coff.get(), n, xoff.get(), ritzv.get(), x.dptr());
-  x.touch();
-  HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  R.value = theta[0];
-  R.ritz = theta;
-  R.normres = resid;
-  R.numiter = cycles;
-  R.converged = converged;
-  return R;
+
+  Ctx* ctx_;
+  Tensor& x_;
+  Tensor& y_;
+  const int kdim_, maxiter_;
+  const double tol_;
+  i64 nb_ = 0, bs_ = 0, nfull_ = 0, n_ = 0;
+  int N2_ = 0, ld_ = 0;
+  std::vector<i64> td_;
+  DBuf<i32> xb0_, yrow_, S_, viol_;
+  DBuf<i64> off_, coff_, xoff_;
+  DBuf<double> V_, part_, small_, dU_, tmp_;
+  bool track_ = false, have_first_ = false, converged_ = false;
+  i32 hv_ = 0;
+  std::vector<double> T_, hh_, theta_, U_;
+  int j_ = 0, cycles_ = 1, m_ = 0;
+  double beta_ = 0, resid_ = 0;
+  EigResult R_;
+};
+
+void check_chain(const Plan* first, const Plan* last, const Tensor& x, const Tensor& y) {
+  if (!first || !last) fail(SB_E_HANDLE, "null plan");
+  if (first->in_a != &x && first->in_b != &x)
+    fail(SB_E_ARGUMENT, "ArgumentError: the first plan does not read the vector x");
+  if (last->out != &y) fail(SB_E_ARGUMENT, "ArgumentError: the last plan does not write y");
+}
+
+}  // namespace
+
+EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tensor& x, Tensor& y, int kdim,
+                           int maxiter, double tol) {
+  if (plans.empty()) fail(SB_E_ARGUMENT, "ArgumentError: eigsolve needs at least one plan");
+  for (auto* p : plans)
+    if (!p) fail(SB_E_HANDLE, "null plan");
+  need_bs(x, "eigsolve");
+  need_bs(y, "eigsolve");
+  check_chain(plans.front(), plans.back(), x, y);
+  Lanczos L(ctx, x, y, kdim, maxiter, tol);
+  auto apply = [&] {
+    for (auto* p : plans) p->run();
+  };
+  L.begin();
+  apply();  // H v0
+  L.first();
+  for (;;) {
+    if (L.apply_needed()) {
+      L.load();
+      apply();
+      L.applied();
+      if (L.tracking()) HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      L.store();
+    }
+    L.orth();
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (L.update()) break;
+  }
+  return L.finish();
+}
+
+std::vector<EigResult> eigsolve_lanczos_batch(Ctx* ctx, Batch& batch, const std::vector<Tensor*>& xs,
+                                              const std::vector<Tensor*>& ys, int kdim, int maxiter, double tol) {
+  const int n = batch.nchains;
+  if ((int)xs.size() != n || (int)ys.size() != n)
+    fail(SB_E_ARGUMENT, "ArgumentError: one vector x and one output y per chain of the batch");
+  for (int c = 0; c < n; ++c) {
+    if (!xs[c] || !ys[c]) fail(SB_E_HANDLE, "null tensor");
+    need_bs(*xs[c], "eigsolve");
+    need_bs(*ys[c], "eigsolve");
+    check_chain(batch.stages.front().plans[c], batch.stages.back().plans[c], *xs[c], *ys[c]);
+    for (int d = 0; d < c; ++d)
+      if (xs[d] == xs[c] || ys[d] == ys[c]) fail(SB_E_ARGUMENT, "ArgumentError: the chains share a vector");
+  }
+  std::vector<std::unique_ptr<Lanczos>> L;
+  for (int c = 0; c < n; ++c) L.emplace_back(std::make_unique<Lanczos>(ctx, *xs[c], *ys[c], kdim, maxiter, tol));
+  std::vector<int> active(n);
+  for (int c = 0; c < n; ++c) active[c] = c;
+  for (int c : active) L[c]->begin();
+  batch.run(active);  // H v0 of every chain
+  for (int c : active) L[c]->first();
+  while (!active.empty()) {
+    std::vector<int> need;
+    for (int c : active)
+      if (L[c]->apply_needed()) need.push_back(c);
+    if (!need.empty()) {
+      for (int c : need) L[c]->load();
+      batch.run(need);
+      bool any_track = false;
+      for (int c : need) {
+        L[c]->applied();
+        any_track = any_track || L[c]->tracking();
+      }
+      if (any_track) HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int c : need) L[c]->store();
+    }
+    for (int c : active) L[c]->orth();
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> still;
+    for (int c : active)
+      if (!L[c]->update()) still.push_back(c);
+    active.swap(still);
+  }
+  std::vector<EigResult> out;
+  for (int c = 0; c < n; ++c) out.push_back(L[c]->finish());
+  return out;
 }
 
 }  // namespace sb

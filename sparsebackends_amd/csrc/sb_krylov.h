@@ -4,6 +4,7 @@
 
 #include <vector>
 
+#include "sb_batch.h"
 #include "sb_contract.h"
 
 namespace sb {
@@ -36,5 +37,12 @@ struct EigResult {
 // On return x holds the normalised Ritz vector.
 EigResult eigsolve_lanczos(Ctx*, const std::vector<const Plan*>& plans, Tensor& x, Tensor& y, int krylovdim,
                            int maxiter, double tol);
+
+// The same solve for every chain c of a batch (operator: chain c, from xs[c]
+// to ys[c]), in lock step: one batched operator application per iteration for
+// the chains still running and one host synchronisation for all of them.
+// Each result (and x) is bit for bit what eigsolve_lanczos gives alone.
+std::vector<EigResult> eigsolve_lanczos_batch(Ctx*, Batch& batch, const std::vector<Tensor*>& xs,
+                                              const std::vector<Tensor*>& ys, int krylovdim, int maxiter, double tol);
 
 }  // namespace sb

@@ -69,3 +69,35 @@ def eigsolve_refined(plans_lo, plans_hi, x_lo, x_hi, *, krylovdim=30, maxiter=10
     x_hi.axpby_(1.0, x_lo, 0.0)
     vals, vecs, info_hi = eigsolve(plans_hi, x_hi, krylovdim=polish_krylovdim, maxiter=polish_maxiter, tol=tol)
     return vals, vecs, info_lo, info_hi
+
+
+def eigsolve_batch(batch, xs, *, ys=None, krylovdim=30, maxiter=100, tol=1e-12):
+    """eigsolve for every chain of a PlanBatch at once: chain c is the
+    operator from xs[c] (its first plan reads it) to ys[c] (by default its
+    last plan's output).  The solves run in lock step, one batched operator
+    application per iteration for the chains still running; each result and
+    Ritz vector (written into xs[c]) equals what eigsolve gives for that
+    chain alone, bit for bit.  Returns (vals, infos) with one entry per chain
+    (infos without Ritz values)."""
+    n = batch.nchains
+    xs = list(xs)
+    if len(xs) != n:
+        raise ValueError("one start vector per chain of the batch")
+    if ys is None:
+        ys = [ch[-1]._ops[0] for ch in batch._chains]
+    k = max(int(krylovdim), 2)
+    ax = (C.c_void_p * n)(*[x.handle.value if hasattr(x.handle, "value") else x.handle for x in xs])
+    ay = (C.c_void_p * n)(*[y.handle.value if hasattr(y.handle, "value") else y.handle for y in ys])
+    vals, nres = np.zeros(n), np.zeros(n)
+    numops = np.zeros(n, dtype=np.int64)
+    numiter, conv = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    f64 = C.POINTER(C.c_double)
+    i32 = C.POINTER(C.c_int)
+    check(load().sb_eigsolve_lanczos_batch(batch.ctx, batch._h, C.cast(ax, C.POINTER(C.c_void_p)),
+                                           C.cast(ay, C.POINTER(C.c_void_p)), n, k, int(maxiter), float(tol),
+                                           vals.ctypes.data_as(f64), nres.ctypes.data_as(f64),
+                                           numops.ctypes.data_as(C.POINTER(C.c_int64)), numiter.ctypes.data_as(i32),
+                                           conv.ctypes.data_as(i32)))
+    infos = [ConvergenceInfo(int(conv[c]), float(nres[c]), int(numiter[c]), int(numops[c]), np.zeros(0))
+             for c in range(n)]
+    return [float(v) for v in vals], infos
