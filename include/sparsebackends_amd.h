@@ -264,6 +264,19 @@ int sb_eigsolve_lanczos_batch(sb_ctx_t ctx, sb_batch_t batch, const sb_tensor_t*
                               int nchains, int krylovdim, int maxiter, double tol, double* values, double* normres,
                               int64_t* numops, int* numiter, int* converged);
 
+/* New (no counterpart in the reference; for serving loops that co-batch the
+ * eigensolves of independent problems held by different host threads): the
+ * lock-step solve of sb_eigsolve_lanczos_batch over chains whose plans and
+ * vectors may belong to other contexts of the same device, one context per
+ * thread.  plans[c*nstages+s] is stage s of chain c.  For the duration of the
+ * call every such context is synchronised and then launches on ctx's stream;
+ * at the end ctx's stream is synchronised and they get their own streams
+ * back.  The other contexts must not be used by other threads during the
+ * call.  Results are bit for bit those of sb_eigsolve_lanczos per chain. */
+int sb_eigsolve_lanczos_multi(sb_ctx_t ctx, const sb_plan_t* plans, int nchains, int nstages, const sb_tensor_t* xs,
+                              const sb_tensor_t* ys, int krylovdim, int maxiter, double tol, double* values,
+                              double* normres, int64_t* numops, int* numiter, int* converged);
+
 /* ------------------------------------------------------------ two-site SVD
  * blocksparse_svd (SparseBackends/src/ops_factorize.jl:50-148; ITensor-level
  * wrapper itensor_blocksparse_svd, tensor_wrappers.jl:975-1005) of a

@@ -99,6 +99,8 @@ SIGNATURES = {
                           PP, I64P, F64P, C.c_int64, I64P, F64P],
     "sb_eigsolve_lanczos_batch": [P, P, PP, PP, C.c_int, C.c_int, C.c_int, C.c_double, F64P, F64P, I64P, I32P,
                                   I32P],
+    "sb_eigsolve_lanczos_multi": [P, PP, C.c_int, C.c_int, PP, PP, C.c_int, C.c_int, C.c_double, F64P, F64P, I64P,
+                                  I32P, I32P],
     "sb_eigsolve_lanczos": [P, PP, C.c_int, P, P, C.c_int, C.c_int, C.c_double, F64P, F64P, I32P, F64P, I64P,
                             I32P, I32P],
 }

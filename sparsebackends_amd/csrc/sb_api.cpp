@@ -657,6 +657,43 @@ int sb_eigsolve_lanczos_batch(sb_ctx_t c, sb_batch_t b, const sb_tensor_t* xs, c
   })
 }
 
+int sb_eigsolve_lanczos_multi(sb_ctx_t c, const sb_plan_t* plans, int nchains, int nstages, const sb_tensor_t* xs,
+                              const sb_tensor_t* ys, int krylovdim, int maxiter, double tol, double* values,
+                              double* normres, int64_t* numops, int* numiter, int* converged) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!plans || !xs || !ys) fail(SB_E_ARGUMENT, "ArgumentError: null argument");
+    if (nchains < 1 || nstages < 1) fail(SB_E_ARGUMENT, "ArgumentError: a batch needs at least one chain and one stage");
+    std::vector<Plan*> v((size_t)nchains * nstages);
+    std::vector<Ctx*> guests;
+    for (size_t i = 0; i < v.size(); ++i) {
+      v[i] = reinterpret_cast<Plan*>(plans[i]);
+      if (!v[i]) fail(SB_E_HANDLE, "null plan");
+      guests.push_back(v[i]->ctx);
+    }
+    std::vector<Tensor*> vx(nchains), vy(nchains);
+    for (int i = 0; i < nchains; ++i) {
+      vx[i] = T_(xs[i], BS);
+      vy[i] = T_(ys[i], BS);
+      guests.push_back(vx[i]->ctx);
+      guests.push_back(vy[i]->ctx);
+    }
+    std::vector<EigResult> r;
+    {
+      StreamLoan loan(ctx, guests);
+      auto bt = make_batch(ctx, v, nchains, nstages);
+      r = eigsolve_lanczos_batch(ctx, *bt, vx, vy, krylovdim, maxiter, tol);
+    }
+    for (int i = 0; i < nchains; ++i) {
+      if (values) values[i] = r[i].value;
+      if (normres) normres[i] = r[i].normres;
+      if (numops) numops[i] = r[i].numops;
+      if (numiter) numiter[i] = r[i].numiter;
+      if (converged) converged[i] = r[i].converged ? 1 : 0;
+    }
+  })
+}
+
 int sb_bs_svd(sb_ctx_t c, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_t mindim, double cutoff,
               sb_tensor_t* U, sb_tensor_t* SV, double* svs, int64_t svs_cap, int64_t* newdim, double* truncerr) {
   SB_TRY({

@@ -66,7 +66,10 @@ std::unique_ptr<Batch> make_batch(Ctx* ctx, const std::vector<Plan*>& plans, int
   if ((i64)plans.size() != (i64)nchains * nstages) fail(SB_E_ARGUMENT, "ArgumentError: batch plan count");
   for (auto* p : plans) {
     if (!p) fail(SB_E_HANDLE, "null plan");
-    if (p->ctx != ctx) fail(SB_E_ARGUMENT, "ArgumentError: every plan of a batch must belong to its context");
+    // a plan of another context takes part while that context runs on this
+    // one's stream (StreamLoan)
+    if (p->ctx != ctx && (p->ctx->device != ctx->device || p->ctx->stream != ctx->stream))
+      fail(SB_E_ARGUMENT, "ArgumentError: every plan of a batch must belong to its context");
   }
   auto b = std::make_unique<Batch>();
   b->ctx = ctx;
@@ -169,6 +172,37 @@ void Batch::run_stage(int s, const std::vector<int>& active) {
   timed(ctx, "box_prepare", [&] { box_prepare_batch(ctx, a); });
   timed(ctx, "grouped_gemm", [&] { box_gemm_batch(ctx, st.prec, a); });
   for (int c : active) st.plans[c]->finish();
+}
+
+StreamLoan::StreamLoan(Ctx* host, const std::vector<Ctx*>& guests) : host_(host) {
+  for (Ctx* g : guests) {
+    if (!g) fail(SB_E_HANDLE, "null context");
+    if (g == host || std::find(guests_.begin(), guests_.end(), g) != guests_.end()) continue;
+    if (g->device != host->device) fail(SB_E_ARGUMENT, "ArgumentError: the chains' contexts are on another device");
+    guests_.push_back(g);
+  }
+  for (Ctx* g : guests_) {
+    // the guest's queued work is done before the host's stream takes over
+    hipError_t e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) {
+      restore();
+      HIP_CHECK(e);
+    }
+    saved_.push_back(g->stream);
+    g->stream = host->stream;
+  }
+}
+
+void StreamLoan::restore() {
+  for (size_t i = 0; i < saved_.size(); ++i) guests_[i]->stream = saved_[i];
+  saved_.clear();
+}
+
+StreamLoan::~StreamLoan() {
+  // whatever the loaned contexts queued on the host's stream is done before
+  // they get their own streams back (their caches then reuse blocks in order)
+  (void)hipStreamSynchronize(host_->stream);
+  restore();
 }
 
 void Batch::run(const std::vector<int>& active_in) {

@@ -37,6 +37,26 @@ struct Batch {
   void run_stage(int s, const std::vector<int>& active);
 };
 
+// Lends the stream of host to the guest contexts for the lifetime of the
+// object: each guest's stream is synchronised, then the guest launches on
+// host's stream (work of the plans and tensors of several contexts can then
+// be ordered and batched on one stream); at the end host's stream is
+// synchronised and the guests get their streams back.  The guests must not be
+// used by other threads meanwhile.
+class StreamLoan {
+ public:
+  StreamLoan(Ctx* host, const std::vector<Ctx*>& guests);
+  ~StreamLoan();
+  StreamLoan(const StreamLoan&) = delete;
+  StreamLoan& operator=(const StreamLoan&) = delete;
+
+ private:
+  void restore();
+  Ctx* host_;
+  std::vector<Ctx*> guests_;
+  std::vector<hipStream_t> saved_;
+};
+
 // plans[c * nstages + s]: stage s of chain c.  Plans of one stage must be
 // independent (no plan reads another chain's output).
 std::unique_ptr<Batch> make_batch(Ctx* ctx, const std::vector<Plan*>& plans, int nchains, int nstages);

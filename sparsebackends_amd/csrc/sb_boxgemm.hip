@@ -16,6 +16,7 @@
 // Per execution: box_prepare (group boxes, per-pair K extents, tile counts,
 // clearing of C entries that fall outside a shrunken box) then box_gemm, a
 // persistent kernel that walks the (group, tile) work list.
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 
@@ -860,12 +861,14 @@ int box_tile_configs() { return kNumTiles; }
 bool box_tile_is_wave(int cfg) { return cfg >= 8; }
 
 template <class K>
-static i64 grid_for(Ctx* ctx, K kern, int& per_cu) {
+static i64 grid_for(Ctx* ctx, K kern, std::atomic<int>& per_cu_cache) {
+  int per_cu = per_cu_cache.load(std::memory_order_relaxed);
   if (!per_cu) {  // resident workgroups per CU (the same on every device of this build)
     int nb = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0));
     per_cu = std::max(1, nb);
     if (const char* e = std::getenv("SB_BOX_WG_PER_CU")) per_cu = std::max(1, std::atoi(e));  // tuning override
+    per_cu_cache.store(per_cu, std::memory_order_relaxed);
   }
   i64 nwg = (i64)ctx->num_cu * per_cu;
   return std::max<i64>(NXCD, nwg / NXCD * NXCD);  // a multiple of the XCD count
@@ -874,7 +877,7 @@ static i64 grid_for(Ctx* ctx, K kern, int& per_cu) {
 template <int BM, int BN, int WM, int WN, int PREC>
 static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
   auto kern = box_gemm_kernel<BM, BN, WM, WN, PREC>;
-  static int per_cu = 0;
+  static std::atomic<int> per_cu{0};
   i64 nwg = grid_for(ctx, kern, per_cu);
   hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.gorder, g.seg, g.aoff,
                      g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox,
@@ -885,7 +888,7 @@ static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const do
 template <int BM, int BN, int WM, int WN, int PREC>
 static void launch_one_batch(Ctx* ctx, const BoxBatchArgs& b) {
   auto kern = box_gemm_batch_kernel<BM, BN, WM, WN, PREC>;
-  static int per_cu = 0;
+  static std::atomic<int> per_cu{0};
   i64 nwg = grid_for(ctx, kern, per_cu);
   hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, b.jobs, b.act, b.gbase, b.nact, b.npos,
                      b.toff, b.trec);
@@ -920,7 +923,7 @@ void box_gemm_batch(Ctx* ctx, unsigned prec, const BoxBatchArgs& b) {
 template <int FM, int FN, int PREC>
 static void launch_wave(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
   auto kern = box_wave_kernel<FM, FN, PREC>;
-  static int per_cu = 0;
+  static std::atomic<int> per_cu{0};
   i64 nwg = grid_for(ctx, kern, per_cu);
   hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.seg, g.aoff, g.boff,
                      g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox);

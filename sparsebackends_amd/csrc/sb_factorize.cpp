@@ -21,7 +21,8 @@ void k_sect_gather(Ctx*, const double* psi, int d1, int d2, i64 Dl, i64 ntask, c
 size_t sect_svd_lds(int n, int len);
 void k_sect_svd(Ctx*, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                double* SVo);
+                double* SVo,
+                const int* glist, double* ws, const i64* wsoff);
 void k_sect_emit(Ctx*, bool isA, i64 nblk, int d, i64 D1, i64 D2, i64 nside, const i64* blk, const i64* sec,
                  const i64* off, const i64* dims, const i64* kept, const int* RC, const i64* uoff_or_soff,
                  const i64* sgoff, const double* sig, const double* F, bool right, double* out);
@@ -207,7 +208,12 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   const i64 ng = (i64)act.size();
   std::vector<int> hR(ng), hC(ng), htr(ng);
   std::vector<i64> xoff(ng + 1, 0), uoff(ng + 1, 0), soff(ng + 1, 0), sgoff(ng + 1, 0);
+  // sectors whose factorisation fits the LDS of a workgroup (lds_g), the
+  // others (big_g) run the same kernel on a global workspace
+  constexpr size_t kLds = 160 * 1024;
   size_t lds = 0;
+  std::vector<int> lds_g, big_g;
+  std::vector<i64> wsoff(1, 0);
   for (i64 g = 0; g < ng; ++g) {
     const i64 m = act[g];
     if (R[m] > INT32_MAX || Cn[m] > INT32_MAX) fail(SB_E_LIMIT, "sector matrix dimension beyond int32");
@@ -215,13 +221,19 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
     hC[g] = (int)Cn[m];
     htr[g] = R[m] > Cn[m];
     const int n = (int)std::min(R[m], Cn[m]), len = (int)std::max(R[m], Cn[m]);
-    lds = std::max(lds, sect_svd_lds(n, len));
+    const size_t need = sect_svd_lds(n, len);
+    if (need <= kLds) {
+      lds = std::max(lds, need);
+      lds_g.push_back((int)g);
+    } else {
+      big_g.push_back((int)g);
+      wsoff.push_back(wsoff.back() + (i64)((need + 255) / 256 * 256 / sizeof(double)));
+    }
     xoff[g + 1] = xoff[g] + R[m] * Cn[m];
     uoff[g + 1] = uoff[g] + R[m] * n;
     soff[g + 1] = soff[g] + (i64)n * Cn[m];
     sgoff[g + 1] = sgoff[g] + n;
   }
-  if (lds > 160 * 1024) fail(SB_E_LIMIT, "sector SVD: a sector matrix does not fit the 160 KB LDS of a workgroup");
   // gather tasks from psi's blocks
   std::vector<i32> hk, hi;
   down(ctx, hk, psi.keys.get(), (size_t)psi.n * 2);
@@ -258,8 +270,25 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   k_zero(ctx, X.get(), (size_t)(xoff[ng] ? xoff[ng] : 1) * sizeof(double));
   k_sect_gather(ctx, psi.dptr(), d1, d2, Dl, ntask, d_task.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(),
                 X.get());
-  k_sect_svd(ctx, ng, lds, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(), d_soff.get(),
-             d_sgoff.get(), sig.get(), Uo.get(), SVo.get());
+  {
+    DBuf<int> d_lg, d_bg;
+    DBuf<i64> d_wsoff;
+    DBuf<double> ws;
+    if (!big_g.empty()) {
+      up(ctx, d_bg, big_g);
+      up(ctx, d_wsoff, wsoff);
+      ws.alloc(ctx, (size_t)wsoff.back());
+    }
+    if (!lds_g.empty() && !big_g.empty()) up(ctx, d_lg, lds_g);
+    if (!lds_g.empty())
+      k_sect_svd(ctx, (i64)lds_g.size(), lds, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(),
+                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), big_g.empty() ? nullptr : d_lg.get(),
+                 nullptr, nullptr);
+    if (!big_g.empty())
+      k_sect_svd(ctx, (i64)big_g.size(), 0, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(),
+                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), d_bg.get(), ws.get(), d_wsoff.get());
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host lists and the workspace go out of scope
+  }
   std::vector<double> hs;
   down(ctx, hs, sig.get(), (size_t)sgoff[ng]);
   HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,7 +38,10 @@ struct Error {
 // ------------------------------------------------------------ allocator
 // Size-class caching allocator: device blocks are kept after release and
 // reused for later requests of the same class, so steady-state contractions
-// call no hipMalloc/hipFree (both synchronise the device).
+// call no hipMalloc/hipFree (both synchronise the device).  Several host
+// threads may share one context: the cache is guarded by a mutex, and since a
+// context has one stream, a block released by one thread and handed to
+// another is reused in stream order.
 class DeviceCache {
  public:
   explicit DeviceCache(int device) : device_(device) {}
@@ -50,6 +54,8 @@ class DeviceCache {
 
  private:
   static size_t cls(size_t b);
+  void trim_locked();
+  std::mutex mu_;
   int device_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> live_;

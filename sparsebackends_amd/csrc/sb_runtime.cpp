@@ -27,6 +27,7 @@ size_t DeviceCache::cls(size_t b) {
 
 void* DeviceCache::get(size_t bytes) {
   size_t c = cls(bytes);
+  std::lock_guard<std::mutex> lk(mu_);
   auto it = free_.lower_bound(c);
   if (it != free_.end() && it->first <= c + c / 4) {
     void* p = it->second;
@@ -41,7 +42,7 @@ void* DeviceCache::get(size_t bytes) {
   hipError_t e = hipMalloc(&p, c);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    trim();
+    trim_locked();
     HIP_CHECK(hipMalloc(&p, c));
   }
   live_[p] = c;
@@ -50,6 +51,7 @@ void* DeviceCache::get(size_t bytes) {
 }
 
 void DeviceCache::put(void* p) {
+  std::lock_guard<std::mutex> lk(mu_);
   auto it = live_.find(p);
   if (it == live_.end()) return;
   free_.emplace(it->second, p);
@@ -59,6 +61,11 @@ void DeviceCache::put(void* p) {
 }
 
 void DeviceCache::trim() {
+  std::lock_guard<std::mutex> lk(mu_);
+  trim_locked();
+}
+
+void DeviceCache::trim_locked() {
   // caller guarantees no queued work uses cached blocks (device synchronised)
   (void)hipDeviceSynchronize();
   for (auto& kv : free_) (void)hipFree(kv.second);

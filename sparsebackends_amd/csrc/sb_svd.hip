@@ -186,13 +186,17 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
                                                              const int* __restrict__ trans,
                                                              const i64* __restrict__ uoff, const i64* __restrict__ soff,
                                                              const i64* __restrict__ sgoff, double* __restrict__ sig,
-                                                             double* __restrict__ Uo, double* __restrict__ SVo) {
+                                                             double* __restrict__ Uo, double* __restrict__ SVo,
+                                                             const int* __restrict__ glist, double* __restrict__ ws,
+                                                             const i64* __restrict__ wsoff) {
   extern __shared__ double lds[];
-  const int g = blockIdx.x;
+  // sector of this workgroup; its working arrays in LDS, or (ws) in a global
+  // workspace for sectors beyond the LDS of a workgroup
+  const int g = glist ? glist[blockIdx.x] : (int)blockIdx.x;
   const bool tr = trans[g] != 0;
   const int Rg = R[g], Cg = Cc[g];
   const int n = tr ? Cg : Rg, len = tr ? Rg : Cg;  // rows of X, row length
-  double* Xs = lds;
+  double* Xs = ws ? ws + wsoff[blockIdx.x] : lds;
   double* Q = Xs + (i64)n * len;  // Q[r + n*c]
   double* nrm = Q + (i64)n * n;
   int* ord = reinterpret_cast<int*>(nrm + n);
@@ -298,11 +302,12 @@ size_t sect_svd_lds(int n, int len) {
 
 void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                double* SVo) {
+                double* SVo, const int* glist, double* ws, const i64* wsoff) {
   if (!ng) return;
-  HIP_CHECK(hipFuncSetAttribute((const void*)sect_svd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (ws) lds = 0;
+  else HIP_CHECK(hipFuncSetAttribute((const void*)sect_svd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(sect_svd_kernel, dim3((unsigned)ng), dim3(SS_THREADS), lds, ctx->stream, X, xoff, R, Cc, trans,
-                     uoff, soff, sgoff, sig, Uo, SVo);
+                     uoff, soff, sgoff, sig, Uo, SVo, glist, ws, wsoff);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -164,9 +164,13 @@ class DMRG:
     Tensors, environments and bonds stay on the device between steps; the
     host keeps the sector structure (charges and extents of every bond)."""
 
-    def __init__(self, model, states, device=0, flags=0):
+    def __init__(self, model, states, device=0, flags=0, eigsolver=None):
+        """eigsolver(plans, psi, krylovdim=, maxiter=, tol=) -> (vals, vecs, info)
+        replaces krylov.eigsolve (a serving loop's co-batching hook)."""
         from .heff import Bond
+        from .krylov import eigsolve
         self.m, self.N, self.device, self.flags = model, len(states), device, flags
+        self.eigsolver = eigsolver if eigsolver is not None else eigsolve
         d, w = model.d, model.w
         zero = tuple(0 for _ in model.charges[0])
         # bonds 0..N: bond j sits between sites j and j+1; charges accumulate from the left
@@ -200,11 +204,10 @@ class DMRG:
     def step(self, b, direction, maxdim, cutoff, krylovdim, maxiter, tol, mindim=1):
         """optimise sites b, b+1 (1-based): eigsolve on H_eff, sector SVD, environment update"""
         from .heff import Bond, heff_plans
-        from .krylov import eigsolve
         from .factorize import svd_sectors
         psi = two_site(self.A[b - 1], self.A[b])
         plans, Hp = heff_plans(psi, self.L[b - 1], self.W, self.W, self.R[b + 2], self.flags)
-        vals, _, info = eigsolve(plans, psi, krylovdim=krylovdim, maxiter=maxiter, tol=tol)
+        vals, _, info = self.eigsolver(plans, psi, krylovdim=krylovdim, maxiter=maxiter, tol=tol)
         Bl, Br = self.bonds[b - 1], self.bonds[b + 1]
         rowsec, colsec, charges = fused_sectors(self.m, Bl, Br)
         A, B, svs, kept, spec = svd_sectors(psi, rowsec, colsec, Bl.dims, Br.dims, len(charges),

@@ -101,3 +101,48 @@ def eigsolve_batch(batch, xs, *, ys=None, krylovdim=30, maxiter=100, tol=1e-12):
     infos = [ConvergenceInfo(int(conv[c]), float(nres[c]), int(numiter[c]), int(numops[c]), np.zeros(0))
              for c in range(n)]
     return [float(v) for v in vals], infos
+
+
+def eigsolve_multi(chains, xs, *, ys=None, ctx=None, krylovdim=30, maxiter=100, tol=1e-12):
+    """eigsolve_batch over chains whose plans and vectors may belong to the
+    contexts of other host threads (one context per thread, same device): the
+    solves run in lock step on the stream of ctx (default: the calling
+    thread's context), which the other contexts borrow for the duration of the
+    call (sb_eigsolve_lanczos_multi).  chains[c] is the plan list of problem c
+    (every chain the same length), xs[c] its start vector (overwritten with the
+    Ritz vector).  The threads owning those contexts must not use them during
+    the call.  Results equal eigsolve per chain, bit for bit.
+    Returns (vals, infos)."""
+    from .storage import context
+    chains = [list(ch) for ch in chains]
+    n = len(chains)
+    if n == 0:
+        return [], []
+    ns = len(chains[0])
+    if ns == 0 or any(len(ch) != ns for ch in chains):
+        raise ValueError("every chain must have the same, nonzero number of plans")
+    xs = list(xs)
+    if len(xs) != n:
+        raise ValueError("one start vector per chain")
+    if ys is None:
+        ys = [ch[-1]._ops[0] for ch in chains]
+    if ctx is None:
+        ctx = context(xs[0].device)
+    k = max(int(krylovdim), 2)
+    hv = lambda t: t.handle.value if hasattr(t.handle, "value") else t.handle  # noqa: E731
+    ap = (C.c_void_p * (n * ns))(*[p._h.value for ch in chains for p in ch])
+    ax = (C.c_void_p * n)(*[hv(x) for x in xs])
+    ay = (C.c_void_p * n)(*[hv(y) for y in ys])
+    vals, nres = np.zeros(n), np.zeros(n)
+    numops = np.zeros(n, dtype=np.int64)
+    numiter, conv = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+    f64 = C.POINTER(C.c_double)
+    i32 = C.POINTER(C.c_int)
+    pp = C.POINTER(C.c_void_p)
+    check(load().sb_eigsolve_lanczos_multi(ctx, C.cast(ap, pp), n, ns, C.cast(ax, pp), C.cast(ay, pp), k,
+                                           int(maxiter), float(tol), vals.ctypes.data_as(f64),
+                                           nres.ctypes.data_as(f64), numops.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           numiter.ctypes.data_as(i32), conv.ctypes.data_as(i32)))
+    infos = [ConvergenceInfo(int(conv[c]), float(nres[c]), int(numiter[c]), int(numops[c]), np.zeros(0))
+             for c in range(n)]
+    return [float(v) for v in vals], infos

@@ -16,10 +16,15 @@ from ._abi import load, check
 
 _ctx_lock = threading.Lock()
 _contexts = {}
+_bound = threading.local()
 
 
 def context(device=0):
-    """The default context (HIP device + stream) of this thread and device."""
+    """The context (HIP device + stream) this thread uses on a device: the one
+    bound by bind_context, else the thread's own default context."""
+    b = getattr(_bound, "ctx", None)
+    if b is not None and device in b:
+        return b[device]
     key = (threading.get_ident(), device)
     with _ctx_lock:
         if key not in _contexts:
@@ -29,8 +34,30 @@ def context(device=0):
         return _contexts[key]
 
 
+def bind_context(handle, device=0):
+    """Make the calling thread use the context handle on device (None: back to
+    its own default).  Threads bound to one context share its stream and its
+    device memory cache, so tensors and plans made by one of them can be used
+    by the others (a tensor belongs to the context that made it); their
+    library calls are ordered by the stream."""
+    b = getattr(_bound, "ctx", None)
+    if b is None:
+        b = _bound.ctx = {}
+    if handle is None:
+        b.pop(device, None)
+    else:
+        b[device] = handle
+
+
 def synchronize(device=0):
     check(load().sb_ctx_synchronize(context(device)))
+
+
+def memory(device=0):
+    """(bytes in use, bytes cached) of this thread's context's device memory cache"""
+    a, b = C.c_int64(), C.c_int64()
+    check(load().sb_ctx_memory(context(device), C.byref(a), C.byref(b)))
+    return int(a.value), int(b.value)
 
 
 def profile(enable=True, device=0):
