@@ -18,7 +18,9 @@ energy difference and time-to-energy against it, and solves/s with p50/p99
 latency of the DMRG-step solve (krylovdim 3, maxiter 1); "dmrg_solve" times
 whole two-site DMRG ground-state solves of XXZ chains (solves/s, p50/p99);
 "batch" runs 64 independent J1-J2 H_eff*psi problems plan by plan and as one
-batch (one box product launch per contraction stage over all problems).
+batch (one box product launch per contraction stage over all problems);
+"serve" runs a stream of DMRG requests through the serving loop against the
+plain one-at-a-time queue (solves/s, p50/p99 latency, FP64 and FP32 products).
 
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
@@ -27,6 +29,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-c
                        [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--no-eig]
                        [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--dmrg-solves N]
                        [--dmrg-sites N] [--dmrg-chi X] [--batch-problems N] [--batch-reps R]
+                       [--serve-requests N] [--serve-slots S] [--serve-sites L,..] [--serve-chis X,..]
                        [--calibrate-copy MB]
 """
 from __future__ import annotations
@@ -72,6 +75,10 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=10)
     ap.add_argument("--batch-kdim", type=int, default=16)
     ap.add_argument("--batch-eig-tol", type=float, default=1e-8, help="0: skip the batched ground-state solves")
+    ap.add_argument("--serve-requests", type=int, default=24, help="DMRG requests of the serving run (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-sites", default="16,24,32", help="chain lengths the requests draw from")
+    ap.add_argument("--serve-chis", default="32,64,128", help="bond dimension caps the requests draw from")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -264,6 +271,71 @@ def batch_eig(sb, a, probs, batch):
                 speedup=t_seq / t_bat, energies_identical=bool(e_seq == e_bat))
 
 
+def serve_section(sb, a):
+    """The serving loop (sparsebackends_amd.serve) on DMRG ground-state requests
+    of XXZ chains (J = 1, Delta uniform in [0.5, 1.5], Neel start, sweeps
+    until the energy changes by less than 1e-10, at most 12), lengths and bond
+    caps drawn from --serve-sites / --serve-chis.  Three measurements on the
+    same request mix:
+    * saturated: all requests at t = 0; plain (one solve at a time, first come
+      first served) vs served (slots on their own contexts and streams), with
+      and without co-batched Krylov steps; solves/s, p50/p99, and whether
+      every energy equals the plain one bit for bit;
+    * Poisson: plain at 0.7 x its saturated rate, served at 3 x and 4 x that
+      rate; solves/s and p99 of each (the target: 3 x the solves/s at no
+      higher p99);
+    * mixed: the served run with FP32 block products (FP64 storage, Lanczos
+      and SVD), solves/s and the largest energy difference to FP64."""
+    import dataclasses
+    from sparsebackends_amd import serve as S
+    sites = tuple(int(x) for x in a.serve_sites.split(","))
+    chis = tuple(int(x) for x in a.serve_chis.split(","))
+    n, slots = a.serve_requests, a.serve_slots
+    base = S.poisson_requests(n, float("inf"), seed=0, sites=sites, chis=chis)
+    S.run_plain(base[:1])  # first-use costs (library load, contexts) out of the timed runs
+    out = dict(model="XXZ J=1, Delta~U[0.5,1.5], Neel start, U(1) Sz, MPO w=5", requests=n, slots=slots,
+               sites=list(sites), chi_max=list(chis), energy_tol=1e-10, krylovdim=3, maxiter=1,
+               note="a reduced sample of the serving configuration (its lengths 32-128 and bond caps 128-1024 "
+                    "take hours per 512 requests); same request distribution family")
+    t0 = time.perf_counter()
+    plain = S.run_plain(base)
+    out["plain_saturated"] = dict(S.summarize(plain), wall_s=time.perf_counter() - t0)
+    ref = {r.rid: r.energy for r in plain}
+    srv = S.DMRGServer(slots=slots, co_batch=False)
+    res = srv.run(base)
+    out["served_saturated"] = dict(S.summarize(res), identical_to_plain=all(r.energy == ref[r.rid] for r in res),
+                                   max_inflight=srv.max_inflight)
+    srv = S.DMRGServer(slots=slots, co_batch=True, window=0.0005)
+    res = srv.run(base)
+    out["served_cobatched_saturated"] = dict(S.summarize(res),
+                                             identical_to_plain=all(r.energy == ref[r.rid] for r in res),
+                                             batches=len(srv.batch_sizes),
+                                             mean_batch=float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0,
+                                             max_batch=int(max(srv.batch_sizes)) if srv.batch_sizes else 0)
+    out["saturated_speedup"] = out["served_saturated"]["solves_per_s"] / out["plain_saturated"]["solves_per_s"]
+    # Poisson arrivals: the same mix (same seed), arrival times scaled by the rate
+    mu = out["plain_saturated"]["solves_per_s"]
+    rp = 0.7 * mu
+    lp = S.summarize(S.run_plain(S.poisson_requests(n, rp, seed=1, sites=sites, chis=chis)))
+    out["poisson"] = dict(plain=dict(lp, rate=rp))
+    for mult in (3.0, 4.0):
+        rs = mult * rp
+        srv = S.DMRGServer(slots=slots, co_batch=False)
+        ls = S.summarize(srv.run(S.poisson_requests(n, rs, seed=1, sites=sites, chis=chis)))
+        out["poisson"][f"served_at_{mult:g}x"] = dict(ls, rate=rs,
+                                                      solves_per_s_ratio=ls["solves_per_s"] / lp["solves_per_s"],
+                                                      p99_ratio=ls["p99_s"] / lp["p99_s"])
+    # mixed: FP32 block products
+    mreq = [dataclasses.replace(r, flags=sb.PREC_FP32) for r in base]
+    srv = S.DMRGServer(slots=slots, co_batch=False)
+    res = srv.run(mreq)
+    out["served_mixed_fp32"] = dict(S.summarize(res),
+                                    max_abs_dE_vs_fp64=float(max(abs(r.energy - ref[r.rid]) for r in res)))
+    out["peak_over_footprint_estimate"] = float(max(r.peak_bytes / r.footprint for r in plain))
+    out["largest_peak_mb"] = float(max(r.peak_bytes for r in plain) / 2 ** 20)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -420,6 +492,9 @@ def main():
 
     if a.batch_problems > 0 and a.prec == "fp64" and not a.no_eig:
         out["batch"] = batch_section(sb, a)
+
+    if a.serve_requests > 0 and a.prec == "fp64" and not a.no_eig:
+        out["serve"] = serve_section(sb, a)
 
     if a.calibrate_copy > 0:
         n = int(a.calibrate_copy * 1e6 / 8)

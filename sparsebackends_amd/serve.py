@@ -54,6 +54,7 @@ class SolveRequest:
     cutoff: float = 1e-12
     krylovdim: int = 3
     maxiter: int = 1
+    flags: int = 0  # precision form of the block products (PREC_FP64 / PREC_FP32 / PREC_BF16X3)
 
 
 @dataclass
@@ -67,7 +68,7 @@ class SolveResult:
     admitted: float = 0.0
     finish: float = 0.0
     footprint: int = 0     # admission estimate, bytes
-    peak_bytes: int = 0    # device bytes in use by the solve's context, largest after any sweep
+    peak_bytes: int = 0    # device bytes the solve held after a sweep, largest over its sweeps
     maxlinkdim: int = 0
 
     @property
@@ -79,10 +80,11 @@ def footprint_bytes(req: SolveRequest, d=2, w=5):
     """Upper estimate of the device bytes a solve holds: site tensors and
     environments of every site at the cap chi_max, plus one step's working set
     (two-site psi, the three H_eff intermediates, Krylov vectors).  Blocks of
-    the sector format are padded to the bond's largest sector, which for the
-    U(1) bond distributions of these chains stays within pad = 3 of the dense
-    size."""
-    chi, n, pad = req.chi_max, req.sites, 3
+    the sector format are padded to the bond's largest sector; pad = 4 times
+    the dense size covers the U(1) bond distributions of these chains (the
+    measured peak of 24 solves of 16-32 sites at chi 32-128 reached 0.87 of
+    it)."""
+    chi, n, pad = req.chi_max, req.sites, 4
     site = d * chi * chi
     env = w * chi * chi
     step = (req.krylovdim + 4) * d * d * chi * chi + 3 * w * d * d * chi * chi
@@ -93,15 +95,18 @@ def neel(n):
     return [0, 1] * (n // 2) + [0] * (n % 2)
 
 
-def solve_dmrg(req: SolveRequest, eigsolver=None, device=0, flags=0):
+def solve_dmrg(req: SolveRequest, eigsolver=None, device=0):
     """Runs one request to completion on the calling thread's context.
-    Returns (energy, energies per sweep, converged, maxlinkdim, peak bytes)."""
-    dm = D.DMRG(H.heisenberg_model(1.0, float(req.delta)), neel(req.sites), device=device, flags=flags,
+    Returns (energy, energies per sweep, converged, maxlinkdim, peak bytes):
+    peak bytes = the context's device bytes in use after a sweep beyond what
+    it held before the solve, at most over the sweeps."""
+    base = memory(device)[0]
+    dm = D.DMRG(H.heisenberg_model(1.0, float(req.delta)), neel(req.sites), device=device, flags=req.flags,
                 eigsolver=eigsolver)
     es, conv, peak = [], False, 0
     for _ in range(req.max_sweeps):
         e = dm.sweep(maxdim=req.chi_max, cutoff=req.cutoff, krylovdim=req.krylovdim, maxiter=req.maxiter, tol=1e-14)
-        peak = max(peak, memory(device)[0])
+        peak = max(peak, memory(device)[0] - base)
         if es and abs(e - es[-1]) < req.tol:
             es.append(e)
             conv = True

@@ -91,6 +91,8 @@ def test_served_equals_alone():
             assert a.energy == b.energy and a.energies == b.energies, (a.rid, a.energy, b.energy)
         if co_batch:
             assert max(srv.batch_sizes) > 1
+    # the admission estimate bounds what a solve holds
+    assert all(0 < r.peak_bytes <= r.footprint for r in plain)
     # the N = 10, chi = 32 chain is exact: the ground energy of the Sz = 0 sector
     last = plain[-1]
     assert last.converged
