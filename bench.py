@@ -10,7 +10,9 @@ charge-allowed entries, seed 0; the MPO is the Hubbard MPO.
 A step is one H_eff*psi: every step recomputes all four contractions
 (box-aware grouped block products, no operand permutes) from the inputs
 resident in HBM.  The block structure is planned once (sb_plan_create), as a
-Krylov solve keeps it fixed; the planning time is reported separately.
+Krylov solve keeps it fixed; the planning time (which includes the tile
+autotuner's timing runs, SB_FLAG_AUTOTUNE) is reported separately, and the
+step with the cost model's tile shapes is timed beside it.
 
 The environments are symmetrised so that H_eff is symmetric; "eigsolve" then
 reports the ground energy by FP64 Lanczos, the mixed-precision solve's
@@ -26,7 +28,7 @@ value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
-                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-cpu] [--no-mixed] [--no-eig]
+                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-autotune] [--no-cpu] [--no-mixed] [--no-eig]
                        [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--dmrg-solves N]
                        [--dmrg-sites N] [--dmrg-chi X] [--batch-problems N] [--batch-reps R]
                        [--serve-requests N] [--serve-slots S] [--serve-sites L,..] [--serve-chis X,..]
@@ -59,6 +61,8 @@ def parse():
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--tile", type=int, default=0,
                     help="force tile shape 1+c of the block products (0: chosen at plan time)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="tile shapes from the padded-work cost model instead of the measured autotuner")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-eig", action="store_true", help="skip the eigensolver measurements")
@@ -345,6 +349,9 @@ def main():
     import sparsebackends_amd as sb
 
     flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec] | (a.tile << 16)
+    tune = not a.no_autotune and a.tile == 0
+    if tune:
+        flags |= sb.AUTOTUNE
     t0 = time.perf_counter()
     prob = make_problem(a.workload)
     t_gen = time.perf_counter() - t0
@@ -361,10 +368,11 @@ def main():
     t_plan = time.perf_counter() - t0
     stats = [p.stats() for p in plans]
     opb = [p.operand_bytes() for p in plans]
+    tiles = [p.box_config() for p in plans]
     for _ in range(a.warmup):
         prob.execute()
     sb.synchronize()
-    sb.profile(True)
+    sb.profile(True, products_only=True)  # events around the block products only
     sb.profile_reset()
     sb.synchronize()
     t0 = time.perf_counter()
@@ -374,6 +382,23 @@ def main():
     dt = time.perf_counter() - t0
     prof = sb.profile_read()
     sb.profile(False)
+    # the same steps as one HIP graph launch each (the chain a Krylov iteration replays)
+    graph = sb.PlanGraph(plans)
+    for _ in range(3):
+        graph.launch()
+    sb.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        graph.launch()
+    sb.synchronize()
+    dtg = time.perf_counter() - t0
+    gstat = graph.stats()
+    # and the plain eager steps without any timing events
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        prob.execute()
+    sb.synchronize()
+    dte = time.perf_counter() - t0
     ms_step = 1e3 * dt / a.steps
     value = alg / (dt / a.steps) / 1e12
 
@@ -441,7 +466,33 @@ def main():
         },
         "kernels": {k: dict(v, per_step_ms=v["total_ms"] / a.steps) for k, v in prof.items()},
         "roofline": roof,
+        "graph": dict(ms_per_step=dtg * 1e3 / a.steps, tflops=alg / (dtg / a.steps) / 1e12,
+                      eager_ms_per_step=dte * 1e3 / a.steps, eager_tflops=alg / (dte / a.steps) / 1e12,
+                      note="the timed region above carries HIP events around the block products; these are the "
+                           "same steps as one graph launch each and as plain eager launches, without events",
+                      **gstat),
     }
+
+    names = ["16x128", "128x16", "32x32", "64x32", "32x64", "64x64", "16x64", "64x16"]
+    out["tiles"] = dict(autotuned=tune, per_contraction=[
+        dict(tile=names[t["tile"]] if 0 <= t["tile"] < len(names) else t["tile"], wg_per_cu=t["wg_per_cu"] or "resident",
+             candidates_timed=t["candidates"], best_ms=t["best_ms"]) for t in tiles])
+    if tune:
+        # the same step with the cost model's tile shapes, for comparison
+        pc = copy.copy(prob).upload().outputs()
+        pc.make_plans(flags & ~sb.AUTOTUNE)
+        pc.execute()
+        sb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            pc.execute()
+        sb.synchronize()
+        dcm = (time.perf_counter() - t0) / a.steps
+        out["tiles"]["cost_model_ms_per_step"] = dcm * 1e3
+        out["tiles"]["cost_model_tiles"] = [names[p.box_config()["tile"]] for p in pc.plans]
+        out["tiles"]["autotuned_speedup"] = dcm / (dt / a.steps)
+        out["tiles"]["same_result"] = bool(np.array_equal(pc.Hpsi.parts()["data"], prob.Hpsi.parts()["data"]))
+        del pc
 
     if not a.no_mixed and a.prec == "fp64":
         # the precision forms on the same problem, with their error vs FP64

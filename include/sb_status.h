@@ -43,6 +43,13 @@
  * (the reference creates one for every entry, contract_coo_dense.jl:113-122
  * in the included file; its test coo_dense.jl:164-189 expects none). */
 #define SB_FLAG_SKIP_ZERO_SLICES  (1u << 1)
+/* Box products (BS x BS contract_shared!, strided route): choose the tile
+ * shape and grid density by timing every candidate on the plan's own work at
+ * plan creation, instead of the padded-work cost model; the choice is cached
+ * per block-size histogram (log2 bins of the output blocks' row and column
+ * extents), so later plans of the same histogram reuse it without timing.
+ * Speed only: every choice gives the same result. */
+#define SB_FLAG_AUTOTUNE          (1u << 2)
 
 /* precision form of the block GEMMs (bits 8..11).  Storage stays float64. */
 #define SB_PREC_MASK    (0xFu << 8)

@@ -37,6 +37,7 @@ typedef struct sb_ctx_s* sb_ctx_t;      /* device + stream + device-memory cache
 typedef struct sb_tensor_s* sb_tensor_t; /* a device-resident tensor of one storage kind */
 typedef struct sb_plan_s* sb_plan_t;     /* a contraction with its block structure fixed */
 typedef struct sb_batch_s* sb_batch_t;   /* independent plan chains executed together */
+typedef struct sb_graph_s* sb_graph_t;   /* a plan chain replayed as a HIP graph */
 
 /* ------------------------------------------------------------ runtime */
 const char* sb_last_error(void);
@@ -52,8 +53,11 @@ int sb_ctx_synchronize(sb_ctx_t ctx);
 int sb_ctx_memory(sb_ctx_t ctx, int64_t* in_use, int64_t* cached);
 int sb_ctx_trim(sb_ctx_t ctx);
 /* New (the reference's only tracing is the SB_TRACE prints, e.g.
- * contract_bs_dense.jl:112-114): time every numeric step of plan executions
- * with HIP events on the context's stream, aggregated per kernel name */
+ * contract_bs_dense.jl:112-114): time the numeric steps of plan executions
+ * with HIP events on the context's stream, aggregated per kernel name.
+ * enable: 0 off, 1 every step, 2 only the block products (grouped_gemm), whose
+ * timing then disturbs the stream least.  Plan chains run eagerly (no graph
+ * replays) while profiling is on. */
 int sb_ctx_profile(sb_ctx_t ctx, int enable);
 int sb_ctx_profile_reset(sb_ctx_t ctx);
 /* synchronises; writes lines "name count total_ms min_ms max_ms" into buf
@@ -189,7 +193,31 @@ int sb_plan_stats(sb_plan_t plan, double* flops, double* gemm_bytes, double* per
                   int64_t* nblocks_out);
 /* distinct payload bytes of the block-product operands (A, B) and result C */
 int sb_plan_operand_bytes(sb_plan_t plan, double* a_bytes, double* b_bytes, double* c_bytes);
+/* New (no counterpart in the reference): the box product's launch shape —
+ * tile configuration (-1: the plan is not one box product), workgroups per
+ * CU (0: as many as are resident), and for a plan made with SB_FLAG_AUTOTUNE
+ * the number of candidates timed (0: the choice came from the histogram
+ * cache or the cost model) and the best time measured in ms. */
+int sb_plan_box_config(sb_plan_t plan, int* tile, int* wg_per_cu, int* candidates, double* best_ms);
+/* forget the autotuner's cached choices (all devices) */
+int sb_autotune_reset(void);
 int sb_plan_free(sb_plan_t plan);
+
+/* New (no counterpart in the reference, which re-derives every contraction on
+ * every call): a chain of plans executed in order as one HIP graph, e.g. the
+ * four contractions of one H_eff*psi.  sb_graph_launch runs the chain: the
+ * first launch eagerly, the second captured into a graph (kernel nodes only),
+ * later ones as one graph launch while the chain's buffers and its host-side
+ * decisions (which operands need a box scan) match the capture; results are
+ * those of executing the plans one by one.  Chains with a plan that is not a
+ * box product, and every launch while the context profiles, run eagerly.
+ * sb_ctx_graphs sets whether eigensolves replay their operator this way:
+ * 1 always, 0 never, -1 (default) when krylovdim * maxiter >= 8. */
+int sb_graph_create(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_graph_t* out);
+int sb_graph_launch(sb_ctx_t ctx, sb_graph_t graph);
+int sb_graph_stats(sb_graph_t graph, int64_t* eager, int64_t* captures, int64_t* replays, int64_t* nodes);
+int sb_graph_free(sb_graph_t graph);
+int sb_ctx_graphs(sb_ctx_t ctx, int mode);
 
 /* New (no counterpart in the reference, which contracts one problem per call;
  * it is the batched schedule the many-small-problems workloads need): a batch

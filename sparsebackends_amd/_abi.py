@@ -80,8 +80,15 @@ SIGNATURES = {
     "sb_plan_create": [P, P, U64P, C.c_int, P, U64P, C.c_int, P, U64P, C.c_int, C.c_uint32, PP],
     "sb_plan_execute": [P, P],
     "sb_plan_stats": [P, F64P, F64P, F64P, I64P, I64P],
+    "sb_plan_box_config": [P, I32P, I32P, I32P, F64P],
+    "sb_autotune_reset": [],
     "sb_plan_operand_bytes": [P, F64P, F64P, F64P],
     "sb_plan_free": [P],
+    "sb_graph_create": [P, PP, C.c_int, PP],
+    "sb_graph_launch": [P, P],
+    "sb_graph_stats": [P, I64P, I64P, I64P, I64P],
+    "sb_graph_free": [P],
+    "sb_ctx_graphs": [P, C.c_int],
     "sb_batch_create": [P, PP, C.c_int, C.c_int, PP],
     "sb_batch_execute": [P, P, I32P, C.c_int],
     "sb_batch_stats": [P, F64P, I32P],

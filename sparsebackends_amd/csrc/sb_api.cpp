@@ -7,6 +7,7 @@
 #include "../../include/sparsebackends_amd.h"
 #include "sb_batch.h"
 #include "sb_factorize.h"
+#include "sb_graph.h"
 #include "sb_krylov.h"
 
 using namespace sb;
@@ -109,7 +110,8 @@ int sb_ctx_profile(sb_ctx_t c, int enable) {
   SB_TRY({
     Ctx* ctx = C_(c);
     if (!enable) ctx->prof_collect();
-    ctx->profile = enable != 0;
+    if (enable < 0 || enable > 2) fail(SB_E_ARGUMENT, "ArgumentError: profile level must be 0, 1 or 2");
+    ctx->profile = enable;
   })
 }
 int sb_ctx_profile_reset(sb_ctx_t c) {
@@ -549,6 +551,19 @@ int sb_plan_stats(sb_plan_t p, double* flops, double* gemm_bytes, double* perm_b
     if (nblocks_out) *nblocks_out = pl->nblocks_out;
   })
 }
+int sb_plan_box_config(sb_plan_t p, int* tile, int* wg_per_cu, int* candidates, double* best_ms) {
+  SB_TRY({
+    if (!p) fail(SB_E_HANDLE, "null plan");
+    Plan* pl = reinterpret_cast<Plan*>(p);
+    if (tile) *tile = pl->box ? pl->box->g.cfg : -1;
+    if (wg_per_cu) *wg_per_cu = pl->box ? pl->box->g.wgcu : 0;
+    if (candidates) *candidates = pl->tune.candidates;
+    if (best_ms) *best_ms = pl->tune.ms;
+  })
+}
+int sb_autotune_reset(void) {
+  SB_TRY({ autotune_cache_clear(); })
+}
 int sb_plan_operand_bytes(sb_plan_t p, double* a_bytes, double* b_bytes, double* c_bytes) {
   SB_TRY({
     if (!p) fail(SB_E_HANDLE, "null plan");
@@ -559,6 +574,45 @@ int sb_plan_operand_bytes(sb_plan_t p, double* a_bytes, double* b_bytes, double*
   })
 }
 int sb_plan_free(sb_plan_t p) { SB_TRY(delete reinterpret_cast<Plan*>(p)) }
+
+int sb_graph_create(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_graph_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!plans || !out) fail(SB_E_ARGUMENT, "ArgumentError: null argument");
+    if (nplans < 1) fail(SB_E_ARGUMENT, "ArgumentError: a plan chain needs at least one plan");
+    std::vector<const Plan*> v(nplans);
+    for (int i = 0; i < nplans; ++i) v[i] = reinterpret_cast<const Plan*>(plans[i]);
+    *out = reinterpret_cast<sb_graph_t>(new ChainGraph(ctx, v));
+  })
+}
+int sb_graph_launch(sb_ctx_t c, sb_graph_t g) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!g) fail(SB_E_HANDLE, "null graph");
+    reinterpret_cast<ChainGraph*>(g)->launch();
+    (void)ctx;
+  })
+}
+int sb_graph_stats(sb_graph_t g, int64_t* eager, int64_t* captures, int64_t* replays, int64_t* nodes) {
+  SB_TRY({
+    if (!g) fail(SB_E_HANDLE, "null graph");
+    ChainGraph* cg = reinterpret_cast<ChainGraph*>(g);
+    if (eager) *eager = cg->eager;
+    if (captures) *captures = cg->captures;
+    if (replays) *replays = cg->replays;
+    if (nodes) *nodes = (int64_t)cg->nodes();
+  })
+}
+int sb_graph_free(sb_graph_t g) {
+  SB_TRY({ delete reinterpret_cast<ChainGraph*>(g); })
+}
+int sb_ctx_graphs(sb_ctx_t c, int mode) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (mode < -1 || mode > 1) fail(SB_E_ARGUMENT, "ArgumentError: graph mode must be -1, 0 or 1");
+    ctx->graphs = mode;
+  })
+}
 
 int sb_batch_create(sb_ctx_t c, const sb_plan_t* plans, int nchains, int nstages, sb_batch_t* out) {
   SB_TRY({

@@ -11,7 +11,7 @@ namespace {
 
 template <class F>
 void timed(Ctx* ctx, const char* name, F&& f) {
-  if (!ctx->profile) {
+  if (!ctx->profiled(name)) {
     f();
     return;
   }

@@ -861,7 +861,8 @@ int box_tile_configs() { return kNumTiles; }
 bool box_tile_is_wave(int cfg) { return cfg >= 8; }
 
 template <class K>
-static i64 grid_for(Ctx* ctx, K kern, std::atomic<int>& per_cu_cache) {
+static i64 grid_for(Ctx* ctx, K kern, std::atomic<int>& per_cu_cache, int want = 0) {
+  if (want > 0) return std::max<i64>(NXCD, (i64)ctx->num_cu * want / NXCD * NXCD);
   int per_cu = per_cu_cache.load(std::memory_order_relaxed);
   if (!per_cu) {  // resident workgroups per CU (the same on every device of this build)
     int nb = 0;
@@ -878,7 +879,7 @@ template <int BM, int BN, int WM, int WN, int PREC>
 static void launch_one(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
   auto kern = box_gemm_kernel<BM, BN, WM, WN, PREC>;
   static std::atomic<int> per_cu{0};
-  i64 nwg = grid_for(ctx, kern, per_cu);
+  i64 nwg = grid_for(ctx, kern, per_cu, g.wgcu);
   hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.gorder, g.seg, g.aoff,
                      g.boff, g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox,
                      g.amf, g.bnf);
@@ -924,7 +925,7 @@ template <int FM, int FN, int PREC>
 static void launch_wave(Ctx* ctx, const BoxGemmArgs& g, const double* A, const double* B, double* C, i32* cflag) {
   auto kern = box_wave_kernel<FM, FN, PREC>;
   static std::atomic<int> per_cu{0};
-  i64 nwg = grid_for(ctx, kern, per_cu);
+  i64 nwg = grid_for(ctx, kern, per_cu, g.wgcu);
   hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, ctx->stream, g.d, g.ng, g.seg, g.aoff, g.boff,
                      g.coff, g.gbox, g.kbox, g.kc, g.toff, g.trec, A, B, C, cflag, g.clear, g.cnew, g.cbox);
   HIP_CHECK(hipGetLastError());

@@ -596,6 +596,13 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
         g.cfg = c;
       }
     }
+    if (flags & SB_FLAG_AUTOTUNE) {
+      const BoxOperands o{TA.box.get(), TA.bflag.get(), TB.box.get(), TB.bflag.get(), C.box.get(), C.bflag.get(),
+                          TA.data.get(), TB.data.get(), C.data.get()};
+      pl.tune = autotune_box(ctx, flags, g, hb, M, N, o);
+      g.cfg = pl.tune.cfg;
+      g.wgcu = pl.tune.wgcu;
+    }
   }
   BoxTile t = box_tile(g.cfg);
   g.max_tiles = ng * ((M + t.BM - 1) / t.BM) * ((N + t.BN - 1) / t.BN);

@@ -36,6 +36,30 @@ struct BoxPlan {
   std::vector<i64> cfull;  // C's full tail extents
 };
 
+// Device pointers one box product reads and writes.
+struct BoxOperands {
+  const i32 *abox, *aflag, *bbox, *bflag;
+  i32 *cbox, *cflag;
+  const double *A, *B;
+  double* C;
+};
+// The autotuner's choice for one box product (sb_autotune.cpp): tile shape,
+// workgroups per CU (0: as many as are resident), whether it came from the
+// histogram cache, the best time measured and the candidates timed.
+struct BoxTuneResult {
+  int cfg = 0, wgcu = 0;
+  bool cached = false;
+  double ms = 0;
+  int candidates = 0;
+};
+// Times the candidate tile shapes and grid densities of the box product g
+// (group boxes hb on the host, full row/column extents M, N) on its operands,
+// or reuses the choice cached for the same block-size histogram.  Runs the
+// product several times: C ends holding its result.
+BoxTuneResult autotune_box(Ctx* ctx, unsigned flags, const BoxGemmArgs& g, const std::vector<i32>& hb, i64 M, i64 N,
+                           const BoxOperands& o);
+void autotune_cache_clear();
+
 struct Plan {
   Ctx* ctx = nullptr;
   std::vector<Step> steps;
@@ -63,13 +87,14 @@ struct Plan {
   const Tensor* in_a = nullptr;  // the operands the plan reads at every execution
   const Tensor* in_b = nullptr;
   std::shared_ptr<BoxPlan> box;  // set when the steps are one box product
+  BoxTuneResult tune;            // the autotuner's choice (candidates == 0: not tuned)
   void finish() const {
     if (!out) return;
     out->touch();
     if (out_box) out->box_version = out->version;
   }
   void run_step(const Step& s) const {
-    if (!ctx->profile) {
+    if (!ctx->profiled(s.name)) {
       s();
       return;
     }

@@ -98,7 +98,11 @@ struct Ctx {
   bool own_stream = true;
   DeviceCache cache;
   int num_cu = 256;
-  bool profile = false;
+  // kernel timing with HIP events on the stream: 0 off, 1 every timed step,
+  // 2 only the block products (grouped_gemm), the fewest events
+  int profile = 0;
+  int graphs = -1;  // eigensolves replay their operator as a HIP graph: 1 always, 0 never, -1 for long solves
+  bool profiled(const char* name) const;
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, ProfAgg> prof;

@@ -133,6 +133,7 @@ struct BoxGemmArgs {
   BoxDesc d;
   i64 ng = 0, np = 0, max_tiles = 0;
   int cfg = 0, amf = 0, bnf = 0;
+  int wgcu = 0;  // persistent workgroups per CU (0: as many as are resident)
   // fixed per plan: pairs in group order
   const i32* seg = nullptr;   // ng + 1
   const i64* aoff = nullptr;  // per pair: element offset of the A block

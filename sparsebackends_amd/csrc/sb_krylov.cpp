@@ -8,6 +8,7 @@
 #include <cstring>
 
 #include "sb_krylov.h"
+#include "sb_graph.h"
 
 namespace sb {
 
@@ -32,7 +33,7 @@ namespace {
 
 template <class F>
 void step(Ctx* ctx, const char* name, F&& f) {
-  if (!ctx->profile) {
+  if (!ctx->profiled(name)) {
     f();
     return;
   }
@@ -585,8 +586,13 @@ EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tens
   need_bs(y, "eigsolve");
   check_chain(plans.front(), plans.back(), x, y);
   Lanczos L(ctx, x, y, kdim, maxiter, tol);
+  std::unique_ptr<ChainGraph> cg;
+  if (eig_uses_graph(ctx, kdim, maxiter)) cg = std::make_unique<ChainGraph>(ctx, plans);
   auto apply = [&] {
-    for (auto* p : plans) p->run();
+    if (cg)
+      cg->launch();
+    else
+      for (auto* p : plans) p->run();
   };
   L.begin();
   apply();  // H v0

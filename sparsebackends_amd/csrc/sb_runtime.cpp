@@ -1,6 +1,7 @@
 // Runtime: error state, contexts (device + stream), caching device allocator.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <algorithm>
 
 #include "sb_internal.h"
@@ -100,14 +101,20 @@ Ctx::~Ctx() {
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
+bool Ctx::profiled(const char* name) const {
+  return profile == 1 || (profile == 2 && std::strcmp(name, "grouped_gemm") == 0);
+}
+
 hipEvent_t Ctx::event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
     event_pool.pop_back();
     return e;
   }
+  // timing-only events: without the system-scope fence on record, which would
+  // write back and invalidate the caches between the kernels being timed
   hipEvent_t e;
-  HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   return e;
 }
 

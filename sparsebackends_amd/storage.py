@@ -60,9 +60,10 @@ def memory(device=0):
     return int(a.value), int(b.value)
 
 
-def profile(enable=True, device=0):
-    """Time every numeric plan step with HIP events on the library's stream."""
-    check(load().sb_ctx_profile(context(device), 1 if enable else 0))
+def profile(enable=True, device=0, products_only=False):
+    """Time every numeric plan step (products_only: only the block products,
+    grouped_gemm) with HIP events on the library's stream."""
+    check(load().sb_ctx_profile(context(device), (2 if products_only else 1) if enable else 0))
 
 
 def profile_reset(device=0):

@@ -12,6 +12,7 @@ from .storage import DenseTensor
 # flags (include/sb_status.h)
 EXACT_JOIN = 1 << 0
 SKIP_ZERO_SLICES = 1 << 1
+AUTOTUNE = 1 << 2
 PREC_FP64 = 0 << 8
 PREC_FP32 = 1 << 8
 PREC_BF16X3 = 2 << 8
@@ -82,6 +83,15 @@ class Plan:
         return dict(flops=f.value, gemm_bytes=gb.value, perm_bytes=pb.value, npairs=npairs.value,
                     nblocks_out=nblk.value)
 
+    def box_config(self):
+        """the box product's launch shape: tile (-1: not a box product),
+        workgroups per CU (0: all resident), autotuner candidates timed (0:
+        not timed) and the best time measured (ms)"""
+        t, w, n = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        check(load().sb_plan_box_config(self._h, C.byref(t), C.byref(w), C.byref(n), C.byref(ms)))
+        return dict(tile=t.value, wg_per_cu=w.value, candidates=n.value, best_ms=ms.value)
+
     def operand_bytes(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(load().sb_plan_operand_bytes(self._h, C.byref(a), C.byref(b), C.byref(c)))
@@ -138,3 +148,44 @@ class PlanBatch:
                 self._h = C.c_void_p()
         except Exception:
             pass
+
+
+class PlanGraph:
+    """A chain of plans executed in order as one HIP graph (sb_graph_*): the
+    first launch() runs eagerly, the second is captured, later ones replay the
+    graph while the chain's buffers and box decisions match the capture.
+    Results equal executing the plans one by one."""
+
+    def __init__(self, plans):
+        plans = list(plans)
+        if not plans:
+            raise ValueError("a plan chain needs at least one plan")
+        self._plans = plans  # the graph refers to the plans
+        self.ctx = plans[0].ctx
+        arr = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
+        h = C.c_void_p()
+        check(load().sb_graph_create(self.ctx, C.cast(arr, C.POINTER(C.c_void_p)), len(plans), C.byref(h)))
+        self._h = h
+
+    def launch(self):
+        check(load().sb_graph_launch(self.ctx, self._h))
+
+    def stats(self):
+        e, c, r, n = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(load().sb_graph_stats(self._h, C.byref(e), C.byref(c), C.byref(r), C.byref(n)))
+        return dict(eager=e.value, captures=c.value, replays=r.value, nodes=n.value)
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().sb_graph_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def set_graphs(mode, device=0):
+    """eigensolves replay their operator as a HIP graph: 1 always, 0 never,
+    -1 (default) when krylovdim * maxiter >= 8"""
+    from .storage import context
+    check(load().sb_ctx_graphs(context(device), int(mode)))
