@@ -343,7 +343,27 @@ __global__ void axpy_groups_kernel(i64 L, const double* __restrict__ alpha, cons
   if (t >= ngroups * L) return;
   i64 grp = t / L, e = t - grp * L;
   double c = 0.0;
-  for (i32 pp = seg[grp]; pp < seg[grp + 1]; ++pp) {
+  const i32 end = seg[grp + 1];
+  i32 pp = seg[grp];
+  // eight terms' loads issued together, then added in order: the same IEEE
+  // operation sequence as one term at a time, without a load latency per term
+  for (; pp + 8 <= end; pp += 8) {
+    i32 pi[8];
+    double a[8], x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pi[u] = order[pp + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = alpha[pi[u]];
+      x[u] = X[xoff[pi[u]] + e];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      double prod = a[u] * x[u];
+      c = c + prod;
+    }
+  }
+  for (; pp < end; ++pp) {
     i32 pi = order[pp];
     double prod = alpha[pi] * X[xoff[pi] + e];
     c = c + prod;
@@ -361,7 +381,25 @@ __global__ void outer_groups_kernel(i64 m, i64 n, const double* __restrict__ A, 
   i64 grp = t / bs, e = t - grp * bs;
   i64 i = e % m, j = e / m;
   double c = 0.0;
-  for (i32 pp = seg[grp]; pp < seg[grp + 1]; ++pp) {
+  const i32 end = seg[grp + 1];
+  i32 pp = seg[grp];
+  for (; pp + 8 <= end; pp += 8) {  // loads of eight terms together, adds in order
+    i32 pi[8];
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pi[u] = order[pp + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = A[aoff[pi[u]] + i];
+      b[u] = B[boff[pi[u]] + j];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      double prod = a[u] * b[u];
+      c = c + prod;
+    }
+  }
+  for (; pp < end; ++pp) {
     i32 pi = order[pp];
     double prod = A[aoff[pi] + i] * B[boff[pi] + j];
     c = c + prod;
