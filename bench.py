@@ -23,13 +23,14 @@ whole two-site DMRG ground-state solves of XXZ chains (solves/s, p50/p99);
 batch (one box product launch per contraction stage over all problems);
 "serve" runs a stream of DMRG requests through the serving loop against the
 plain one-at-a-time queue (solves/s, p50/p99 latency, FP64 and FP32 products);
-"coo" times the element-sparse workload (PXP chain, COO x Dense at chi=128).
+"coo" times the element-sparse workload (PXP chain, COO x Dense at chi=128);
+"sweep" times one two-site DMRG sweep of the N=32 Heisenberg chain at chi=256.
 
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-chi2048|heisenberg-chi256]
-                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-autotune] [--no-cpu] [--no-coo] [--no-mixed] [--no-eig]
+                       [--prec fp64|fp32|bf16x3] [--tile T] [--no-autotune] [--no-cpu] [--no-coo] [--no-sweep] [--no-mixed] [--no-eig]
                        [--krylovdim K] [--eig-tol T] [--eig-tol-lo T] [--solves N] [--dmrg-solves N]
                        [--dmrg-sites N] [--dmrg-chi X] [--batch-problems N] [--batch-reps R]
                        [--serve-requests N] [--serve-slots S] [--serve-sites L,..] [--serve-chis X,..]
@@ -66,6 +67,7 @@ def parse():
                     help="tile shapes from the padded-work cost model instead of the measured autotuner")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-coo", action="store_true", help="skip the element-sparse (PXP, COO) workload")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the N=32 chi=256 Heisenberg DMRG sweep")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-eig", action="store_true", help="skip the eigensolver measurements")
     ap.add_argument("--krylovdim", type=int, default=30)
@@ -177,6 +179,33 @@ def dmrg_section(sb, a):
                 solves_per_s=len(lat) / lat.sum(), p50_s=float(np.percentile(lat, 50)),
                 p99_s=float(np.percentile(lat, 99)), max_energy_error_vs_chi256=float(np.max(np.abs(errs))),
                 note="sequential solves, each recomputing its plans at every step")
+
+
+def sweep_section(sb, a):
+    """One two-site DMRG sweep of the plain configuration's chain: S=1/2
+    Heisenberg J=1, N=32, U(1) Sz, MPO w=5, bond dimension 256, Krylov
+    dimension 3 (maxiter 1), cutoff 1e-10.  The MPS is grown from the Neel
+    state by untimed sweeps until its bonds reach 256 (the configuration's
+    random QR-orthonormalised start is replaced by a converging state of the
+    same bond dimension); then one full sweep (left to right and back, 62
+    two-site steps) is timed, and its energy compared with the chain's exact
+    ground energy is reported."""
+    from sparsebackends_amd import heff as H
+    from sparsebackends_amd import dmrg as D
+    N, chi = 32, 256
+    dm = D.DMRG(H.heisenberg_model(1.0), [0, 1] * (N // 2))
+    warm = 0
+    while dm.maxlinkdim() < chi and warm < 8:
+        dm.sweep(maxdim=chi, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
+        warm += 1
+    sb.synchronize()
+    t0 = time.perf_counter()
+    e = dm.sweep(maxdim=chi, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
+    sb.synchronize()
+    ts = time.perf_counter() - t0
+    return dict(model="Heisenberg S=1/2 J=1, N=32, U(1) Sz, MPO w=5", chi=chi, maxlinkdim=int(dm.maxlinkdim()),
+                warmup_sweeps=warm, krylovdim=3, maxiter=1, cutoff=1e-10, steps=2 * (N - 1), sweep_s=ts,
+                ms_per_step=ts / (2 * (N - 1)) * 1e3, energy=e, truncerr=dm.truncerr)
 
 
 def batch_section(sb, a):
@@ -536,7 +565,8 @@ def main():
                       **gstat),
     }
 
-    names = ["16x128", "128x16", "32x32", "64x32", "32x64", "64x64", "16x64", "64x16"]
+    names = ["16x128", "128x16", "32x32", "64x32", "32x64", "64x64", "16x64", "64x16",
+             "wave16x16", "wave32x16", "wave16x32", "wave32x32", "wave64x16", "wave16x64"]
     out["tiles"] = dict(autotuned=tune, per_contraction=[
         dict(tile=names[t["tile"]] if 0 <= t["tile"] < len(names) else t["tile"], wg_per_cu=t["wg_per_cu"] or "resident",
              candidates_timed=t["candidates"], best_ms=t["best_ms"]) for t in tiles])
@@ -615,6 +645,9 @@ def main():
 
     if a.batch_problems > 0 and a.prec == "fp64" and not a.no_eig:
         out["batch"] = batch_section(sb, a)
+
+    if not a.no_sweep and a.prec == "fp64" and not a.no_eig:
+        out["sweep"] = sweep_section(sb, a)
 
     if not a.no_coo and a.prec == "fp64":
         out["coo"] = coo_section(sb, a)

@@ -3,8 +3,9 @@
 // The padded-work cost model picks a tile from the output blocks' boxes alone;
 // what actually runs fastest also depends on the K extents, the operand reuse
 // through L2 and how many persistent workgroups share a CU.  With
-// SB_FLAG_AUTOTUNE a box plan instead times every workgroup tile shape at
-// three grid densities on its own operands at plan creation (one warm-up and
+// SB_FLAG_AUTOTUNE a box plan instead times every tile shape (workgroup tiles
+// and one-wave tiles) at three grid densities on its own operands at plan
+// creation (one warm-up and
 // three timed executions of the prepare + product pair each, the fastest
 // kept), and remembers the winner under the plan's block-size histogram, so a
 // later plan with the same histogram reuses it without timing.
@@ -78,7 +79,6 @@ BoxTuneResult autotune_box(Ctx* ctx, unsigned flags, const BoxGemmArgs& g0, cons
   }
   i64 maxt = 1;
   for (int c = 0; c < box_tile_configs(); ++c) {
-    if (box_tile_is_wave(c)) continue;
     const BoxTile t = box_tile(c);
     maxt = std::max<i64>(maxt, g0.ng * ((M + t.BM - 1) / t.BM) * ((N + t.BN - 1) / t.BN));
   }
@@ -93,8 +93,7 @@ BoxTuneResult autotune_box(Ctx* ctx, unsigned flags, const BoxGemmArgs& g0, cons
   };
   double best = 1e300;
   const int dens[3] = {0, 1, 2};  // resident workgroups per CU: as many as fit, one, two
-  for (int c = 0; c < box_tile_configs(); ++c) {
-    if (box_tile_is_wave(c)) continue;
+  for (int c = 0; c < box_tile_configs(); ++c) {  // workgroup tiles and one-wave tiles
     for (int w : dens) {
       g.cfg = c;
       g.wgcu = w;

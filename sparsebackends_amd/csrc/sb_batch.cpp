@@ -89,17 +89,20 @@ std::unique_ptr<Batch> make_batch(Ctx* ctx, const std::vector<Plan*>& plans, int
     std::map<int, double> vote;
     const unsigned prec0 = st.plans[0]->box ? (st.plans[0]->box->flags & SB_PREC_MASK) : 0;
     for (auto* p : st.plans) {
-      if (!p->box || p->steps.size() != 3 || box_tile_is_wave(p->box->g.cfg) ||
-          (p->box->flags & SB_PREC_MASK) != prec0) {
+      if (!p->box || p->steps.size() != 3 || (p->box->flags & SB_PREC_MASK) != prec0) {
         ok = false;
         break;
       }
-      vote[p->box->g.cfg] += p->flops + 1.0;
+      // the batched launch takes workgroup tiles: plans that chose a one-wave tile do not vote
+      if (!box_tile_is_wave(p->box->g.cfg)) vote[p->box->g.cfg] += p->flops + 1.0;
     }
     if (!ok) continue;
     st.fused = true;
     st.prec = prec0;
-    st.cfg = std::max_element(vote.begin(), vote.end(), [](auto& x, auto& y) { return x.second < y.second; })->first;
+    st.cfg = vote.empty() ? 3  // 64 x 32
+                          : std::max_element(vote.begin(), vote.end(), [](auto& x, auto& y) {
+                              return x.second < y.second;
+                            })->first;
     const BoxTile t = box_tile(st.cfg);
     std::vector<BoxJob> hj;
     i64 npos = 0, maxt = 0;

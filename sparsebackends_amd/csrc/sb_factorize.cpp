@@ -21,8 +21,10 @@ void k_sect_gather(Ctx*, const double* psi, int d1, int d2, i64 Dl, i64 ntask, c
 size_t sect_svd_lds(int n, int len);
 void k_sect_svd(Ctx*, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                double* SVo,
-                const int* glist, double* ws, const i64* wsoff);
+                double* SVo, const int* glist);
+void k_sect_svd_blocks(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn, const std::vector<int>& ll,
+                       double* X, const i64* xoff, const int* R, const int* Cc, const int* trans, const i64* uoff,
+                       const i64* soff, const i64* sgoff, double* sig, double* Uo, double* SVo);
 void k_sect_emit(Ctx*, bool isA, i64 nblk, int d, i64 D1, i64 D2, i64 nside, const i64* blk, const i64* sec,
                  const i64* off, const i64* dims, const i64* kept, const int* RC, const i64* uoff_or_soff,
                  const i64* sgoff, const double* sig, const double* F, bool right, double* out);
@@ -208,12 +210,12 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   const i64 ng = (i64)act.size();
   std::vector<int> hR(ng), hC(ng), htr(ng);
   std::vector<i64> xoff(ng + 1, 0), uoff(ng + 1, 0), soff(ng + 1, 0), sgoff(ng + 1, 0);
-  // sectors whose factorisation fits the LDS of a workgroup (lds_g), the
-  // others (big_g) run the same kernel on a global workspace
-  constexpr size_t kLds = 160 * 1024;
+  // sectors whose rows and rotations fit the LDS of a workgroup (lds_g) are
+  // factorised there, one workgroup each; larger ones (big_g) by block Jacobi
+  // over many workgroups
+  constexpr size_t kLds = 159 * 1024;
   size_t lds = 0;
-  std::vector<int> lds_g, big_g;
-  std::vector<i64> wsoff(1, 0);
+  std::vector<int> lds_g, big_g, big_n, big_len;
   for (i64 g = 0; g < ng; ++g) {
     const i64 m = act[g];
     if (R[m] > INT32_MAX || Cn[m] > INT32_MAX) fail(SB_E_LIMIT, "sector matrix dimension beyond int32");
@@ -221,13 +223,13 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
     hC[g] = (int)Cn[m];
     htr[g] = R[m] > Cn[m];
     const int n = (int)std::min(R[m], Cn[m]), len = (int)std::max(R[m], Cn[m]);
-    const size_t need = sect_svd_lds(n, len);
-    if (need <= kLds) {
-      lds = std::max(lds, need);
+    if (sect_svd_lds(n, len) <= kLds) {
+      lds = std::max(lds, sect_svd_lds(n, len));
       lds_g.push_back((int)g);
     } else {
       big_g.push_back((int)g);
-      wsoff.push_back(wsoff.back() + (i64)((need + 255) / 256 * 256 / sizeof(double)));
+      big_n.push_back(n);
+      big_len.push_back(len);
     }
     xoff[g + 1] = xoff[g] + R[m] * Cn[m];
     uoff[g + 1] = uoff[g] + R[m] * n;
@@ -271,23 +273,15 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   k_sect_gather(ctx, psi.dptr(), d1, d2, Dl, ntask, d_task.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(),
                 X.get());
   {
-    DBuf<int> d_lg, d_bg;
-    DBuf<i64> d_wsoff;
-    DBuf<double> ws;
-    if (!big_g.empty()) {
-      up(ctx, d_bg, big_g);
-      up(ctx, d_wsoff, wsoff);
-      ws.alloc(ctx, (size_t)wsoff.back());
-    }
-    if (!lds_g.empty() && !big_g.empty()) up(ctx, d_lg, lds_g);
-    if (!lds_g.empty())
+    DBuf<int> d_lg;
+    if (!lds_g.empty()) {
+      up(ctx, d_lg, lds_g);
       k_sect_svd(ctx, (i64)lds_g.size(), lds, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(),
-                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), big_g.empty() ? nullptr : d_lg.get(),
-                 nullptr, nullptr);
-    if (!big_g.empty())
-      k_sect_svd(ctx, (i64)big_g.size(), 0, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(),
-                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), d_bg.get(), ws.get(), d_wsoff.get());
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host lists and the workspace go out of scope
+                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), d_lg.get());
+    }
+    k_sect_svd_blocks(ctx, big_g, big_n, big_len, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(),
+                      d_uoff.get(), d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get());
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host list goes out of scope
   }
   std::vector<double> hs;
   down(ctx, hs, sig.get(), (size_t)sgoff[ng]);

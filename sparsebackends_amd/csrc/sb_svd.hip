@@ -16,6 +16,52 @@ __device__ inline double wave_sum(double v) {
 
 // W rows of group g: row r at woff[g] + r * ncols_g, ncols_g = (gstart[g+1]-gstart[g]) * d2,
 // column j*d2 + t = A[bsrc[j] + r + d1*t] (block j of the group, column-major d1 x d2 block)
+// Convergence of one-sided Jacobi: a pair of rows counts as orthogonal when
+// |x.y| <= tol |x| |y|.  The computed dot product of rows of length len
+// carries a rounding error up to about len * eps |x| |y|, so a tolerance
+// below that never holds for long rows and every sweep would rotate again;
+// tol = max(1e-15, 2 len eps) (eps = 2^-53).
+__device__ inline double jacobi_tol(i64 len) { return fmax(1e-15, 2.0 * (double)len * 1.1102230246251565e-16); }
+// A row whose norm is below 1e-15 |X|_F (|X|_F is invariant under the
+// rotations) is a zero row to working precision: a rank-deficient matrix
+// rotates its null rows into rounding noise, whose direction is random, so
+// the relative test above would fail for them at every sweep.  Pairs with
+// such a row are left alone; the singular values they stand for lie far
+// below any truncation cutoff.
+__device__ inline double jacobi_null2(double fro2) { return 1e-30 * fro2; }
+
+// One one-sided Jacobi rotation of the rows xp, xq (length len), applied to
+// the columns up, uq (length n) of the accumulated rotation; one wave.
+// Returns whether it rotated.
+__device__ inline bool jacobi_pair(double* xp, double* xq, int len, double* up, double* uq, int n, int lane,
+                                   double ctol, double null2) {
+  double a = 0, b = 0, c = 0;
+  for (int j = lane; j < len; j += 64) {
+    const double x = xp[j], y = xq[j];
+    a = fma(x, x, a);
+    b = fma(y, y, b);
+    c = fma(x, y, c);
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  if (c == 0.0 || a <= null2 || b <= null2 || fabs(c) <= ctol * sqrt(a) * sqrt(b)) return false;
+  const double zeta = (b - a) / (2.0 * c);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+  for (int j = lane; j < len; j += 64) {
+    const double x = xp[j], y = xq[j];
+    xp[j] = cs * x - sn * y;
+    xq[j] = sn * x + cs * y;
+  }
+  for (int r = lane; r < n; r += 64) {
+    const double u = up[r], v = uq[r];
+    up[r] = cs * u - sn * v;
+    uq[r] = sn * u + cs * v;
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict__ A, int d1, int d2,
                                                        const i64* __restrict__ gstart, const i64* __restrict__ bsrc,
                                                        const i64* __restrict__ woff, double* __restrict__ W,
@@ -36,6 +82,10 @@ __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict
   }
   for (int e = lane; e < d1 * d1; e += 64) U[e] = (e % d1 == e / d1) ? 1.0 : 0.0;
   __syncthreads();
+  const double ctol = jacobi_tol(nc);
+  double fro2 = 0;
+  for (i64 e = lane; e < (i64)d1 * nc; e += 64) fro2 = fma(Wg[e], Wg[e], fro2);
+  const double null2 = jacobi_null2(wave_sum(fro2));
   for (int sweep = 0; sweep < 40; ++sweep) {
     bool rotated = false;
     for (int p = 0; p < d1 - 1; ++p)
@@ -52,7 +102,7 @@ __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict
         a = wave_sum(a);
         b = wave_sum(b);
         c = wave_sum(c);
-        if (c == 0.0 || fabs(c) <= 1e-15 * sqrt(a) * sqrt(b)) continue;
+        if (c == 0.0 || a <= null2 || b <= null2 || fabs(c) <= ctol * sqrt(a) * sqrt(b)) continue;
         const double zeta = (b - a) / (2.0 * c);
         const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
@@ -181,74 +231,13 @@ void k_sect_gather(Ctx* ctx, const double* psi, int d1, int d2, i64 Dl, i64 ntas
   HIP_CHECK(hipGetLastError());
 }
 
-__global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __restrict__ X, const i64* __restrict__ xoff,
-                                                             const int* __restrict__ R, const int* __restrict__ Cc,
-                                                             const int* __restrict__ trans,
-                                                             const i64* __restrict__ uoff, const i64* __restrict__ soff,
-                                                             const i64* __restrict__ sgoff, double* __restrict__ sig,
-                                                             double* __restrict__ Uo, double* __restrict__ SVo,
-                                                             const int* __restrict__ glist, double* __restrict__ ws,
-                                                             const i64* __restrict__ wsoff) {
-  extern __shared__ double lds[];
-  // sector of this workgroup; its working arrays in LDS, or (ws) in a global
-  // workspace for sectors beyond the LDS of a workgroup
-  const int g = glist ? glist[blockIdx.x] : (int)blockIdx.x;
-  const bool tr = trans[g] != 0;
-  const int Rg = R[g], Cg = Cc[g];
-  const int n = tr ? Cg : Rg, len = tr ? Rg : Cg;  // rows of X, row length
-  double* Xs = ws ? ws + wsoff[blockIdx.x] : lds;
-  double* Q = Xs + (i64)n * len;  // Q[r + n*c]
-  double* nrm = Q + (i64)n * n;
-  int* ord = reinterpret_cast<int*>(nrm + n);
-  int* flag = ord + n;
+// The triplets of a converged sector (rows of X mutually orthogonal, Q the
+// accumulated rotations, nrm / ord scratch of n entries in LDS): sigma = row
+// norms in descending order (stable), U (Rg x n, column-major), sigma V^T
+// (n x Cg, row-major).  One workgroup of SS_THREADS.
+__device__ void sect_emit(const double* Xs, const double* Q, int n, int len, bool tr, int Rg, int Cg, double* nrm,
+                          int* ord, double* sig, double* Ug, double* Sg) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
-  const double* Xg = X + xoff[g];
-  for (i64 e = tid; e < (i64)n * len; e += SS_THREADS) Xs[e] = Xg[e];
-  for (i64 e = tid; e < (i64)n * n; e += SS_THREADS) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
-  __syncthreads();
-  const int n2 = n + (n & 1), m = n2 - 1;
-  for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
-    if (tid == 0) *flag = 0;
-    __syncthreads();
-    for (int rd = 0; rd < m; ++rd) {
-      for (int pi = wave; pi < n2 / 2; pi += nwave) {
-        const int p = pi == 0 ? 0 : 1 + (rd + pi) % m;
-        const int q = pi == 0 ? 1 + rd : 1 + (rd - pi + m) % m;
-        if (p >= n || q >= n) continue;
-        double* xp = Xs + (i64)p * len;
-        double* xq = Xs + (i64)q * len;
-        double a = 0, b = 0, c = 0;
-        for (int j = lane; j < len; j += 64) {
-          const double x = xp[j], y = xq[j];
-          a = fma(x, x, a);
-          b = fma(y, y, b);
-          c = fma(x, y, c);
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        c = wave_sum(c);
-        if (c == 0.0 || fabs(c) <= 1e-15 * sqrt(a) * sqrt(b)) continue;
-        const double zeta = (b - a) / (2.0 * c);
-        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-        for (int j = lane; j < len; j += 64) {
-          const double x = xp[j], y = xq[j];
-          xp[j] = cs * x - sn * y;
-          xq[j] = sn * x + cs * y;
-        }
-        for (int r = lane; r < n; r += 64) {
-          const double up = Q[r + (i64)n * p], uq = Q[r + (i64)n * q];
-          Q[r + (i64)n * p] = cs * up - sn * uq;
-          Q[r + (i64)n * q] = sn * up + cs * uq;
-        }
-        if (lane == 0) *flag = 1;
-      }
-      __syncthreads();
-    }
-    const int f = *flag;
-    __syncthreads();
-    if (!f) break;
-  }
   for (int i = wave; i < n; i += nwave) {
     const double* xi = Xs + (i64)i * len;
     double a = 0;
@@ -270,10 +259,7 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
     }
   }
   __syncthreads();
-  // k = n triplets: U (Rg x n, column-major), sigma * V^T (n x Cg, row-major), sigma
-  for (int c = tid; c < n; c += SS_THREADS) sig[sgoff[g] + c] = nrm[ord[c]];
-  double* Ug = Uo + uoff[g];
-  double* Sg = SVo + soff[g];
+  for (int c = tid; c < n; c += SS_THREADS) sig[c] = nrm[ord[c]];
   if (!tr) {  // X = M: U = Q, sigma V^T = rows of X
     for (i64 e = tid; e < (i64)Rg * n; e += SS_THREADS) {
       const int r = (int)(e % Rg), c = (int)(e / Rg);
@@ -296,19 +282,258 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
   }
 }
 
-size_t sect_svd_lds(int n, int len) {
+// |X|_F^2 of the n x len rows at Xg (one workgroup; every thread gets it)
+__device__ double sect_fro2(const double* Xg, i64 cnt) {
+  __shared__ double part[SS_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double f = 0;
+  for (i64 e = tid; e < cnt; e += SS_THREADS) f = fma(Xg[e], Xg[e], f);
+  f = wave_sum(f);
+  if (lane == 0) part[wave] = f;
+  __syncthreads();
+  double fro2 = 0;
+  for (int w = 0; w < SS_THREADS / 64; ++w) fro2 += part[w];
+  __syncthreads();
+  return fro2;
+}
+
+// Sectors whose rows and rotations fit the LDS of a workgroup: one workgroup
+// per sector (glist[blockIdx.x]) runs the whole one-sided Jacobi in LDS.
+__global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __restrict__ X, const i64* __restrict__ xoff,
+                                                             const int* __restrict__ R, const int* __restrict__ Cc,
+                                                             const int* __restrict__ trans,
+                                                             const i64* __restrict__ uoff, const i64* __restrict__ soff,
+                                                             const i64* __restrict__ sgoff, double* __restrict__ sig,
+                                                             double* __restrict__ Uo, double* __restrict__ SVo,
+                                                             const int* __restrict__ glist) {
+  extern __shared__ double lds[];
+  const int g = glist[blockIdx.x];
+  const bool tr = trans[g] != 0;
+  const int Rg = R[g], Cg = Cc[g];
+  const int n = tr ? Cg : Rg, len = tr ? Rg : Cg;  // rows of X, row length
+  const int n2 = n + (n & 1), m = n2 - 1, np = n2 / 2;  // round-robin pairs per round
+  double* Xs = lds;
+  double* Q = Xs + (i64)n * len;  // Q[r + n*c]
+  double* nrm = Q + (i64)n * n;
+  int* ord = reinterpret_cast<int*>(nrm + n);
+  int* flag = ord + n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
+  const double* Xg = X + xoff[g];
+  for (i64 e = tid; e < (i64)n * len; e += SS_THREADS) Xs[e] = Xg[e];
+  for (i64 e = tid; e < (i64)n * n; e += SS_THREADS) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
+  const double null2 = jacobi_null2(sect_fro2(Xg, (i64)n * len));
+  const double ctol = jacobi_tol(len);
+  for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int rd = 0; rd < m; ++rd) {
+      for (int pi = wave; pi < np; pi += nwave) {
+        const int p = pi == 0 ? 0 : 1 + (rd + pi) % m;
+        const int q = pi == 0 ? 1 + rd : 1 + (rd - pi + m) % m;
+        if (p >= n || q >= n) continue;
+        if (jacobi_pair(Xs + (i64)p * len, Xs + (i64)q * len, len, Q + (i64)n * p, Q + (i64)n * q, n, lane, ctol,
+                        null2) &&
+            lane == 0)
+          *flag = 1;
+      }
+      __syncthreads();
+    }
+    const int f = *flag;
+    __syncthreads();
+    if (!f) break;
+  }
+  sect_emit(Xs, Q, n, len, tr, Rg, Cg, nrm, ord, sig + sgoff[g], Uo + uoff[g], SVo + soff[g]);
+}
+
+size_t sect_svd_lds(int n, int len) {  // LDS bytes of a sector in sect_svd_kernel
   return ((size_t)n * len + (size_t)n * n + n) * sizeof(double) + (size_t)(n + 1) * sizeof(int);
 }
 
 void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                double* SVo, const int* glist, double* ws, const i64* wsoff) {
+                double* SVo, const int* glist) {
   if (!ng) return;
-  if (ws) lds = 0;
-  else HIP_CHECK(hipFuncSetAttribute((const void*)sect_svd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_svd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(sect_svd_kernel, dim3((unsigned)ng), dim3(SS_THREADS), lds, ctx->stream, X, xoff, R, Cc, trans,
-                     uoff, soff, sgoff, sig, Uo, SVo, glist, ws, wsoff);
+                     uoff, soff, sgoff, sig, Uo, SVo, glist);
   HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- block Jacobi
+// Sectors too large for one workgroup's LDS: block one-sided Jacobi over many
+// workgroups.  The n rows of a sector's X (in place in the gathered sector
+// matrix) are cut into B blocks of bs (8) rows (B even, the last ones short or
+// empty); a sweep is B - 1 rounds of the round-robin pairing of blocks, one
+// launch per round over every large sector, one workgroup per block pair: it
+// brings the pair's rows and the matching columns of Q into LDS, orthogonalises
+// the rows against each other (one round-robin sweep over them) and writes
+// both back.  The host reads the sectors' rotation flags after each sweep.
+
+struct BjSector {  // one large sector
+  int g;           // sector number
+  int n, len, bs, B;
+  i64 qoff;        // offset of its Q in the workspace
+};
+
+__global__ __launch_bounds__(SS_THREADS) void sect_bj_init_kernel(const BjSector* __restrict__ sec,
+                                                                 const double* __restrict__ X,
+                                                                 const i64* __restrict__ xoff, double* __restrict__ Qw,
+                                                                 double* __restrict__ null2, int* __restrict__ flag) {
+  const BjSector s = sec[blockIdx.x];
+  double* Q = Qw + s.qoff;
+  for (i64 e = threadIdx.x; e < (i64)s.n * s.n; e += SS_THREADS) Q[e] = (e % s.n == e / s.n) ? 1.0 : 0.0;
+  const double f = sect_fro2(X + xoff[s.g], (i64)s.n * s.len);
+  if (threadIdx.x == 0) {
+    null2[blockIdx.x] = jacobi_null2(f);
+    flag[blockIdx.x] = 0;
+  }
+}
+
+__global__ __launch_bounds__(SS_THREADS) void sect_bj_round_kernel(const BjSector* __restrict__ sec, int maxpairs,
+                                                                  int rd, double* __restrict__ X,
+                                                                  const i64* __restrict__ xoff, double* __restrict__ Qw,
+                                                                  const double* __restrict__ null2,
+                                                                  int* __restrict__ flag) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x / maxpairs, pi = blockIdx.x % maxpairs;
+  const BjSector s = sec[b];
+  const int mB = s.B - 1;
+  if (pi >= s.B / 2 || rd >= mB) return;
+  const int pb = pi == 0 ? 0 : 1 + (rd + pi) % mB;
+  const int qb = pi == 0 ? 1 + rd : 1 + (rd - pi + mB) % mB;
+  // the pair's rows: block pb then block qb (either may be short or empty)
+  const int p0 = pb * s.bs, p1 = min(s.n, p0 + s.bs), q0 = qb * s.bs, q1 = min(s.n, q0 + s.bs);
+  const int np_ = max(0, p1 - p0), nq_ = max(0, q1 - q0), nl = np_ + nq_;
+  if (nl < 2) return;
+  const int n = s.n, len = s.len;
+  double* Xg = X + xoff[s.g];
+  double* Q = Qw + s.qoff;
+  double* Xl = lds;                    // nl rows of len
+  double* Ql = Xl + (i64)nl * len;     // nl columns of n
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
+  auto grow = [&](int l) { return l < np_ ? p0 + l : q0 + (l - np_); };
+  for (i64 e = tid; e < (i64)nl * len; e += SS_THREADS) {
+    const int l = (int)(e / len), j = (int)(e % len);
+    Xl[e] = Xg[(i64)grow(l) * len + j];
+  }
+  for (i64 e = tid; e < (i64)nl * n; e += SS_THREADS) {
+    const int l = (int)(e / n), r = (int)(e % n);
+    Ql[e] = Q[r + (i64)n * grow(l)];
+  }
+  __syncthreads();
+  const double ctol = jacobi_tol(len), nu = null2[b];
+  const int n2 = nl + (nl & 1), m = n2 - 1, npair = n2 / 2;
+  bool any = false;
+  for (int r = 0; r < m; ++r) {
+    for (int qi = wave; qi < npair; qi += nwave) {
+      const int p = qi == 0 ? 0 : 1 + (r + qi) % m;
+      const int q = qi == 0 ? 1 + r : 1 + (r - qi + m) % m;
+      if (p >= nl || q >= nl) continue;
+      if (jacobi_pair(Xl + (i64)p * len, Xl + (i64)q * len, len, Ql + (i64)n * p, Ql + (i64)n * q, n, lane, ctol, nu))
+        any = true;
+    }
+    __syncthreads();
+  }
+  for (i64 e = tid; e < (i64)nl * len; e += SS_THREADS) {
+    const int l = (int)(e / len), j = (int)(e % len);
+    Xg[(i64)grow(l) * len + j] = Xl[e];
+  }
+  for (i64 e = tid; e < (i64)nl * n; e += SS_THREADS) {
+    const int l = (int)(e / n), r = (int)(e % n);
+    Q[r + (i64)n * grow(l)] = Ql[e];
+  }
+  if (any && lane == 0) flag[b] = 1;
+}
+
+__global__ __launch_bounds__(SS_THREADS) void sect_bj_finish_kernel(const BjSector* __restrict__ sec,
+                                                                   const double* __restrict__ X,
+                                                                   const i64* __restrict__ xoff,
+                                                                   const double* __restrict__ Qw,
+                                                                   const int* __restrict__ R, const int* __restrict__ Cc,
+                                                                   const int* __restrict__ trans,
+                                                                   const i64* __restrict__ uoff,
+                                                                   const i64* __restrict__ soff,
+                                                                   const i64* __restrict__ sgoff,
+                                                                   double* __restrict__ sig, double* __restrict__ Uo,
+                                                                   double* __restrict__ SVo) {
+  extern __shared__ double lds[];
+  const BjSector s = sec[blockIdx.x];
+  const int g = s.g;
+  double* nrm = lds;
+  int* ord = reinterpret_cast<int*>(nrm + s.n);
+  sect_emit(X + xoff[g], Qw + s.qoff, s.n, s.len, trans[g] != 0, R[g], Cc[g], nrm, ord, sig + sgoff[g],
+            Uo + uoff[g], SVo + soff[g]);
+}
+
+// rows per block of a large sector: the pair's rows and Q columns fit the LDS
+int sect_bj_rows(int n, int len) {
+  const size_t lim = 159 * 1024;
+  // eight rows per block: a block pair's round-robin sweep is 15 rounds of
+  // one pair per wave, the sequential depth of a launch
+  int bs = 8;
+  while (bs > 1 && (size_t)2 * bs * (len + n) * sizeof(double) > lim) --bs;
+  return bs;
+}
+
+// The block Jacobi of the large sectors gl (sector numbers), their (n, len):
+// factorises them in place in X (workspace for Q allocated here) and writes
+// their triplets like k_sect_svd.
+void k_sect_svd_blocks(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn,
+                       const std::vector<int>& ll, double* X, const i64* xoff, const int* R, const int* Cc,
+                       const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
+                       double* SVo) {
+  const int nb = (int)gl.size();
+  if (!nb) return;
+  std::vector<BjSector> hs(nb);
+  i64 qtot = 0;
+  int maxpairs = 1, maxrounds = 0, maxn = 1;
+  size_t lds = 0;
+  for (int b = 0; b < nb; ++b) {
+    BjSector& s = hs[b];
+    s.g = gl[b];
+    s.n = nn[b];
+    s.len = ll[b];
+    s.bs = sect_bj_rows(s.n, s.len);
+    s.B = (s.n + s.bs - 1) / s.bs;
+    s.B += s.B & 1;
+    if (s.B < 2) s.B = 2;
+    s.qoff = qtot;
+    qtot += (i64)s.n * s.n;
+    maxpairs = std::max(maxpairs, s.B / 2);
+    maxrounds = std::max(maxrounds, s.B - 1);
+    maxn = std::max(maxn, s.n);
+    lds = std::max(lds, (size_t)2 * s.bs * (s.len + s.n) * sizeof(double));
+  }
+  DBuf<BjSector> d_sec(ctx, nb);
+  HIP_CHECK(hipMemcpyAsync(d_sec.get(), hs.data(), nb * sizeof(BjSector), hipMemcpyHostToDevice, ctx->stream));
+  DBuf<double> Qw(ctx, (size_t)qtot), nul(ctx, nb);
+  DBuf<int> flag(ctx, nb);
+  hipLaunchKernelGGL(sect_bj_init_kernel, dim3(nb), dim3(SS_THREADS), 0, ctx->stream, d_sec.get(), X, xoff, Qw.get(),
+                     nul.get(), flag.get());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_round_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  std::vector<int> hf(nb);
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    for (int rd = 0; rd < maxrounds; ++rd) {
+      hipLaunchKernelGGL(sect_bj_round_kernel, dim3((unsigned)(nb * maxpairs)), dim3(SS_THREADS), lds, ctx->stream,
+                         d_sec.get(), maxpairs, rd, X, xoff, Qw.get(), nul.get(), flag.get());
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipMemcpyAsync(hf.data(), flag.get(), nb * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    bool any = false;
+    for (int f : hf) any = any || f;
+    if (!any) break;
+    k_zero(ctx, flag.get(), nb * sizeof(int));
+  }
+  const size_t flds = (size_t)maxn * (sizeof(double) + sizeof(int));
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_finish_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)flds));
+  hipLaunchKernelGGL(sect_bj_finish_kernel, dim3(nb), dim3(SS_THREADS), flds, ctx->stream, d_sec.get(), X, xoff,
+                     Qw.get(), R, Cc, trans, uoff, soff, sgoff, sig, Uo, SVo);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the workspace and the sector table go out of scope
 }
 
 // factors of the kept triplets.  A block b: (il, sector g) -> [s1, l, mm]

@@ -18,6 +18,9 @@ def problems():
                                       H.heisenberg_bond(20, qmax=3, width=4.0), seed=3)
     yield "hubbard", H.HeffProblem(H.hubbard_model(1.0, 4.0), H.hubbard_bond(40, n0=0, sigma=1.2),
                                    H.hubbard_bond(36, n0=2, sigma=1.2), seed=5)
+    # fused sectors up to ~120 x 120: the rows in LDS, the rotations in a global workspace
+    yield "heisenberg-mid", H.HeffProblem(H.heisenberg_model(1.0), H.heisenberg_bond(240, qmax=3, width=4.0),
+                                          H.heisenberg_bond(230, qmax=3, width=4.0), seed=9)
     # fused sectors up to ~200 x 200: beyond the LDS of a workgroup, factorised in a global workspace
     yield "heisenberg-big", H.HeffProblem(H.heisenberg_model(1.0), H.heisenberg_bond(400, qmax=3, width=4.0),
                                           H.heisenberg_bond(380, qmax=3, width=4.0), seed=7)
@@ -35,7 +38,7 @@ def dense_factors(prob, A, B, charges, kept):
     return Ad, Bd
 
 
-@pytest.mark.parametrize("name,prob", list(problems()), ids=["heisenberg", "hubbard", "heisenberg-big"])
+@pytest.mark.parametrize("name,prob", list(problems()), ids=["heisenberg", "hubbard", "heisenberg-mid", "heisenberg-big"])
 def test_oracle_sector_svd(name, prob):
     rowsec, colsec, charges = D.fused_sectors(prob.m, prob.Bl, prob.Br)
     psi, _ = expand(prob.psi_parts, [prob.Bl, prob.Br], [None, None, 0, 1])  # [s1, s2, l, r]
@@ -61,7 +64,7 @@ def test_oracle_sector_svd(name, prob):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,prob", list(problems()), ids=["heisenberg", "hubbard", "heisenberg-big"])
+@pytest.mark.parametrize("name,prob", list(problems()), ids=["heisenberg", "hubbard", "heisenberg-mid", "heisenberg-big"])
 @pytest.mark.parametrize("ortho", ["left", "right"])
 @pytest.mark.parametrize("trunc", [{}, {"maxdim": 17}, {"cutoff": 0.05, "mindim": 3}])
 def test_gpu_sector_svd(name, prob, ortho, trunc):
