@@ -213,9 +213,11 @@ def batch_section(sb, a):
     two-site H_eff*psi of a J1-J2 chain's centre bond for each of
     a.batch_problems seeds (J2 uniform in [0, 0.5], chi log-uniform in
     [64, 1024] in 17 Sz sectors).  One matvec of the batch = the four
-    contractions of every problem; run plan by plan (sequential) and as a
-    PlanBatch (one box product launch per stage over all problems), FP64 and
-    FP32 forms; the two give identical bits."""
+    contractions of every problem; run plan by plan (sequential), as a
+    PlanBatch (one box product launch per stage over all problems) with the
+    cost model's tiles, with tiles measured per stage over all problems
+    (autotune), and as one HIP graph launch per matvec; FP64 and FP32 forms;
+    all give identical bits."""
     from sparsebackends_amd import heff as H
     t0 = time.perf_counter()
     probs = H.j1j2_batch_problems(a.batch_problems)
@@ -251,22 +253,43 @@ def batch_section(sb, a):
                 p.execute()
         t_seq = timed(seq, a.batch_reps)
         ref = [p.Hpsi.parts()["data"].copy() for p, _, _ in probs]
-        sb.profile(True)
-        sb.profile_reset()
         t_bat = timed(batch.execute, a.batch_reps)
+        same = all(np.array_equal(p.Hpsi.parts()["data"], r) for (p, _, _), r in zip(probs, ref))
+        names = ["16x128", "128x16", "32x32", "64x32", "32x64", "64x64", "16x64", "64x16"]
+        tiles_vote = [names[t] if t >= 0 else None for t, _ in batch.stage_tiles()]
+        # the batch's tile shapes measured (every workgroup shape timed per stage over all problems)
+        t0 = time.perf_counter()
+        batch.autotune()
+        t_tune = time.perf_counter() - t0
+        sb.profile(True, products_only=True)
+        sb.profile_reset()
+        t_tuned = timed(batch.execute, a.batch_reps)
         prof = sb.profile_read()
         sb.profile(False)
-        same = all(np.array_equal(p.Hpsi.parts()["data"], r) for (p, _, _), r in zip(probs, ref))
         gemm_ms = prof.get("grouped_gemm", {}).get("total_ms", 0.0) / (a.batch_reps + 1)
+        same_tuned = all(np.array_equal(p.Hpsi.parts()["data"], r) for (p, _, _), r in zip(probs, ref))
+        # one HIP graph launch per matvec of the whole batch
+        graph = sb.PlanGraph(batch)
+        for _ in range(3):
+            graph.launch()
+        t_graph = timed(graph.launch, a.batch_reps)
+        same_graph = all(np.array_equal(p.Hpsi.parts()["data"], r) for (p, _, _), r in zip(probs, ref))
         r = dict(sequential_ms=t_seq * 1e3, batched_ms=t_bat * 1e3, sequential_tflops=alg / t_seq / 1e12,
                  batched_tflops=alg / t_bat / 1e12, speedup=t_seq / t_bat, bit_identical=bool(same),
                  fused_stages=batch.stats()["fused_stages"], plan_s=t_plan,
-                 kernels_ms_per_matvec={k: v["total_ms"] / (a.batch_reps + 1) for k, v in prof.items()})
+                 kernels_ms_per_matvec={k: v["total_ms"] / (a.batch_reps + 1) for k, v in prof.items()},
+                 tiles_cost_model=tiles_vote,
+                 tiles_autotuned=[names[t] if t >= 0 else None for t, _ in batch.stage_tiles()],
+                 autotune_s=t_tune, autotuned_ms=t_tuned * 1e3, autotuned_tflops=alg / t_tuned / 1e12,
+                 autotuned_bit_identical=bool(same_tuned),
+                 graph_ms=t_graph * 1e3, graph_tflops=alg / t_graph / 1e12, graph_bit_identical=bool(same_graph),
+                 graph_stats=graph.stats())
+        del graph
         if gemm_ms > 0:
             peak = PEAK[name]
             r["gemm_roofline"] = dict(bound="mfma", achieved=alg / (gemm_ms * 1e-3) / 1e12, peak=peak,
                                       unit="TFLOP/s", frac=alg / (gemm_ms * 1e-3) / 1e12 / peak,
-                                      note="the four batched grouped_gemm launches of one matvec")
+                                      note="the four batched grouped_gemm launches of one matvec, autotuned tiles")
         if name == "fp64" and a.batch_eig_tol > 0:
             r["eigsolve"] = batch_eig(sb, a, probs, batch)
         res[name] = r

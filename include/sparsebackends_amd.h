@@ -214,6 +214,10 @@ int sb_plan_free(sb_plan_t plan);
  * sb_ctx_graphs sets whether eigensolves replay their operator this way:
  * 1 always, 0 never, -1 (default) when krylovdim * maxiter >= 8. */
 int sb_graph_create(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_graph_t* out);
+/* the same for one execution of every chain of a batch (sb_batch_execute with
+ * all chains): its fused stages replayed as one graph launch; the batched
+ * eigensolve uses it while all its chains are running */
+int sb_batch_graph_create(sb_ctx_t ctx, sb_batch_t batch, sb_graph_t* out);
 int sb_graph_launch(sb_ctx_t ctx, sb_graph_t graph);
 int sb_graph_stats(sb_graph_t graph, int64_t* eager, int64_t* captures, int64_t* replays, int64_t* nodes);
 int sb_graph_free(sb_graph_t graph);
@@ -234,6 +238,12 @@ int sb_batch_create(sb_ctx_t ctx, const sb_plan_t* plans, int nchains, int nstag
 int sb_batch_execute(sb_ctx_t ctx, sb_batch_t batch, const int* active, int nactive);
 /* flops of one execution of every chain; number of fused stages */
 int sb_batch_stats(sb_batch_t batch, double* flops, int* fused_stages);
+/* New: time every workgroup tile shape for each fused stage's joint launch
+ * over all chains and keep the fastest (also done at creation when a stage's
+ * plans were made with SB_FLAG_AUTOTUNE); the stage's shape (-1: not fused)
+ * and its measured time */
+int sb_batch_autotune(sb_ctx_t ctx, sb_batch_t batch);
+int sb_batch_stage_tile(sb_batch_t batch, int stage, int* tile, double* tuned_ms);
 int sb_batch_free(sb_batch_t batch);
 
 /* ------------------------------------------------------------ Krylov vector operations

@@ -86,6 +86,7 @@ SIGNATURES = {
     "sb_plan_free": [P],
     "sb_graph_create": [P, PP, C.c_int, PP],
     "sb_graph_launch": [P, P],
+    "sb_batch_graph_create": [P, P, PP],
     "sb_graph_stats": [P, I64P, I64P, I64P, I64P],
     "sb_graph_free": [P],
     "sb_ctx_graphs": [P, C.c_int],
@@ -93,6 +94,8 @@ SIGNATURES = {
     "sb_batch_execute": [P, P, I32P, C.c_int],
     "sb_batch_stats": [P, F64P, I32P],
     "sb_batch_free": [P],
+    "sb_batch_autotune": [P, P],
+    "sb_batch_stage_tile": [P, C.c_int, I32P, F64P],
     "sb_bs_norm": [P, P, F64P],
     "sb_bs_copy": [P, P, PP],
     "sb_bs_similar": [P, P, PP],

@@ -585,6 +585,15 @@ int sb_graph_create(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_graph_t* 
     *out = reinterpret_cast<sb_graph_t>(new ChainGraph(ctx, v));
   })
 }
+int sb_batch_graph_create(sb_ctx_t c, sb_batch_t b, sb_graph_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!b || !out) fail(SB_E_ARGUMENT, "ArgumentError: null argument");
+    Batch* bt = reinterpret_cast<Batch*>(b);
+    if (bt->ctx != ctx) fail(SB_E_ARGUMENT, "ArgumentError: the batch belongs to another context");
+    *out = reinterpret_cast<sb_graph_t>(batch_graph(ctx, *bt).release());
+  })
+}
 int sb_graph_launch(sb_ctx_t c, sb_graph_t g) {
   SB_TRY({
     Ctx* ctx = C_(c);
@@ -644,6 +653,26 @@ int sb_batch_stats(sb_batch_t b, double* flops, int* fused_stages) {
       for (auto& s : bt->stages) f += s.fused;
       *fused_stages = f;
     }
+  })
+}
+int sb_batch_autotune(sb_ctx_t c, sb_batch_t b) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!b) fail(SB_E_HANDLE, "null batch");
+    Batch* bt = reinterpret_cast<Batch*>(b);
+    if (bt->ctx != ctx) fail(SB_E_ARGUMENT, "ArgumentError: the batch belongs to another context");
+    for (int s = 0; s < bt->nstages; ++s)
+      if (bt->stages[s].fused) bt->tune_stage(s);
+  })
+}
+int sb_batch_stage_tile(sb_batch_t b, int stage, int* tile, double* tuned_ms) {
+  SB_TRY({
+    if (!b) fail(SB_E_HANDLE, "null batch");
+    Batch* bt = reinterpret_cast<Batch*>(b);
+    if (stage < 0 || stage >= bt->nstages) fail(SB_E_BOUNDS, "BoundsError: stage out of range");
+    const BatchStage& st = bt->stages[stage];
+    if (tile) *tile = st.fused ? st.cfg : -1;
+    if (tuned_ms) *tuned_ms = st.tuned_ms;
   })
 }
 int sb_batch_free(sb_batch_t b) { SB_TRY(delete reinterpret_cast<Batch*>(b)) }

@@ -103,17 +103,26 @@ std::unique_ptr<Batch> make_batch(Ctx* ctx, const std::vector<Plan*>& plans, int
                           : std::max_element(vote.begin(), vote.end(), [](auto& x, auto& y) {
                               return x.second < y.second;
                             })->first;
-    const BoxTile t = box_tile(st.cfg);
+    bool tune = false;  // a plan asked for measured tile shapes: time the stage's joint launch
+    for (auto* p : st.plans) tune = tune || (p->box->flags & SB_FLAG_AUTOTUNE);
     std::vector<BoxJob> hj;
     i64 npos = 0, maxt = 0;
     for (auto* p : st.plans) {
       p->steps[0]();  // the operands' boxes exist (and are current) before their addresses are taken
       hj.push_back(job_of_plan(*p->box));
-      const BoxDesc& d = p->box->g.d;
-      const i64 M = prod_ext(d.m_ext, d.nm), N = prod_ext(d.n_ext, d.nn);
       st.ng.push_back(p->box->g.ng);
       npos += p->box->g.ng;
-      maxt += p->box->g.ng * ((M + t.BM - 1) / t.BM) * ((N + t.BN - 1) / t.BN);
+    }
+    for (int c = 0; c < box_tile_configs(); ++c) {  // work list room for every shape the stage may take
+      if (box_tile_is_wave(c)) continue;
+      const BoxTile t = box_tile(c);
+      i64 mt = 0;
+      for (auto* p : st.plans) {
+        const BoxDesc& d = p->box->g.d;
+        const i64 M = prod_ext(d.m_ext, d.nm), N = prod_ext(d.n_ext, d.nn);
+        mt += p->box->g.ng * ((M + t.BM - 1) / t.BM) * ((N + t.BN - 1) / t.BN);
+      }
+      maxt = std::max(maxt, mt);
     }
     SB_ASSERT(maxt < (1ll << 31), "batched box GEMM: work list too large");
     st.jobs.alloc(ctx, hj.size());
@@ -131,8 +140,43 @@ std::unique_ptr<Batch> make_batch(Ctx* ctx, const std::vector<Plan*>& plans, int
     st.scan_bytes = box_scan_tmp_bytes(ctx, std::max<i64>(1, npos));
     st.scan_tmp.alloc(ctx, st.scan_bytes);
     HIP_CHECK(hipStreamSynchronize(ctx->stream));  // hj is uploaded
+    if (tune) b->tune_stage(s);
   }
   return b;
+}
+
+// the fastest workgroup tile for stage s's joint launch over every chain:
+// one warm-up and three timed executions of the stage per shape
+void Batch::tune_stage(int s) {
+  BatchStage& st = stages[s];
+  std::vector<int> all(nchains);
+  for (int c = 0; c < nchains; ++c) all[c] = c;
+  hipEvent_t ea = ctx->event(), eb = ctx->event();
+  double best = 1e300;
+  int bestc = st.cfg;
+  for (int c = 0; c < box_tile_configs(); ++c) {
+    if (box_tile_is_wave(c)) continue;
+    st.cfg = c;
+    run_stage(s, all);
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      HIP_CHECK(hipEventRecord(ea, ctx->stream));
+      run_stage(s, all);
+      HIP_CHECK(hipEventRecord(eb, ctx->stream));
+      HIP_CHECK(hipEventSynchronize(eb));
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, ea, eb));
+      tmin = std::min(tmin, ms);
+    }
+    if (tmin < best) {
+      best = tmin;
+      bestc = c;
+    }
+  }
+  ctx->event_pool.push_back(ea);
+  ctx->event_pool.push_back(eb);
+  st.cfg = bestc;
+  st.tuned_ms = best;
 }
 
 void Batch::run_stage(int s, const std::vector<int>& active) {

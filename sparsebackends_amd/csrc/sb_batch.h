@@ -14,6 +14,7 @@ struct BatchStage {
   std::vector<Plan*> plans;  // plan of every chain at this stage
   bool fused = false;        // every plan is one box product of one precision form
   int cfg = 0;
+  double tuned_ms = 0;  // the stage's time with cfg when measured (SB_FLAG_AUTOTUNE)
   unsigned prec = 0;
   std::vector<i64> ng;       // groups per job
   DBuf<BoxJob> jobs;
@@ -35,6 +36,8 @@ struct Batch {
   void run(const std::vector<int>& active);
   // the chains of one stage: the fused launch, or the plans one by one
   void run_stage(int s, const std::vector<int>& active);
+  // time every workgroup tile for stage s over all chains and keep the fastest
+  void tune_stage(int s);
 };
 
 // Lends the stream of host to the guest contexts for the lifetime of the

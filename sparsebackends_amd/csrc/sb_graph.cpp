@@ -14,6 +14,8 @@
 // graph is updated in place where HIP allows it).
 #include "sb_graph.h"
 
+#include "sb_batch.h"
+
 #include <algorithm>
 #include <cstring>
 
@@ -47,6 +49,20 @@ ChainGraph::ChainGraph(Ctx* c, std::vector<const Plan*> ps) : ctx(c), plans(std:
     if (p->ctx != ctx) fail(SB_E_ARGUMENT, "ArgumentError: every plan of a chain must belong to its context");
     if (!graphable(p)) eager_only = true;  // only box-product chains are captured
   }
+}
+
+ChainGraph::ChainGraph(Ctx* c, std::vector<const Plan*> ps, std::function<void()> run,
+                       std::function<uintptr_t()> extra)
+    : ChainGraph(c, std::move(ps)) {
+  run_ = std::move(run);
+  extra_ = std::move(extra);
+}
+
+void ChainGraph::run_eager() {
+  if (run_)
+    run_();
+  else
+    for (auto* p : plans) p->run();
 }
 
 ChainGraph::~ChainGraph() {
@@ -84,6 +100,7 @@ std::vector<uintptr_t> ChainGraph::state() const {
     s.push_back(tr.get(b.C) ? 1 : 0);  // C box fill
     tr.set(b.C);                       // the product leaves C's boxes valid
   }
+  if (extra_) s.push_back(extra_());
   return s;
 }
 
@@ -109,7 +126,7 @@ struct Saved {
 
 void ChainGraph::launch() {
   if (eager_only || ctx->profile) {
-    for (auto* p : plans) p->run();
+    run_eager();
     ++eager;
     return;
   }
@@ -120,8 +137,11 @@ void ChainGraph::launch() {
     ++replays;
     return;
   }
-  if (eager == 0) {  // the first execution allocates what the chain needs: not captured
-    for (auto* p : plans) p->run();
+  // the first execution allocates what the chain needs, and one whose extra
+  // state is not the replayable one (e.g. a batch whose active lists must be
+  // uploaded again) synchronises: neither is captured
+  if (eager == 0 || (extra_ && st.back() == 0)) {
+    run_eager();
     ++eager;
     return;
   }
@@ -136,7 +156,7 @@ void ChainGraph::launch() {
   hipGraph_t g = nullptr;
   HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   try {
-    for (auto* p : plans) p->run();
+    run_eager();
   } catch (const Error& e) {
     (void)hipStreamEndCapture(ctx->stream, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -147,7 +167,7 @@ void ChainGraph::launch() {
     }
     if (e.code != SB_E_DEVICE) throw;
     eager_only = true;  // a call the capture does not take: run eagerly from now on
-    for (auto* p : plans) p->run();
+    run_eager();
     ++eager;
     return;
   }
@@ -173,7 +193,7 @@ void ChainGraph::launch() {
       s.t->box_version = s.box_version;
     }
     eager_only = true;
-    for (auto* p : plans) p->run();
+    run_eager();
     ++eager;
     return;
   }
@@ -200,6 +220,21 @@ void ChainGraph::launch() {
 bool ChainGraph::state_after_ok(const std::vector<uintptr_t>& before) const {
   const std::vector<uintptr_t> now = buffers();
   return before.size() >= now.size() && std::equal(now.begin(), now.end(), before.begin());
+}
+
+std::unique_ptr<ChainGraph> batch_graph(Ctx* ctx, Batch& batch) {
+  std::vector<const Plan*> plans;
+  for (auto& st : batch.stages)
+    for (auto* p : st.plans) plans.push_back(p);
+  Batch* b = &batch;
+  // a replay reads the active-chain lists the stages last uploaded: they must
+  // be the full ones the capture saw
+  auto full = [b]() -> uintptr_t {
+    for (auto& st : b->stages)
+      if (st.fused && (int)st.act_host.size() != b->nchains) return 0;
+    return 1;
+  };
+  return std::make_unique<ChainGraph>(ctx, plans, [b] { b->run({}); }, full);
 }
 
 bool eig_uses_graph(const Ctx* ctx, int kdim, int maxiter) {

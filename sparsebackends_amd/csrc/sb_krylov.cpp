@@ -629,8 +629,17 @@ std::vector<EigResult> eigsolve_lanczos_batch(Ctx* ctx, Batch& batch, const std:
   for (int c = 0; c < n; ++c) L.emplace_back(std::make_unique<Lanczos>(ctx, *xs[c], *ys[c], kdim, maxiter, tol));
   std::vector<int> active(n);
   for (int c = 0; c < n; ++c) active[c] = c;
+  // the batch's operator as one graph launch while every chain is still running
+  std::unique_ptr<ChainGraph> bg;
+  if (eig_uses_graph(ctx, kdim, maxiter)) bg = batch_graph(ctx, batch);
+  auto apply = [&](const std::vector<int>& act) {
+    if (bg && (int)act.size() == n)
+      bg->launch();
+    else
+      batch.run(act);
+  };
   for (int c : active) L[c]->begin();
-  batch.run(active);  // H v0 of every chain
+  apply(active);  // H v0 of every chain
   for (int c : active) L[c]->first();
   while (!active.empty()) {
     std::vector<int> need;
@@ -638,7 +647,7 @@ std::vector<EigResult> eigsolve_lanczos_batch(Ctx* ctx, Batch& batch, const std:
       if (L[c]->apply_needed()) need.push_back(c);
     if (!need.empty()) {
       for (int c : need) L[c]->load();
-      batch.run(need);
+      apply(need);
       bool any_track = false;
       for (int c : need) {
         L[c]->applied();

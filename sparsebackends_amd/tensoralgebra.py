@@ -141,6 +141,20 @@ class PlanBatch:
         check(load().sb_batch_stats(self._h, C.byref(f), C.byref(n)))
         return dict(flops=f.value, fused_stages=n.value)
 
+    def autotune(self):
+        """time every workgroup tile shape for each fused stage over all chains, keep the fastest"""
+        check(load().sb_batch_autotune(self.ctx, self._h))
+        return self
+
+    def stage_tiles(self):
+        """per stage: (tile configuration or -1 if not fused, measured ms or 0)"""
+        out = []
+        for st in range(self.nstages):
+            t, ms = C.c_int(), C.c_double()
+            check(load().sb_batch_stage_tile(self._h, st, C.byref(t), C.byref(ms)))
+            out.append((t.value, ms.value))
+        return out
+
     def __del__(self):
         try:
             if self._h:
@@ -157,14 +171,21 @@ class PlanGraph:
     Results equal executing the plans one by one."""
 
     def __init__(self, plans):
-        plans = list(plans)
-        if not plans:
-            raise ValueError("a plan chain needs at least one plan")
-        self._plans = plans  # the graph refers to the plans
-        self.ctx = plans[0].ctx
-        arr = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
+        """plans: a list of Plan objects, or a PlanBatch (one execution of all
+        its chains per launch)"""
         h = C.c_void_p()
-        check(load().sb_graph_create(self.ctx, C.cast(arr, C.POINTER(C.c_void_p)), len(plans), C.byref(h)))
+        if isinstance(plans, PlanBatch):
+            self._plans = plans  # the graph refers to the batch
+            self.ctx = plans.ctx
+            check(load().sb_batch_graph_create(self.ctx, plans._h, C.byref(h)))
+        else:
+            plans = list(plans)
+            if not plans:
+                raise ValueError("a plan chain needs at least one plan")
+            self._plans = plans  # the graph refers to the plans
+            self.ctx = plans[0].ctx
+            arr = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
+            check(load().sb_graph_create(self.ctx, C.cast(arr, C.POINTER(C.c_void_p)), len(plans), C.byref(h)))
         self._h = h
 
     def launch(self):
