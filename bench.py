@@ -24,7 +24,8 @@ batch (one box product launch per contraction stage over all problems);
 "serve" runs a stream of DMRG requests through the serving loop against the
 plain one-at-a-time queue (solves/s, p50/p99 latency, FP64 and FP32 products);
 "coo" times the element-sparse workload (PXP chain, COO x Dense at chi=128);
-"sweep" times one two-site DMRG sweep of the N=32 Heisenberg chain at chi=256.
+"sweep" times one two-site DMRG sweep of the N=32 Heisenberg chain at chi=256;
+"permute" times permutedims of the chi=2048 psi (GB/s against the HBM peak).
 
 value = algorithmic FLOPs per H_eff*psi (sum of 2*M*K*N over the matched
 block pairs with the actual sector extents) / time per step, in TFLOP/s.
@@ -387,10 +388,36 @@ def serve_section(sb, a):
     mreq = [dataclasses.replace(r, flags=sb.PREC_FP32) for r in base]
     srv = S.DMRGServer(slots=slots, co_batch=False)
     res = srv.run(mreq)
+    terr = {r.rid: r.truncerr for r in plain}
     out["served_mixed_fp32"] = dict(S.summarize(res),
-                                    max_abs_dE_vs_fp64=float(max(abs(r.energy - ref[r.rid]) for r in res)))
+                                    max_abs_dE_vs_fp64=float(max(abs(r.energy - ref[r.rid]) for r in res)),
+                                    max_truncerr=float(max(r.truncerr for r in res)),
+                                    max_truncerr_fp64=float(max(terr.values())),
+                                    max_abs_dtruncerr_vs_fp64=float(max(abs(r.truncerr - terr[r.rid]) for r in res)))
     out["peak_over_footprint_estimate"] = float(max(r.peak_bytes / r.footprint for r in plain))
     out["largest_peak_mb"] = float(max(r.peak_bytes for r in plain) / 2 ** 20)
+    return out
+
+
+def permute_section(sb, a, prob):
+    """permutedims (storage.jl:188-269) of the chi=2048 Hubbard psi [Ql, Qr |
+    s1, s2, l, r] (4430 blocks of 4 x 4 x 13 x 13): a tail permutation that
+    keeps the fastest axis (-> s1, l, s2, r: element kernel) and one that
+    moves it (-> l, r, s1, s2: LDS-tiled kernel).  Bytes = payload read and
+    written + keys; time per call includes the key remap and sort."""
+    nb, bs = int(prob.psi.nblocks), int(prob.psi.blksize)
+    moved = 2 * nb * bs * 8 + 2 * nb * 2 * 4
+    out = dict(blocks=nb, blksize=bs, bytes_per_call=moved)
+    for name, perm in (("fastest_kept", [1, 2, 3, 5, 4, 6]), ("fastest_moved", [1, 2, 5, 6, 3, 4])):
+        prob.psi.permutedims(perm)
+        sb.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            prob.psi.permutedims(perm)
+        sb.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = dict(perm=perm, ms=dt * 1e3, gbs=moved / dt / 1e9, hbm_frac=moved / dt / 1e9 / HBM_PEAK_GBS)
     return out
 
 
@@ -674,6 +701,7 @@ def main():
 
     if not a.no_coo and a.prec == "fp64":
         out["coo"] = coo_section(sb, a)
+        out["permute"] = permute_section(sb, a, prob)
 
     if a.serve_requests > 0 and a.prec == "fp64" and not a.no_eig:
         out["serve"] = serve_section(sb, a)

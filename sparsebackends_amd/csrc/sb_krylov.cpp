@@ -631,7 +631,10 @@ std::vector<EigResult> eigsolve_lanczos_batch(Ctx* ctx, Batch& batch, const std:
   for (int c = 0; c < n; ++c) active[c] = c;
   // the batch's operator as one graph launch while every chain is still running
   std::unique_ptr<ChainGraph> bg;
-  if (eig_uses_graph(ctx, kdim, maxiter)) bg = batch_graph(ctx, batch);
+  bool own = true;  // graphs only over this context's own plans (not over contexts on loan)
+  for (auto& st : batch.stages)
+    for (auto* p : st.plans) own = own && p->ctx == ctx;
+  if (own && eig_uses_graph(ctx, kdim, maxiter)) bg = batch_graph(ctx, batch);
   auto apply = [&](const std::vector<int>& act) {
     if (bg && (int)act.size() == n)
       bg->launch();

@@ -12,6 +12,10 @@
 
 namespace sb {
 
+struct FArr {
+  float v[MAXR];
+};
+
 struct PermArgs {
   int nd;
   int dims[MAXR];   // old (input) order
@@ -48,6 +52,47 @@ __global__ void permute_elem_kernel(const double* __restrict__ in, const i64* __
       src += c * pa.istr[pj.v[j]];
     }
     out[t] = in[src];
+  }
+}
+
+// q = n / d for 0 <= n < 2^24, exactly: the float quotient is within one of
+// the true one, then corrected with the remainder
+__device__ inline int div_small(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  int r = n - q * d;
+  if (r >= d) ++q;
+  else if (r < 0) --q;
+  return q;
+}
+
+// many blocks of a modest size: one workgroup per block (no division to find
+// the block), the in-block index decoded with 32-bit exact division; when the
+// fastest axis moves, a block of at most kStageMax values is read contiguously
+// into LDS first, so both the HBM read and the HBM write are coalesced
+constexpr i64 kStageMax = 8192;  // 64 KB of LDS
+__global__ __launch_bounds__(256) void permute_block_kernel(const double* __restrict__ in,
+                                                            const i64* __restrict__ src_off, i64 bs, PermArgs pa,
+                                                            IArr newdims, IArr pj, FArr inv, int stage,
+                                                            double* __restrict__ out) {
+  extern __shared__ double blk[];  // the whole input block when it fits (bs <= kStageMax)
+  const i64 o = blockIdx.x;
+  const i64 src0 = src_off ? src_off[o] : o * bs;
+  double* dst = out + o * bs;
+  const bool staged = stage != 0;
+  if (staged) {  // one contiguous read of the block, then the permuted order out of LDS
+    for (int t = threadIdx.x; t < (int)bs; t += 256) blk[t] = in[src0 + t];
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < (int)bs; t += 256) {
+    int r = t;
+    i64 src = 0;
+    for (int j = 0; j < pa.nd; ++j) {
+      const int d = (int)newdims.v[j];
+      const int q = div_small(r, d, inv.v[j]);
+      src += (i64)(r - q * d) * pa.istr[pj.v[j]];
+      r = q;
+    }
+    dst[t] = staged ? blk[src] : in[src0 + src];
   }
 }
 
@@ -142,7 +187,16 @@ void permute_blocks(Ctx* ctx, const double* in, const i64* src_off, i64 nblocks,
     s *= newdims.v[j];
   }
   int b = pj2[0];
-  if (b == 0 || pa.dims[0] < 8 || pa.dims[b] < 8) {
+  if ((b == 0 || pa.dims[0] < 8 || pa.dims[b] < 8) && bs < (1 << 24) && nblocks >= 256 && nblocks < (1ll << 31)) {
+    FArr inv{};
+    for (int j = 0; j < nz; ++j) inv.v[j] = 1.0f / (float)newdims.v[j];
+    // staged through LDS when the fastest axis moves (strided reads otherwise); a
+    // kept fastest axis already reads contiguously
+    const int stage = b != 0 && bs <= kStageMax;
+    const size_t lds = stage ? (size_t)bs * sizeof(double) : 0;
+    hipLaunchKernelGGL(permute_block_kernel, dim3((unsigned)nblocks), dim3(256), lds, ctx->stream, in, src_off, bs,
+                       pa, newdims, pj, inv, stage, out);
+  } else if (b == 0 || pa.dims[0] < 8 || pa.dims[b] < 8) {
     hipLaunchKernelGGL(permute_elem_kernel, dim3(grid_for(nblocks * bs)), dim3(256), 0, ctx->stream, in, src_off,
                        nblocks, bs, pa, newdims, pj, out);
   } else {

@@ -67,6 +67,7 @@ class SolveResult:
     arrival: float = 0.0
     admitted: float = 0.0
     finish: float = 0.0
+    truncerr: float = 0.0  # largest discarded weight of the last sweep
     footprint: int = 0     # admission estimate, bytes
     peak_bytes: int = 0    # device bytes the solve held after a sweep, largest over its sweeps
     maxlinkdim: int = 0
@@ -97,7 +98,8 @@ def neel(n):
 
 def solve_dmrg(req: SolveRequest, eigsolver=None, device=0):
     """Runs one request to completion on the calling thread's context.
-    Returns (energy, energies per sweep, converged, maxlinkdim, peak bytes):
+    Returns (energy, energies per sweep, converged, maxlinkdim, peak bytes,
+    largest discarded weight of the last sweep):
     peak bytes = the context's device bytes in use after a sweep beyond what
     it held before the solve, at most over the sweeps."""
     base = memory(device)[0]
@@ -113,7 +115,7 @@ def solve_dmrg(req: SolveRequest, eigsolver=None, device=0):
             break
         es.append(e)
     synchronize(device)
-    return es[-1], es, conv, dm.maxlinkdim(), peak
+    return es[-1], es, conv, dm.maxlinkdim(), peak, dm.truncerr
 
 
 class KrylovBatcher:
@@ -212,7 +214,7 @@ class DMRGServer:
     def __init__(self, slots=8, hbm_budget=200 << 30, co_batch=True, window=0.0005, device=0, solver=None,
                  batcher_args=None):
         """solver(request, eigsolver, device) -> (energy, energies, converged,
-        maxlinkdim, peak bytes) replaces solve_dmrg; batcher_args are extra
+        maxlinkdim, peak bytes, truncation error) replaces solve_dmrg; batcher_args are extra
         KrylovBatcher arguments (both for tests of the schedule)."""
         if slots < 1:
             raise ValueError("a server needs at least one slot")
@@ -250,9 +252,9 @@ class DMRGServer:
                     return
                 r, t_adm, fp = item
                 try:
-                    e, es, conv, mld, peak = self.solver(r, eig, self.device)
-                    res = SolveResult(r.rid, e, len(es), conv, es, r.arrival, t_adm, time.perf_counter() - t0, fp,
-                                      int(peak), mld)
+                    e, es, conv, mld, peak, terr = self.solver(r, eig, self.device)
+                    res = SolveResult(r.rid, e, len(es), conv, es, r.arrival, t_adm, time.perf_counter() - t0, terr,
+                                      fp, int(peak), mld)
                 except Exception as ex:  # reported by run()
                     res = None
                     with cv:
@@ -321,8 +323,8 @@ def run_plain(requests, device=0, solver=None, eigsolver=None):
         if r.arrival > now:
             time.sleep(r.arrival - now)
         t_adm = time.perf_counter() - t0
-        e, es, conv, mld, peak = solver(r, eigsolver if eigsolver is not None else eigsolve, device)
-        out[r.rid] = SolveResult(r.rid, e, len(es), conv, es, r.arrival, t_adm, time.perf_counter() - t0,
+        e, es, conv, mld, peak, terr = solver(r, eigsolver if eigsolver is not None else eigsolve, device)
+        out[r.rid] = SolveResult(r.rid, e, len(es), conv, es, r.arrival, t_adm, time.perf_counter() - t0, terr,
                                  footprint_bytes(r), int(peak), mld)
     return [out[r.rid] for r in sorted(reqs, key=lambda r: r.rid)]
 

@@ -52,7 +52,7 @@ def fake_solver(req, eig, device):
             vals, _, _ = eig(("plans", req.rid, sweep, b), FakeVec(req.rid), krylovdim=3, maxiter=1, tol=1e-14)
             e = vals[0]
         es.append(e)
-    return es[-1], es, True, req.chi_max, 0
+    return es[-1], es, True, req.chi_max, 0, 0.0
 
 
 def reqs(n=12, rate=400.0, seed=0):
