@@ -212,7 +212,10 @@ int sb_plan_free(sb_plan_t plan);
  * those of executing the plans one by one.  Chains with a plan that is not a
  * box product, and every launch while the context profiles, run eagerly.
  * sb_ctx_graphs sets whether eigensolves replay their operator this way:
- * 1 always, 0 never, -1 (default) when krylovdim * maxiter >= 8. */
+ * 1 always, 0 never, -1 (default) when krylovdim * maxiter >= 8.  A capture
+ * takes in whatever is launched on the context's stream meanwhile: a context
+ * used by several threads at once must have graphs off (mode 0) and must not
+ * launch chain graphs. */
 int sb_graph_create(sb_ctx_t ctx, const sb_plan_t* plans, int nplans, sb_graph_t* out);
 /* the same for one execution of every chain of a batch (sb_batch_execute with
  * all chains): its fused stages replayed as one graph launch; the batched

@@ -24,6 +24,7 @@ with.
 """
 from __future__ import annotations
 
+import gc
 import threading
 import time
 from collections import deque
@@ -34,7 +35,7 @@ import numpy as np
 from . import dmrg as D
 from . import heff as H
 from .krylov import eigsolve, eigsolve_multi
-from .storage import memory, synchronize
+from .storage import memory, synchronize, trim
 
 
 @dataclass
@@ -221,6 +222,9 @@ class DMRGServer:
         self.slots, self.budget, self.co_batch = int(slots), int(hbm_budget), bool(co_batch)
         self.window, self.device = window, device
         self.solver = solver if solver is not None else solve_dmrg
+        # what a slot thread does with its context when the run ends (the library's
+        # solves: return the device memory its context caches)
+        self.release = trim if solver is None else None
         self.batcher_args = dict(batcher_args or {})
         self.batch_sizes, self.batch_solve_s = [], 0.0
         self.max_reserved = 0
@@ -236,6 +240,7 @@ class DMRGServer:
         cv = threading.Condition()
         state = dict(busy=0, reserved=0, done=0, error=None)
         inbox = [deque() for _ in range(self.slots)]
+        used = [False] * self.slots
         idle = list(range(self.slots))
         results = {}
         batcher = KrylovBatcher(lambda: state["busy"], self.window, **self.batcher_args) if self.co_batch else None
@@ -249,7 +254,11 @@ class DMRGServer:
                         cv.wait()
                     item = inbox[i].popleft()
                 if item is None:
+                    if used[i] and self.release:  # its solves' tensors are gone: give the cached memory back
+                        gc.collect()
+                        self.release(self.device)
                     return
+                used[i] = True
                 r, t_adm, fp = item
                 try:
                     e, es, conv, mld, peak, terr = self.solver(r, eig, self.device)

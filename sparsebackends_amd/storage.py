@@ -39,13 +39,19 @@ def bind_context(handle, device=0):
     its own default).  Threads bound to one context share its stream and its
     device memory cache, so tensors and plans made by one of them can be used
     by the others (a tensor belongs to the context that made it); their
-    library calls are ordered by the stream."""
+    library calls are ordered by the stream.  Binding turns the context's
+    graph replays of eigensolve operators off (a capture on the shared stream
+    would take in the other threads' launches); PlanGraph must not be used on
+    a context that several threads use."""
     b = getattr(_bound, "ctx", None)
     if b is None:
         b = _bound.ctx = {}
     if handle is None:
         b.pop(device, None)
     else:
+        # a graph capture on the shared stream would take in the other threads'
+        # launches: the shared context's eigensolves run without graphs
+        check(load().sb_ctx_graphs(handle, 0))
         b[device] = handle
 
 
@@ -58,6 +64,12 @@ def memory(device=0):
     a, b = C.c_int64(), C.c_int64()
     check(load().sb_ctx_memory(context(device), C.byref(a), C.byref(b)))
     return int(a.value), int(b.value)
+
+
+def trim(device=0):
+    """return the device memory this thread's context keeps cached for reuse
+    (blocks of live tensors stay); synchronises the device"""
+    check(load().sb_ctx_trim(context(device)))
 
 
 def profile(enable=True, device=0, products_only=False):
