@@ -145,6 +145,17 @@ int bits_for(u64 maxval) {
   return b == 0 ? 1 : b;
 }
 
+// keys and values back from the sort's alternate buffers: one launch instead
+// of two blit copies
+template <class K>
+__global__ void copy_pairs_kernel(K* __restrict__ kd, const K* __restrict__ ks, i32* __restrict__ vd,
+                                  const i32* __restrict__ vs, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  kd[i] = ks[i];
+  vd[i] = vs[i];
+}
+
 void sort_pairs(Ctx* ctx, u64* keys, i32* vals, i64 n, int bits) {
   if (n <= 1) return;
   DBuf<u64> k2(ctx, n);
@@ -156,8 +167,9 @@ void sort_pairs(Ctx* ctx, u64* keys, i32* vals, i64 n, int bits) {
   DBuf<unsigned char> tmp(ctx, tb ? tb : 1);
   HIP_CHECK(rocprim::radix_sort_pairs(tmp.get(), tb, kb, vb, (size_t)n, 0, bits, ctx->stream));
   if (kb.current() != keys) {
-    HIP_CHECK(hipMemcpyAsync(keys, kb.current(), n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_CHECK(hipMemcpyAsync(vals, vb.current(), n * sizeof(i32), hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(copy_pairs_kernel<u64>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, keys,
+                       kb.current(), vals, vb.current(), n);
+    HIP_CHECK(hipGetLastError());
   }
 }
 
@@ -172,8 +184,9 @@ void sort_pairs32(Ctx* ctx, unsigned* keys, i32* vals, i64 n, int bits) {
   DBuf<unsigned char> tmp(ctx, tb ? tb : 1);
   HIP_CHECK(rocprim::radix_sort_pairs(tmp.get(), tb, kb, vb, (size_t)n, 0, bits, ctx->stream));
   if (kb.current() != keys) {
-    HIP_CHECK(hipMemcpyAsync(keys, kb.current(), n * sizeof(unsigned), hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_CHECK(hipMemcpyAsync(vals, vb.current(), n * sizeof(i32), hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(copy_pairs_kernel<unsigned>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       keys, kb.current(), vals, vb.current(), n);
+    HIP_CHECK(hipGetLastError());
   }
 }
 
