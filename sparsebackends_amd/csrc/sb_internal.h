@@ -49,13 +49,19 @@ class DeviceCache {
   void* get(size_t bytes);
   void put(void* p);
   void trim();
-  size_t bytes_in_use() const { return in_use_; }
-  size_t bytes_cached() const { return cached_; }
+  size_t bytes_in_use() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return in_use_;
+  }
+  size_t bytes_cached() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cached_;
+  }
 
  private:
   static size_t cls(size_t b);
   void trim_locked();
-  std::mutex mu_;
+  mutable std::mutex mu_;
   int device_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> live_;

@@ -234,6 +234,8 @@ class DMRGServer:
         """Serves the requests (arrival times relative to the call) and returns
         their results in request order."""
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        if len({r.rid for r in reqs}) != len(reqs):
+            raise ValueError("request ids must be distinct")
         for r in reqs:
             if footprint_bytes(r) > self.budget:
                 raise ValueError(f"request {r.rid} needs {footprint_bytes(r)} bytes, above the budget {self.budget}")

@@ -116,6 +116,12 @@ def test_fifo_admission_and_memory_budget():
     assert all(adm[i] <= adm[i + 1] for i in range(len(adm) - 1))
 
 
+def test_duplicate_ids_rejected():
+    rq = [S.SolveRequest(1, 16, 32, 1.0), S.SolveRequest(1, 16, 32, 0.5)]
+    with pytest.raises(ValueError):
+        S.DMRGServer(slots=2, solver=fake_solver).run(rq)
+
+
 def test_oversized_request_rejected():
     rq = [S.SolveRequest(0, 128, 1024, 1.0)]
     with pytest.raises(ValueError):
