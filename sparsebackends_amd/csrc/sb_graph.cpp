@@ -153,41 +153,11 @@ void ChainGraph::launch() {
       Tensor* m = const_cast<Tensor*>(t);
       saved.push_back({m, m->version, m->box_version});
     }
-  hipGraph_t g = nullptr;
-  HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  try {
-    run_eager();
-  } catch (const Error& e) {
-    (void)hipStreamEndCapture(ctx->stream, &g);
+  // the captured run executed nothing: undo its host effects, then run the
+  // chain eagerly from now on
+  auto abandon = [&](hipGraph_t g) {
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();
-    for (auto& s : saved) {
-      s.t->version = s.version;
-      s.t->box_version = s.box_version;
-    }
-    if (e.code != SB_E_DEVICE) throw;
-    eager_only = true;  // a call the capture does not take: run eagerly from now on
-    run_eager();
-    ++eager;
-    return;
-  }
-  HIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
-  bool ok = state_after_ok(st);
-  // only kernel nodes: a memset node is not replayed reliably on this device
-  size_t nn = 0;
-  if (ok) HIP_CHECK(hipGraphGetNodes(g, nullptr, &nn));
-  if (ok && nn) {
-    std::vector<hipGraphNode_t> nodes(nn);
-    HIP_CHECK(hipGraphGetNodes(g, nodes.data(), &nn));
-    for (auto n : nodes) {
-      hipGraphNodeType ty;
-      HIP_CHECK(hipGraphNodeGetType(n, &ty));
-      if (ty != hipGraphNodeTypeKernel && ty != hipGraphNodeTypeEmpty) ok = false;
-    }
-  }
-  nodes_ = nn;
-  if (!ok) {
-    (void)hipGraphDestroy(g);
     for (auto& s : saved) {
       s.t->version = s.version;
       s.t->box_version = s.box_version;
@@ -195,6 +165,48 @@ void ChainGraph::launch() {
     eager_only = true;
     run_eager();
     ++eager;
+  };
+  hipGraph_t g = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    run_eager();
+  } catch (const Error& e) {
+    (void)hipStreamEndCapture(ctx->stream, &g);
+    if (e.code != SB_E_DEVICE) {  // not a call the capture refused
+      if (g) (void)hipGraphDestroy(g);
+      for (auto& s : saved) {
+        s.t->version = s.version;
+        s.t->box_version = s.box_version;
+      }
+      throw;
+    }
+    abandon(g);
+    return;
+  } catch (...) {
+    (void)hipStreamEndCapture(ctx->stream, &g);  // never leave the stream capturing
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  if (hipStreamEndCapture(ctx->stream, &g) != hipSuccess || !g) {
+    abandon(g);
+    return;
+  }
+  bool ok = state_after_ok(st);
+  // only kernel nodes: a memset node is not replayed reliably on this device
+  size_t nn = 0;
+  if (ok) ok = hipGraphGetNodes(g, nullptr, &nn) == hipSuccess;
+  if (ok && nn) {
+    std::vector<hipGraphNode_t> nodes(nn);
+    ok = hipGraphGetNodes(g, nodes.data(), &nn) == hipSuccess;
+    for (size_t i = 0; ok && i < nn; ++i) {
+      hipGraphNodeType ty;
+      ok = hipGraphNodeGetType(nodes[i], &ty) == hipSuccess &&
+           (ty == hipGraphNodeTypeKernel || ty == hipGraphNodeTypeEmpty);
+    }
+  }
+  nodes_ = nn;
+  if (!ok) {
+    abandon(g);
     return;
   }
   bool updated = false;
@@ -208,7 +220,11 @@ void ChainGraph::launch() {
       exec = nullptr;
     }
   }
-  if (!exec) HIP_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  if (!exec && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    exec = nullptr;
+    abandon(g);
+    return;
+  }
   (void)hipGraphDestroy(g);
   cap_state = st;
   ++captures;
