@@ -14,6 +14,37 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// v + the v of the lane that the DPP pattern CTRL names (both 32-bit halves
+// moved by the data-parallel primitives, within a row of 16 lanes)
+template <int CTRL>
+__device__ inline double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return v + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// lane l's v, wave-uniform
+__device__ inline double read_lane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)lo);
+}
+
+// The wave's sum, wave-uniform, for the Jacobi pair step whose latency is its
+// reductions: within each row of 16 lanes by DPP (lane ^ 1, lane ^ 2, then the
+// half-row and row mirrors, each pairing the sums of two disjoint lane groups;
+// partner lanes add the same two values, so a row's lanes hold the same bits),
+// then the four row sums read from lanes 0, 16, 32, 48 and added pairwise.
+__device__ inline double wave_sum_dpp(double v) {
+  v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141>(v);  // row_half_mirror
+  v = dpp_add<0x140>(v);  // row_mirror
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
 // W rows of group g: row r at woff[g] + r * ncols_g, ncols_g = (gstart[g+1]-gstart[g]) * d2,
 // column j*d2 + t = A[bsrc[j] + r + d1*t] (block j of the group, column-major d1 x d2 block)
 // Convergence of one-sided Jacobi: a pair of rows counts as orthogonal when
@@ -42,9 +73,9 @@ __device__ inline bool jacobi_pair(double* xp, double* xq, int len, double* up, 
     b = fma(y, y, b);
     c = fma(x, y, c);
   }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  c = wave_sum(c);
+  a = wave_sum_dpp(a);
+  b = wave_sum_dpp(b);
+  c = wave_sum_dpp(c);
   if (c == 0.0 || a <= null2 || b <= null2 || fabs(c) <= ctol * sqrt(a) * sqrt(b)) return false;
   const double zeta = (b - a) / (2.0 * c);
   const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -260,24 +291,21 @@ __device__ void sect_emit(const double* Xs, const double* Q, int n, int len, boo
   }
   __syncthreads();
   for (int c = tid; c < n; c += SS_THREADS) sig[c] = nrm[ord[c]];
+  // one wave per output column of U / row of sigma V^T (no index division)
   if (!tr) {  // X = M: U = Q, sigma V^T = rows of X
-    for (i64 e = tid; e < (i64)Rg * n; e += SS_THREADS) {
-      const int r = (int)(e % Rg), c = (int)(e / Rg);
-      Ug[e] = Q[r + (i64)n * ord[c]];
-    }
-    for (i64 e = tid; e < (i64)n * Cg; e += SS_THREADS) {
-      const int c = (int)(e / Cg), j = (int)(e % Cg);
-      Sg[e] = Xs[(i64)ord[c] * len + j];
+    for (int c = wave; c < n; c += nwave) {
+      const double* qc = Q + (i64)n * ord[c];
+      const double* xc = Xs + (i64)ord[c] * len;
+      for (int r = lane; r < Rg; r += 64) Ug[r + (i64)Rg * c] = qc[r];
+      for (int j = lane; j < Cg; j += 64) Sg[(i64)c * Cg + j] = xc[j];
     }
   } else {  // X = M^T: U columns = rows of X / sigma, sigma V^T row c = sigma * column ord[c] of Q
-    for (i64 e = tid; e < (i64)Rg * n; e += SS_THREADS) {
-      const int r = (int)(e % Rg), c = (int)(e / Rg);
+    for (int c = wave; c < n; c += nwave) {
       const double s = nrm[ord[c]];
-      Ug[e] = s > 0 ? Xs[(i64)ord[c] * len + r] / s : 0.0;
-    }
-    for (i64 e = tid; e < (i64)n * Cg; e += SS_THREADS) {
-      const int c = (int)(e / Cg), j = (int)(e % Cg);
-      Sg[e] = nrm[ord[c]] * Q[j + (i64)n * ord[c]];
+      const double* qc = Q + (i64)n * ord[c];
+      const double* xc = Xs + (i64)ord[c] * len;
+      for (int r = lane; r < Rg; r += 64) Ug[r + (i64)Rg * c] = s > 0 ? xc[r] / s : 0.0;
+      for (int j = lane; j < Cg; j += 64) Sg[(i64)c * Cg + j] = s * qc[j];
     }
   }
 }
@@ -320,7 +348,8 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
   const double* Xg = X + xoff[g];
   for (i64 e = tid; e < (i64)n * len; e += SS_THREADS) Xs[e] = Xg[e];
-  for (i64 e = tid; e < (i64)n * n; e += SS_THREADS) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
+  for (int c = wave; c < n; c += nwave)
+    for (int r = lane; r < n; r += 64) Q[r + (i64)n * c] = r == c ? 1.0 : 0.0;
   const double null2 = jacobi_null2(sect_fro2(Xg, (i64)n * len));
   const double ctol = jacobi_tol(len);
   for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
@@ -365,9 +394,13 @@ void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, 
 // matrix) are cut into B blocks of bs (8) rows (B even, the last ones short or
 // empty); a sweep is B - 1 rounds of the round-robin pairing of blocks, one
 // launch per round over every large sector, one workgroup per block pair: it
-// brings the pair's rows and the matching columns of Q into LDS, orthogonalises
-// the rows against each other (one round-robin sweep over them) and writes
-// both back.  The host reads the sectors' rotation flags after each sweep.
+// brings the pair's rows and the matching columns of Q into LDS, rotates the
+// row pairs of the sweep that fall to it, and writes both back.  Every row pair
+// is visited once per sweep: in the first round (where every block meets one
+// partner) a workgroup sweeps all pairs of its rows, in the later rounds only
+// the pairs across its two blocks (a row of one block with a row of the
+// other), in max(rows of the two blocks) steps of one pair per wave.  The host
+// reads the sectors' rotation flags after each sweep.
 
 struct BjSector {  // one large sector
   int g;           // sector number
@@ -381,7 +414,8 @@ __global__ __launch_bounds__(SS_THREADS) void sect_bj_init_kernel(const BjSector
                                                                  double* __restrict__ null2, int* __restrict__ flag) {
   const BjSector s = sec[blockIdx.x];
   double* Q = Qw + s.qoff;
-  for (i64 e = threadIdx.x; e < (i64)s.n * s.n; e += SS_THREADS) Q[e] = (e % s.n == e / s.n) ? 1.0 : 0.0;
+  for (int c = threadIdx.x >> 6; c < s.n; c += SS_THREADS / 64)
+    for (int r = threadIdx.x & 63; r < s.n; r += 64) Q[r + (i64)s.n * c] = r == c ? 1.0 : 0.0;
   const double f = sect_fro2(X + xoff[s.g], (i64)s.n * s.len);
   if (threadIdx.x == 0) {
     null2[blockIdx.x] = jacobi_null2(f);
@@ -412,35 +446,48 @@ __global__ __launch_bounds__(SS_THREADS) void sect_bj_round_kernel(const BjSecto
   double* Ql = Xl + (i64)nl * len;     // nl columns of n
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
   auto grow = [&](int l) { return l < np_ ? p0 + l : q0 + (l - np_); };
-  for (i64 e = tid; e < (i64)nl * len; e += SS_THREADS) {
-    const int l = (int)(e / len), j = (int)(e % len);
-    Xl[e] = Xg[(i64)grow(l) * len + j];
-  }
-  for (i64 e = tid; e < (i64)nl * n; e += SS_THREADS) {
-    const int l = (int)(e / n), r = (int)(e % n);
-    Ql[e] = Q[r + (i64)n * grow(l)];
+  // one wave per row of the pair (no index division)
+  for (int l = wave; l < nl; l += nwave) {
+    const double* xg = Xg + (i64)grow(l) * len;
+    const double* qg = Q + (i64)n * grow(l);
+    for (int j = lane; j < len; j += 64) Xl[(i64)l * len + j] = xg[j];
+    for (int r = lane; r < n; r += 64) Ql[(i64)l * n + r] = qg[r];
   }
   __syncthreads();
   const double ctol = jacobi_tol(len), nu = null2[b];
-  const int n2 = nl + (nl & 1), m = n2 - 1, npair = n2 / 2;
   bool any = false;
-  for (int r = 0; r < m; ++r) {
-    for (int qi = wave; qi < npair; qi += nwave) {
-      const int p = qi == 0 ? 0 : 1 + (r + qi) % m;
-      const int q = qi == 0 ? 1 + r : 1 + (r - qi + m) % m;
-      if (p >= nl || q >= nl) continue;
-      if (jacobi_pair(Xl + (i64)p * len, Xl + (i64)q * len, len, Ql + (i64)n * p, Ql + (i64)n * q, n, lane, ctol, nu))
-        any = true;
+  if (rd == 0) {  // every pair of the 2 bs rows (round-robin)
+    const int n2 = nl + (nl & 1), m = n2 - 1, npair = n2 / 2;
+    for (int r = 0; r < m; ++r) {
+      for (int qi = wave; qi < npair; qi += nwave) {
+        const int p = qi == 0 ? 0 : 1 + (r + qi) % m;
+        const int q = qi == 0 ? 1 + r : 1 + (r - qi + m) % m;
+        if (p >= nl || q >= nl) continue;
+        if (jacobi_pair(Xl + (i64)p * len, Xl + (i64)q * len, len, Ql + (i64)n * p, Ql + (i64)n * q, n, lane, ctol,
+                        nu))
+          any = true;
+      }
+      __syncthreads();
     }
-    __syncthreads();
+  } else {  // the pairs across the blocks: step k pairs row i of pb with row (i + k) mod L of qb
+    const int L = max(np_, nq_);
+    for (int k = 0; k < L; ++k) {
+      for (int i = wave; i < L; i += nwave) {
+        const int j = (i + k) % L;
+        if (i >= np_ || j >= nq_) continue;
+        const int p = i, q = np_ + j;
+        if (jacobi_pair(Xl + (i64)p * len, Xl + (i64)q * len, len, Ql + (i64)n * p, Ql + (i64)n * q, n, lane, ctol,
+                        nu))
+          any = true;
+      }
+      __syncthreads();
+    }
   }
-  for (i64 e = tid; e < (i64)nl * len; e += SS_THREADS) {
-    const int l = (int)(e / len), j = (int)(e % len);
-    Xg[(i64)grow(l) * len + j] = Xl[e];
-  }
-  for (i64 e = tid; e < (i64)nl * n; e += SS_THREADS) {
-    const int l = (int)(e / n), r = (int)(e % n);
-    Q[r + (i64)n * grow(l)] = Ql[e];
+  for (int l = wave; l < nl; l += nwave) {
+    double* xg = Xg + (i64)grow(l) * len;
+    double* qg = Q + (i64)n * grow(l);
+    for (int j = lane; j < len; j += 64) xg[j] = Xl[(i64)l * len + j];
+    for (int r = lane; r < n; r += 64) qg[r] = Ql[(i64)l * n + r];
   }
   if (any && lane == 0) flag[b] = 1;
 }
