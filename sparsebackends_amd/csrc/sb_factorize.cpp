@@ -213,12 +213,13 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   // sectors with fewer than 32 rows to orthogonalise (lds_g) are factorised
   // in the LDS of one workgroup each; the others (big_g) by block Jacobi over
   // many workgroups, whose sequential depth per sweep is smaller from about
-  // 32 rows on (measured on the chi = 256 Heisenberg chain: 0.64 s per sweep
-  // against 0.74 s with the cut at 64 rows and 0.89 s with only the sectors
-  // beyond the LDS in blocks)
+  // 32 rows on (measured on the chi = 256 Heisenberg chain with the LDS
+  // factorisation running beside the blocks: 0.64 s per sweep against 0.74 s
+  // with the cut at 64 rows and 0.89 s with only the sectors beyond the LDS
+  // in blocks)
   constexpr size_t kLds = 159 * 1024;
   size_t lds = 0;
-  std::vector<int> lds_g, big_g, big_n, big_len;
+  std::vector<int> lds_g, lds_n, lds_len, big_g, big_n, big_len;
   for (i64 g = 0; g < ng; ++g) {
     const i64 m = act[g];
     if (R[m] > INT32_MAX || Cn[m] > INT32_MAX) fail(SB_E_LIMIT, "sector matrix dimension beyond int32");
@@ -229,6 +230,8 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
     if (n < 32 && sect_svd_lds(n, len) <= kLds) {
       lds = std::max(lds, sect_svd_lds(n, len));
       lds_g.push_back((int)g);
+      lds_n.push_back(n);
+      lds_len.push_back(len);
     } else {
       big_g.push_back((int)g);
       big_n.push_back(n);
@@ -238,6 +241,15 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
     uoff[g + 1] = uoff[g] + R[m] * n;
     soff[g + 1] = soff[g] + (i64)n * Cn[m];
     sgoff[g + 1] = sgoff[g] + n;
+  }
+  // while block Jacobi sweeps run, the small sectors ride along in their
+  // launches (one round per sweep each) instead of a launch of their own that
+  // the large sectors would wait for
+  if (!big_g.empty()) {
+    big_g.insert(big_g.end(), lds_g.begin(), lds_g.end());
+    big_n.insert(big_n.end(), lds_n.begin(), lds_n.end());
+    big_len.insert(big_len.end(), lds_len.begin(), lds_len.end());
+    lds_g.clear();
   }
   // gather tasks from psi's blocks
   std::vector<i32> hk, hi;

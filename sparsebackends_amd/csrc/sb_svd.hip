@@ -389,8 +389,8 @@ void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, 
 }
 
 // ---------------------------------------------------------------- block Jacobi
-// Sectors too large for one workgroup's LDS: block one-sided Jacobi over many
-// workgroups.  The n rows of a sector's X (in place in the gathered sector
+// Sectors with 32 rows or more to orthogonalise, or too large for one
+// workgroup's LDS: block one-sided Jacobi over many workgroups.  The n rows of a sector's X (in place in the gathered sector
 // matrix) are cut into B blocks of bs (8) rows (B even, the last ones short or
 // empty); a sweep is B - 1 rounds of the round-robin pairing of blocks, one
 // launch per round over every large sector, one workgroup per block pair: it
@@ -423,7 +423,12 @@ __global__ __launch_bounds__(SS_THREADS) void sect_bj_init_kernel(const BjSector
   }
 }
 
-__global__ __launch_bounds__(SS_THREADS) void sect_bj_round_kernel(const BjSector* __restrict__ sec, int maxpairs,
+// eight waves: one row pair per wave at every step of a pair of 8-row blocks
+// (measured: 16-row blocks on sixteen waves take 25 % longer per sweep of the
+// chi = 256 Heisenberg chain's sectors, though with half the launches)
+constexpr int BJ_THREADS = 512;
+
+__global__ __launch_bounds__(BJ_THREADS) void sect_bj_round_kernel(const BjSector* __restrict__ sec, int maxpairs,
                                                                   int rd, double* __restrict__ X,
                                                                   const i64* __restrict__ xoff, double* __restrict__ Qw,
                                                                   const double* __restrict__ null2,
@@ -444,7 +449,7 @@ __global__ __launch_bounds__(SS_THREADS) void sect_bj_round_kernel(const BjSecto
   double* Q = Qw + s.qoff;
   double* Xl = lds;                    // nl rows of len
   double* Ql = Xl + (i64)nl * len;     // nl columns of n
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = BJ_THREADS / 64;
   auto grow = [&](int l) { return l < np_ ? p0 + l : q0 + (l - np_); };
   // one wave per row of the pair (no index division)
   for (int l = wave; l < nl; l += nwave) {
@@ -515,9 +520,9 @@ __global__ __launch_bounds__(SS_THREADS) void sect_bj_finish_kernel(const BjSect
 // rows per block of a large sector: the pair's rows and Q columns fit the LDS
 int sect_bj_rows(int n, int len) {
   const size_t lim = 159 * 1024;
-  // eight rows per block: a block pair's round-robin sweep is 15 rounds of
-  // one pair per wave, the sequential depth of a launch
-  int bs = 8;
+  // eight rows per block, one row pair per wave of the workgroup at every
+  // step; fewer where the pair's rows and Q columns would not fit the LDS
+  int bs = BJ_THREADS / 64;
   while (bs > 1 && (size_t)2 * bs * (len + n) * sizeof(double) > lim) --bs;
   return bs;
 }
@@ -563,7 +568,7 @@ void k_sect_svd_blocks(Ctx* ctx, const std::vector<int>& gl, const std::vector<i
   std::vector<int> hf(nb);
   for (int sweep = 0; sweep < 40; ++sweep) {
     for (int rd = 0; rd < maxrounds; ++rd) {
-      hipLaunchKernelGGL(sect_bj_round_kernel, dim3((unsigned)(nb * maxpairs)), dim3(SS_THREADS), lds, ctx->stream,
+      hipLaunchKernelGGL(sect_bj_round_kernel, dim3((unsigned)(nb * maxpairs)), dim3(BJ_THREADS), lds, ctx->stream,
                          d_sec.get(), maxpairs, rd, X, xoff, Qw.get(), nul.get(), flag.get());
       HIP_CHECK(hipGetLastError());
     }
