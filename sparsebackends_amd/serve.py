@@ -232,8 +232,9 @@ class DMRGServer:
 
     def run(self, requests):
         """Serves the requests (arrival times relative to the call) and returns
-        their results in request order."""
-        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        their results in the order the requests were given."""
+        given = list(requests)
+        reqs = sorted(given, key=lambda r: (r.arrival, r.rid))
         if len({r.rid for r in reqs}) != len(reqs):
             raise ValueError("request ids must be distinct")
         for r in reqs:
@@ -318,15 +319,19 @@ class DMRGServer:
                 self.batch_solve_s = batcher.solve_s
         if state["error"] is not None:
             raise state["error"]
-        return [results[r.rid] for r in sorted(reqs, key=lambda r: r.rid)]
+        return [results[r.rid] for r in given]
 
 
 def run_plain(requests, device=0, solver=None, eigsolver=None):
     """The plain GPU baseline: one solve at a time on the calling thread, first
     come first served, each eigensolve alone.  Same results and timing fields.
-    solver / eigsolver replace solve_dmrg / krylov.eigsolve (tests)."""
+    solver / eigsolver replace solve_dmrg / krylov.eigsolve (tests).  Results in
+    the order the requests were given."""
     solver = solver if solver is not None else solve_dmrg
-    reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+    given = list(requests)
+    if len({r.rid for r in given}) != len(given):
+        raise ValueError("request ids must be distinct")
+    reqs = sorted(given, key=lambda r: (r.arrival, r.rid))
     out = {}
     t0 = time.perf_counter()
     for r in reqs:
@@ -337,7 +342,7 @@ def run_plain(requests, device=0, solver=None, eigsolver=None):
         e, es, conv, mld, peak, terr = solver(r, eigsolver if eigsolver is not None else eigsolve, device)
         out[r.rid] = SolveResult(r.rid, e, len(es), conv, es, r.arrival, t_adm, time.perf_counter() - t0, terr,
                                  footprint_bytes(r), int(peak), mld)
-    return [out[r.rid] for r in sorted(reqs, key=lambda r: r.rid)]
+    return [out[r.rid] for r in given]
 
 
 def poisson_requests(n, rate, seed=0, sites=(32, 64, 96, 128), chis=(128, 256, 512, 1024), delta=(0.5, 1.5),

@@ -122,6 +122,16 @@ def test_duplicate_ids_rejected():
         S.DMRGServer(slots=2, solver=fake_solver).run(rq)
 
 
+def test_results_in_given_order():
+    rq = [S.SolveRequest(i, 16, 32, 1.0) for i in (5, 2, 9, 0)]
+    solo = Recorder().solo_fn
+    for res in (S.DMRGServer(slots=2, co_batch=False, solver=fake_solver, batcher_args={"solo": solo}).run(rq),
+                S.run_plain(rq, solver=fake_solver, eigsolver=Recorder().solo_fn)):
+        assert [r.rid for r in res] == [5, 2, 9, 0]
+    with pytest.raises(ValueError):
+        S.run_plain(rq + rq[:1], solver=fake_solver, eigsolver=Recorder().solo_fn)
+
+
 def test_oversized_request_rejected():
     rq = [S.SolveRequest(0, 128, 1024, 1.0)]
     with pytest.raises(ValueError):
