@@ -21,7 +21,11 @@ _bound = threading.local()
 
 def context(device=0):
     """The context (HIP device + stream) this thread uses on a device: the one
-    bound by bind_context, else the thread's own default context."""
+    bound by bind_context, else the thread's own default context.  Default
+    contexts are kept by thread identifier, which Python reuses once a thread
+    has ended: a new thread may continue an ended thread's context (its stream,
+    its cached device memory and the tensors still alive from it), which is
+    correct because a context is only ever used by one live thread at a time."""
     b = getattr(_bound, "ctx", None)
     if b is not None and device in b:
         return b[device]
