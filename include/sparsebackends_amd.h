@@ -273,6 +273,16 @@ int sb_bs_add(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out);
 /* inner product over blocks of equal key (the scalar ITensors' inner gets
  * from the P_C = 0 contraction, tensor_wrappers.jl:646-652) */
 int sb_bs_dot(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, double* out);
+/* New, with no counterpart in the reference: sb_bs_norm for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Writes norm(A.data) into
+ * out_device[0] (device memory) on the context's stream, without allocating
+ * and without synchronising: the reduction's 1024 partial sums live in a
+ * fixed 8 KiB scratch array per GPU, so every payload length fits.  The
+ * scratch is shared by all calls on one GPU: calls that overlap on one GPU
+ * (several streams or threads) race on it.  Same kernels as sb_bs_norm, so
+ * the same value bit for bit. */
+int sb_bs_norm_noalloc(sb_ctx_t ctx, sb_tensor_t A, double* out_device);
 
 /* ------------------------------------------------------------ eigensolver
  * Replaces KrylovKit's eigsolve(A, x0, 1, :SR; ishermitian=true, krylovdim,

@@ -72,7 +72,14 @@ int sb_ctx_create(int device, sb_ctx_t* out) {
     int n = 0;
     HIP_CHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) fail(SB_E_ARGUMENT, "no such HIP device");
-    *out = reinterpret_cast<sb_ctx_t>(new Ctx(device));
+    Ctx* ctx = new Ctx(device);  // makes the device current
+    try {
+      noalloc_setup(device);
+    } catch (...) {
+      delete ctx;
+      throw;
+    }
+    *out = reinterpret_cast<sb_ctx_t>(ctx);
   })
 }
 int sb_ctx_destroy(sb_ctx_t c) { SB_TRY(delete C_(c)) }
@@ -692,6 +699,9 @@ int sb_bs_add(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out) {
 }
 int sb_bs_dot(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, double* out) {
   SB_TRY(*out = bs_dot(C_(c), *T_(A, BS), *T_(B, BS)))
+}
+int sb_bs_norm_noalloc(sb_ctx_t c, sb_tensor_t A, double* out_device) {
+  SB_TRY(bs_norm_noalloc(C_(c), *T_(A), out_device))
 }
 
 int sb_eigsolve_lanczos(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_tensor_t x, sb_tensor_t y, int krylovdim,

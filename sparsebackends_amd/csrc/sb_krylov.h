@@ -18,6 +18,12 @@ void bs_axpby(Ctx*, double alpha, const Tensor& A, double beta, Tensor& R);
 TP bs_add(Ctx*, const Tensor& A, const Tensor& B);
 double bs_dot(Ctx*, const Tensor& A, const Tensor& B);
 
+// allocation-free variants (sb_noalloc.hip): noalloc_setup finds the device's
+// scratch array once per device (that device current); bs_norm_noalloc writes
+// norm(A.data) into out[0], device memory, asynchronously
+void noalloc_setup(int device);
+void bs_norm_noalloc(Ctx*, const Tensor& A, double* out);
+
 // eigenvalues (ascending) and eigenvectors (columns) of a symmetric
 // column-major m x m matrix, cyclic Jacobi
 void sym_eig(int m, const std::vector<double>& A, std::vector<double>& w, std::vector<double>& U);

@@ -291,6 +291,12 @@ class NewBlockSparseSorted(_Tensor):
         check(load().sb_bs_norm(self.ctx, self._h, C.byref(v)))
         return v.value
 
+    def norm_noalloc(self, out_device_ptr):
+        """norm(data) written to out_device_ptr[0] (device memory) on the
+        context's stream, with no device allocation and no synchronisation
+        (sb_bs_norm_noalloc; for graph capture and allocation-free loops)."""
+        check(load().sb_bs_norm_noalloc(self.ctx, self._h, C.c_void_p(out_device_ptr)))
+
     def copy(self):
         """copy: keys, ids and data (tensor_wrappers.jl:817-823)."""
         out = C.c_void_p()
