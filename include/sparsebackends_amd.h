@@ -187,6 +187,25 @@ int sb_plan_create(sb_ctx_t ctx, sb_tensor_t C, const uint64_t* labelsC, int nC,
                    const uint64_t* labelsA, int nA, sb_tensor_t B, const uint64_t* labelsB, int nB, uint32_t flags,
                    sb_plan_t* out);
 int sb_plan_execute(sb_ctx_t ctx, sb_plan_t plan);
+/* New (no counterpart in the reference, which applies the two-site effective
+ * Hamiltonian of DMRG as four contract! calls psi*L*W1*W2*R, ITensorMPS's
+ * product(ProjMPO, psi) as test_product_dmrg_state.jl contracts it): the
+ * whole H_eff*psi as one plan whose execution is one fused kernel, with no
+ * intermediate tensor.  Layouts (those of the four-call chain):
+ *   psi [Ql, Qr | s1, s2, l, r]  L [Ql', Ql | wl, l', l]  R [Qr', Qr | wr, r', r]
+ *   W1 [ | wl, s1', s1, wm]  W2 [ | wm, s2', s2, wr]  (one dense block each)
+ * Hpsi, made by sb_bs_new with psi's dims and N2 = 4, gets the block list the
+ * chain's last contract! gives it (keys and ids bit for bit) and at every
+ * execution the value of the chain for psi's current values (summation order
+ * of its own: FP64 to rounding).  The sector extents are the environments'
+ * nonzero reach at creation: entries of psi beyond it are not read (the chain
+ * multiplies them by zero environment entries); L, R, W1 and W2 must not change
+ * afterwards (an execution after a library call changed them fails).  flags:
+ * SB_PREC_FP64 or SB_PREC_FP32 (products in FP32, storage FP64).  SB_E_LIMIT
+ * when an output block reaches beyond 4096 entries or the LDS tiles: then the
+ * four contract! plans do the work. */
+int sb_heff_plan_create(sb_ctx_t ctx, sb_tensor_t Hpsi, sb_tensor_t psi, sb_tensor_t L, sb_tensor_t W1,
+                        sb_tensor_t W2, sb_tensor_t R, uint32_t flags, sb_plan_t* out);
 /* flops: sum of 2*M*K*N over contributing block pairs (actual extents);
  * gemm_bytes / perm_bytes: algorithmic bytes of the block products / permutes */
 int sb_plan_stats(sb_plan_t plan, double* flops, double* gemm_bytes, double* perm_bytes, int64_t* npairs,

@@ -79,6 +79,7 @@ SIGNATURES = {
     "sb_contract_bs_bs_to_dense": [P, P, U64P, C.c_int, P, U64P, C.c_int, P, U64P, C.c_int, C.c_uint32],
     "sb_plan_create": [P, P, U64P, C.c_int, P, U64P, C.c_int, P, U64P, C.c_int, C.c_uint32, PP],
     "sb_plan_execute": [P, P],
+    "sb_heff_plan_create": [P, P, P, P, P, P, P, C.c_uint32, PP],
     "sb_plan_stats": [P, F64P, F64P, F64P, I64P, I64P],
     "sb_plan_box_config": [P, I32P, I32P, I32P, F64P],
     "sb_autotune_reset": [],

@@ -539,6 +539,16 @@ int sb_plan_create(sb_ctx_t c, sb_tensor_t C, const uint64_t* labelsC, int nC, s
     *out = reinterpret_cast<sb_plan_t>(pl.release());
   })
 }
+int sb_heff_plan_create(sb_ctx_t c, sb_tensor_t Hpsi, sb_tensor_t psi, sb_tensor_t L, sb_tensor_t W1, sb_tensor_t W2,
+                        sb_tensor_t R, uint32_t flags, sb_plan_t* out) {
+  SB_TRY({
+    Ctx* ctx = C_(c);
+    if (!out) fail(SB_E_ARGUMENT, "ArgumentError: null output handle");
+    auto pl = heff_plan(ctx, *T_(Hpsi), *T_(psi), *T_(L), *T_(W1), *T_(W2), *T_(R), flags);
+    pl->run_pending();
+    *out = reinterpret_cast<sb_plan_t>(pl.release());
+  })
+}
 int sb_plan_execute(sb_ctx_t c, sb_plan_t p) {
   SB_TRY({
     C_(c);

@@ -116,8 +116,8 @@ BoxTuneResult autotune_box(Ctx* ctx, unsigned flags, const BoxGemmArgs& g0, cons
       }
     }
   }
-  ctx->event_pool.push_back(ea);
-  ctx->event_pool.push_back(eb);
+  ctx->event_return(ea);
+  ctx->event_return(eb);
   r.ms = best;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -19,7 +19,7 @@ void timed(Ctx* ctx, const char* name, F&& f) {
   (void)hipEventRecord(a, ctx->stream);
   f();
   (void)hipEventRecord(b, ctx->stream);
-  ctx->prof_pending.push_back({name, a, b});
+  ctx->prof_push({name, a, b});
 }
 
 i64 prod_ext(const int* e, int n) {
@@ -173,8 +173,8 @@ void Batch::tune_stage(int s) {
       bestc = c;
     }
   }
-  ctx->event_pool.push_back(ea);
-  ctx->event_pool.push_back(eb);
+  ctx->event_return(ea);
+  ctx->event_return(eb);
   st.cfg = bestc;
   st.tuned_ms = best;
 }

@@ -102,7 +102,7 @@ struct Plan {
     (void)hipEventRecord(a, ctx->stream);
     s();
     (void)hipEventRecord(b, ctx->stream);
-    ctx->prof_pending.push_back({s.name, a, b});
+    ctx->prof_push({s.name, a, b});
   }
   // every step (a replay with refreshed operand values)
   void run() const {
@@ -125,5 +125,11 @@ TP al_permute(Ctx* ctx, const Tensor& A, const std::vector<i64>& perm);
 // plan execution.  The plan keeps pointers to A's and B's payloads.
 std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, const Tensor& A,
                                     const Labels& labelsA, const Tensor& B, const Labels& labelsB, unsigned flags);
+
+// The two-site H_eff*psi (the four-call chain psi*L*W1*W2*R of two-site DMRG)
+// as one plan with one fused kernel (sb_heff.hip); out gets the chain's block
+// list, and every execution psi's current H_eff image.
+std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const Tensor& L, const Tensor& W1,
+                                const Tensor& W2, const Tensor& R, unsigned flags);
 
 }  // namespace sb

@@ -109,12 +109,17 @@ struct Ctx {
   int profile = 0;
   int graphs = -1;  // eigensolves replay their operator as a HIP graph: 1 always, 0 never, -1 for long solves
   bool profiled(const char* name) const;
+  // event pool and profiling records: several host threads may share a
+  // context (bind_context), so every access holds prof_mu
+  std::mutex prof_mu;
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, ProfAgg> prof;
   explicit Ctx(int dev);
   ~Ctx();
   hipEvent_t event();
+  void event_return(hipEvent_t e);
+  void prof_push(const ProfRec& r);
   void prof_collect();  // synchronises, folds finished records into prof
 };
 

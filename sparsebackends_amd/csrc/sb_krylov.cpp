@@ -41,7 +41,7 @@ void step(Ctx* ctx, const char* name, F&& f) {
   (void)hipEventRecord(a, ctx->stream);
   f();
   (void)hipEventRecord(b, ctx->stream);
-  ctx->prof_pending.push_back({name, a, b});
+  ctx->prof_push({name, a, b});
 }
 
 void need_bs(const Tensor& t, const char* what) {

@@ -102,10 +102,13 @@ Ctx::~Ctx() {
 }
 
 bool Ctx::profiled(const char* name) const {
-  return profile == 1 || (profile == 2 && std::strcmp(name, "grouped_gemm") == 0);
+  // level 2: the block products only (the box products and the fused H_eff kernel)
+  return profile == 1 ||
+         (profile == 2 && (std::strcmp(name, "grouped_gemm") == 0 || std::strcmp(name, "heff_fused") == 0));
 }
 
 hipEvent_t Ctx::event() {
+  std::lock_guard<std::mutex> lk(prof_mu);
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
     event_pool.pop_back();
@@ -118,8 +121,19 @@ hipEvent_t Ctx::event() {
   return e;
 }
 
+void Ctx::event_return(hipEvent_t e) {
+  std::lock_guard<std::mutex> lk(prof_mu);
+  event_pool.push_back(e);
+}
+
+void Ctx::prof_push(const ProfRec& r) {
+  std::lock_guard<std::mutex> lk(prof_mu);
+  prof_pending.push_back(r);
+}
+
 void Ctx::prof_collect() {
   HIP_CHECK(hipStreamSynchronize(stream));
+  std::lock_guard<std::mutex> lk(prof_mu);
   for (auto& r : prof_pending) {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));

@@ -27,7 +27,7 @@ from __future__ import annotations
 import numpy as np
 
 from .storage import NewBlockSparseSorted
-from .tensoralgebra import Plan, contract_
+from .tensoralgebra import HeffPlan, Plan, contract_
 
 # ------------------------------------------------------------------ sectors
 
@@ -359,6 +359,13 @@ class HeffProblem:
         self.plans.append(Plan(self.T3, self.l_T3, self.T2, self.l_T2, self.W2, self.l_W2, flags))
         self.plans.append(Plan(self.Hpsi, self.l_out, self.T3, self.l_T3, self.R, self.l_R, flags))
         return self.plans
+
+    def make_fused_plan(self, flags=0):
+        """H_eff*psi as ONE plan with one fused kernel (HeffPlan), writing
+        self.Hpsi_fused: the chain's block list, the chain's values"""
+        self.Hpsi_fused = NewBlockSparseSorted.new(self.dims_out, 4, device=self.device)
+        self.fused = HeffPlan(self.Hpsi_fused, self.psi, self.L, self.W1, self.W2, self.R, flags)
+        return self.fused
 
     def execute(self):
         for p in self.plans:

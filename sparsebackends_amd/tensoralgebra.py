@@ -106,6 +106,24 @@ class Plan:
             pass
 
 
+class HeffPlan(Plan):
+    """The two-site H_eff*psi of DMRG (the four contract! calls psi*L*W1*W2*R)
+    as one plan executing one fused kernel (sb_heff_plan_create): Hpsi (made
+    with NewBlockSparseSorted.new(psi.dims, 4)) gets the chain's block list and,
+    at every execute(), the chain's value for psi's current values.  L, R, W1
+    and W2 must keep their values; flags: PREC_FP64 or PREC_FP32.  Raises
+    SBError of kind "LimitExceeded" (SB_E_LIMIT) when a sector block is too large for the
+    fused kernel; the four Plan objects then do the work."""
+
+    def __init__(self, Hpsi, psi, L, W1, W2, R, flags=0):
+        self._ops = (Hpsi, psi, L, W1, W2, R)  # Hpsi first: eigsolve's default output
+        h = C.c_void_p()
+        check(load().sb_heff_plan_create(Hpsi.ctx, Hpsi.handle, psi.handle, L.handle, W1.handle, W2.handle,
+                                         R.handle, flags, C.byref(h)))
+        self._h = h
+        self.ctx = Hpsi.ctx
+
+
 class PlanBatch:
     """Independent plan chains executed together: chains[c] is the list of
     plans of problem c (every chain the same length).  Stage s of all chains
