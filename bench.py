@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--tile", type=int, default=0,
                     help="force tile shape 1+c of the block products (0: chosen at plan time)")
+    ap.add_argument("--path", default="fused", choices=["fused", "chain"],
+                    help="the timed H_eff*psi: the fused plan (one kernel) or the four contract! plans")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the contract!-per-call (drop-in) eigensolve")
     ap.add_argument("--no-autotune", action="store_true",
                     help="tile shapes from the padded-work cost model instead of the measured autotuner")
     ap.add_argument("--no-cpu", action="store_true")
@@ -104,11 +107,65 @@ def rel_err(x, y):
     return float(np.linalg.norm(x - y) / (n if n else 1.0))
 
 
-def eig_section(sb, a, prob, plans):
-    """Ground state of the (symmetric) H_eff: FP64 Lanczos against the
-    mixed-precision solve (FP32 or bf16x3 block products, then an FP64
-    second stage from its Ritz vector), and the DMRG-step solve (krylovdim 3,
-    maxiter 1, the reference DMRG's call) as solves/s with its latency."""
+def dropin_eigsolve(sb, prob, x0d, krylovdim, tol, maxops=400):
+    """The FP64 drop-in's time-to-energy: Lanczos (Gram-Schmidt twice,
+    restart from the Ritz vector) whose every operator application is the four
+    contract! calls ITensors issues through the C ABI (block structure derived
+    on every call), the image kept in the vector's block space, and whose
+    vector operations are the Krylov hooks (norm, dot, scale, key-union add:
+    new tensors, as ITensors' out-of-place arithmetic makes them) -- what
+    KrylovKit's eigsolve on H_eff runs with the backend dropped in.
+    Returns (energy, seconds, applications)."""
+    from sparsebackends_amd.storage import NewBlockSparseSorted as BS
+    p = prob
+
+    def apply(x):
+        T1 = BS.new(p.dims_T1, 5)
+        sb.contract_(T1, p.l_T1, x, p.l_psi, p.L, p.l_L)
+        T2 = BS.new(p.dims_T2, 5)
+        sb.contract_(T2, p.l_T2, T1, p.l_T1, p.W1, p.l_W1)
+        T3 = BS.new(p.dims_T3, 5)
+        sb.contract_(T3, p.l_T3, T2, p.l_T2, p.W2, p.l_W2)
+        H = BS.new(p.dims_out, 4)
+        sb.contract_(H, p.l_out, T3, p.l_T3, p.R, p.l_R)
+        return H
+
+    sb.synchronize()
+    t0 = time.perf_counter()
+    x = (1.0 / x0d.norm()) * x0d
+    nops, theta, done = 0, None, False
+    while nops < maxops and not done:
+        V, al, be = [x], [], []
+        for j in range(krylovdim):
+            w = x0d.restrict(apply(V[j]))  # H v in the vector's block space, as the library's eigensolve keeps it
+            nops += 1
+            al.append(w.dot(V[j]))
+            for _ in range(2):
+                for v in V:
+                    w = w + (-w.dot(v)) * v
+            b = w.norm()
+            ev, U = np.linalg.eigh(np.diag(al) + np.diag(be, 1) + np.diag(be, -1))
+            theta = float(ev[0])
+            done = abs(b * U[-1, 0]) <= tol or b <= 1e-13
+            if done or j == krylovdim - 1:
+                break
+            be.append(b)
+            V.append((1.0 / b) * w)
+        y = float(U[0, 0]) * V[0]
+        for c, v in zip(U[1:, 0], V[1:len(U)]):
+            y = y + float(c) * v
+        x = (1.0 / y.norm()) * y
+    sb.synchronize()
+    return theta, time.perf_counter() - t0, nops
+
+
+def eig_section(sb, a, prob, plans, fused64, fused32):
+    """Ground state of the (symmetric) H_eff.  FP64 Lanczos with the fused
+    H_eff plan as the operator, and with the four contract! plans; the
+    mixed-precision solve (FP32 products, then an FP64 second stage from its
+    Ritz vector) on the fused plans; the drop-in's time-to-energy (contract!
+    per call); and the DMRG-step solve (krylovdim 3, maxiter 1, the reference
+    DMRG's call) as solves/s with its latency."""
     x0 = prob.psi_parts["data"]
     res = {}
 
@@ -121,26 +178,47 @@ def eig_section(sb, a, prob, plans):
 
     prob.psi.set_data_(x0)
     x0d = prob.psi.copy()  # the start vector, kept on the device
-    (vals, _, info), t64 = timed(lambda: sb.eigsolve(plans, prob.psi, krylovdim=a.krylovdim, maxiter=50, tol=a.eig_tol))
+    op64 = [fused64] if fused64 is not None else plans
+    (vals, _, info), t64 = timed(lambda: sb.eigsolve(op64, prob.psi, krylovdim=a.krylovdim, maxiter=50, tol=a.eig_tol))
     e64 = vals[0]
     res["tol"] = a.eig_tol
     res["krylovdim"] = a.krylovdim
+    res["operator"] = "fused H_eff plan" if fused64 is not None else "four contract! plans"
     res["fp64"] = dict(energy=e64, seconds=t64, numops=info.numops, converged=info.converged, normres=info.normres)
-    for name, fl in (("fp32", sb.PREC_FP32), ("bf16x3", sb.PREC_BF16X3)):
-        pm = copy.copy(prob).upload().outputs()
-        plans_lo = pm.make_plans(fl)
+    if fused64 is not None:
+        prob.psi.axpby_(1.0, x0d, 0.0)
+        (vc, _, ic), tc = timed(lambda: sb.eigsolve(plans, prob.psi, krylovdim=a.krylovdim, maxiter=50, tol=a.eig_tol))
+        res["fp64_chain_plans"] = dict(energy=vc[0], seconds=tc, numops=ic.numops, abs_dE_vs_fused=abs(vc[0] - e64))
+    lo_forms = [("fp32", sb.PREC_FP32, fused32)] if fused32 is not None else []
+    lo_forms += [("fp32_chain", sb.PREC_FP32, None), ("bf16x3_chain", sb.PREC_BF16X3, None)]
+    for name, fl, fplan in lo_forms:
+        if fplan is not None:
+            pm, plans_lo = fplan
+        else:
+            pm = copy.copy(prob).upload().outputs()
+            plans_lo = pm.make_plans(fl)
         pm.psi.axpby_(1.0, x0d, 0.0)
+        prob.psi.axpby_(1.0, x0d, 0.0)
         tol_lo = a.eig_tol_lo * max(1.0, abs(e64))
-        (r, t) = timed(lambda: sb.eigsolve_refined(plans_lo, plans, pm.psi, prob.psi, krylovdim=a.krylovdim,
+        (r, t) = timed(lambda: sb.eigsolve_refined(plans_lo, op64, pm.psi, prob.psi, krylovdim=a.krylovdim,
                                                    maxiter=50, tol_lo=tol_lo, tol=a.eig_tol))
         vals, _, ilo, ihi = r
         res[name] = dict(energy=vals[0], abs_dE_vs_fp64=abs(vals[0] - e64), seconds=t, time_to_energy_vs_fp64=t64 / t,
                          numops_low=ilo.numops, numops_fp64=ihi.numops, converged=ihi.converged, tol_low=tol_lo)
         del pm, plans_lo
+    if not a.no_dropin:
+        ed, td, nd = dropin_eigsolve(sb, prob, x0d, a.krylovdim, a.eig_tol)
+        res["fp64_dropin"] = dict(energy=ed, seconds=td, numops=nd, abs_dE_vs_fp64=abs(ed - e64),
+                                  note="contract! per call (block structure derived on every call) and the Krylov "
+                                       "hooks, as KrylovKit drives the dropped-in backend")
+        for name in ("fp32",):
+            if name in res:
+                res[name]["time_to_energy_vs_dropin"] = td / res[name]["seconds"]
+        res["fp64"]["time_to_energy_vs_dropin"] = td / t64
     lat = []
     for _ in range(a.solves):
         prob.psi.axpby_(1.0, x0d, 0.0)
-        _, t = timed(lambda: sb.eigsolve(plans, prob.psi, krylovdim=3, maxiter=1, tol=1e-14))
+        _, t = timed(lambda: sb.eigsolve(op64, prob.psi, krylovdim=3, maxiter=1, tol=1e-14))
         lat.append(t)
     lat = np.array(lat)
     res["dmrg_step_solve"] = dict(krylovdim=3, maxiter=1, solves=a.solves, solves_per_s=len(lat) / lat.sum(),
@@ -509,77 +587,96 @@ def main():
     sb.synchronize()
     t_plan = time.perf_counter() - t0
     stats = [p.stats() for p in plans]
-    opb = [p.operand_bytes() for p in plans]
     tiles = [p.box_config() for p in plans]
-    for _ in range(a.warmup):
-        prob.execute()
-    sb.synchronize()
-    sb.profile(True, products_only=True)  # events around the block products only
-    sb.profile_reset()
-    sb.synchronize()
+    fprec = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32}.get(a.prec)
+    fused, t_fplan = None, None
+    if a.path == "fused" and fprec is not None:
+        t0 = time.perf_counter()
+        try:
+            fused = prob.make_fused_plan(fprec)
+        except sb.SBError as e:  # sectors too large for the fused kernel: the chain does the work
+            print(f"fused plan unavailable ({e}); timing the contract! plans", file=sys.stderr)
+        sb.synchronize()
+        t_fplan = time.perf_counter() - t0
+    step = fused.execute if fused is not None else prob.execute
+    out_t = prob.Hpsi_fused if fused is not None else prob.Hpsi
+
+    def timed_steps(fn, n, products_only=True):
+        for _ in range(a.warmup):
+            fn()
+        sb.synchronize()
+        sb.profile(True, products_only=products_only)  # events around the block products only
+        sb.profile_reset()
+        sb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        sb.synchronize()
+        d = time.perf_counter() - t0
+        pr = sb.profile_read()
+        sb.profile(False)
+        return d, pr
+
+    dt, prof = timed_steps(step, a.steps)
+    # the plain eager steps without any timing events
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        prob.execute()
-    sb.synchronize()
-    dt = time.perf_counter() - t0
-    prof = sb.profile_read()
-    sb.profile(False)
-    # the same steps as one HIP graph launch each (the chain a Krylov iteration replays)
-    graph = sb.PlanGraph(plans)
-    for _ in range(3):
-        graph.launch()
-    sb.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        graph.launch()
-    sb.synchronize()
-    dtg = time.perf_counter() - t0
-    gstat = graph.stats()
-    # and the plain eager steps without any timing events
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        prob.execute()
+        step()
     sb.synchronize()
     dte = time.perf_counter() - t0
     ms_step = 1e3 * dt / a.steps
     value = alg / (dt / a.steps) / 1e12
-
-    # dominant kernel: the kernel name with the largest total time
-    dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
-    dom_name, dom_rec = dom
-    launches_per_step = dom_rec["count"] / a.steps
-    avg_ms = dom_rec["total_ms"] / dom_rec["count"]
     alg_bytes = prob.algorithmic_bytes()["total"]
-    if dom_name == "grouped_gemm":
-        # per launch: the four block-product launches of a step, averaged; the
-        # algorithmic bytes are the operands read once and the result written
-        # once with the actual sector extents
+
+    def chain_roofline(prof_c, dt_c):
+        """the box products of the four contract! plans: HBM roofline of grouped_gemm"""
+        rec = prof_c["grouped_gemm"]
+        lps = rec["count"] / a.steps
+        avg = rec["total_ms"] / rec["count"]
         ai = alg / alg_bytes
-        if ai * HBM_PEAK_GBS / 1e3 < PEAK[a.prec]:
-            roof = dict(bound="hbm", achieved=alg_bytes / launches_per_step / (avg_ms * 1e-3) / 1e9,
-                        peak=HBM_PEAK_GBS, unit="GB/s")
-        else:
-            roof = dict(bound="mfma", achieved=alg / launches_per_step / (avg_ms * 1e-3) / 1e12, peak=PEAK[a.prec],
-                        unit="TFLOP/s")
-        roof["note"] = (f"average over the {int(launches_per_step)} grouped_gemm launches of a step; "
-                        f"{alg_bytes / 1e9:.3f} GB algorithmic per step, arithmetic intensity {ai:.2f} FLOP/B")
+        r = dict(bound="hbm", achieved=alg_bytes / lps / (avg * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                 kernel="grouped_gemm", avg_launch_ms=avg,
+                 note=f"average over the {int(lps)} grouped_gemm launches of a step; {alg_bytes / 1e9:.3f} GB "
+                      f"algorithmic per step, arithmetic intensity {ai:.2f} FLOP/B")
+        r["frac"] = r["achieved"] / r["peak"]
+        return r
+
+    if fused is not None:
+        # dominant kernel: the fused H_eff kernel; one application is one plan
+        # step (its size-class launches run concurrently, forked from and
+        # joined to the context's stream), timed by the events around it
+        fst = fused.stats()
+        mf = fused.mfma_flops()
+        rec = prof["heff_fused"]
+        avg_ms = rec["total_ms"] / rec["count"]
+        roof = dict(bound="mfma", achieved=mf / (avg_ms * 1e-3) / 1e12, peak=PEAK[a.prec], unit="TFLOP/s",
+                    kernel="heff_fused (heff_kernel launches of one application)", avg_launch_ms=avg_ms,
+                    note=(f"matrix-core FLOPs the launches execute per application ({mf / 1e9:.3f} GFLOP, tiles "
+                          f"padded to 16) over the application's time; useful FLOPs of the fused algorithm "
+                          f"{fst['flops'] / 1e9:.3f} GFLOP ({fst['flops'] / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s); "
+                          f"the four-call chain's algorithmic {alg / 1e9:.3f} GFLOP "
+                          f"({alg / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s); {fst['gemm_bytes'] / 1e6:.1f} MB "
+                          f"algorithmic traffic ({fst['gemm_bytes'] / (avg_ms * 1e-3) / 1e9:.0f} GB/s)"),
+                    useful_gflop=fst["flops"] / 1e9, mfma_gflop=mf / 1e9, algorithmic_bytes=fst["gemm_bytes"])
+        roof["frac"] = roof["achieved"] / roof["peak"]
     else:
-        perm_bytes = sum(s["perm_bytes"] for s in stats) / max(1.0, launches_per_step)
-        roof = dict(bound="hbm", achieved=perm_bytes / (avg_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                    note=f"average over the {int(launches_per_step)} {dom_name} launches of a step")
-    roof["kernel"] = dom_name
-    roof["frac"] = roof["achieved"] / roof["peak"]
+        roof = chain_roofline(prof, dt)
     roof["traffic"] = None
-    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-    # workload (kept beside the profile summary); used when it covers this run
-    tf = os.path.join(ROOT, "profiles", "box_gemm_heff_hubbard_chi2048.traffic.json")
-    if dom_name == "grouped_gemm" and a.prec == "fp64" and a.tile == 0 and os.path.exists(tf):
+    # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload, kept beside the profile summary; taken only when it was
+    # measured on this very library build (its sha256), else null
+    tf = os.path.join(ROOT, "profiles", "heff_fused_hubbard_chi2048.traffic.json")
+    if fused is not None and a.prec == "fp64" and os.path.exists(tf):
+        import hashlib
         with open(tf) as fh:
             t = json.load(fh)
-        if t.get("workload") == prob_desc:
-            roof["traffic"] = (t["fetch_bytes_per_step"] + t["write_bytes_per_step"]) / t["launches_per_step"]
-            roof["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/)"
-    roof["avg_launch_ms"] = avg_ms
+        with open(sb.LIB_PATH, "rb") as fh:
+            lib_sha = hashlib.sha256(fh.read()).hexdigest()
+        if t.get("workload") == prob_desc and t.get("library_sha256") == lib_sha:
+            roof["traffic"] = t["fetch_bytes"] + t["write_bytes"]
+            roof["traffic_unit"] = "bytes per application (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/)"
+        else:
+            roof["traffic_note"] = "profiles/ traffic was measured on another build: not used"
 
     out = {
         "metric": "H_eff·ψ TFLOP/s FP64 vs mixed at |ΔE|≤1e-10; DMRG solves/s + p99 latency",
@@ -597,23 +694,46 @@ def main():
                 "symmetrised with the MPO link involution (H_eff symmetric), exact model MPO",
         "config": {
             "workload": prob_desc,
+            "path": "fused H_eff plan (one kernel per application)" if fused is not None else "four contract! plans",
             "format": "sector + in-sector axes, dense tail padded to the largest sector",
             "psi_blocks": int(len(prob.psi_parts["ids"])),
             "algorithmic_gflop_per_step": alg / 1e9,
+            "algorithmic_gflop_note": "sum of 2*M*K*N over the matched block pairs of the four contract! calls, "
+                                      "actual sector extents (the work unit of value)",
             "algorithmic_gb_per_step": alg_bytes / 1e9,
             "padded_gflop_per_step": pad / 1e9,
             "block_pairs_per_step": int(sum(s["npairs"] for s in stats)),
-            "plan_s": t_plan,
+            "plan_s": t_fplan if fused is not None else t_plan,
+            "chain_plan_s": t_plan,
             "generate_s": t_gen,
         },
         "kernels": {k: dict(v, per_step_ms=v["total_ms"] / a.steps) for k, v in prof.items()},
         "roofline": roof,
-        "graph": dict(ms_per_step=dtg * 1e3 / a.steps, tflops=alg / (dtg / a.steps) / 1e12,
-                      eager_ms_per_step=dte * 1e3 / a.steps, eager_tflops=alg / (dte / a.steps) / 1e12,
-                      note="the timed region above carries HIP events around the block products; these are the "
-                           "same steps as one graph launch each and as plain eager launches, without events",
-                      **gstat),
+        "eager_no_events": dict(ms_per_step=dte * 1e3 / a.steps, tflops=alg / (dte / a.steps) / 1e12,
+                                note="the same steps without the HIP events of the timed region"),
     }
+    dtc = dt
+    if fused is not None:
+        out["config"]["fused_useful_gflop_per_step"] = fst["flops"] / 1e9
+        out["config"]["fused_sector_pairs"] = fst["npairs"]
+        # the four contract! plans on the same problem: the round-1 path, and its HBM roofline
+        dtc, profc = timed_steps(prob.execute, a.steps)
+        graph = sb.PlanGraph(plans)
+        for _ in range(3):
+            graph.launch()
+        sb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            graph.launch()
+        sb.synchronize()
+        dtg = time.perf_counter() - t0
+        out["chain"] = dict(ms_per_step=dtc * 1e3 / a.steps, tflops=alg / (dtc / a.steps) / 1e12,
+                            graph_ms_per_step=dtg * 1e3 / a.steps, roofline=chain_roofline(profc, dtc),
+                            kernels={k: dict(v, per_step_ms=v["total_ms"] / a.steps) for k, v in profc.items()},
+                            fused_speedup=dtc / dt,
+                            fused_vs_chain_rel_err=rel_err(prob.Hpsi_fused.parts()["data"], prob.Hpsi.parts()["data"]),
+                            same_block_list=bool(np.array_equal(prob.Hpsi_fused.parts()["ids"],
+                                                                prob.Hpsi.parts()["ids"])))
 
     names = ["16x128", "128x16", "32x32", "64x32", "32x64", "64x64", "16x64", "64x16",
              "wave16x16", "wave32x16", "wave16x32", "wave32x32", "wave64x16", "wave16x64"]
@@ -633,27 +753,41 @@ def main():
         dcm = (time.perf_counter() - t0) / a.steps
         out["tiles"]["cost_model_ms_per_step"] = dcm * 1e3
         out["tiles"]["cost_model_tiles"] = [names[p.box_config()["tile"]] for p in pc.plans]
-        out["tiles"]["autotuned_speedup"] = dcm / (dt / a.steps)
+        out["tiles"]["autotuned_speedup"] = dcm / ((dtc if fused is not None else dt) / a.steps)
         out["tiles"]["same_result"] = bool(np.array_equal(pc.Hpsi.parts()["data"], prob.Hpsi.parts()["data"]))
         del pc
 
+    fused32 = None
     if not a.no_mixed and a.prec == "fp64":
         # the precision forms on the same problem, with their error vs FP64
-        ref = prob.Hpsi.parts()["data"]
+        ref = out_t.parts()["data"]
+        ref_chain = prob.Hpsi.parts()["data"]
         mixed = {}
-        for name, fl in (("fp32", sb.PREC_FP32), ("bf16x3", sb.PREC_BF16X3)):
+        forms = [("fp32_chain", sb.PREC_FP32, False), ("bf16x3_chain", sb.PREC_BF16X3, False)]
+        if fused is not None:
+            forms.insert(0, ("fp32", sb.PREC_FP32, True))
+        for name, fl, fz in forms:
             pm = copy.copy(prob).upload().outputs()  # same host data, fresh device tensors
-            pm.make_plans(fl)
-            pm.execute()
+            if fz:
+                plan_lo = pm.make_fused_plan(fl)
+                run, res_t, r = plan_lo.execute, pm.Hpsi_fused, ref
+            else:
+                pm.make_plans(fl)
+                run, res_t, r = pm.execute, pm.Hpsi, ref_chain
+            run()
             sb.synchronize()
             t0 = time.perf_counter()
             for _ in range(max(3, a.steps // 2)):
-                pm.execute()
+                run()
             sb.synchronize()
             d2 = (time.perf_counter() - t0) / max(3, a.steps // 2)
             mixed[name] = dict(tflops=alg / d2 / 1e12, ms_per_step=d2 * 1e3,
-                               rel_err_vs_fp64=rel_err(pm.Hpsi.parts()["data"], ref))
-            del pm
+                               rel_err_vs_fp64=rel_err(res_t.parts()["data"], r),
+                               path="fused H_eff plan" if fz else "four contract! plans")
+            if fz:
+                fused32 = (pm, [plan_lo])
+            else:
+                del pm
         out["mixed"] = mixed
 
     if not a.no_cpu:
@@ -666,12 +800,28 @@ def main():
         t0 = time.perf_counter()
         hp = h.run()
         tc = time.perf_counter() - t0
+        blas = O.use_blas(True, threads=1)
+        blas_cfg = None
+        try:
+            import ctypes
+            f = ctypes.CDLL(blas).openblas_get_config
+            f.restype = ctypes.c_char_p
+            blas_cfg = f().decode()
+        except Exception:
+            pass
+        cpu_model = None
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu_model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+        except Exception:
+            pass
         cpu = dict(value=alg / tc / 1e12, unit="TFLOP/s", cores=1, kind="port",
                    sample="one full two-site H_eff*psi of the same workload (4 contract! calls), "
-                          "oracle C++ restatement with OpenBLAS 0.3.29 dgemm on 1 thread",
-                   seconds=tc)
+                          "oracle C++ restatement with the loaded BLAS's dgemm on 1 thread",
+                   seconds=tc, blas_path=blas, blas_config=blas_cfg, cpu_model=cpu_model,
+                   host_cores=os.cpu_count())
         q = hp.parts()
-        p = prob.Hpsi.parts()
+        p = out_t.parts()
         if np.array_equal(p["keys"], q["keys"]) and np.array_equal(p["ids"], q["ids"]):
             cpu["gpu_vs_cpu_rel_err"] = rel_err(p["data"], q["data"])
         else:
@@ -689,7 +839,7 @@ def main():
 
     if not a.no_eig and a.prec == "fp64":
         # last: the solves overwrite psi and Hpsi
-        out["eigsolve"] = eig_section(sb, a, prob, plans)
+        out["eigsolve"] = eig_section(sb, a, prob, plans, fused, fused32)
     if a.dmrg_solves > 0 and a.prec == "fp64" and not a.no_eig:
         out["dmrg_solve"] = dmrg_section(sb, a)
 

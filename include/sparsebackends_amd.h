@@ -220,6 +220,10 @@ int sb_plan_operand_bytes(sb_plan_t plan, double* a_bytes, double* b_bytes, doub
 int sb_plan_box_config(sb_plan_t plan, int* tile, int* wg_per_cu, int* candidates, double* best_ms);
 /* forget the autotuner's cached choices (all devices) */
 int sb_autotune_reset(void);
+/* New (no counterpart in the reference): the matrix-core FLOPs one
+ * execution of the plan issues, tile padding included (the fused H_eff plan
+ * counts them; 0 for plans that do not) */
+int sb_plan_mfma_flops(sb_plan_t plan, double* out);
 int sb_plan_free(sb_plan_t plan);
 
 /* New (no counterpart in the reference, which re-derives every contraction on
@@ -292,6 +296,11 @@ int sb_bs_add(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out);
 /* inner product over blocks of equal key (the scalar ITensors' inner gets
  * from the P_C = 0 contraction, tensor_wrappers.jl:646-652) */
 int sb_bs_dot(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, double* out);
+/* New (a helper for Krylov loops outside the library, no counterpart in the
+ * reference): a new tensor with A's keys and ids holding B's block of each of
+ * those keys (zero where B has none), the projection of an operator's image
+ * onto the vector's block space that sb_eigsolve_lanczos applies */
+int sb_bs_restrict(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out);
 /* New, with no counterpart in the reference: sb_bs_norm for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Writes norm(A.data) into

@@ -84,6 +84,7 @@ SIGNATURES = {
     "sb_plan_box_config": [P, I32P, I32P, I32P, F64P],
     "sb_autotune_reset": [],
     "sb_plan_operand_bytes": [P, F64P, F64P, F64P],
+    "sb_plan_mfma_flops": [P, F64P],
     "sb_plan_free": [P],
     "sb_graph_create": [P, PP, C.c_int, PP],
     "sb_graph_launch": [P, P],
@@ -106,6 +107,7 @@ SIGNATURES = {
     "sb_bs_add": [P, P, P, PP],
     "sb_bs_dot": [P, P, P, F64P],
     "sb_bs_norm_noalloc": [P, P, P],
+    "sb_bs_restrict": [P, P, P, PP],
     "sb_bs_svd": [P, P, C.c_int, C.c_int64, C.c_int64, C.c_double, PP, PP, F64P, C.c_int64, I64P, F64P],
     "sb_bs_svd_sectors": [P, P, C.c_int64, I64P, I64P, I64P, I64P, C.c_int, C.c_int64, C.c_int64, C.c_double, PP,
                           PP, I64P, F64P, C.c_int64, I64P, F64P],

@@ -590,6 +590,12 @@ int sb_plan_operand_bytes(sb_plan_t p, double* a_bytes, double* b_bytes, double*
     if (c_bytes) *c_bytes = pl->c_bytes;
   })
 }
+int sb_plan_mfma_flops(sb_plan_t p, double* out) {
+  SB_TRY({
+    if (!p || !out) fail(SB_E_HANDLE, "null plan or output");
+    *out = reinterpret_cast<Plan*>(p)->mfma_flops;
+  })
+}
 int sb_plan_free(sb_plan_t p) { SB_TRY(delete reinterpret_cast<Plan*>(p)) }
 
 int sb_graph_create(sb_ctx_t c, const sb_plan_t* plans, int nplans, sb_graph_t* out) {
@@ -709,6 +715,9 @@ int sb_bs_add(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out) {
 }
 int sb_bs_dot(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, double* out) {
   SB_TRY(*out = bs_dot(C_(c), *T_(A, BS), *T_(B, BS)))
+}
+int sb_bs_restrict(sb_ctx_t c, sb_tensor_t A, sb_tensor_t B, sb_tensor_t* out) {
+  SB_TRY(*out = H_(bs_restrict(C_(c), *T_(A, BS), *T_(B, BS))))
 }
 int sb_bs_norm_noalloc(sb_ctx_t c, sb_tensor_t A, double* out_device) {
   SB_TRY(bs_norm_noalloc(C_(c), *T_(A), out_device))

@@ -69,6 +69,7 @@ struct Plan {
   double gemm_bytes = 0;   // algorithmic bytes of the block products (operands read once + C written)
   double perm_bytes = 0;   // bytes moved by operand permutes (read + write)
   i64 npairs = 0, nblocks_out = 0;
+  double mfma_flops = 0;   // matrix-core FLOPs one execution issues, padding included (0: not counted)
   std::string family;
 
   template <class T>

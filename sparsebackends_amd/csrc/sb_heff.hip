@@ -659,6 +659,28 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
     cls_span.push_back({n, mx, tpw});
   }
   if (items.size() >= (size_t)1 << 31) fail(SB_E_LIMIT, "H_eff plan: too many workgroups");
+  // matrix-core work one execution issues (16 x 16 x 4 products, 2048 FLOPs each)
+  double mfma = 0;
+  {
+    size_t it = 0;
+    for (auto& cs : cls_span)
+      for (size_t j = 0; j < cs.count; ++j, ++it) {
+        const int o = items[it].x, t0 = items[it].y;
+        const int dlp = o_dlp[(size_t)o], drp = o_drp[(size_t)o];
+        const int RT = up16((int)(P * dlp)) / 16, ML = up16(dlp) / 16, CT = up16(drp) / 16;
+        const int ntile = std::min(RT * CT - t0, kWaves * cs.tpw);
+        for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q) {
+          const HeffPair& hp = pairs[(size_t)q];
+          const int K1 = up4(hp.dl), N1 = up16((int)(P * hp.dr)), K2 = up4(hp.dr);
+          for (int wl = 0; wl < w; ++wl) {
+            if (!(hp.wlm >> wl & 1)) continue;
+            mfma += (double)ML * (N1 / 16) * (K1 / 4);
+            for (int wr = 0; wr < w; ++wr)
+              if ((hp.wrm >> wr & 1) && lidx[(size_t)(wl * w + wr)] >= 0) mfma += (double)ntile * (K2 / 4 + 4);
+          }
+        }
+      }
+  }
 
   // ---- install Hpsi's block list (overwrite semantics), payload zero
   out.n = nout;
@@ -799,6 +821,7 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   pl->b_bytes = benv;
   pl->c_bytes = bytes_out;
   pl->npairs = (i64)pairs.size();
+  pl->mfma_flops = 2048.0 * mfma;
   pl->nblocks_out = nout;
   pl->out = &out;
   pl->out_box = true;

@@ -244,6 +244,21 @@ double bs_dot(Ctx* ctx, const Tensor& A, const Tensor& B) {
   return v;
 }
 
+// B's values in A's block structure: a tensor with A's keys and ids whose
+// block of key k is B's block of key k (zero where B has none).  The block
+// space projection the eigensolver applies to H x (blocks of y absent from x
+// are left out), for Krylov loops run outside the library.
+TP bs_restrict(Ctx* ctx, const Tensor& A, const Tensor& B) {
+  need_bs(A, "restrict");
+  need_bs(B, "restrict");
+  if (A.dims != B.dims || A.N2 != B.N2) fail(SB_E_ARGUMENT, "ArgumentError: restrict of tensors with different dims");
+  DBuf<i64> off;
+  match_blocks(ctx, A, B, off);
+  TP C = copy_structure(ctx, A, false);
+  k_gather_blocks_by_key(ctx, A.n, A.blksize, A.ids.get(), off.get(), B.dptr(), C->data.get());
+  return C;
+}
+
 // ---------------------------------------------------------------- symmetric eigenproblem (host)
 
 // Cyclic Jacobi (Rutishauser's rotation formulas) on an m x m symmetric

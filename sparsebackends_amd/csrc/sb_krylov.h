@@ -17,6 +17,7 @@ void bs_fill(Ctx*, Tensor& A, double v);
 void bs_axpby(Ctx*, double alpha, const Tensor& A, double beta, Tensor& R);
 TP bs_add(Ctx*, const Tensor& A, const Tensor& B);
 double bs_dot(Ctx*, const Tensor& A, const Tensor& B);
+TP bs_restrict(Ctx*, const Tensor& A, const Tensor& B);
 
 // allocation-free variants (sb_noalloc.hip): noalloc_setup finds the device's
 // scratch array once per device (that device current); bs_norm_noalloc writes

@@ -332,6 +332,13 @@ class NewBlockSparseSorted(_Tensor):
         check(load().sb_bs_add(self.ctx, self._h, other.handle, C.byref(out)))
         return NewBlockSparseSorted(out, self.device)
 
+    def restrict(self, other):
+        """other's blocks in this tensor's block structure (zero where other
+        has no block of a key): the projection onto this vector's block space"""
+        out = C.c_void_p()
+        check(load().sb_bs_restrict(self.ctx, self._h, other.handle, C.byref(out)))
+        return NewBlockSparseSorted(out, self.device)
+
     def dot(self, other):
         """inner product over blocks of equal key."""
         v = C.c_double()

@@ -92,6 +92,12 @@ class Plan:
         check(load().sb_plan_box_config(self._h, C.byref(t), C.byref(w), C.byref(n), C.byref(ms)))
         return dict(tile=t.value, wg_per_cu=w.value, candidates=n.value, best_ms=ms.value)
 
+    def mfma_flops(self):
+        """matrix-core FLOPs one execution issues, tile padding included (0: not counted)"""
+        v = C.c_double()
+        check(load().sb_plan_mfma_flops(self._h, C.byref(v)))
+        return v.value
+
     def operand_bytes(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         check(load().sb_plan_operand_bytes(self._h, C.byref(a), C.byref(b), C.byref(c)))
