@@ -1,0 +1,100 @@
+"""GPU tests at the sizes of the benchmark configurations.
+
+* Batching (64 independent S=1/2 J1-J2 problems, J2 ~ U[0, 0.5], chi
+  log-uniform in [64, 1024] in 17 Sz sectors, seeds 0..63): one batch
+  execution equals every chain executed alone, bit for bit, in FP64 and FP32;
+  two members against the oracle's four contract! calls at 1e-12.
+* Precision (Hubbard t=1 U=4, chi=2048 in 520 (n, 2Sz) sectors): the FP32 and
+  bf16x3 products of the four-call chain and the fused plan's FP32 form
+  within 2e-6 of the oracle's FP64 result.
+* Serving (XXZ requests at N=32, chi_max=128 and N=64, chi_max=256): every
+  served energy equals the one-at-a-time run's bit for bit.
+"""
+import numpy as np
+import pytest
+
+import sparsebackends_amd as sb
+from sparsebackends_amd import heff as H
+from oracle import oracle as O
+from oracle.heff_oracle import HeffOracle
+from refs import rel_frobenius
+
+pytestmark = pytest.mark.gpu
+
+
+def outputs(p):
+    return p.Hpsi.parts()["data"].copy()
+
+
+@pytest.mark.parametrize("flags", [sb.PREC_FP64, sb.PREC_FP32], ids=["fp64", "fp32"])
+def test_batch_64_j1j2_equals_alone(flags):
+    print(f"[config batch] 64 J1-J2 problems flags={flags}", flush=True)
+    probs = H.j1j2_batch_problems(64)
+    chis = [c for _, _, c in probs]
+    assert len(probs) == 64 and min(chis) >= 64 and max(chis) <= 1024 and max(chis) > 512
+    for p, _, _ in probs:
+        p.upload().outputs().make_plans(flags)
+    alone = []
+    for p, _, _ in probs:
+        p.execute()
+        alone.append(outputs(p))
+        p.Hpsi.fill_(0.0)
+    batch = sb.PlanBatch([p.plans for p, _, _ in probs])
+    assert batch.stats()["fused_stages"] == 4
+    batch.execute()
+    for c, (p, _, _) in enumerate(probs):
+        assert np.array_equal(outputs(p), alone[c]), c
+    if flags == sb.PREC_FP64:
+        # two members against the oracle: the smallest bond and one of chi ~ 256
+        order = np.argsort(chis)
+        pick = [int(order[0]), int(order[np.searchsorted(np.sort(chis), 256)])]
+        O.use_blas(True, threads=8)
+        try:
+            for c in pick:
+                p = probs[c][0]
+                q = HeffOracle(p).run().parts()
+                hp = p.Hpsi.parts()
+                assert np.array_equal(hp["keys"], q["keys"]) and np.array_equal(hp["ids"], q["ids"])
+                assert rel_frobenius(hp["data"], q["data"]) <= 1e-12, (c, chis[c])
+        finally:
+            O.use_blas(False)
+
+
+def test_precision_forms_hubbard_chi2048():
+    print("[config precision] hubbard chi=2048", flush=True)
+    prob = H.hubbard_chi2048_problem(0)
+    O.use_blas(True, threads=8)
+    try:
+        q = HeffOracle(prob).run().parts()
+    finally:
+        O.use_blas(False)
+    errs = {}
+    for name, fl in (("fp32", sb.PREC_FP32), ("bf16x3", sb.PREC_BF16X3)):
+        prob.upload().outputs()
+        prob.make_plans(fl)
+        got = prob.Hpsi.parts()
+        assert np.array_equal(got["keys"], q["keys"]) and np.array_equal(got["ids"], q["ids"])
+        errs[name] = rel_frobenius(got["data"], q["data"])
+    prob.upload()
+    prob.make_fused_plan(sb.PREC_FP32)
+    got = prob.Hpsi_fused.parts()
+    assert np.array_equal(got["keys"], q["keys"]) and np.array_equal(got["ids"], q["ids"])
+    errs["fused_fp32"] = rel_frobenius(got["data"], q["data"])
+    print("[config precision] relative errors", errs, flush=True)
+    for name, e in errs.items():
+        assert 0 < e <= 2e-6, (name, e)
+
+
+@pytest.mark.parametrize("sites,chi", [(32, 128), (64, 256)])
+def test_serving_sizes_equal_alone(sites, chi):
+    from sparsebackends_amd import serve as S
+    print(f"[config serving] N={sites} chi_max={chi}", flush=True)
+    rq = [S.SolveRequest(i, sites, chi, delta, 0.0, 1e-10, 8) for i, delta in enumerate((0.7, 1.3))]
+    plain = S.run_plain(rq)
+    res = S.DMRGServer(slots=2).run(rq)
+    for a, b in zip(plain, res):
+        assert a.energy == b.energy and a.energies == b.energies, (a.rid, a.energy, b.energy)
+        assert a.maxlinkdim <= chi and np.isfinite(a.energy)
+    print("[config serving] maxlinkdim", [r.maxlinkdim for r in plain], "sweeps", [r.sweeps for r in plain],
+          flush=True)
+    assert all(r.maxlinkdim >= 64 for r in plain)  # bonds well beyond the small serving tests' 16-32
