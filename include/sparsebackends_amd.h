@@ -389,6 +389,30 @@ int sb_bs_svd_sectors(sb_ctx_t ctx, sb_tensor_t psi, int64_t nsec, const int64_t
                       double cutoff, sb_tensor_t* A, sb_tensor_t* B, int64_t* kept, double* svs, int64_t svs_cap,
                       int64_t* newdim, double* truncerr);
 
+/* ------------------------------------------------------------ QR
+ * blocksparse_left_qr (SparseBackends/src/ops_factorize.jl:8-43) of a
+ * NewBlockSparseSorted{T,3,1,2} A with prefix (l, r) and dense s: the dense
+ * (n_l d_s) x n_r matrix of its nonzero l and r values, thin Householder QR
+ * with LAPACK's sign conventions.  Q {T,3,2} (dims (A.dims[1], d_s, k)) has
+ * one block per l value (keys ascending, ids 1..n_l); R is a dense k x n_r
+ * array (columns in ascending r value).  An A without blocks fails with
+ * ArgumentError. */
+int sb_bs_left_qr(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t* Q, sb_tensor_t* R);
+/* left_factorize (tensor_wrappers.jl:954-970) below its ITensor layer: Q as
+ * above and R scattered into a dense k x d_r array (column r of the r value r,
+ * zero where A has no block with that r) */
+int sb_bs_left_factorize(sb_ctx_t ctx, sb_tensor_t A, sb_tensor_t* Q, sb_tensor_t* R);
+/* New (the sector-format counterpart of blocksparse_left_qr, as
+ * sb_bs_svd_sectors is of blocksparse_svd): QR of a site tensor
+ * A [Ql, Qr | s, l, r] sector by sector.  For right sector ir the matrix has
+ * rows (Ql, s, l) over A's blocks (il, ir) with rowsec[il + nl*s] == ir
+ * (ascending il, s, then l < dl[il]) and columns r < dr[ir]; one thin QR per
+ * sector.  Q has A's block list and the k = min(rows, dr) columns of every
+ * sector's factor (kept[ir] = k); R [Qr, Qr | m, r] holds the k x dr factors
+ * on its diagonal keys. */
+int sb_bs_qr_sectors(sb_ctx_t ctx, sb_tensor_t A, const int64_t* rowsec, const int64_t* dl, const int64_t* dr,
+                     sb_tensor_t* Q, sb_tensor_t* R, int64_t* kept);
+
 #ifdef __cplusplus
 }
 #endif

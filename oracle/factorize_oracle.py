@@ -21,6 +21,16 @@ the per-group singular values of numpy.linalg.svd, the truncation rules on
 hand-built spectra, and a literal key/id case worked by hand from the loop
 order of the reference.
 
+blocksparse_left_qr (ops_factorize.jl:8-43) and left_factorize
+(tensor_wrappers.jl:954-970): the dense (n_l d_s) x n_r matrix of the
+nonzero l and r values (row (li-1) d_s + s, column ri, :13-25), numpy's qr
+(LAPACK geqrf / orgqr, the Householder QR with the sign conventions of the
+reference's qr), thin to k = min(n_l d_s, n_r) (:28-31), Q keyed by l with one
+block per l value in ascending order (setindex! creates every block, :34-41),
+R scattered to k x d_r by left_factorize.  Pinned (tests/test_oracle_qr.py)
+by reconstruction, orthonormal Q, a hand-worked literal case whose signs are
+LAPACK's, and the key/id rule.
+
 svd_sectors: the same factorisation for the sector format (psi
 [Ql, Qr | s1, s2, l, r], blocks by fused charge), which the reference does not
 have; pinned (tests/test_sector_svd.py) by exact reconstruction over the full
@@ -30,6 +40,40 @@ whole (s1, l) x (s2, r) matrix.
 from __future__ import annotations
 
 import numpy as np
+
+
+def blocksparse_left_qr(parts):
+    """returns (Q parts, R (k x n_r), l_vals, r_vals); parts with dims
+    (n_l, n_r, d_s), N2 = 1"""
+    assert parts["N2"] == 1 and len(parts["dims"]) == 3
+    keys = np.asarray(parts["keys"]).reshape(-1, 2)
+    ids = np.asarray(parts["ids"])
+    data = np.asarray(parts["data"], dtype=np.float64)
+    ds = int(parts["dims"][2])
+    l_vals = np.unique(keys[:, 0])
+    r_vals = np.unique(keys[:, 1])
+    li = {int(v): i for i, v in enumerate(l_vals)}
+    ri = {int(v): i for i, v in enumerate(r_vals)}
+    M = np.zeros((len(l_vals) * ds, len(r_vals)))
+    for (l, r), i in zip(keys, ids):
+        M[li[int(l)] * ds:(li[int(l)] + 1) * ds, ri[int(r)]] = data[(int(i) - 1) * ds:int(i) * ds]
+    Q, R = np.linalg.qr(M, mode="reduced")
+    k = min(M.shape)
+    Q, R = Q[:, :k], R[:k, :]
+    qdata = np.zeros(len(l_vals) * ds * k)
+    for n, _ in enumerate(l_vals):
+        qdata[n * ds * k:(n + 1) * ds * k] = Q[n * ds:(n + 1) * ds, :].reshape(-1, order="F")
+    qparts = dict(dims=(int(parts["dims"][0]), ds, k), N2=2, keys=l_vals.reshape(-1, 1).astype(np.int64),
+                  ids=np.arange(1, len(l_vals) + 1), data=qdata, blksize=ds * k)
+    return qparts, R, l_vals.astype(np.int64), r_vals.astype(np.int64)
+
+
+def left_factorize(parts):
+    """returns (Q parts, R_full (k x d_r)): R's column ri at column r_vals[ri] - 1"""
+    q, R, _, r_vals = blocksparse_left_qr(parts)
+    full = np.zeros((R.shape[0], int(parts["dims"][1])))
+    full[:, r_vals - 1] = R
+    return q, full
 
 
 def blocksparse_svd(parts, ortho="left", maxdim=None, mindim=1, cutoff=0.0):

@@ -108,6 +108,9 @@ SIGNATURES = {
     "sb_bs_dot": [P, P, P, F64P],
     "sb_bs_norm_noalloc": [P, P, P],
     "sb_bs_restrict": [P, P, P, PP],
+    "sb_bs_left_qr": [P, P, PP, PP],
+    "sb_bs_left_factorize": [P, P, PP, PP],
+    "sb_bs_qr_sectors": [P, P, I64P, I64P, I64P, PP, PP, I64P],
     "sb_bs_svd": [P, P, C.c_int, C.c_int64, C.c_int64, C.c_double, PP, PP, F64P, C.c_int64, I64P, F64P],
     "sb_bs_svd_sectors": [P, P, C.c_int64, I64P, I64P, I64P, I64P, C.c_int, C.c_int64, C.c_int64, C.c_double, PP,
                           PP, I64P, F64P, C.c_int64, I64P, F64P],

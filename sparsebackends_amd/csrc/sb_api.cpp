@@ -820,6 +820,36 @@ int sb_bs_svd(sb_ctx_t c, sb_tensor_t A, int ortho_right, int64_t maxdim, int64_
   })
 }
 
+int sb_bs_left_qr(sb_ctx_t c, sb_tensor_t A, sb_tensor_t* Q, sb_tensor_t* R) {
+  SB_TRY({
+    if (!Q || !R) fail(SB_E_ARGUMENT, "ArgumentError: null output handle");
+    LeftQrResult r = bs_left_qr(C_(c), *T_(A));
+    *Q = H_(std::move(r.Q));
+    *R = H_(std::move(r.R));
+  })
+}
+int sb_bs_left_factorize(sb_ctx_t c, sb_tensor_t A, sb_tensor_t* Q, sb_tensor_t* R) {
+  SB_TRY({
+    if (!Q || !R) fail(SB_E_ARGUMENT, "ArgumentError: null output handle");
+    LeftQrResult r = bs_left_factorize(C_(c), *T_(A));
+    *Q = H_(std::move(r.Q));
+    *R = H_(std::move(r.R));
+  })
+}
+int sb_bs_qr_sectors(sb_ctx_t c, sb_tensor_t A, const int64_t* rowsec, const int64_t* dl, const int64_t* dr,
+                     sb_tensor_t* Q, sb_tensor_t* R, int64_t* kept) {
+  SB_TRY({
+    if (!Q || !R) fail(SB_E_ARGUMENT, "ArgumentError: null output handle");
+    if (!rowsec || !dl || !dr) fail(SB_E_ARGUMENT, "ArgumentError: null sector table");
+    Tensor* t = T_(A);
+    SectQrResult r = bs_qr_sectors(C_(c), *t, rowsec, dl, dr);
+    if (kept)
+      for (size_t i = 0; i < r.kept.size(); ++i) kept[i] = r.kept[i];
+    *Q = H_(std::move(r.Q));
+    *R = H_(std::move(r.R));
+  })
+}
+
 int sb_bs_svd_sectors(sb_ctx_t c, sb_tensor_t psi, int64_t nsec, const int64_t* rowsec, const int64_t* colsec,
                       const int64_t* dl, const int64_t* dr, int ortho_right, int64_t maxdim, int64_t mindim,
                       double cutoff, sb_tensor_t* A, sb_tensor_t* B, int64_t* kept, double* svs, int64_t svs_cap,

@@ -393,3 +393,236 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
 }
 
 }  // namespace sb
+
+namespace sb {
+
+namespace {
+
+// one thin QR per job on the device; A is a scratch copy, Q / R / tau sized by the jobs
+struct QrRun {
+  DBuf<double> A, Q, R, tau;
+  DBuf<QrJob> jobs;
+};
+
+void run_qr(Ctx* ctx, QrRun& q, std::vector<QrJob>& jobs, i64 na, i64 nq, i64 nr, i64 nt) {
+  q.Q.alloc(ctx, (size_t)std::max<i64>(nq, 1));
+  q.R.alloc(ctx, (size_t)std::max<i64>(nr, 1));
+  q.tau.alloc(ctx, (size_t)std::max<i64>(nt, 1));
+  (void)na;
+  up(ctx, q.jobs, jobs);
+  k_qr_batch(ctx, q.jobs.get(), (int)jobs.size(), q.A.get(), q.Q.get(), q.R.get(), q.tau.get());
+}
+
+void copy_map(Ctx* ctx, const std::vector<i64>& smap, const double* src, const std::vector<i64>& dmap, double* dst) {
+  DBuf<i64> s, d;
+  up(ctx, s, smap);
+  up(ctx, d, dmap);
+  k_copy_map(ctx, (i64)smap.size(), s.get(), src, d.get(), dst);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the index buffers go back to the cache
+}
+
+}  // namespace
+
+// blocksparse_left_qr (ops_factorize.jl:8-43): A {T,3,1,2} with prefix (l, r)
+// and dense s; M[(li-1) d_s + s, ri] = A[l_vals[li], r_vals[ri], s], thin QR,
+// Q {T,3,2} keyed by l with dense (s, new bond), one block per l value in
+// ascending order (the reference's setindex! creates them in that order), R
+// the k x n_r matrix.
+LeftQrResult bs_left_qr(Ctx* ctx, const Tensor& A) {
+  if (A.kind != BS || A.N() != 3 || A.N2 != 1)
+    fail(SB_E_METHOD, "MethodError: blocksparse_left_qr needs a NewBlockSparseSorted{T,3,1,2}");
+  const i64 n = A.n, ds = A.dims[2];
+  if (n == 0) fail(SB_E_ARGUMENT, "ArgumentError: blocksparse_left_qr of a tensor without blocks");
+  std::vector<i32> hk, hi;
+  down(ctx, hk, A.keys.get(), (size_t)n * 2);
+  down(ctx, hi, A.ids.get(), (size_t)n);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  LeftQrResult res;
+  for (i64 i = 0; i < n; ++i) {
+    res.l_vals.push_back(hk[(size_t)(2 * i)]);
+    res.r_vals.push_back(hk[(size_t)(2 * i + 1)]);
+  }
+  auto uniq = [](std::vector<i64>& v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  uniq(res.l_vals);
+  uniq(res.r_vals);
+  std::map<i64, i64> li, ri;
+  for (size_t i = 0; i < res.l_vals.size(); ++i) li[res.l_vals[i]] = (i64)i;
+  for (size_t i = 0; i < res.r_vals.size(); ++i) ri[res.r_vals[i]] = (i64)i;
+  const i64 nl = (i64)res.l_vals.size(), nr = (i64)res.r_vals.size();
+  const i64 m = nl * ds, k = std::min(m, nr);
+  if (m >= (1ll << 31) || nr >= (1ll << 31)) fail(SB_E_LIMIT, "blocksparse_left_qr: matrix too large");
+  QrRun q;
+  q.A.alloc(ctx, (size_t)(m * nr));
+  k_zero(ctx, q.A.get(), (size_t)(m * nr) * sizeof(double));
+  std::vector<i64> sm, dm;
+  for (i64 i = 0; i < n; ++i) {
+    const i64 r0 = li[hk[(size_t)(2 * i)]] * ds, c = ri[hk[(size_t)(2 * i + 1)]];
+    for (i64 s = 0; s < ds; ++s) {
+      sm.push_back((i64)(hi[(size_t)i] - 1) * A.blksize + s);
+      dm.push_back(r0 + s + m * c);
+    }
+  }
+  copy_map(ctx, sm, A.dptr(), dm, q.A.get());
+  std::vector<QrJob> jobs{QrJob{(int)m, (int)nr, 0, 0, 0, 0}};
+  run_qr(ctx, q, jobs, m * nr, m * k, k * nr, k);
+  // Q: block li holds Q_mat[(li) d_s + s, b] at s + d_s b
+  res.Q = bs_make(ctx, {A.dims[0], ds, k}, 2);
+  Tensor& Q = *res.Q;
+  Q.n = nl;
+  std::vector<i32> qk(res.l_vals.begin(), res.l_vals.end()), qi((size_t)nl);
+  for (i64 i = 0; i < nl; ++i) qi[(size_t)i] = (i32)(i + 1);
+  up(ctx, Q.keys, qk);
+  up(ctx, Q.ids, qi);
+  Q.data.alloc(ctx, (size_t)std::max<i64>(nl * Q.blksize, 1));
+  sm.clear();
+  dm.clear();
+  for (i64 l = 0; l < nl; ++l)
+    for (i64 b = 0; b < k; ++b)
+      for (i64 s = 0; s < ds; ++s) {
+        sm.push_back(l * ds + s + m * b);
+        dm.push_back(l * Q.blksize + s + ds * b);
+      }
+  copy_map(ctx, sm, q.Q.get(), dm, Q.data.get());
+  res.R = dense_new(ctx, {k, nr});
+  HIP_CHECK(hipMemcpyAsync(res.R->dptr(), q.R.get(), (size_t)(k * nr) * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  Q.touch();
+  return res;
+}
+
+// left_factorize (tensor_wrappers.jl:954-970) below the ITensor layer: Q of
+// blocksparse_left_qr and R scattered to the full k x d_r (column r_vals[ri] of
+// R_full = column ri of R)
+LeftQrResult bs_left_factorize(Ctx* ctx, const Tensor& A) {
+  LeftQrResult res = bs_left_qr(ctx, A);
+  const i64 k = res.R->dims[0], nr = res.R->dims[1], dr = A.dims[1];
+  TP full = dense_new(ctx, {k, dr});
+  k_zero(ctx, full->dptr(), (size_t)(k * dr) * sizeof(double));
+  std::vector<i64> sm, dm;
+  for (i64 c = 0; c < nr; ++c)
+    for (i64 b = 0; b < k; ++b) {
+      sm.push_back(b + k * c);
+      dm.push_back(b + k * (res.r_vals[(size_t)c] - 1));
+    }
+  copy_map(ctx, sm, res.R->dptr(), dm, full->dptr());
+  res.R = std::move(full);
+  return res;
+}
+
+// Sector QR of a sector-format site tensor A [Ql, Qr | s, l, r]: per right
+// sector ir the matrix with rows (Ql, s, l) over the blocks (il, ir) of A whose
+// (il, s) has rowsec[il + nl s] == ir (ascending il, then s, then l < dl[il])
+// and columns r < dr[ir]; thin QR each (one workgroup per sector).  Q has A's
+// keys and ids with the columns b < kept[ir] = min(rows, dr[ir]); R [Qr, Qr | m, r]
+// holds the sectors' k x dr factors on its diagonal keys, ids ascending.
+SectQrResult bs_qr_sectors(Ctx* ctx, const Tensor& A, const i64* rowsec, const i64* dl, const i64* dr) {
+  if (A.kind != BS || A.N() != 5 || A.N2 != 3)
+    fail(SB_E_METHOD, "MethodError: the sector QR needs a site tensor [Ql, Qr | s, l, r]");
+  const i64 nl = A.dims[0], nr = A.dims[1], d = A.dims[2], Dl = A.dims[3], Dr = A.dims[4];
+  for (i64 i = 0; i < nl; ++i)
+    if (dl[i] < 0 || dl[i] > Dl) fail(SB_E_BOUNDS, "BoundsError: sector extent beyond the padded axis");
+  for (i64 i = 0; i < nr; ++i)
+    if (dr[i] < 0 || dr[i] > Dr) fail(SB_E_BOUNDS, "BoundsError: sector extent beyond the padded axis");
+  for (i64 i = 0; i < nl * d; ++i)
+    if (rowsec[i] < -1 || rowsec[i] >= nr) fail(SB_E_BOUNDS, "BoundsError: row sector out of range");
+  std::vector<i32> hk, hi;
+  down(ctx, hk, A.keys.get(), (size_t)A.n * 2);
+  down(ctx, hi, A.ids.get(), (size_t)A.n);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  std::vector<std::vector<std::pair<i64, i64>>> by_r((size_t)nr);  // ir -> (il, block row)
+  for (i64 i = 0; i < A.n; ++i) by_r[(size_t)(hk[(size_t)(2 * i + 1)] - 1)].push_back({hk[(size_t)(2 * i)] - 1, i});
+  SectQrResult res;
+  res.kept.assign((size_t)nr, 0);
+  std::vector<QrJob> jobs;
+  std::vector<i64> sm, dm, jsec;
+  i64 na = 0, nq = 0, nrr = 0, nt = 0;
+  struct Row {
+    i64 blk, s, l;
+  };
+  std::vector<std::vector<Row>> rows((size_t)nr);
+  for (i64 ir = 0; ir < nr; ++ir) {
+    auto& v = by_r[(size_t)ir];
+    std::sort(v.begin(), v.end());
+    for (auto& b : v)
+      for (i64 s = 0; s < d; ++s)
+        if (rowsec[b.first + nl * s] == ir)
+          for (i64 l = 0; l < dl[b.first]; ++l) rows[(size_t)ir].push_back({b.second, s, l});
+    const i64 m = (i64)rows[(size_t)ir].size(), n = dr[ir], k = std::min(m, n);
+    if (k == 0) continue;
+    if (m >= (1ll << 31)) fail(SB_E_LIMIT, "sector QR: sector matrix too large");
+    jobs.push_back(QrJob{(int)m, (int)n, na, nq, nrr, nt});
+    jsec.push_back(ir);
+    for (i64 c = 0; c < n; ++c)
+      for (i64 x = 0; x < m; ++x) {
+        const Row& rw = rows[(size_t)ir][(size_t)x];
+        sm.push_back((i64)(hi[(size_t)rw.blk] - 1) * A.blksize + rw.s + d * (rw.l + Dl * c));
+        dm.push_back(na + x + m * c);
+      }
+    res.kept[(size_t)ir] = k;
+    na += m * n;
+    nq += m * k;
+    nrr += k * n;
+    nt += k;
+  }
+  QrRun q;
+  q.A.alloc(ctx, (size_t)std::max<i64>(na, 1));
+  if (na) k_zero(ctx, q.A.get(), (size_t)na * sizeof(double));
+  copy_map(ctx, sm, A.dptr(), dm, q.A.get());
+  run_qr(ctx, q, jobs, na, nq, nrr, nt);
+  // Q: A's block list, zero payload, the factors' columns scattered back
+  res.Q = bs_make(ctx, A.dims, 3);
+  Tensor& Q = *res.Q;
+  Q.n = A.n;
+  up(ctx, Q.keys, hk);
+  up(ctx, Q.ids, hi);
+  Q.data.alloc(ctx, (size_t)std::max<i64>(A.n * A.blksize, 1));
+  if (A.n) k_zero(ctx, Q.data.get(), (size_t)(A.n * A.blksize) * sizeof(double));
+  sm.clear();
+  dm.clear();
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const QrJob& J = jobs[j];
+    const i64 ir = jsec[j], k = std::min(J.m, J.n);
+    for (i64 b = 0; b < k; ++b)
+      for (i64 x = 0; x < J.m; ++x) {
+        const Row& rw = rows[(size_t)ir][(size_t)x];
+        sm.push_back(J.qoff + x + (i64)J.m * b);
+        dm.push_back((i64)(hi[(size_t)rw.blk] - 1) * A.blksize + rw.s + d * (rw.l + Dl * b));
+      }
+  }
+  copy_map(ctx, sm, q.Q.get(), dm, Q.data.get());
+  // R [Qr, Qr | m, r]: one diagonal block per factored sector
+  res.R = bs_make(ctx, {nr, nr, Dr, Dr}, 2);
+  Tensor& R = *res.R;
+  const i64 nb = (i64)jobs.size();
+  R.n = nb;
+  std::vector<i32> rk((size_t)(2 * nb)), ri((size_t)nb);
+  for (i64 j = 0; j < nb; ++j) {
+    rk[(size_t)(2 * j)] = rk[(size_t)(2 * j + 1)] = (i32)(jsec[(size_t)j] + 1);
+    ri[(size_t)j] = (i32)(j + 1);
+  }
+  up(ctx, R.keys, rk);
+  up(ctx, R.ids, ri);
+  R.data.alloc(ctx, (size_t)std::max<i64>(nb * R.blksize, 1));
+  if (nb) k_zero(ctx, R.data.get(), (size_t)(nb * R.blksize) * sizeof(double));
+  sm.clear();
+  dm.clear();
+  for (i64 j = 0; j < nb; ++j) {
+    const QrJob& J = jobs[(size_t)j];
+    const i64 k = std::min(J.m, J.n);
+    for (i64 c = 0; c < J.n; ++c)
+      for (i64 b = 0; b < k; ++b) {
+        sm.push_back(J.roff + b + k * c);
+        dm.push_back(j * R.blksize + b + Dr * c);
+      }
+  }
+  copy_map(ctx, sm, q.R.get(), dm, R.data.get());
+  Q.touch();
+  R.touch();
+  return res;
+}
+
+}  // namespace sb

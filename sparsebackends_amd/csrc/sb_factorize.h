@@ -32,4 +32,21 @@ struct SectSvdResult {
 SectSvdResult bs_svd_sectors(Ctx*, const Tensor& psi, i64 nsec, const i64* rowsec, const i64* colsec, const i64* dl,
                              const i64* dr, bool right, i64 maxdim, i64 mindim, double cutoff);
 
+struct LeftQrResult {
+  TP Q, R;                      // Q {3, N2 = 2} keyed by l; R dense (k x n_r), or (k x d_r) from left_factorize
+  std::vector<i64> l_vals, r_vals;
+};
+// blocksparse_left_qr (SparseBackends/src/ops_factorize.jl:8-43) of a NewBlockSparseSorted{T,3,1,2}
+LeftQrResult bs_left_qr(Ctx*, const Tensor& A);
+// left_factorize (tensor_wrappers.jl:954-970) below the ITensor layer: R scattered to k x d_r
+LeftQrResult bs_left_factorize(Ctx*, const Tensor& A);
+
+struct SectQrResult {
+  TP Q, R;                  // Q with A's block list; R [Qr, Qr | m, r] on the diagonal keys
+  std::vector<i64> kept;    // new extent of every right sector (0: no rows)
+};
+// sector QR of a site tensor [Ql, Qr | s, l, r]: rowsec[il + nl s] = the right
+// sector of (Ql, s) or -1; dl, dr the sector extents
+SectQrResult bs_qr_sectors(Ctx*, const Tensor& A, const i64* rowsec, const i64* dl, const i64* dr);
+
 }  // namespace sb

@@ -257,4 +257,13 @@ void k_i32_to_i64(Ctx*, const i32* a, i64 n, i64* b);
 void k_take_col(Ctx*, const i32* keys, int w, int col0, const i32* idx, i64 n, i32* out);
 void k_rv_lin(Ctx*, const i32* v, i64 n, u64* lin);
 void k_scatter_i32(Ctx*, const i32* idx, const i32* vals, i64 n, i32* out);
+// thin Householder QR (LAPACK conventions) of a batch of column-major
+// matrices, one workgroup each (sb_qr.hip); A is overwritten
+struct QrJob {
+  int m, n;
+  i64 aoff, qoff, roff, toff;
+};
+void k_qr_batch(Ctx*, const QrJob* jobs, int njobs, double* A, double* Q, double* R, double* tau);
+// dst[dmap[t]] = src[smap[t]] for t < n
+void k_copy_map(Ctx*, i64 n, const i64* smap, const double* src, const i64* dmap, double* dst);
 }  // namespace sb

@@ -157,14 +157,76 @@ def two_site(A, B):
     return X.permutedims([1, 2, 3, 5, 4, 6])
 
 
+def config_bonds(N, chi=256, qmax=8, width=32.0):
+    """The bond layout of the plain configuration (S=1/2 chain, q = 2 Sz, total
+    Sz = 0): bond k (k sites to its left) holds the sectors q in {-qmax..qmax}
+    that the left block reaches with the right block still able to cancel it
+    (q = k mod 2, |q| <= min(k, N - k)), with dims proportional to
+    exp(-q^2 / width) summing (largest remainder) to min(chi, 2^k, 2^(N-k)),
+    each capped by the states the two blocks have in that sector
+    (binomial counts); bonds 0 and N are the trivial sector of dim 1."""
+    from math import comb
+    from .heff import Bond, largest_remainder
+    out = []
+    for k in range(N + 1):
+        qs = [q for q in range(-qmax, qmax + 1) if (q - k) % 2 == 0 and abs(q) <= min(k, N - k)]
+        cap = min(chi, 2 ** min(k, 60), 2 ** min(N - k, 60))
+        dims = largest_remainder([np.exp(-q * q / width) for q in qs], cap) if qs else []
+        dims = [min(int(dm), comb(k, (k + q) // 2), comb(N - k, (N - k - q) // 2)) for q, dm in zip(qs, dims)]
+        out.append(Bond([(q,) for q in qs], dims))
+    return out
+
+
+def random_mps(model, bonds, seed=0, device=0):
+    """site tensors A_j [Ql, Qr | s, l, r] on the bonds' sectors with i.i.d.
+    N(0, 1) entries (numpy PCG64, the given seed) on the charge-allowed ones"""
+    rng = np.random.default_rng(seed)
+    return [upload(site_parts(model, bonds[j], bonds[j + 1], rng=rng), device) for j in range(len(bonds) - 1)]
+
+
+def left_orthonormalize(model, A, bonds):
+    """Left-orthonormalise an MPS by QR, sector by sector (sb_bs_qr_sectors):
+    A_j = Q_j R_j, A_{j+1} <- R_j A_{j+1}, from the first site to the last;
+    the last site's R (the state's norm) is dropped, so the state ends
+    normalised.  Returns (the new site tensors, the new bonds)."""
+    from .factorize import qr_sectors
+    from .heff import Bond, qadd
+    A = list(A)
+    bonds = list(bonds)
+    d = model.d
+    lR = [lab("Qm"), lab("Qr"), lab("m"), lab("r")]
+    lA = [lab("Qr"), lab("Qr2"), lab("s"), lab("r"), lab("r2")]
+    lC = [lab("Qm"), lab("Qr2"), lab("m"), lab("s"), lab("r2")]
+    for j in range(len(A)):
+        Bl, Br = bonds[j], bonds[j + 1]
+        rowsec = np.full((Bl.nsec, d), -1, dtype=np.int64)
+        for il in range(Bl.nsec):
+            for s_ in range(d):
+                q = qadd(Bl.charges[il], model.charges[s_])
+                rowsec[il, s_] = Br.index.get(q, -1)
+        Q, R, kept = qr_sectors(A[j], rowsec, Bl.dims, Br.dims)
+        if np.any(kept == 0):
+            raise ValueError(f"bond {j + 1}: a sector without rows (unreachable charge)")
+        A[j] = Q
+        bonds[j + 1] = Bond(Br.charges, [int(x) for x in kept])
+        if j + 1 < len(A):
+            nx = A[j + 1]
+            dims = nx.dims
+            C_ = NewBlockSparseSorted.new((dims[0], dims[1], dims[3], d, dims[4]), 3, device=nx.device)
+            contract_(C_, lC, R, lR, nx, lA)  # (R A)[Qm, Qr2 | m, s, r2]
+            A[j + 1] = C_.permutedims([1, 2, 4, 3, 5])
+    return A, bonds
+
+
 class DMRG:
     """Two-site DMRG of an open chain with a uniform site model, in a fixed
-    total-charge sector, from a product state.
+    total-charge sector, from a product state (or from given site tensors,
+    DMRG.from_mps).
 
     Tensors, environments and bonds stay on the device between steps; the
     host keeps the sector structure (charges and extents of every bond)."""
 
-    def __init__(self, model, states, device=0, flags=0, eigsolver=None):
+    def __init__(self, model, states, device=0, flags=0, eigsolver=None, environments=True):
         """eigsolver(plans, psi, krylovdim=, maxiter=, tol=) -> (vals, vecs, info)
         replaces krylov.eigsolve (a serving loop's co-batching hook)."""
         from .heff import Bond
@@ -196,10 +258,23 @@ class DMRG:
         e[0, 0, 0] = 1.0
         self.R = {self.N + 1: upload(dict(dims=(1, 1, w, 1, 1), N2=3, keys=np.array([[1, 1]]), ids=np.array([1]),
                                           data=e.reshape(-1, order="F")), device)}
-        for j in range(self.N, 2, -1):  # R_j from R_{j+1} and site j (1-based)
-            self.R[j] = make_R(self.R[j + 1], self.A[j - 1], self.W, flags)[0]
+        if environments:
+            for j in range(self.N, 2, -1):  # R_j from R_{j+1} and site j (1-based)
+                self.R[j] = make_R(self.R[j + 1], self.A[j - 1], self.W, flags)[0]
         self.energies = []
         self.truncerr = 0.0
+
+    @classmethod
+    def from_mps(cls, model, A, bonds, device=0, flags=0, eigsolver=None):
+        """DMRG from a left-orthonormal MPS (site tensors A, bonds): the right
+        environments are built from it, as from a product state"""
+        self = cls(model, [0] * len(A), device=device, flags=flags, eigsolver=eigsolver, environments=False)
+        self.A = list(A)
+        self.bonds = list(bonds)
+        self.total = bonds[-1].charges[0]
+        for j in range(self.N, 2, -1):
+            self.R[j] = make_R(self.R[j + 1], self.A[j - 1], self.W, flags)[0]
+        return self
 
     def step(self, b, direction, maxdim, cutoff, krylovdim, maxiter, tol, mindim=1):
         """optimise sites b, b+1 (1-based): eigsolve on H_eff, sector SVD, environment update"""

@@ -1,6 +1,8 @@
-"""Two-site SVD with truncation on block-sparse storage:
-blocksparse_svd (SparseBackends/src/ops_factorize.jl:50-148) through the
-library's sb_bs_svd."""
+"""Factorisations on block-sparse storage: blocksparse_left_qr and
+left_factorize (SparseBackends/src/ops_factorize.jl:8-43,
+tensor_wrappers.jl:954-970), the two-site SVD with truncation
+blocksparse_svd (ops_factorize.jl:50-148), and their sector-format
+counterparts, through the library's sb_bs_left_qr / sb_bs_svd family."""
 from __future__ import annotations
 
 import ctypes as C
@@ -9,10 +11,47 @@ from collections import namedtuple
 import numpy as np
 
 from ._abi import check, load
-from .storage import NewBlockSparseSorted
+from .storage import DenseTensor, NewBlockSparseSorted
 
 # ITensors.Spectrum(eigs, truncerr): eigs = kept singular values squared
 Spectrum = namedtuple("Spectrum", ["eigs", "truncerr"])
+
+
+def blocksparse_left_qr(A):
+    """A: NewBlockSparseSorted with prefix (l, r) and dense s.  Returns
+    (Q, R, l_vals, r_vals) like the reference: Q keyed by l with dense
+    (s, new bond), R the k x n_r matrix (numpy) over the ascending nonzero r
+    values, l_vals / r_vals the sorted distinct l and r values (1-based)."""
+    Q, R = C.c_void_p(), C.c_void_p()
+    check(load().sb_bs_left_qr(A.ctx, A.handle, C.byref(Q), C.byref(R)))
+    keys = A.parts()["keys"]
+    l_vals = np.unique(keys[:, 0]).astype(np.int64)
+    r_vals = np.unique(keys[:, 1]).astype(np.int64)
+    return NewBlockSparseSorted(Q, A.device), DenseTensor(R, A.device).numpy(), l_vals, r_vals
+
+
+def left_factorize(A):
+    """left_factorize below its ITensor layer: (Q, R_full) with R_full the
+    k x d_r array (numpy) whose column r holds R's column of r value r"""
+    Q, R = C.c_void_p(), C.c_void_p()
+    check(load().sb_bs_left_factorize(A.ctx, A.handle, C.byref(Q), C.byref(R)))
+    return NewBlockSparseSorted(Q, A.device), DenseTensor(R, A.device).numpy()
+
+
+def qr_sectors(A, rowsec, dl, dr):
+    """Sector QR of a site tensor A [Ql, Qr | s, l, r] (sb_bs_qr_sectors):
+    rowsec[il, s] = the right sector of (Ql, s) or -1 (shape (nl, d)); dl, dr
+    the sector extents.  Returns (Q with A's block list, R [Qr, Qr | m, r],
+    kept: the new extent of every right sector)."""
+    i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64))  # noqa: E731
+    rs = i64(np.asarray(rowsec).reshape(-1, order="F"))
+    dl_, dr_ = i64(dl), i64(dr)
+    kept = np.zeros(max(1, len(dr_)), dtype=np.int64)
+    Q, R = C.c_void_p(), C.c_void_p()
+    p64 = C.POINTER(C.c_int64)
+    check(load().sb_bs_qr_sectors(A.ctx, A.handle, rs.ctypes.data_as(p64), dl_.ctypes.data_as(p64),
+                                  dr_.ctypes.data_as(p64), C.byref(Q), C.byref(R), kept.ctypes.data_as(p64)))
+    return NewBlockSparseSorted(Q, A.device), NewBlockSparseSorted(R, A.device), kept[:len(dr_)].copy()
 
 
 def blocksparse_svd(A, ortho="left", maxdim=None, mindim=1, cutoff=0.0):

@@ -98,3 +98,58 @@ def test_serving_sizes_equal_alone(sites, chi):
     print("[config serving] maxlinkdim", [r.maxlinkdim for r in plain], "sweeps", [r.sweeps for r in plain],
           flush=True)
     assert all(r.maxlinkdim >= 64 for r in plain)  # bonds well beyond the small serving tests' 16-32
+
+
+def oracle_heff(psi, L, W, R):
+    """the oracle's four contract! calls of H_eff*psi on the device tensors' host parts"""
+    def bs(t):
+        p = t.parts()
+        return O.BS.from_parts(p["dims"], p["N2"], p["keys"], p["ids"], p["data"])
+    nl, nr, d1, d2, Dl, Dr = psi.dims
+    w, wr, wm = W.dims[0], W.dims[3], W.dims[3]
+    opsi, oL, oW, oR = bs(psi), bs(L), bs(W), bs(R)
+    T1 = O.BS.new((nl, nr, d1, d2, Dr, w, Dl), 5)
+    O.contract(T1, H.L_T1, opsi, H.L_PSI, oL, H.L_L)
+    T2 = O.BS.new((nl, nr, d1, wm, d2, Dr, Dl), 5)
+    O.contract(T2, H.L_T2, T1, H.L_T1, oW, H.L_W1)
+    T3 = O.BS.new((nl, nr, d2, wr, d1, Dr, Dl), 5)
+    O.contract(T3, H.L_T3, T2, H.L_T2, oW, H.L_W2)
+    Hp = O.BS.new((nl, nr, d1, d2, Dl, Dr), 4)
+    O.contract(Hp, H.L_OUT, T3, H.L_T3, oR, H.L_R)
+    return Hp.parts()
+
+
+def test_config1_sweep_from_qr_start():
+    """S=1/2 Heisenberg J=1, N=32, chi=256 in Sz sectors, MPS i.i.d. N(0,1)
+    from seed 0 left-orthonormalised by QR, Krylov dim 3, cutoff 1e-10"""
+    from sparsebackends_amd import dmrg as D
+    print("[config 1] N=32 chi=256 sweep from the QR-orthonormalised random MPS", flush=True)
+    m = H.heisenberg_model(1.0)
+    bonds = D.config_bonds(32, chi=256)
+    A, bonds = D.left_orthonormalize(m, D.random_mps(m, bonds, seed=0), bonds)
+    assert max(b.dim for b in bonds) == 256
+    dm = D.DMRG.from_mps(m, A, bonds)
+    # the first two-site step's H_eff*psi against the oracle, at the bond of the largest sectors
+    b = 16
+    for bb in range(1, b):
+        dm.step(bb, +1, 256, 1e-10, 3, 1, 1e-14)
+    psi = D.two_site(dm.A[b - 1], dm.A[b])
+    plans, Hp = H.heff_plans(psi, dm.L[b - 1], dm.W, dm.W, dm.R[b + 2])
+    got = Hp.parts()
+    O.use_blas(True, threads=8)
+    try:
+        q = oracle_heff(psi, dm.L[b - 1], dm.W, dm.R[b + 2])
+    finally:
+        O.use_blas(False)
+    assert np.array_equal(got["keys"], q["keys"]) and np.array_equal(got["ids"], q["ids"])
+    assert rel_frobenius(got["data"], q["data"]) <= 1e-12
+    # one sweep at chi = 256, and a chi = 512 run from the same start: variational order
+    dm = D.DMRG.from_mps(m, A, bonds)
+    e256 = dm.sweep(maxdim=256, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
+    ref = D.DMRG.from_mps(m, A, bonds)
+    e512 = None
+    for _ in range(3):
+        e512 = ref.sweep(maxdim=512, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
+    print(f"[config 1] E(chi 256, one sweep) = {e256:.12f}, E(chi 512, three sweeps) = {e512:.12f}", flush=True)
+    assert -0.45 * 32 < e512 <= e256 + 1e-10
+    assert dm.maxlinkdim() <= 256
