@@ -261,30 +261,85 @@ def dmrg_section(sb, a):
 
 
 def sweep_section(sb, a):
-    """One two-site DMRG sweep of the plain configuration's chain: S=1/2
-    Heisenberg J=1, N=32, U(1) Sz, MPO w=5, bond dimension 256, Krylov
-    dimension 3 (maxiter 1), cutoff 1e-10.  The MPS is grown from the Neel
-    state by untimed sweeps until its bonds reach 256 (the configuration's
-    random QR-orthonormalised start is replaced by a converging state of the
-    same bond dimension); then one full sweep (left to right and back, 62
-    two-site steps) is timed, and its energy compared with the chain's exact
-    ground energy is reported."""
+    """One two-site DMRG sweep of the plain configuration: S=1/2 Heisenberg
+    J=1, N=32, U(1) Sz, MPO w=5, bond dimension 256 (sectors q = 2Sz of the
+    bond's parity, dims ~ exp(-q^2/32)), the MPS i.i.d. N(0,1) from seed 0
+    left-orthonormalised by QR (sector by sector, on the device), Krylov
+    dimension 3 (maxiter 1), cutoff 1e-10.  The sweep (left to right and back,
+    62 two-site steps) is timed.  CPU leg: the oracle (reference algorithm
+    restated in C++, BLAS on one thread) on the three centre steps' work that
+    dominates a step -- three H_eff*psi applications (krylovdim 3) and the
+    sector SVD -- timed on the host, per step."""
     from sparsebackends_amd import heff as H
     from sparsebackends_amd import dmrg as D
     N, chi = 32, 256
-    dm = D.DMRG(H.heisenberg_model(1.0), [0, 1] * (N // 2))
-    warm = 0
-    while dm.maxlinkdim() < chi and warm < 8:
-        dm.sweep(maxdim=chi, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
-        warm += 1
+    m = H.heisenberg_model(1.0)
+    t0 = time.perf_counter()
+    bonds = D.config_bonds(N, chi=chi)
+    A, bonds = D.left_orthonormalize(m, D.random_mps(m, bonds, seed=0), bonds)
+    dm = D.DMRG.from_mps(m, A, bonds)
     sb.synchronize()
+    t_init = time.perf_counter() - t0
     t0 = time.perf_counter()
     e = dm.sweep(maxdim=chi, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
     sb.synchronize()
     ts = time.perf_counter() - t0
-    return dict(model="Heisenberg S=1/2 J=1, N=32, U(1) Sz, MPO w=5", chi=chi, maxlinkdim=int(dm.maxlinkdim()),
-                warmup_sweeps=warm, krylovdim=3, maxiter=1, cutoff=1e-10, steps=2 * (N - 1), sweep_s=ts,
-                ms_per_step=ts / (2 * (N - 1)) * 1e3, energy=e, truncerr=dm.truncerr)
+    res = dict(model="Heisenberg S=1/2 J=1, N=32, U(1) Sz, MPO w=5", chi=chi, maxlinkdim=int(dm.maxlinkdim()),
+               start="random N(0,1) MPS, seed 0, left-orthonormalised by sector QR on the device",
+               init_s=t_init, krylovdim=3, maxiter=1, cutoff=1e-10, steps=2 * (N - 1), sweep_s=ts,
+               ms_per_step=ts / (2 * (N - 1)) * 1e3, energy=e, truncerr=dm.truncerr)
+    if not a.no_cpu:
+        from oracle import oracle as O
+        from oracle.factorize_oracle import svd_sectors as osvd
+        O.use_blas(True, threads=1)
+
+        def obs(t):
+            p = t.parts()
+            return O.BS.from_parts(p["dims"], p["N2"], p["keys"], p["ids"], p["data"]), p
+
+        # after the sweep the left environments are a half sweep old: left-to-right
+        # steps 1..14 rebuild them (untimed), so that the sampled steps see the
+        # consistent state the sweep's own forward steps see
+        kw = dict(maxdim=chi, cutoff=1e-10, krylovdim=3, maxiter=1, tol=1e-14)
+        for b in range(1, 15):
+            dm.step(b, +1, **kw)
+        tcpu, tgpu = [], []
+        for b in (15, 16, 17):
+            psi = D.two_site(dm.A[b - 1], dm.A[b])
+            plans, Hp = H.heff_plans(psi, dm.L[b - 1], dm.W, dm.W, dm.R[b + 2])
+            sb.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(3):
+                for pl in plans:
+                    pl.execute()
+            sb.synchronize()
+            tgpu.append(time.perf_counter() - t1)
+            (opsi, pp), (oL, _), (oW, _), (oR, _) = obs(psi), obs(dm.L[b - 1]), obs(dm.W), obs(dm.R[b + 2])
+            nl, nr, d1, d2, Dl, Dr = psi.dims
+            w = dm.W.dims[0]
+            t1 = time.perf_counter()
+            for _ in range(3):
+                T1 = O.BS.new((nl, nr, d1, d2, Dr, w, Dl), 5)
+                O.contract(T1, H.L_T1, opsi, H.L_PSI, oL, H.L_L)
+                T2 = O.BS.new((nl, nr, d1, w, d2, Dr, Dl), 5)
+                O.contract(T2, H.L_T2, T1, H.L_T1, oW, H.L_W1)
+                T3 = O.BS.new((nl, nr, d2, w, d1, Dr, Dl), 5)
+                O.contract(T3, H.L_T3, T2, H.L_T2, oW, H.L_W2)
+                Ho = O.BS.new((nl, nr, d1, d2, Dl, Dr), 4)
+                O.contract(Ho, H.L_OUT, T3, H.L_T3, oR, H.L_R)
+            Bl, Br = dm.bonds[b - 1], dm.bonds[b + 1]
+            rowsec, colsec, charges = D.fused_sectors(m, Bl, Br)
+            osvd(pp, rowsec, colsec, Bl.dims, Br.dims, len(charges), maxdim=chi, cutoff=1e-10)
+            tcpu.append(time.perf_counter() - t1)
+            dm.step(b, +1, **kw)
+        res["cpu_baseline_sample"] = dict(
+            kind="port", cores=1, steps=[15, 16, 17], cpu_s_per_step=float(np.mean(tcpu)),
+            gpu_heff_s_per_step=float(np.mean(tgpu)),
+            sample="three centre two-site steps: 3 H_eff*psi applications (four contract! calls each, oracle "
+                   "C++ restatement, BLAS on 1 thread) + the sector SVD (numpy LAPACK)",
+            sweep_estimate_s=float(np.mean(tcpu)) * 2 * (N - 1),
+            note="the estimate scales the centre steps (the largest bonds) to the sweep's 62 steps")
+    return res
 
 
 def batch_section(sb, a):
