@@ -91,6 +91,17 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-sites", default="16,24,32", help="chain lengths the requests draw from")
     ap.add_argument("--serve-chis", default="32,64,128", help="bond dimension caps the requests draw from")
+    ap.add_argument("--serve-config5", type=int, default=0,
+                    help="run only the serving measurement at the serving configuration's sizes with this many "
+                         "requests (N in 32..128, chi_max in 128..1024; minutes per request) and print its JSON")
+    ap.add_argument("--serve5-seed", type=int, default=0)
+    ap.add_argument("--serve5-max-sweeps", type=int, default=20)
+    ap.add_argument("--serve5-parts", default="plain,served,mixed")
+    ap.add_argument("--serve5-rates", default="1,10,50", help="Poisson arrival rates (requests/s) of the served runs")
+    ap.add_argument("--serve5-mixed-rates", default="10", help="rates of the served mixed-precision runs")
+    ap.add_argument("--serve5-plain-json", default="",
+                    help="reuse the plain run of an earlier --serve-config5 output (same requests) instead of "
+                         "running it")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -532,6 +543,118 @@ def serve_section(sb, a):
     return out
 
 
+def plain_replay(reqs, service):
+    """Latencies of the plain server (one solve at a time, first come first
+    served) for the given arrivals: finish_i = max(arrival_i, finish_{i-1}) +
+    service_i, with the service times measured in its saturated run."""
+    order = sorted(reqs, key=lambda r: (r.arrival, r.rid))
+    fin, lat = 0.0, {}
+    for r in order:
+        fin = max(r.arrival, fin) + service[r.rid]
+        lat[r.rid] = fin - r.arrival
+    la = np.array([lat[r.rid] for r in reqs])
+    span = fin - min(r.arrival for r in reqs)
+    return dict(solves=len(reqs), span_s=float(span), solves_per_s=len(reqs) / span, p50_s=float(np.percentile(la, 50)),
+                p99_s=float(np.percentile(la, 99)), mean_s=float(la.mean()))
+
+
+def serve5_section(sb, a):
+    """The serving loop at the serving configuration's sizes: XXZ chains (J =
+    1, Delta uniform in [0.5, 1.5], Neel start, U(1) Sz, MPO w=5), N drawn
+    from {32, 64, 96, 128} and chi_max from {128, 256, 512, 1024}, energy
+    tolerance 1e-10 between sweeps (at most --serve5-max-sweeps), cutoff 1e-12, Krylov
+    dimension 3.  A sample of --serve-config5 requests (seed --serve5-seed)
+    of the configuration's 512: one request there takes seconds to minutes.
+    * plain: one solve at a time, FCFS, every request at t = 0 (its service
+      times); at each Poisson rate its latencies follow exactly from those
+      service times (plain_replay: the plain server idles only when no request
+      is waiting);
+    * served: the continuous-batching server (8 slots, admission by the HBM
+      footprint estimate), measured at each Poisson rate;
+    * mixed: the served run with FP32 block products and an FP64 polish
+      (sweeps with FP64 products once the FP32 energy settles to 1e-6
+      relative), measured at --serve5-mixed-rates.
+    Served and plain energies are compared bit for bit, mixed ones against
+    the FP64 plain energies."""
+    import dataclasses
+    from sparsebackends_amd import serve as S
+    n = a.serve_config5
+    parts = set(a.serve5_parts.split(","))
+    base = S.poisson_requests(n, float("inf"), seed=a.serve5_seed, max_sweeps=a.serve5_max_sweeps)
+    out = dict(model="XXZ J=1, Delta~U[0.5,1.5], Neel start, U(1) Sz, MPO w=5", requests=n,
+               sample=f"{n} requests drawn with seed {a.serve5_seed} from N in (32,64,96,128) x chi_max in "
+                      "(128,256,512,1024) (the configuration's stream has 512)",
+               mix=[dict(rid=r.rid, sites=r.sites, chi_max=r.chi_max, delta=round(r.delta, 6)) for r in base],
+               energy_tol=1e-10, max_sweeps=base[0].max_sweeps, cutoff=base[0].cutoff, krylovdim=3, maxiter=1,
+               slots=8)
+    S.run_plain([S.SolveRequest(10 ** 6, 8, 16, 1.0)])  # first-use costs out of the timed runs
+    t_start = [time.perf_counter()]
+
+    def solver(r, eig, dev):  # the library's solve, with a line per finished request
+        res = S.solve_dmrg(r, eig, dev)
+        print(f"[serve5]   request {r.rid} (N={r.sites}, chi_max={r.chi_max}) done at "
+              f"{time.perf_counter() - t_start[0]:.1f} s, {len(res[1])} sweeps", flush=True)
+        return res
+
+    def server():
+        srv = S.DMRGServer(slots=8, co_batch=False, solver=solver)
+        srv.release = S.trim  # as the library's own solves do
+        t_start[0] = time.perf_counter()
+        return srv
+    print(f"[serve5] {n} requests: " + ", ".join(f"({r.sites},{r.chi_max})" for r in base), flush=True)
+    if a.serve5_plain_json:
+        with open(a.serve5_plain_json) as f:
+            pl = json.load(f)["plain"]
+        service = {int(k): v for k, v in pl["service_s"].items()}
+        ref = {int(k): v for k, v in pl["energies"].items()}
+        out["plain"] = pl
+    else:
+        t0 = t_start[0] = time.perf_counter()
+        plain = S.run_plain(base, solver=solver)
+        service = {r.rid: r.finish - r.admitted for r in plain}
+        ref = {r.rid: r.energy for r in plain}
+        out["plain"] = dict(S.summarize(plain), wall_s=time.perf_counter() - t0, service_s=service, energies=ref,
+                            sweeps={r.rid: r.sweeps for r in plain},
+                            peak_gib={r.rid: r.peak_bytes / 2 ** 30 for r in plain},
+                            footprint_gib={r.rid: r.footprint / 2 ** 30 for r in plain})
+        print(f"[serve5] plain saturated: {out['plain']['solves_per_s']:.4f} solves/s, "
+              f"converged {out['plain']['converged']}/{n}", flush=True)
+    rates = [float(x) for x in a.serve5_rates.split(",") if x]
+    out["rates"] = {}
+    for rate in rates:
+        reqs = S.poisson_requests(n, rate, seed=a.serve5_seed, max_sweeps=a.serve5_max_sweeps)
+        row = dict(plain=plain_replay(reqs, service))
+        if "served" in parts:
+            srv = server()
+            t0 = time.perf_counter()
+            res = srv.run(reqs)
+            row["served"] = dict(S.summarize(res), wall_s=time.perf_counter() - t0, max_inflight=srv.max_inflight,
+                                 identical_to_plain=all(r.energy == ref[r.rid] for r in res))
+            row["solves_per_s_ratio"] = row["served"]["solves_per_s"] / row["plain"]["solves_per_s"]
+            row["p99_ratio"] = row["served"]["p99_s"] / row["plain"]["p99_s"]
+            print(f"[serve5] rate {rate:g}/s served: {row['served']['solves_per_s']:.4f} solves/s "
+                  f"(x{row['solves_per_s_ratio']:.2f}), p99 x{row['p99_ratio']:.2f}", flush=True)
+        out["rates"][f"{rate:g}"] = row
+    if "mixed" in parts:
+        for rate in [float(x) for x in a.serve5_mixed_rates.split(",") if x]:
+            reqs = [dataclasses.replace(r, flags=sb.PREC_FP32) for r in S.poisson_requests(n, rate, seed=a.serve5_seed,
+                                                                                     max_sweeps=a.serve5_max_sweeps)]
+            srv = server()
+            t0 = time.perf_counter()
+            res = srv.run(reqs)
+            pr = plain_replay(reqs, service)
+            m = dict(S.summarize(res), wall_s=time.perf_counter() - t0,
+                     max_abs_dE_vs_fp64=float(max(abs(r.energy - ref[r.rid]) for r in res)),
+                     sweeps={r.rid: r.sweeps for r in res}, plain=pr,
+                     solves_per_s_ratio_vs_plain=0.0, p99_ratio_vs_plain=0.0)
+            m["solves_per_s_ratio_vs_plain"] = m["solves_per_s"] / pr["solves_per_s"]
+            m["p99_ratio_vs_plain"] = m["p99_s"] / pr["p99_s"]
+            out.setdefault("mixed", {})[f"{rate:g}"] = m
+            print(f"[serve5] rate {rate:g}/s mixed: {m['solves_per_s']:.4f} solves/s, max |dE| "
+                  f"{m['max_abs_dE_vs_fp64']:.2e}, converged {m['converged']}/{n}", flush=True)
+    return out
+
+
 def permute_section(sb, a, prob):
     """permutedims (storage.jl:188-269) of the chi=2048 Hubbard psi [Ql, Qr |
     s1, s2, l, r] (4430 blocks of 4 x 4 x 13 x 13): a tail permutation that
@@ -623,6 +746,9 @@ def main():
         sys.exit(2)
     import sparsebackends_amd as sb
 
+    if a.serve_config5 > 0:
+        print(json.dumps(dict(serving_config5=serve5_section(sb, a))), flush=True)
+        return
     flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec] | (a.tile << 16)
     tune = not a.no_autotune and a.tile == 0
     if tune:

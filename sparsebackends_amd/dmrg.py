@@ -291,10 +291,17 @@ class DMRG:
         live = [i for i in range(len(charges)) if kept[i] > 0]
         self.bonds[b] = Bond([charges[i] for i in live], [int(kept[i]) for i in live])
         self.A[b - 1], self.A[b] = A, B
+        # the environment this step used on the side it leaves behind is rebuilt
+        # before its next use (by the steps of the opposite direction), so it is
+        # freed now: only one side's environments are resident at a time
         if direction > 0:
             self.L[b] = make_L(self.L[b - 1], A, self.W, self.flags)[0]
+            if b + 2 <= self.N:
+                self.R.pop(b + 2, None)
         else:
             self.R[b + 1] = make_R(self.R[b + 2], B, self.W, self.flags)[0]
+            if b - 1 >= 1:
+                self.L.pop(b - 1, None)
         self.truncerr = max(self.truncerr, spec.truncerr)
         return vals[0], info
 

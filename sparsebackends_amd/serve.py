@@ -56,6 +56,11 @@ class SolveRequest:
     krylovdim: int = 3
     maxiter: int = 1
     flags: int = 0  # precision form of the block products (PREC_FP64 / PREC_FP32 / PREC_BF16X3)
+    # a low-precision request sweeps with those products until the energy changes
+    # by less than polish_tol * |E| between sweeps (or polish_sweeps sweeps are
+    # left), then with FP64 products until tol: its energy is an FP64 result
+    polish_tol: float = 1e-6
+    polish_sweeps: int = 4
 
 
 @dataclass
@@ -80,13 +85,14 @@ class SolveResult:
 
 def footprint_bytes(req: SolveRequest, d=2, w=5):
     """Upper estimate of the device bytes a solve holds: site tensors and
-    environments of every site at the cap chi_max, plus one step's working set
-    (two-site psi, the three H_eff intermediates, Krylov vectors).  Blocks of
-    the sector format are padded to the bond's largest sector; pad = 4 times
-    the dense size covers the U(1) bond distributions of these chains (the
-    measured peak of 24 solves of 16-32 sites at chi 32-128 reached 0.87 of
-    it)."""
-    chi, n, pad = req.chi_max, req.sites, 4
+    environments of every site at the cap chi_max (a sweep keeps one side's
+    environments), plus one step's working set (two-site psi, the three H_eff
+    intermediates, Krylov vectors).  Blocks of the sector format are padded to
+    the bond's largest sector; pad = 6 times the dense size covers the U(1)
+    bond distributions of these chains: measured peaks (to convergence or 12-20
+    sweeps) were 0.29-0.81 of it at (sites, chi_max) from (32, 128) to
+    (128, 512) and (64, 1024), the largest at (128, 512)."""
+    chi, n, pad = req.chi_max, req.sites, 6
     site = d * chi * chi
     env = w * chi * chi
     step = (req.krylovdim + 4) * d * d * chi * chi + 3 * w * d * d * chi * chi
@@ -107,14 +113,21 @@ def solve_dmrg(req: SolveRequest, eigsolver=None, device=0):
     dm = D.DMRG(H.heisenberg_model(1.0, float(req.delta)), neel(req.sites), device=device, flags=req.flags,
                 eigsolver=eigsolver)
     es, conv, peak = [], False, 0
-    for _ in range(req.max_sweeps):
+    low = req.flags != 0  # in the low-precision stage
+    prev = None  # the previous sweep's energy in the current stage
+    for k in range(req.max_sweeps):
         e = dm.sweep(maxdim=req.chi_max, cutoff=req.cutoff, krylovdim=req.krylovdim, maxiter=req.maxiter, tol=1e-14)
         peak = max(peak, memory(device)[0] - base)
-        if es and abs(e - es[-1]) < req.tol:
-            es.append(e)
+        es.append(e)
+        settled = prev is not None and abs(e - prev) < (req.polish_tol * abs(e) if low else req.tol)
+        prev = e
+        if low and (settled or req.max_sweeps - 1 - k <= req.polish_sweeps):
+            low, prev = False, None
+            dm.flags = 0  # the FP64 polish: the next sweeps' plans use FP64 products
+            continue
+        if settled:
             conv = True
             break
-        es.append(e)
     synchronize(device)
     return es[-1], es, conv, dm.maxlinkdim(), peak, dm.truncerr
 
@@ -349,9 +362,11 @@ def poisson_requests(n, rate, seed=0, sites=(32, 64, 96, 128), chis=(128, 256, 5
                      tol=1e-10, max_sweeps=12):
     """n requests with exponential inter-arrival times of mean 1/rate (rate in
     requests per second; rate = inf: all at time 0), lengths and bond caps
-    drawn uniformly from sites / chis, anisotropy uniform in delta."""
+    drawn uniformly from sites / chis, anisotropy uniform in delta.  One seed
+    gives the same requests at every rate; only the arrival times scale."""
     rng = np.random.default_rng(seed)
-    gaps = np.zeros(n) if not np.isfinite(rate) else rng.exponential(1.0 / rate, n)
+    gaps = rng.standard_exponential(n)  # drawn at every rate, so the mix of a seed does not depend on the rate
+    gaps = gaps / rate if np.isfinite(rate) else np.zeros(n)
     t = np.cumsum(gaps) - (gaps[0] if n else 0.0)
     return [SolveRequest(i, int(rng.choice(sites)), int(rng.choice(chis)), float(rng.uniform(*delta)), float(t[i]),
                          tol, max_sweeps) for i in range(n)]

@@ -9,33 +9,79 @@ from __future__ import annotations
 
 import ctypes as C
 import threading
+import weakref
 
 import numpy as np
 
 from ._abi import load, check
 
-_ctx_lock = threading.Lock()
-_contexts = {}
-_bound = threading.local()
+_local = threading.local()  # per thread: .own {device: _Context}, .bound {device: context}
+_registry = weakref.WeakValueDictionary()  # handle value -> its _Context
+_pending = []  # handles of contexts no longer referenced, destroyed at the next context() call
+_pending_lock = threading.Lock()
+
+
+class _Context:
+    """A library context (HIP device + stream + device-memory cache) owned by
+    Python: the thread that made it holds it in thread-local storage, and every
+    tensor and plan made on it holds it too, so it lives until the thread has
+    ended and the last of them is gone.  Then it is destroyed (sb_ctx_destroy)
+    at the next context() call of any thread: not in __del__ itself, because
+    a garbage-collection pass may finalise the context before tensors of the
+    same pass whose device memory its cache still owns."""
+
+    __slots__ = ("h", "device", "owned", "__weakref__")
+
+    def __init__(self, device, handle=None):
+        if handle is None:
+            h = C.c_void_p()
+            check(load().sb_ctx_create(device, C.byref(h)))
+            self.h, self.owned = h, True
+        else:  # a context made outside this module: used, never destroyed here
+            self.h, self.owned = handle, False
+        self.device = device
+        if self.owned:
+            _registry[self.h.value] = self
+
+    def __del__(self):
+        try:
+            if self.owned and self.h:
+                with _pending_lock:
+                    _pending.append(self.h)
+        except Exception:
+            pass
+
+
+def _drain():
+    with _pending_lock:
+        hs = list(_pending)
+        _pending.clear()
+    for h in hs:
+        load().sb_ctx_destroy(h)
+
+
+def _context_obj(device=0):
+    if _pending:
+        _drain()
+    b = getattr(_local, "bound", None)
+    if b is not None and device in b:
+        return b[device]
+    own = getattr(_local, "own", None)
+    if own is None:
+        own = _local.own = {}
+    c = own.get(device)
+    if c is None:
+        c = own[device] = _Context(device)
+    return c
 
 
 def context(device=0):
-    """The context (HIP device + stream) this thread uses on a device: the one
-    bound by bind_context, else the thread's own default context.  Default
-    contexts are kept by thread identifier, which Python reuses once a thread
-    has ended: a new thread may continue an ended thread's context (its stream,
-    its cached device memory and the tensors still alive from it), which is
-    correct because a context is only ever used by one live thread at a time."""
-    b = getattr(_bound, "ctx", None)
-    if b is not None and device in b:
-        return b[device]
-    key = (threading.get_ident(), device)
-    with _ctx_lock:
-        if key not in _contexts:
-            h = C.c_void_p()
-            check(load().sb_ctx_create(device, C.byref(h)))
-            _contexts[key] = h
-        return _contexts[key]
+    """The context handle (HIP device + stream) this thread uses on a device:
+    the one bound by bind_context, else the thread's own default context,
+    made at its first use.  A thread's default context is its own (a thread
+    started later never continues it) and is destroyed once the thread has
+    ended and no tensor or plan made on it is left."""
+    return _context_obj(device).h
 
 
 def bind_context(handle, device=0):
@@ -43,20 +89,26 @@ def bind_context(handle, device=0):
     its own default).  Threads bound to one context share its stream and its
     device memory cache, so tensors and plans made by one of them can be used
     by the others (a tensor belongs to the context that made it); their
-    library calls are ordered by the stream.  Binding turns the context's
-    graph replays of eigensolve operators off (a capture on the shared stream
-    would take in the other threads' launches); PlanGraph must not be used on
-    a context that several threads use."""
-    b = getattr(_bound, "ctx", None)
+    library calls are ordered by the stream, and the binding keeps the context
+    alive.  Binding turns the context's graph replays of eigensolve operators
+    off (a capture on the shared stream would take in the other threads'
+    launches); PlanGraph must not be used on a context that several threads
+    use.  Timing measurements on a shared context (the autotuner's candidate
+    runs, profile()) take in the other threads' work on the same stream."""
+    b = getattr(_local, "bound", None)
     if b is None:
-        b = _bound.ctx = {}
+        b = _local.bound = {}
     if handle is None:
         b.pop(device, None)
-    else:
-        # a graph capture on the shared stream would take in the other threads'
-        # launches: the shared context's eigensolves run without graphs
-        check(load().sb_ctx_graphs(handle, 0))
-        b[device] = handle
+        return
+    h = handle if isinstance(handle, C.c_void_p) else C.c_void_p(handle)
+    obj = _registry.get(h.value)
+    if obj is None:
+        obj = _Context(device, h)
+    # a graph capture on the shared stream would take in the other threads'
+    # launches: the shared context's eigensolves run without graphs
+    check(load().sb_ctx_graphs(obj.h, 0))
+    b[device] = obj
 
 
 def synchronize(device=0):
@@ -113,6 +165,7 @@ class _Tensor:
     def __init__(self, handle, device=0):
         self._h = handle if isinstance(handle, C.c_void_p) else C.c_void_p(handle)
         self.device = device
+        self._owner = _context_obj(device)  # the context whose cache holds the device memory
 
     @property
     def handle(self):

@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 from ._abi import load, check
-from .storage import DenseTensor
+from .storage import DenseTensor, _context_obj
 
 # flags (include/sb_status.h)
 EXACT_JOIN = 1 << 0
@@ -71,7 +71,8 @@ class Plan:
                                     la.ctypes.data_as(u), len(labelsA), self._ops[2].handle, lb.ctypes.data_as(u),
                                     len(labelsB), flags, C.byref(h)))
         self._h = h
-        self.ctx = C_.ctx
+        self._owner = _context_obj(dev)  # the context the plan's buffers and stream belong to
+        self.ctx = self._owner.h
 
     def execute(self):
         check(load().sb_plan_execute(self.ctx, self._h))
@@ -127,7 +128,8 @@ class HeffPlan(Plan):
         check(load().sb_heff_plan_create(Hpsi.ctx, Hpsi.handle, psi.handle, L.handle, W1.handle, W2.handle,
                                          R.handle, flags, C.byref(h)))
         self._h = h
-        self.ctx = Hpsi.ctx
+        self._owner = _context_obj(Hpsi.device)
+        self.ctx = self._owner.h
 
 
 class PlanBatch:

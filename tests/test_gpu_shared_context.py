@@ -63,26 +63,24 @@ def test_serving_slots_release_cache():
     lock = threading.Lock()
 
     def release(dev):
-        # thread ids are reused, so a slot thread may carry on the context of a
-        # thread of an earlier test, whose leftover tensors Python's collector
-        # may free into that context's cache at any time: collect first, then
-        # keep the collector off while the figures are read (one slot at a time)
+        # every slot thread has a context of its own (made at its first call):
+        # after its solves, collect its garbage and trim, with the collector
+        # kept off while the figures are read
         with lock:
             gc.collect()
             gc.disable()
             try:
-                seen.append(sb.memory(dev))
+                before = sb.memory(dev)
                 sb.trim(dev)
-                seen.append(sb.memory(dev))
+                seen.append((before, sb.memory(dev)))
             finally:
                 gc.enable()
 
     srv = S.DMRGServer(slots=2, co_batch=False)
     srv.release = release
     srv.run(S.poisson_requests(3, float("inf"), seed=1, sites=(8,), chis=(16,), max_sweeps=3))
-    assert len(seen) == 4  # each slot released its context once
-    # the trim hands the cached blocks back.  Python's collector may free a
-    # slot's tensors from the other slot's thread while the slot trims, and
-    # those blocks go back to the slot's cache, so a single slot's figures need
-    # not fall; with two slots at least one trim shows the cache shrinking.
-    assert any(after[1] < before[1] for before, after in zip(seen[0::2], seen[1::2])), seen
+    assert len(seen) == 2, seen  # each slot released its context once
+    for before, after in seen:
+        # the solves' tensors are gone: nothing in use, the cache held blocks and the trim returned them all
+        assert before[0] == 0 and after[0] == 0, seen
+        assert before[1] > 0 and after[1] == 0, seen
