@@ -22,9 +22,9 @@ size_t sect_svd_lds(int n, int len);
 void k_sect_svd(Ctx*, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
                 double* SVo, const int* glist);
-void k_sect_svd_blocks(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn, const std::vector<int>& ll,
-                       double* X, const i64* xoff, const int* R, const int* Cc, const int* trans, const i64* uoff,
-                       const i64* soff, const i64* sgoff, double* sig, double* Uo, double* SVo);
+void k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& Rs, const std::vector<int>& Cs,
+                     double* X, const i64* xoff, const i64* uoff, const i64* soff, const i64* sgoff, double* sig,
+                     double* Uo, double* SVo);
 void k_sect_emit(Ctx*, bool isA, i64 nblk, int d, i64 D1, i64 D2, i64 nside, const i64* blk, const i64* sec,
                  const i64* off, const i64* dims, const i64* kept, const int* RC, const i64* uoff_or_soff,
                  const i64* sgoff, const double* sig, const double* F, bool right, double* out);
@@ -211,15 +211,12 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   std::vector<int> hR(ng), hC(ng), htr(ng);
   std::vector<i64> xoff(ng + 1, 0), uoff(ng + 1, 0), soff(ng + 1, 0), sgoff(ng + 1, 0);
   // sectors with fewer than 32 rows to orthogonalise (lds_g) are factorised
-  // in the LDS of one workgroup each; the others (big_g) by block Jacobi over
-  // many workgroups, whose sequential depth per sweep is smaller from about
-  // 32 rows on (measured on the chi = 256 Heisenberg chain with the LDS
-  // factorisation running beside the blocks: 0.64 s per sweep against 0.74 s
-  // with the cut at 64 rows and 0.89 s with only the sectors beyond the LDS
-  // in blocks)
+  // in the LDS of one workgroup each; the others (big_g) by QR with column
+  // pivoting and block Jacobi over many workgroups on the rows of R, whose
+  // sequential depth per sweep is smaller from about 32 rows on
   constexpr size_t kLds = 159 * 1024;
   size_t lds = 0;
-  std::vector<int> lds_g, lds_n, lds_len, big_g, big_n, big_len;
+  std::vector<int> lds_g, big_g;
   for (i64 g = 0; g < ng; ++g) {
     const i64 m = act[g];
     if (R[m] > INT32_MAX || Cn[m] > INT32_MAX) fail(SB_E_LIMIT, "sector matrix dimension beyond int32");
@@ -230,12 +227,8 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
     if (n < 32 && sect_svd_lds(n, len) <= kLds) {
       lds = std::max(lds, sect_svd_lds(n, len));
       lds_g.push_back((int)g);
-      lds_n.push_back(n);
-      lds_len.push_back(len);
     } else {
       big_g.push_back((int)g);
-      big_n.push_back(n);
-      big_len.push_back(len);
     }
     xoff[g + 1] = xoff[g] + R[m] * Cn[m];
     uoff[g + 1] = uoff[g] + R[m] * n;
@@ -244,13 +237,19 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   }
   // while block Jacobi sweeps run, the small sectors ride along in their
   // launches (one round per sweep each) instead of a launch of their own that
-  // the large sectors would wait for
+  // the large sectors would wait for; those sectors are gathered column-major
+  // (M itself) for the QR preconditioning
+  std::vector<int> qp_R, qp_C;
   if (!big_g.empty()) {
     big_g.insert(big_g.end(), lds_g.begin(), lds_g.end());
-    big_n.insert(big_n.end(), lds_n.begin(), lds_n.end());
-    big_len.insert(big_len.end(), lds_len.begin(), lds_len.end());
     lds_g.clear();
+    for (int g : big_g) {
+      qp_R.push_back(hR[g]);
+      qp_C.push_back(hC[g]);
+    }
   }
+  std::vector<int> gtr(htr);
+  if (!big_g.empty()) std::fill(gtr.begin(), gtr.end(), 1);
   // gather tasks from psi's blocks
   std::vector<i32> hk, hi;
   down(ctx, hk, psi.keys.get(), (size_t)psi.n * 2);
@@ -282,10 +281,12 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   up(ctx, d_R, hR);
   up(ctx, d_C, hC);
   up(ctx, d_tr, htr);
+  DBuf<int> d_gtr;
+  up(ctx, d_gtr, gtr);
   DBuf<double> X(ctx, xoff[ng] ? xoff[ng] : 1), Uo(ctx, uoff[ng] ? uoff[ng] : 1), SVo(ctx, soff[ng] ? soff[ng] : 1),
       sig(ctx, sgoff[ng] ? sgoff[ng] : 1);
   k_zero(ctx, X.get(), (size_t)(xoff[ng] ? xoff[ng] : 1) * sizeof(double));
-  k_sect_gather(ctx, psi.dptr(), d1, d2, Dl, ntask, d_task.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(),
+  k_sect_gather(ctx, psi.dptr(), d1, d2, Dl, ntask, d_task.get(), d_xoff.get(), d_R.get(), d_C.get(), d_gtr.get(),
                 X.get());
   {
     DBuf<int> d_lg;
@@ -294,8 +295,8 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
       k_sect_svd(ctx, (i64)lds_g.size(), lds, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(),
                  d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), d_lg.get());
     }
-    k_sect_svd_blocks(ctx, big_g, big_n, big_len, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(),
-                      d_uoff.get(), d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get());
+    k_sect_svd_qrcp(ctx, big_g, qp_R, qp_C, X.get(), d_xoff.get(), d_uoff.get(), d_soff.get(), d_sgoff.get(),
+                    sig.get(), Uo.get(), SVo.get());
     HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host list goes out of scope
   }
   std::vector<double> hs;

@@ -277,17 +277,17 @@ __device__ void sect_emit(const double* Xs, const double* Q, int n, int len, boo
     if (lane == 0) nrm[i] = sqrt(a);
   }
   __syncthreads();
-  if (tid == 0) {
-    for (int i = 0; i < n; ++i) ord[i] = i;
-    for (int i = 1; i < n; ++i) {  // stable insertion sort, descending
-      const int v = ord[i];
-      int k = i - 1;
-      while (k >= 0 && nrm[ord[k]] < nrm[v]) {
-        ord[k + 1] = ord[k];
-        --k;
-      }
-      ord[k + 1] = v;
+  // stable sort, descending: row i goes to its rank, the number of rows with a
+  // larger norm plus those of equal norm before it (a NaN ranks as -inf, so
+  // the ranks stay a permutation)
+  for (int i = tid; i < n; i += SS_THREADS) {
+    const double v = isnan(nrm[i]) ? -INFINITY : nrm[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double u = isnan(nrm[j]) ? -INFINITY : nrm[j];
+      rank += (u > v) || (u == v && j < i);
     }
+    ord[rank] = i;
   }
   __syncthreads();
   for (int c = tid; c < n; c += SS_THREADS) sig[c] = nrm[ord[c]];
@@ -390,23 +390,51 @@ void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, 
 
 // ---------------------------------------------------------------- block Jacobi
 // Sectors with 32 rows or more to orthogonalise, or too large for one
-// workgroup's LDS: block one-sided Jacobi over many workgroups.  The n rows of a sector's X (in place in the gathered sector
-// matrix) are cut into B blocks of bs (8) rows (B even, the last ones short or
-// empty); a sweep is B - 1 rounds of the round-robin pairing of blocks, one
-// launch per round over every large sector, one workgroup per block pair: it
-// brings the pair's rows and the matching columns of Q into LDS, rotates the
-// row pairs of the sweep that fall to it, and writes both back.  Every row pair
-// is visited once per sweep: in the first round (where every block meets one
-// partner) a workgroup sweeps all pairs of its rows, in the later rounds only
-// the pairs across its two blocks (a row of one block with a row of the
-// other), in max(rows of the two blocks) steps of one pair per wave.  The host
-// reads the sectors' rotation flags after each sweep.
+// workgroup's LDS: block one-sided Jacobi over many workgroups.  The n rows of
+// a sector's X (in place in the gathered sector matrix) are cut into B blocks
+// of bs rows (B even, the last ones short or empty); a sweep is B - 1 rounds of
+// the round-robin pairing of blocks, one launch per round over every large
+// sector, one workgroup per block pair: it brings the pair's rows into LDS,
+// rotates the row pairs of the sweep that fall to it, and writes them back.
+// Every row pair is visited once per sweep: in the first round (where every
+// block meets one partner) a workgroup sweeps all pairs of its rows, in the
+// later rounds only the pairs across its two blocks (a row of one block with a
+// row of the other), in max(rows of the two blocks) steps of one pair per wave.
+//
+// The accumulated rotation Q (n x n, column c rotated like row c of X) stays
+// out of the rounds: a workgroup records the product of its rotations as the
+// (2 bs) x (2 bs) matrix T (row j of the new rows = sum_i T[j][i] old row i)
+// and, once per sweep, one launch applies every round's T to Q in round order,
+// each workgroup on a slice of Q's entries (the rotations act on the column
+// index only, so slices are independent).  The rounds then move only X's rows
+// and fit larger blocks.  The host reads the sectors' rotation flags after each
+// sweep.
 
 struct BjSector {  // one large sector
   int g;           // sector number
   int n, len, bs, B;
   i64 qoff;        // offset of its Q in the workspace
+  i64 toff;        // offset of its round matrices T (one sweep)
+  int foff;        // offset of its (round, pair) rotation flags
+  int S;           // Q entries per workgroup of the apply launch
+  int slice0;      // its first workgroup in the apply launch
 };
+
+// the Jacobi of a set of sectors: its sector table and accumulated rotations
+struct BjRun {
+  std::vector<BjSector> hs;
+  DBuf<BjSector> d_sec;
+  DBuf<double> Qw;
+};
+
+constexpr int BJ_MAXBS = 8;  // rows per block: eight waves, one row pair each per step
+
+// the two blocks of pair pi in round rd of the round-robin pairing of B blocks
+__device__ inline void bj_blocks(int B, int rd, int pi, int& pb, int& qb) {
+  const int mB = B - 1;
+  pb = pi == 0 ? 0 : 1 + (rd + pi) % mB;
+  qb = pi == 0 ? 1 + rd : 1 + (rd - pi + mB) % mB;
+}
 
 __global__ __launch_bounds__(SS_THREADS) void sect_bj_init_kernel(const BjSector* __restrict__ sec,
                                                                  const double* __restrict__ X,
@@ -423,41 +451,73 @@ __global__ __launch_bounds__(SS_THREADS) void sect_bj_init_kernel(const BjSector
   }
 }
 
-// eight waves: one row pair per wave at every step of a pair of 8-row blocks
-// (measured: 16-row blocks on sixteen waves take 25 % longer per sweep of the
-// chi = 256 Heisenberg chain's sectors, though with half the launches)
-constexpr int BJ_THREADS = 512;
+constexpr int BJ_THREADS = 64 * BJ_MAXBS;
+
+// jacobi_pair with the rotation recorded in the rows tp, tq (length nt) of T
+__device__ inline bool jacobi_pair_t(double* xp, double* xq, int len, double* tp, double* tq, int nt, int lane,
+                                     double ctol, double null2) {
+  double a = 0, b = 0, c = 0;
+  for (int j = lane; j < len; j += 64) {
+    const double x = xp[j], y = xq[j];
+    a = fma(x, x, a);
+    b = fma(y, y, b);
+    c = fma(x, y, c);
+  }
+  a = wave_sum_dpp(a);
+  b = wave_sum_dpp(b);
+  c = wave_sum_dpp(c);
+  if (c == 0.0 || a <= null2 || b <= null2 || fabs(c) <= ctol * sqrt(a) * sqrt(b)) return false;
+  const double zeta = (b - a) / (2.0 * c);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+  for (int j = lane; j < len; j += 64) {
+    const double x = xp[j], y = xq[j];
+    xp[j] = cs * x - sn * y;
+    xq[j] = sn * x + cs * y;
+  }
+  if (lane < nt) {
+    const double u = tp[lane], v = tq[lane];
+    tp[lane] = cs * u - sn * v;
+    tq[lane] = sn * u + cs * v;
+  }
+  return true;
+}
 
 __global__ __launch_bounds__(BJ_THREADS) void sect_bj_round_kernel(const BjSector* __restrict__ sec, int maxpairs,
                                                                   int rd, double* __restrict__ X,
-                                                                  const i64* __restrict__ xoff, double* __restrict__ Qw,
+                                                                  const i64* __restrict__ xoff,
                                                                   const double* __restrict__ null2,
-                                                                  int* __restrict__ flag) {
+                                                                  int* __restrict__ flag, double* __restrict__ Tb,
+                                                                  int* __restrict__ tflag) {
   extern __shared__ double lds[];
+  __shared__ int wg_any;
   const int b = blockIdx.x / maxpairs, pi = blockIdx.x % maxpairs;
   const BjSector s = sec[b];
   const int mB = s.B - 1;
   if (pi >= s.B / 2 || rd >= mB) return;
-  const int pb = pi == 0 ? 0 : 1 + (rd + pi) % mB;
-  const int qb = pi == 0 ? 1 + rd : 1 + (rd - pi + mB) % mB;
+  int pb, qb;
+  bj_blocks(s.B, rd, pi, pb, qb);
   // the pair's rows: block pb then block qb (either may be short or empty)
   const int p0 = pb * s.bs, p1 = min(s.n, p0 + s.bs), q0 = qb * s.bs, q1 = min(s.n, q0 + s.bs);
   const int np_ = max(0, p1 - p0), nq_ = max(0, q1 - q0), nl = np_ + nq_;
-  if (nl < 2) return;
-  const int n = s.n, len = s.len;
-  double* Xg = X + xoff[s.g];
-  double* Q = Qw + s.qoff;
-  double* Xl = lds;                    // nl rows of len
-  double* Ql = Xl + (i64)nl * len;     // nl columns of n
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = BJ_THREADS / 64;
+  int* tf = tflag + s.foff + rd * (s.B / 2) + pi;
+  if (nl < 2) {  // nothing to rotate: the apply launch skips this pair
+    if (tid == 0) *tf = 0;
+    return;
+  }
+  const int len = s.len, td = 2 * s.bs;
+  double* Xg = X + xoff[s.g];
+  double* Xl = lds;                 // nl rows of len
+  double* Tl = Xl + (i64)nl * len;  // T[j * td + i]
   auto grow = [&](int l) { return l < np_ ? p0 + l : q0 + (l - np_); };
   // one wave per row of the pair (no index division)
   for (int l = wave; l < nl; l += nwave) {
     const double* xg = Xg + (i64)grow(l) * len;
-    const double* qg = Q + (i64)n * grow(l);
     for (int j = lane; j < len; j += 64) Xl[(i64)l * len + j] = xg[j];
-    for (int r = lane; r < n; r += 64) Ql[(i64)l * n + r] = qg[r];
   }
+  for (int e = tid; e < td * td; e += BJ_THREADS) Tl[e] = (e / td == e % td) ? 1.0 : 0.0;
+  if (tid == 0) wg_any = 0;
   __syncthreads();
   const double ctol = jacobi_tol(len), nu = null2[b];
   bool any = false;
@@ -468,8 +528,7 @@ __global__ __launch_bounds__(BJ_THREADS) void sect_bj_round_kernel(const BjSecto
         const int p = qi == 0 ? 0 : 1 + (r + qi) % m;
         const int q = qi == 0 ? 1 + r : 1 + (r - qi + m) % m;
         if (p >= nl || q >= nl) continue;
-        if (jacobi_pair(Xl + (i64)p * len, Xl + (i64)q * len, len, Ql + (i64)n * p, Ql + (i64)n * q, n, lane, ctol,
-                        nu))
+        if (jacobi_pair_t(Xl + (i64)p * len, Xl + (i64)q * len, len, Tl + p * td, Tl + q * td, nl, lane, ctol, nu))
           any = true;
       }
       __syncthreads();
@@ -481,97 +540,317 @@ __global__ __launch_bounds__(BJ_THREADS) void sect_bj_round_kernel(const BjSecto
         const int j = (i + k) % L;
         if (i >= np_ || j >= nq_) continue;
         const int p = i, q = np_ + j;
-        if (jacobi_pair(Xl + (i64)p * len, Xl + (i64)q * len, len, Ql + (i64)n * p, Ql + (i64)n * q, n, lane, ctol,
-                        nu))
+        if (jacobi_pair_t(Xl + (i64)p * len, Xl + (i64)q * len, len, Tl + p * td, Tl + q * td, nl, lane, ctol, nu))
           any = true;
       }
       __syncthreads();
     }
   }
+  if (any && lane == 0) wg_any = 1;
   for (int l = wave; l < nl; l += nwave) {
     double* xg = Xg + (i64)grow(l) * len;
-    double* qg = Q + (i64)n * grow(l);
     for (int j = lane; j < len; j += 64) xg[j] = Xl[(i64)l * len + j];
-    for (int r = lane; r < n; r += 64) qg[r] = Ql[(i64)l * n + r];
   }
-  if (any && lane == 0) flag[b] = 1;
+  double* tg = Tb + s.toff + (i64)(rd * (s.B / 2) + pi) * td * td;
+  for (int e = tid; e < td * td; e += BJ_THREADS) tg[e] = Tl[e];
+  __syncthreads();
+  if (tid == 0) {
+    *tf = wg_any;
+    if (wg_any) flag[b] = 1;
+  }
 }
 
-__global__ __launch_bounds__(SS_THREADS) void sect_bj_finish_kernel(const BjSector* __restrict__ sec,
-                                                                   const double* __restrict__ X,
-                                                                   const i64* __restrict__ xoff,
-                                                                   const double* __restrict__ Qw,
-                                                                   const int* __restrict__ R, const int* __restrict__ Cc,
-                                                                   const int* __restrict__ trans,
-                                                                   const i64* __restrict__ uoff,
-                                                                   const i64* __restrict__ soff,
-                                                                   const i64* __restrict__ sgoff,
-                                                                   double* __restrict__ sig, double* __restrict__ Uo,
-                                                                   double* __restrict__ SVo) {
+// The later rounds (rd >= 1) of rows up to 64 CH long: wave i keeps row i of
+// block pb in registers for the whole round and meets row (i + k) mod L of
+// block qb (in LDS) at step k, so a rotation reads and writes only the LDS row
+// (the LDS traffic per rotation, which bounds a step, falls from six row
+// passes to two).  Same pairs, same order and same arithmetic as the rd >= 1
+// branch of sect_bj_round_kernel.
+template <int CH>
+__global__ __launch_bounds__(BJ_THREADS) void sect_bj_cross_kernel(const BjSector* __restrict__ sec, int maxpairs,
+                                                                  int rd, double* __restrict__ X,
+                                                                  const i64* __restrict__ xoff,
+                                                                  const double* __restrict__ null2,
+                                                                  int* __restrict__ flag, double* __restrict__ Tb,
+                                                                  int* __restrict__ tflag) {
   extern __shared__ double lds[];
-  const BjSector s = sec[blockIdx.x];
-  const int g = s.g;
-  double* nrm = lds;
-  int* ord = reinterpret_cast<int*>(nrm + s.n);
-  sect_emit(X + xoff[g], Qw + s.qoff, s.n, s.len, trans[g] != 0, R[g], Cc[g], nrm, ord, sig + sgoff[g],
-            Uo + uoff[g], SVo + soff[g]);
+  __shared__ int wg_any;
+  const int b = blockIdx.x / maxpairs, pi = blockIdx.x % maxpairs;
+  const BjSector s = sec[b];
+  const int mB = s.B - 1;
+  if (pi >= s.B / 2 || rd >= mB) return;
+  int pb, qb;
+  bj_blocks(s.B, rd, pi, pb, qb);
+  const int p0 = pb * s.bs, p1 = min(s.n, p0 + s.bs), q0 = qb * s.bs, q1 = min(s.n, q0 + s.bs);
+  const int np_ = max(0, p1 - p0), nq_ = max(0, q1 - q0), nl = np_ + nq_;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = BJ_THREADS / 64;
+  int* tf = tflag + s.foff + rd * (s.B / 2) + pi;
+  if (np_ == 0 || nq_ == 0) {  // no pair across the blocks: the apply launch skips this pair
+    if (tid == 0) *tf = 0;
+    return;
+  }
+  const int len = s.len, td = 2 * s.bs;
+  double* Xg = X + xoff[s.g];
+  double* Yl = lds;                  // the nq_ rows of block qb
+  double* Tl = Yl + (i64)nq_ * len;  // T[j * td + i], rows 0..np_-1 of pb, np_.. of qb
+  for (int l = wave; l < nq_; l += nwave) {
+    const double* yg = Xg + (i64)(q0 + l) * len;
+    for (int j = lane; j < len; j += 64) Yl[(i64)l * len + j] = yg[j];
+  }
+  double xr[CH];
+  const bool hasp = wave < np_;  // np_ <= bs <= nwave
+  const double* xg = Xg + (i64)(p0 + wave) * len;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const int j = lane + 64 * c;
+    xr[c] = hasp && j < len ? xg[j] : 0.0;
+  }
+  for (int e = tid; e < td * td; e += BJ_THREADS) Tl[e] = (e / td == e % td) ? 1.0 : 0.0;
+  if (tid == 0) wg_any = 0;
+  __syncthreads();
+  const double ctol = jacobi_tol(len), nu = null2[b];
+  bool any = false;
+  const int L = max(np_, nq_);
+  for (int k = 0; k < L; ++k) {
+    const int j = (wave + k) % L;
+    if (hasp && j < nq_) {
+      double* y = Yl + (i64)j * len;
+      double yr[CH];
+      double a = 0, bb = 0, cc = 0;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int e = lane + 64 * c;
+        yr[c] = e < len ? y[e] : 0.0;
+        a = fma(xr[c], xr[c], a);
+        bb = fma(yr[c], yr[c], bb);
+        cc = fma(xr[c], yr[c], cc);
+      }
+      a = wave_sum_dpp(a);
+      bb = wave_sum_dpp(bb);
+      cc = wave_sum_dpp(cc);
+      if (!(cc == 0.0 || a <= nu || bb <= nu || fabs(cc) <= ctol * sqrt(a) * sqrt(bb))) {
+        const double zeta = (bb - a) / (2.0 * cc);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const int e = lane + 64 * c;
+          const double x = xr[c], yv = yr[c];
+          xr[c] = cs * x - sn * yv;
+          if (e < len) y[e] = sn * x + cs * yv;
+        }
+        double* tp = Tl + wave * td;
+        double* tq = Tl + (np_ + j) * td;
+        if (lane < nl) {
+          const double u = tp[lane], v = tq[lane];
+          tp[lane] = cs * u - sn * v;
+          tq[lane] = sn * u + cs * v;
+        }
+        any = true;
+      }
+    }
+    __syncthreads();
+  }
+  if (any && lane == 0) wg_any = 1;
+  if (hasp) {
+    double* xo = Xg + (i64)(p0 + wave) * len;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int e = lane + 64 * c;
+      if (e < len) xo[e] = xr[c];
+    }
+  }
+  for (int l = wave; l < nq_; l += nwave) {
+    double* yg = Xg + (i64)(q0 + l) * len;
+    for (int jj = lane; jj < len; jj += 64) yg[jj] = Yl[(i64)l * len + jj];
+  }
+  double* tg = Tb + s.toff + (i64)(rd * (s.B / 2) + pi) * td * td;
+  for (int e = tid; e < td * td; e += BJ_THREADS) tg[e] = Tl[e];
+  __syncthreads();
+  if (tid == 0) {
+    *tf = wg_any;
+    if (wg_any) flag[b] = 1;
+  }
 }
 
-// rows per block of a large sector: the pair's rows and Q columns fit the LDS
+// Q <- the sweep's rotations: for every round in order, the rows of each pair
+// that rotated get T times their old values, on this workgroup's slice of S
+// entries of every row of Q (Q row c = column c of the accumulated rotation).
+// The round's T matrices are staged in LDS; one wave per pair forms the new
+// rows (up to 16) x (up to 32 entries) as two 16 x 16 tiles on the matrix
+// cores (T's rows against the old rows, K = 16 in four steps).
+constexpr int BJ_APPLY_THREADS = 1024;
+typedef double bj_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(BJ_APPLY_THREADS) void sect_bj_apply_kernel(const BjSector* __restrict__ sec, int nsec,
+                                                                        double* __restrict__ Qw,
+                                                                        const double* __restrict__ Tb,
+                                                                        const int* __restrict__ tflag) {
+  extern __shared__ double lds[];
+  int b = 0;
+  while (b + 1 < nsec && sec[b + 1].slice0 <= (int)blockIdx.x) ++b;
+  const BjSector s = sec[b];
+  const int n = s.n, S = s.S, td = 2 * s.bs, half = s.B / 2;
+  const int r0 = ((int)blockIdx.x - s.slice0) * S, rs = min(S, n - r0);
+  if (rs <= 0) return;
+  double* Q = Qw + s.qoff;
+  double* Qs = lds;                  // Qs[c * S + rl]: entries r0 + rl of Q row c
+  double* Ts = Qs + (i64)n * S;      // the round's T matrices, half x td x td
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = BJ_APPLY_THREADS / 64;
+  for (int e = tid; e < n * S; e += BJ_APPLY_THREADS) {
+    const int c = e / S, rl = e - c * S;
+    if (rl < rs) Qs[e] = Q[(i64)n * c + r0 + rl];
+  }
+  const int j = lane & 15, kk = lane >> 4;
+  for (int rd = 0; rd < s.B - 1; ++rd) {
+    const double* Tg = Tb + s.toff + (i64)rd * half * td * td;
+    for (int e = tid; e < half * td * td; e += BJ_APPLY_THREADS) Ts[e] = Tg[e];
+    __syncthreads();
+    for (int pi = wave; pi < half; pi += nwave) {
+      if (!tflag[s.foff + rd * half + pi]) continue;
+      int pb, qb;
+      bj_blocks(s.B, rd, pi, pb, qb);
+      const int p0 = pb * s.bs, p1 = min(n, p0 + s.bs), q0 = qb * s.bs, q1 = min(n, q0 + s.bs);
+      const int np_ = max(0, p1 - p0), nq_ = max(0, q1 - q0), nl = np_ + nq_;
+      const double* T = Ts + pi * td * td;
+      bj_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int l = 4 * k + kk;  // A[j][l] = T[j][l], B[l][col] = old row l, entry col of the tile
+        const double a = (j < nl && l < nl) ? T[j * td + l] : 0.0;
+        const int row = l < nl ? (l < np_ ? p0 + l : q0 + (l - np_)) : -1;
+        const double b0 = (row >= 0 && j < rs) ? Qs[row * S + j] : 0.0;
+        const double b1 = (row >= 0 && 16 + j < rs) ? Qs[row * S + 16 + j] : 0.0;
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {  // register v holds new row 4 v + lane / 16, entry lane & 15 of the tile
+        const int jr = 4 * v + kk;
+        if (jr >= nl) continue;
+        const int row = jr < np_ ? p0 + jr : q0 + (jr - np_);
+        if (j < rs) Qs[row * S + j] = acc0[v];
+        if (16 + j < rs) Qs[row * S + 16 + j] = acc1[v];
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * S; e += BJ_APPLY_THREADS) {
+    const int c = e / S, rl = e - c * S;
+    if (rl < rs) Q[(i64)n * c + r0 + rl] = Qs[e];
+  }
+}
+
+constexpr size_t kBjLds = 159 * 1024;
+
+// rows per block of a large sector: the pair's rows and its T fit the LDS
 int sect_bj_rows(int n, int len) {
-  const size_t lim = 159 * 1024;
-  // eight rows per block, one row pair per wave of the workgroup at every
-  // step; fewer where the pair's rows and Q columns would not fit the LDS
-  int bs = BJ_THREADS / 64;
-  while (bs > 1 && (size_t)2 * bs * (len + n) * sizeof(double) > lim) --bs;
+  (void)n;
+  int bs = BJ_MAXBS;
+  while (bs > 1 && (size_t)(2 * bs * (size_t)len + 4 * bs * bs) * sizeof(double) > kBjLds) --bs;
   return bs;
 }
 
-// The block Jacobi of the large sectors gl (sector numbers), their (n, len):
-// factorises them in place in X (workspace for Q allocated here) and writes
-// their triplets like k_sect_svd.
-void k_sect_svd_blocks(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn,
-                       const std::vector<int>& ll, double* X, const i64* xoff, const int* R, const int* Cc,
-                       const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                       double* SVo) {
+// Q entries per workgroup of the apply launch (at most 32, two matrix-core
+// tiles): n rows of S entries and one round's T matrices in LDS
+static size_t sect_bj_apply_lds(int n, int S, int B, int bs) {
+  return ((size_t)n * S + (size_t)(B / 2) * 4 * bs * bs) * sizeof(double);
+}
+static int sect_bj_slice(int n, int B, int bs) {
+  int S = 32;
+  while (S > 1 && sect_bj_apply_lds(n, S, B, bs) > kBjLds) S >>= 1;
+  return S;
+}
+
+// The block Jacobi of the sectors gl (indices into xoff), their (n rows, len):
+// orthogonalises their rows in place in X and leaves the accumulated rotations
+// (Q[r + n c], c = the row it follows) in run.Qw at run.hs[b].qoff.
+void bj_jacobi(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn, const std::vector<int>& ll,
+               double* X, const i64* xoff, BjRun& run) {
   const int nb = (int)gl.size();
   if (!nb) return;
-  std::vector<BjSector> hs(nb);
-  i64 qtot = 0;
-  int maxpairs = 1, maxrounds = 0, maxn = 1;
-  size_t lds = 0;
+  std::vector<BjSector>& hs = run.hs;
+  hs.assign(nb, BjSector{});
+  i64 qtot = 0, ttot = 0;
+  int ftot = 0, slices = 0;
+  int maxpairs = 1, maxrounds = 0;
+  size_t lds = 0, alds = 0;
   for (int b = 0; b < nb; ++b) {
     BjSector& s = hs[b];
     s.g = gl[b];
     s.n = nn[b];
     s.len = ll[b];
+    if (s.n > (1 << 20) || s.len > (1 << 24))
+      fail(SB_E_LIMIT, "sector SVD: a sector matrix beyond 2^20 rows or 2^24 columns");
     s.bs = sect_bj_rows(s.n, s.len);
+    if (2 * (size_t)s.bs * s.len * sizeof(double) + 4 * (size_t)s.bs * s.bs * sizeof(double) > kBjLds)
+      fail(SB_E_LIMIT, "sector SVD: a sector's rows are too long for the LDS of a workgroup");
     s.B = (s.n + s.bs - 1) / s.bs;
     s.B += s.B & 1;
     if (s.B < 2) s.B = 2;
     s.qoff = qtot;
     qtot += (i64)s.n * s.n;
+    s.toff = ttot;
+    ttot += (i64)(s.B - 1) * (s.B / 2) * 4 * s.bs * s.bs;
+    s.foff = ftot;
+    ftot += (s.B - 1) * (s.B / 2);
+    s.S = sect_bj_slice(s.n, s.B, s.bs);
+    if (sect_bj_apply_lds(s.n, s.S, s.B, s.bs) > kBjLds)
+      fail(SB_E_LIMIT, "sector SVD: a sector's rotation rows and round matrices are too large for the LDS of a "
+                       "workgroup (more than about 1200 rows to orthogonalise)");
+    s.slice0 = slices;
+    slices += (s.n + s.S - 1) / s.S;
     maxpairs = std::max(maxpairs, s.B / 2);
     maxrounds = std::max(maxrounds, s.B - 1);
-    maxn = std::max(maxn, s.n);
-    lds = std::max(lds, (size_t)2 * s.bs * (s.len + s.n) * sizeof(double));
+    lds = std::max(lds, (size_t)(2 * s.bs * (size_t)s.len + 4 * s.bs * s.bs) * sizeof(double));
+    alds = std::max(alds, sect_bj_apply_lds(s.n, s.S, s.B, s.bs));
   }
-  DBuf<BjSector> d_sec(ctx, nb);
-  HIP_CHECK(hipMemcpyAsync(d_sec.get(), hs.data(), nb * sizeof(BjSector), hipMemcpyHostToDevice, ctx->stream));
-  DBuf<double> Qw(ctx, (size_t)qtot), nul(ctx, nb);
-  DBuf<int> flag(ctx, nb);
-  hipLaunchKernelGGL(sect_bj_init_kernel, dim3(nb), dim3(SS_THREADS), 0, ctx->stream, d_sec.get(), X, xoff, Qw.get(),
-                     nul.get(), flag.get());
+  run.d_sec.alloc(ctx, nb);
+  HIP_CHECK(hipMemcpyAsync(run.d_sec.get(), hs.data(), nb * sizeof(BjSector), hipMemcpyHostToDevice, ctx->stream));
+  run.Qw.alloc(ctx, (size_t)std::max<i64>(qtot, 1));
+  DBuf<double> nul(ctx, nb), Tb(ctx, (size_t)std::max<i64>(ttot, 1));
+  DBuf<int> flag(ctx, nb), tflag(ctx, (size_t)std::max(ftot, 1));
+  hipLaunchKernelGGL(sect_bj_init_kernel, dim3(nb), dim3(SS_THREADS), 0, ctx->stream, run.d_sec.get(), X, xoff,
+                     run.Qw.get(), nul.get(), flag.get());
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_round_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)alds));
+  // the later rounds with one block's rows in registers when every row fits 64 CH doubles per wave
+  int maxlen = 0;
+  for (auto& s : hs) maxlen = std::max(maxlen, s.len);
+  const int ch = maxlen <= 256 ? 4 : maxlen <= 512 ? 8 : maxlen <= 1024 ? 16 : 0;
+  size_t clds = 0;
+  for (auto& s : hs) clds = std::max(clds, (size_t)(s.bs * (size_t)s.len + 4 * s.bs * s.bs) * sizeof(double));
+  if (ch == 4) HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_cross_kernel<4>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
+  if (ch == 8) HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_cross_kernel<8>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
+  if (ch == 16) HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_cross_kernel<16>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
   std::vector<int> hf(nb);
   for (int sweep = 0; sweep < 40; ++sweep) {
+    // every (round, pair) flag of the sweep is written by its round launch
     for (int rd = 0; rd < maxrounds; ++rd) {
-      hipLaunchKernelGGL(sect_bj_round_kernel, dim3((unsigned)(nb * maxpairs)), dim3(BJ_THREADS), lds, ctx->stream,
-                         d_sec.get(), maxpairs, rd, X, xoff, Qw.get(), nul.get(), flag.get());
+      const dim3 grid((unsigned)(nb * maxpairs)), block(BJ_THREADS);
+      if (rd > 0 && ch == 4)
+        hipLaunchKernelGGL(sect_bj_cross_kernel<4>, grid, block, clds, ctx->stream, run.d_sec.get(), maxpairs, rd, X,
+                           xoff, nul.get(), flag.get(), Tb.get(), tflag.get());
+      else if (rd > 0 && ch == 8)
+        hipLaunchKernelGGL(sect_bj_cross_kernel<8>, grid, block, clds, ctx->stream, run.d_sec.get(), maxpairs, rd, X,
+                           xoff, nul.get(), flag.get(), Tb.get(), tflag.get());
+      else if (rd > 0 && ch == 16)
+        hipLaunchKernelGGL(sect_bj_cross_kernel<16>, grid, block, clds, ctx->stream, run.d_sec.get(), maxpairs, rd,
+                           X, xoff, nul.get(), flag.get(), Tb.get(), tflag.get());
+      else
+        hipLaunchKernelGGL(sect_bj_round_kernel, grid, block, lds, ctx->stream, run.d_sec.get(), maxpairs, rd, X,
+                           xoff, nul.get(), flag.get(), Tb.get(), tflag.get());
       HIP_CHECK(hipGetLastError());
     }
+    hipLaunchKernelGGL(sect_bj_apply_kernel, dim3((unsigned)slices), dim3(BJ_APPLY_THREADS), alds, ctx->stream,
+                       run.d_sec.get(), nb, run.Qw.get(), Tb.get(), tflag.get());
+    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(hf.data(), flag.get(), nb * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
     bool any = false;
@@ -579,13 +858,446 @@ void k_sect_svd_blocks(Ctx* ctx, const std::vector<int>& gl, const std::vector<i
     if (!any) break;
     k_zero(ctx, flag.get(), nb * sizeof(int));
   }
-  const size_t flds = (size_t)maxn * (sizeof(double) + sizeof(int));
-  HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_finish_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)flds));
-  hipLaunchKernelGGL(sect_bj_finish_kernel, dim3(nb), dim3(SS_THREADS), flds, ctx->stream, d_sec.get(), X, xoff,
-                     Qw.get(), R, Cc, trans, uoff, soff, sgoff, sig, Uo, SVo);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the round workspaces go out of scope
+}
+
+// ------------------------------------------------ QR-preconditioned sector SVD
+// Sectors solved by block Jacobi are first factorised M P = Q1 R (Householder
+// QR with column pivoting, LAPACK's dgeqp3 conventions: the column of largest
+// remaining norm first, ties to the lowest index, norms downdated and
+// recomputed when cancellation leaves less than sqrt(eps) of them).  The
+// factorisation stops at the numerical rank r, where every remaining column
+// has norm <= 1e-15 |M|_F (the threshold below which the Jacobi rotations
+// leave a row alone).  The rows of R (r of them, length C) are then
+// orthogonalised by the block Jacobi: R's rows are graded, which halves the
+// sweeps, and only r <= min(R, C) rows take part.  With R = Qacc Rf:
+// M = Q1 diag(Qacc, I) [Rf; 0] P^T, so U = Q1 diag(Qacc, I) (its first
+// min(R, C) columns, orthonormal also beyond the rank) and
+// sigma V^T = [Rf; 0] P^T; the triplets beyond the rank have sigma = 0.
+//
+// One launch per reflector over all sectors, each workgroup on a chunk of
+// QP_CHUNK columns (one wave per column): every workgroup of a sector picks
+// the same pivot and forms the same reflector from the pivot column (the same
+// reductions in the same order), applies it to its own columns and downdates
+// their norms; the sector's first workgroup stores the reflector.  Columns are
+// not moved: a column keeps its place, its position in the pivot order is
+// recorded, and the pivot column is left as it is from its step on (its rows
+// above the step hold its column of R, the diagonal entry is kept apart).
+struct QpSector {
+  int g;           // sector number (xoff, uoff, soff, sgoff)
+  int R, C, k;     // rows, columns, min(R, C)
+  int wg0;         // first workgroup of its column chunks in the step launches
+  i64 toff;        // offset of its tau / beta (k)
+  i64 poff;        // offset of its per-column arrays (C): norms, position, perm
+  i64 voff;        // offset of its reflectors V (R x k, column-major, unit diagonal)
+  i64 yoff;        // offset of its Jacobi rows Y (r x C, row-major)
+  i64 ufoff;       // offset of its U (R x k, column-major)
+  i64 qoff;        // offset of its accumulated rotation (r x r) in the Jacobi workspace, -1: identity
+  double thr;      // 1e-15 |M|_F
+  int r;           // numerical rank (k until the factorisation stops)
+};
+
+constexpr int QP_THREADS = 1024;
+constexpr int QP_CHUNK = QP_THREADS / 64;  // columns per workgroup of a step launch
+
+__device__ inline int qp_find(const QpSector* sec, int nsec) {
+  int b = 0;
+  while (b + 1 < nsec && sec[b + 1].wg0 <= (int)blockIdx.x) ++b;
+  return b;
+}
+
+// column norms, positions -1 (not yet a pivot)
+__global__ __launch_bounds__(QP_THREADS) void sect_qp_init_kernel(const QpSector* __restrict__ sec, int nsec,
+                                                                 const double* __restrict__ X,
+                                                                 const i64* __restrict__ xoff,
+                                                                 double* __restrict__ vn, int* __restrict__ pos) {
+  const QpSector s = sec[qp_find(sec, nsec)];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = ((int)blockIdx.x - s.wg0) * QP_CHUNK + wave;
+  if (j >= s.C) return;
+  const double* col = X + xoff[s.g] + (i64)j * s.R;
+  double a = 0;
+  for (int i = lane; i < s.R; i += 64) a = fma(col[i], col[i], a);
+  a = sqrt(wave_sum(a));
+  if (lane == 0) {
+    vn[4 * s.poff + j] = a;
+    vn[4 * s.poff + s.C + j] = a;
+    pos[s.poff + j] = -1;
+  }
+}
+
+// |M|_F (from the column norms) and the rank threshold; one workgroup per sector
+__global__ __launch_bounds__(SS_THREADS) void sect_qp_thr_kernel(QpSector* __restrict__ sec,
+                                                                const double* __restrict__ vn) {
+  __shared__ double part[SS_THREADS / 64];
+  QpSector& s = sec[blockIdx.x];
+  double f = 0;
+  for (int j = threadIdx.x; j < s.C; j += SS_THREADS) f = fma(vn[4 * s.poff + j], vn[4 * s.poff + j], f);
+  f = wave_sum(f);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = f;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < SS_THREADS / 64; ++w) t += part[w];
+    s.thr = 1e-15 * sqrt(t);
+    s.r = s.k;
+  }
+}
+
+// step i of every sector's factorisation
+__global__ __launch_bounds__(QP_THREADS) void sect_qp_step_kernel(QpSector* __restrict__ sec, int nsec, int i,
+                                                                 double* __restrict__ X, const i64* __restrict__ xoff,
+                                                                 double* __restrict__ vn, int* __restrict__ pos,
+                                                                 double* __restrict__ V, double* __restrict__ tb) {
+  __shared__ double redv[QP_THREADS / 64];
+  __shared__ int redi[QP_THREADS / 64];
+  const int b = qp_find(sec, nsec);
+  const QpSector s = sec[b];
+  if (i >= s.k || i > s.r) return;  // finished (or stopped at its rank in an earlier step)
+  const int R = s.R, C = s.C;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = QP_THREADS / 64;
+  double* M = X + xoff[s.g];
+  // the norms of step i are read from buffer i % 2 and the downdated ones
+  // written to the other, so that no workgroup reads a norm another one has
+  // already changed in this launch; a column made a pivot in this launch
+  // (position i) still counts as a candidate here, pivots of earlier steps not
+  const double* vn1 = vn + 4 * s.poff + 2 * (i & 1) * C;
+  const double* vn2 = vn1 + C;
+  double* wn1 = vn + 4 * s.poff + 2 * ((i + 1) & 1) * C;
+  double* wn2 = wn1 + C;
+  int* ps = pos + s.poff;
+  // the pivot: largest norm among the columns not yet pivots, the first of equal ones
+  double bv = -1.0;
+  int bi = C;
+  for (int j = tid; j < C; j += QP_THREADS)
+    if ((ps[j] < 0 || ps[j] >= i) && vn1[j] > bv) {
+      bv = vn1[j];
+      bi = j;
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ov > bv || (ov == bv && oi < bi)) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  if (lane == 0) {
+    redv[wave] = bv;
+    redi[wave] = bi;
+  }
+  __syncthreads();
+  bv = redv[0];
+  bi = redi[0];
+  for (int w = 1; w < nwave; ++w)
+    if (redv[w] > bv || (redv[w] == bv && redi[w] < bi)) {
+      bv = redv[w];
+      bi = redi[w];
+    }
+  __syncthreads();
+  const bool first = (int)blockIdx.x == s.wg0;
+  if (!(bv > s.thr)) {  // every remaining column is zero to working precision: rank i
+    if (first && tid == 0) sec[b].r = i;
+    return;
+  }
+  const int c = bi;
+  const double* pc = M + (i64)c * R;
+  // the reflector of the pivot column's rows i.. (dlarfg), formed alike in every workgroup
+  double x2 = 0;
+  for (int e = i + 1 + tid; e < R; e += QP_THREADS) x2 = fma(pc[e], pc[e], x2);
+  x2 = wave_sum(x2);
+  if (lane == 0) redv[wave] = x2;
+  __syncthreads();
+  x2 = 0;
+  for (int w = 0; w < nwave; ++w) x2 += redv[w];
+  const double alpha = pc[i];
+  double ti = 0.0, sc = 0.0, beta = alpha;
+  if (x2 != 0.0) {
+    beta = -copysign(hypot(alpha, sqrt(x2)), alpha);
+    ti = (beta - alpha) / beta;
+    sc = 1.0 / (alpha - beta);
+  }
+  if (first) {
+    double* v = V + s.voff + (i64)i * R;
+    for (int e = tid; e < R; e += QP_THREADS) v[e] = e < i ? 0.0 : e == i ? 1.0 : (ti != 0.0 ? pc[e] * sc : 0.0);
+    if (tid == 0) {
+      tb[s.toff + i] = ti;
+      tb[s.toff + s.k + i] = beta;
+      ps[c] = i;
+    }
+  }
+  // H_i to this workgroup's columns (not pivots, not the pivot of this step), then their norms
+  const int j = ((int)blockIdx.x - s.wg0) * QP_CHUNK + wave;
+  if (j >= C || j == c || (ps[j] >= 0 && ps[j] < i)) return;
+  double* y = M + (i64)j * R;
+  if (ti != 0.0) {
+    double d = lane == 0 ? y[i] : 0.0;
+    for (int e = i + 1 + lane; e < R; e += 64) d = fma(pc[e] * sc, y[e], d);
+    d = wave_sum(d);
+    const double f = ti * d;
+    if (lane == 0) y[i] -= f;
+    for (int e = i + 1 + lane; e < R; e += 64) y[e] -= f * (pc[e] * sc);
+  }
+  const double n1 = vn1[j], n2 = vn2[j];
+  double o1 = 0.0, o2 = 0.0;
+  if (n1 != 0.0) {  // every lane reads the same values: the choice is wave-uniform
+    const double yi = __shfl(lane == 0 ? y[i] : 0.0, 0, 64);
+    double t = fabs(yi) / n1;
+    t = fmax(0.0, (1.0 + t) * (1.0 - t));
+    const double q = n1 / n2;
+    if (t * q * q <= sqrt(1.1102230246251565e-16)) {
+      double a = 0;
+      for (int e = i + 1 + lane; e < R; e += 64) a = fma(y[e], y[e], a);
+      o1 = o2 = sqrt(wave_sum(a));
+    } else {
+      o1 = n1 * sqrt(t);
+      o2 = n2;
+    }
+  }
+  if (lane == 0) {
+    wn1[j] = o1;
+    wn2[j] = o2;
+  }
+}
+
+// positions of the columns that were never pivots (r, r+1, .. in column order),
+// perm[position] = column; one workgroup per sector
+__global__ void sect_qp_perm_kernel(const QpSector* __restrict__ sec, int* __restrict__ pos,
+                                    int* __restrict__ perm) {
+  const QpSector s = sec[blockIdx.x];
+  int* ps = pos + s.poff;
+  int* pm = perm + s.poff;
+  if (threadIdx.x == 0) {
+    int nxt = s.r;
+    for (int j = 0; j < s.C; ++j)
+      if (ps[j] < 0) ps[j] = nxt++;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < s.C; j += blockDim.x) pm[ps[j]] = j;
+}
+
+// Y (r x C, row-major, columns in pivot order): row a of R
+__global__ void sect_qp_rows_kernel(const QpSector* __restrict__ sec, const double* __restrict__ X,
+                                    const i64* __restrict__ xoff, const int* __restrict__ perm,
+                                    const double* __restrict__ tb, double* __restrict__ Y) {
+  const QpSector s = sec[blockIdx.y];
+  const i64 n = (i64)s.r * s.C;
+  const double* M = X + xoff[s.g];
+  const int* pm = perm + s.poff;
+  double* y = Y + s.yoff;
+  for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (i64)gridDim.x * blockDim.x) {
+    const int a = (int)(e / s.C), p = (int)(e - (i64)a * s.C);
+    y[e] = p < a ? 0.0 : p == a ? tb[s.toff + s.k + a] : M[(i64)pm[p] * s.R + a];
+  }
+}
+
+// U (R x k, column-major) = Q1 diag(Qacc, I): one wave per column, QP_BT_COLS
+// columns per workgroup, the column in LDS; the reflectors H_{r-1} .. H_0
+// applied in turn (each column on its own: no barrier between reflectors)
+constexpr int QP_BT_COLS = 8;
+
+__global__ __launch_bounds__(64 * QP_BT_COLS) void sect_qp_back_kernel(const QpSector* __restrict__ sec, int nsec,
+                                                                      const int* __restrict__ wg0,
+                                                                      const double* __restrict__ Vr,
+                                                                      const double* __restrict__ tau,
+                                                                      const double* __restrict__ Qw,
+                                                                      double* __restrict__ Uf) {
+  extern __shared__ double lds[];
+  int b = 0;
+  while (b + 1 < nsec && wg0[b + 1] <= (int)blockIdx.x) ++b;
+  const QpSector s = sec[b];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = ((int)blockIdx.x - wg0[b]) * QP_BT_COLS + wave;
+  if (c >= s.k) return;
+  const int R = s.R, r = s.r;
+  double* u = lds + (i64)wave * R;
+  for (int e = lane; e < R; e += 64) {
+    double v0;
+    if (c < r) v0 = e < r ? (s.qoff >= 0 ? Qw[s.qoff + e + (i64)r * c] : (e == c ? 1.0 : 0.0)) : 0.0;
+    else v0 = e == c ? 1.0 : 0.0;
+    u[e] = v0;
+  }
+  const double* tu = tau + s.toff;
+  for (int j = r - 1; j >= 0; --j) {
+    const double tj = tu[j];
+    if (tj == 0.0) continue;
+    const double* v = Vr + s.voff + (i64)j * R;  // v = [0..; 1; v[j+1..]]
+    double d = lane == 0 ? u[j] : 0.0;
+    for (int e = j + 1 + lane; e < R; e += 64) d = fma(v[e], u[e], d);
+    d = wave_sum(d);
+    const double f = tj * d;
+    if (lane == 0) u[j] -= f;
+    for (int e = j + 1 + lane; e < R; e += 64) u[e] -= f * v[e];
+  }
+  double* out = Uf + s.ufoff + (i64)R * c;
+  for (int e = lane; e < R; e += 64) out[e] = u[e];
+}
+
+// sigma (descending, stable), U columns and sigma V^T rows of a sector
+__global__ __launch_bounds__(SS_THREADS) void sect_qp_emit_kernel(const QpSector* __restrict__ sec,
+                                                                 const double* __restrict__ Y,
+                                                                 const double* __restrict__ Uf,
+                                                                 const int* __restrict__ perm,
+                                                                 const i64* __restrict__ uoff,
+                                                                 const i64* __restrict__ soff,
+                                                                 const i64* __restrict__ sgoff,
+                                                                 double* __restrict__ sig, double* __restrict__ Uo,
+                                                                 double* __restrict__ SVo) {
+  extern __shared__ double lds[];
+  const QpSector s = sec[blockIdx.x];
+  const int R = s.R, C = s.C, k = s.k, r = s.r;
+  double* nrm = lds;
+  int* ord = reinterpret_cast<int*>(nrm + k);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = SS_THREADS / 64;
+  const double* y = Y + s.yoff;
+  for (int i = wave; i < k; i += nwave) {
+    double a = 0;
+    if (i < r)
+      for (int j = lane; j < C; j += 64) a = fma(y[(i64)i * C + j], y[(i64)i * C + j], a);
+    a = wave_sum(a);
+    if (lane == 0) nrm[i] = sqrt(a);
+  }
+  __syncthreads();
+  for (int i = tid; i < k; i += SS_THREADS) {  // stable descending rank (a NaN ranks as -inf)
+    const double v = isnan(nrm[i]) ? -INFINITY : nrm[i];
+    int rank = 0;
+    for (int j = 0; j < k; ++j) {
+      const double w = isnan(nrm[j]) ? -INFINITY : nrm[j];
+      rank += (w > v) || (w == v && j < i);
+    }
+    ord[rank] = i;
+  }
+  __syncthreads();
+  double* sg = sig + sgoff[s.g];
+  for (int c = tid; c < k; c += SS_THREADS) sg[c] = nrm[ord[c]];
+  const int* pm = perm + s.poff;
+  double* Ug = Uo + uoff[s.g];
+  double* Sg = SVo + soff[s.g];
+  for (int c = wave; c < k; c += nwave) {
+    const int o = ord[c];
+    const double* uc = Uf + s.ufoff + (i64)R * o;
+    for (int e = lane; e < R; e += 64) Ug[e + (i64)R * c] = uc[e];
+    double* sv = Sg + (i64)c * C;
+    if (o < r)
+      for (int j = lane; j < C; j += 64) sv[pm[j]] = y[(i64)o * C + j];
+    else
+      for (int j = lane; j < C; j += 64) sv[j] = 0.0;
+  }
+}
+
+// The sectors gl (sector numbers, R x C column-major at xoff): their triplets
+// like k_sect_svd, through the QR preconditioning and the block Jacobi.
+void k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& Rs, const std::vector<int>& Cs,
+                     double* X, const i64* xoff, const i64* uoff, const i64* soff, const i64* sgoff, double* sig,
+                     double* Uo, double* SVo) {
+  const int nb = (int)gl.size();
+  if (!nb) return;
+  std::vector<QpSector> hq(nb);
+  i64 ttot = 0, ptot = 0, vtot = 0, uftot = 0;
+  int maxk = 1, maxR = 1, wgs = 0;
+  for (int b = 0; b < nb; ++b) {
+    QpSector& q = hq[b];
+    q = QpSector{};
+    q.g = gl[b];
+    q.R = Rs[b];
+    q.C = Cs[b];
+    q.k = std::min(q.R, q.C);
+    q.wg0 = wgs;
+    wgs += (q.C + QP_CHUNK - 1) / QP_CHUNK;
+    q.toff = ttot;
+    ttot += 2 * (i64)q.k;
+    q.poff = ptot;
+    ptot += q.C;
+    q.voff = vtot;
+    vtot += (i64)q.R * q.k;
+    q.ufoff = uftot;
+    uftot += (i64)q.R * q.k;
+    q.qoff = -1;
+    q.r = q.k;
+    maxk = std::max(maxk, q.k);
+    maxR = std::max(maxR, q.R);
+  }
+  if ((size_t)QP_BT_COLS * maxR * sizeof(double) > kBjLds)
+    fail(SB_E_LIMIT, "sector SVD: a sector with more rows than the LDS of a workgroup holds for eight columns");
+  DBuf<QpSector> d_q(ctx, nb);
+  HIP_CHECK(hipMemcpyAsync(d_q.get(), hq.data(), nb * sizeof(QpSector), hipMemcpyHostToDevice, ctx->stream));
+  DBuf<double> tb(ctx, (size_t)std::max<i64>(ttot, 1)), vn(ctx, (size_t)std::max<i64>(4 * ptot, 1)),
+      V(ctx, (size_t)std::max<i64>(vtot, 1));
+  DBuf<int> pos(ctx, (size_t)std::max<i64>(ptot, 1)), perm(ctx, (size_t)std::max<i64>(ptot, 1));
+  hipLaunchKernelGGL(sect_qp_init_kernel, dim3((unsigned)wgs), dim3(QP_THREADS), 0, ctx->stream, d_q.get(), nb, X,
+                     xoff, vn.get(), pos.get());
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the workspace and the sector table go out of scope
+  hipLaunchKernelGGL(sect_qp_thr_kernel, dim3(nb), dim3(SS_THREADS), 0, ctx->stream, d_q.get(), vn.get());
+  HIP_CHECK(hipGetLastError());
+  // the steps, in batches between which the host reads the ranks: a sector
+  // that reached its rank stops, and so do the launches once all have
+  std::vector<QpSector> hr(nb);
+  for (int i0 = 0; i0 < maxk; i0 += 32) {
+    for (int i = i0; i < std::min(maxk, i0 + 32); ++i) {
+      hipLaunchKernelGGL(sect_qp_step_kernel, dim3((unsigned)wgs), dim3(QP_THREADS), 0, ctx->stream, d_q.get(), nb,
+                         i, X, xoff, vn.get(), pos.get(), V.get(), tb.get());
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipMemcpyAsync(hr.data(), d_q.get(), nb * sizeof(QpSector), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    bool more = false;
+    for (int b = 0; b < nb; ++b) more = more || (hr[b].r > i0 + 31 && hr[b].k > i0 + 32);
+    if (!more) break;
+  }
+  // the Jacobi rows of every sector of rank 2 or more
+  i64 ytot = 0;
+  std::vector<i64> hyoff;
+  std::vector<int> jg, jn, jl, jb;
+  for (int b = 0; b < nb; ++b) {
+    QpSector& q = hq[b];
+    q.r = hr[b].r;
+    q.thr = hr[b].thr;
+    if (q.r < 0 || q.r > q.k) fail(SB_E_DEVICE, "sector SVD: the QR factorisation returned an invalid rank");
+    q.yoff = ytot;
+    ytot += (i64)q.r * q.C;
+    if (q.r >= 2) {
+      jg.push_back((int)jb.size());
+      jn.push_back(q.r);
+      jl.push_back(q.C);
+      jb.push_back(b);
+      hyoff.push_back(q.yoff);
+    }
+  }
+  HIP_CHECK(hipMemcpyAsync(d_q.get(), hq.data(), nb * sizeof(QpSector), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(sect_qp_perm_kernel, dim3(nb), dim3(256), 0, ctx->stream, d_q.get(), pos.get(), perm.get());
+  HIP_CHECK(hipGetLastError());
+  DBuf<double> Y(ctx, (size_t)std::max<i64>(ytot, 1));
+  hipLaunchKernelGGL(sect_qp_rows_kernel, dim3(64, nb), dim3(256), 0, ctx->stream, d_q.get(), X, xoff, perm.get(),
+                     tb.get(), Y.get());
+  HIP_CHECK(hipGetLastError());
+  BjRun run;
+  DBuf<i64> d_yoff;
+  if (!jb.empty()) {
+    d_yoff.alloc(ctx, hyoff.size());
+    HIP_CHECK(hipMemcpyAsync(d_yoff.get(), hyoff.data(), hyoff.size() * sizeof(i64), hipMemcpyHostToDevice,
+                             ctx->stream));
+    bj_jacobi(ctx, jg, jn, jl, Y.get(), d_yoff.get(), run);
+    for (size_t t = 0; t < jb.size(); ++t) hq[jb[t]].qoff = run.hs[t].qoff;
+    HIP_CHECK(hipMemcpyAsync(d_q.get(), hq.data(), nb * sizeof(QpSector), hipMemcpyHostToDevice, ctx->stream));
+  }
+  std::vector<int> wg0(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) wg0[b + 1] = wg0[b] + (hq[b].k + QP_BT_COLS - 1) / QP_BT_COLS;
+  DBuf<int> d_wg0(ctx, nb + 1);
+  HIP_CHECK(hipMemcpyAsync(d_wg0.get(), wg0.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  DBuf<double> Uf(ctx, (size_t)std::max<i64>(uftot, 1));
+  const size_t blds = (size_t)QP_BT_COLS * maxR * sizeof(double);
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_qp_back_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)blds));
+  hipLaunchKernelGGL(sect_qp_back_kernel, dim3((unsigned)wg0[nb]), dim3(64 * QP_BT_COLS), blds, ctx->stream,
+                     d_q.get(), nb, d_wg0.get(), V.get(), tb.get(), jb.empty() ? nullptr : run.Qw.get(), Uf.get());
+  HIP_CHECK(hipGetLastError());
+  const size_t elds = (size_t)maxk * (sizeof(double) + sizeof(int));
+  HIP_CHECK(hipFuncSetAttribute((const void*)sect_qp_emit_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)elds));
+  hipLaunchKernelGGL(sect_qp_emit_kernel, dim3(nb), dim3(SS_THREADS), elds, ctx->stream, d_q.get(), Y.get(),
+                     Uf.get(), perm.get(), uoff, soff, sgoff, sig, Uo, SVo);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the workspaces go out of scope
 }
 
 // factors of the kept triplets.  A block b: (il, sector g) -> [s1, l, mm]

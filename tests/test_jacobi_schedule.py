@@ -4,9 +4,13 @@ in round 0 a workgroup sweeps every pair of its rows, in later rounds only the
 pairs across its two blocks.  One-sided Jacobi converges when a sweep visits
 every row pair, and a step may rotate a row in one pair only.  The index
 formulas below are the kernel's; the test checks those two properties for
-every sector size the kernel meets in practice."""
+every sector size the kernel meets in practice.  The rotations of a
+workgroup are recorded as one matrix T per (round, pair) and applied to the
+accumulated rotation Q once per sweep (sect_bj_apply_kernel, in round order):
+a model of that scheme must still give X_0 = Q X_final with Q orthogonal."""
 from collections import Counter
 
+import numpy as np
 import pytest
 
 
@@ -56,7 +60,7 @@ def sweep_steps(n, bs):
                     yield rd, pi, step
 
 
-@pytest.mark.parametrize("bs", [1, 2, 3, 5, 7, 8])
+@pytest.mark.parametrize("bs", [1, 2, 3, 5, 7, 8, 11, 16])
 def test_every_pair_once_per_sweep(bs):
     for n in range(2, 160):
         seen = Counter()
@@ -76,3 +80,70 @@ def test_sequential_depth_148_rows():
     steps = Counter((rd, pi) for rd, pi, _ in sweep_steps(148, 8))
     depth = sum(max(c for (r, _), c in steps.items() if r == rd) for rd in {r for r, _ in steps})
     assert depth == 15 + 18 * 8
+
+
+def rounds(n, bs):
+    """per round: the list of (pi, rows of the pair in the kernel's local order, steps)"""
+    B = blocks(n, bs)
+    out = {}
+    for rd, pi, step in sweep_steps(n, bs):
+        out.setdefault(rd, {}).setdefault(pi, []).append(step)
+    return B, out
+
+
+def pair_rows(n, bs, B, rd, pi):
+    mB = B - 1
+    pb = 0 if pi == 0 else 1 + (rd + pi) % mB
+    qb = 1 + rd if pi == 0 else 1 + (rd - pi + mB) % mB
+    rows = list(range(pb * bs, min(n, pb * bs + bs))) + list(range(qb * bs, min(n, qb * bs + bs)))
+    return rows
+
+
+@pytest.mark.parametrize("n,bs,ncol", [(2, 16, 5), (17, 16, 20), (37, 5, 37), (40, 16, 45), (64, 16, 70), (23, 3, 30)])
+def test_recorded_rotations_rebuild_q(n, bs, ncol):
+    rng = np.random.default_rng(n * 100 + bs)
+    X0 = rng.standard_normal((n, ncol)) * np.exp(-np.arange(n) / 7.0)[:, None]
+    X = X0.copy()
+    Q = np.eye(n)  # Q[:, c] rotates like row c of X
+    tol = max(1e-15, 2 * ncol * 2.0 ** -53)
+    null2 = 1e-30 * (X0 ** 2).sum()  # rows the kernel treats as zero (rows <= columns here, as in the kernel)
+    for sweep in range(40):
+        B, rnd = rounds(n, bs)
+        Ts = {}
+        rotated = False
+        for rd in sorted(rnd):
+            for pi, steps in rnd[rd].items():
+                rows = pair_rows(n, bs, B, rd, pi)
+                loc = {r: l for l, r in enumerate(rows)}
+                T = np.eye(2 * bs)
+                for step in steps:
+                    for p, q in step:
+                        x, y = X[p].copy(), X[q].copy()
+                        a, b, c = x @ x, y @ y, x @ y
+                        if c == 0 or a <= null2 or b <= null2 or abs(c) <= tol * np.sqrt(a) * np.sqrt(b):
+                            continue
+                        zeta = (b - a) / (2 * c)
+                        t = (1.0 if zeta >= 0 else -1.0) / (abs(zeta) + np.sqrt(1 + zeta * zeta))
+                        cs = 1 / np.sqrt(1 + t * t)
+                        sn = cs * t
+                        X[p], X[q] = cs * x - sn * y, sn * x + cs * y
+                        tp, tq = T[loc[p]].copy(), T[loc[q]].copy()
+                        T[loc[p]], T[loc[q]] = cs * tp - sn * tq, sn * tp + cs * tq
+                        rotated = True
+                Ts[(rd, pi)] = (rows, T)
+        # the apply launch: every round in order, each pair's rows of Q^T get T times the old ones
+        Qt = Q.T.copy()
+        for rd in sorted(rnd):
+            for pi in rnd[rd]:
+                rows, T = Ts[(rd, pi)]
+                nl = len(rows)
+                Qt[rows] = T[:nl, :nl] @ Qt[rows]
+        Q = Qt.T
+        if not rotated:
+            break
+    np.testing.assert_allclose(Q @ X, X0, atol=1e-12 * np.abs(X0).max())
+    np.testing.assert_allclose(Q.T @ Q, np.eye(n), atol=1e-12)
+    G = X @ X.T
+    off = G - np.diag(np.diag(G))
+    d = np.sqrt(np.diag(G))
+    assert np.all(np.abs(off) <= 1e-12 * np.outer(d, d) + 1e-300)
