@@ -463,9 +463,14 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
   pl.b_bytes += 8.0 * TB.n * TB.blksize;
   pl.c_bytes += 8.0 * ng * C.blksize;
   if (!ng) return;
-  // C's boxes: unknown payload until the first execution clears it
+  // C's fresh payload is zeroed by one contiguous fill and its boxes start
+  // empty, so the first execution clears nothing (a clear of stale entries
+  // walks every padded block element by element)
   C.bflag.alloc(ctx, ng);
   C.box.alloc(ctx, (size_t)(ng * C.N2 ? ng * C.N2 : 1));
+  k_zero(ctx, C.data.get(), (size_t)ng * C.blksize * sizeof(double));
+  k_zero(ctx, C.box.get(), (size_t)ng * C.N2 * sizeof(i32));
+  C.box_version = C.version;
   // pairs in group order: tensor rows and payload offsets of their blocks
   DBuf<i32> ar(ctx, np ? np : 1), br(ctx, np ? np : 1);
   if (np) {
@@ -573,7 +578,6 @@ static void shared_bs_bs(Plan& pl, Tensor& C, const Labels& labelsC, Oper& A, Op
   } else {
     ensure_box(ctx, TA);
     ensure_box(ctx, TB);
-    k_fill_rows(ctx, C.box.get(), ng, C.N2, cfull.data());
     box_prepare(ctx, g, TA.box.get(), TA.bflag.get(), TB.box.get(), TB.bflag.get(), C.box.get(), C.bflag.get(),
                 C.data.get());
     C.box_version = C.version;  // entries outside the boxes are zero now

@@ -130,10 +130,22 @@ __global__ void zero_kernel(unsigned* p, i64 nw) {
   i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nw) p[i] = 0u;
 }
+__global__ void zero16_kernel(uint4* p, i64 n) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) p[i] = z;
+}
 void k_zero(Ctx* ctx, void* p, size_t bytes) {
   // a fill kernel rather than hipMemsetAsync, so it can be captured in a graph
   if (!bytes) return;
   SB_ASSERT(bytes % 4 == 0, "k_zero: size must be a multiple of 4 bytes");
+  if (bytes % 16 == 0 && reinterpret_cast<uintptr_t>(p) % 16 == 0 && bytes >= (1u << 20)) {
+    // large fills: 16-byte stores, a grid of a few waves per CU striding the buffer
+    const i64 n = (i64)(bytes / 16);
+    const i64 grid = std::min<i64>((n + 255) / 256, (i64)ctx->num_cu * 8);
+    hipLaunchKernelGGL(zero16_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, (uint4*)p, n);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   i64 nw = (i64)(bytes / 4);
   LAUNCH(zero_kernel, nw, (unsigned*)p, nw);
 }
