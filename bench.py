@@ -102,6 +102,8 @@ def parse():
     ap.add_argument("--serve5-plain-json", default="",
                     help="reuse the plain run of an earlier --serve-config5 output (same requests) instead of "
                          "running it")
+    ap.add_argument("--only", default="", choices=["", "batch"],
+                    help="run one section alone (batch: the batching configuration, for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
     return ap.parse_args()
@@ -738,6 +740,17 @@ def coo_section(sb, a):
                 cpu_oracle_us=tcpu * 1e6, cpu_note="oracle C++ restatement, one host core")
 
 
+def calibration_copy(sb, mb):
+    """one plain copy kernel of mb MB each way (an identity permute), the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters"""
+    n = int(mb * 1e6 / 8)
+    T = sb.DenseTensor.from_numpy(np.zeros(n))
+    sb.synchronize()
+    T.permutedims([1])
+    sb.synchronize()
+    return n * 8
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -748,6 +761,12 @@ def main():
 
     if a.serve_config5 > 0:
         print(json.dumps(dict(serving_config5=serve5_section(sb, a))), flush=True)
+        return
+    if a.only == "batch":
+        res = dict(batch=batch_section(sb, a))
+        if a.calibrate_copy > 0:
+            res["calibration_copy_bytes"] = calibration_copy(sb, a.calibrate_copy)
+        print(json.dumps(res), flush=True)
         return
     flags = {"fp64": sb.PREC_FP64, "fp32": sb.PREC_FP32, "bf16x3": sb.PREC_BF16X3}[a.prec] | (a.tile << 16)
     tune = not a.no_autotune and a.tile == 0
@@ -1038,12 +1057,7 @@ def main():
         out["serve"] = serve_section(sb, a)
 
     if a.calibrate_copy > 0:
-        n = int(a.calibrate_copy * 1e6 / 8)
-        T = sb.DenseTensor.from_numpy(np.zeros(n))
-        sb.synchronize()
-        T.permutedims([1])  # identity permute = one plain copy kernel of n*8 bytes each way
-        sb.synchronize()
-        out["calibration_copy_bytes"] = n * 8
+        out["calibration_copy_bytes"] = calibration_copy(sb, a.calibrate_copy)
 
     print(json.dumps(out))
 
