@@ -1,4 +1,4 @@
-// Two-site H_eff*psi of DMRG as one plan with one fused kernel.
+// Two-site H_eff*psi of DMRG as one plan: a pack kernel and one fused kernel.
 //
 // ITensorMPS applies the two-site effective Hamiltonian as four contract!
 // calls, psi*L*W1*W2*R (product(ProjMPO, psi); test_product_dmrg_state.jl
@@ -11,12 +11,27 @@
 //   Hpsi[p', l', r'] = sum over link pairs (wl, wr), sector pairs (Ql, Qr):
 //       sum_p W12[wl,wr][p', p] sum_l L[wl, l', l] sum_r psi[p, l, r] R[wr, r', r]
 //
-// with p = (s1, s2) and W12[wl,wr][p', p] = sum_wm W1[wl, s1', s1, wm] W2[wm, s2', s2, wr]
-// (nonzero entries only).  Per sector pair the psi block is staged in LDS,
-// A = L_wl * psi per left link, then every output element accumulates
-// sum_r R_wr[r', r] (W12 row p' . A[:, l', r]) in registers.  No
-// intermediate touches HBM: the kernel reads psi, L, R (all small enough to
-// stay in the caches) and writes Hpsi once.
+// with p = (s1, s2) and W12[wl,wr][p', p] = sum_wm W1[wl, s1', s1, wm] W2[wm, s2', s2, wr].
+//
+// Physical rows.  A psi block (Ql, Qr) of a charge-conserving state holds
+// nonzeros only in the rows p whose charge closes q(Ql) + q(p) = q(Qr): 1, 2
+// or 4 of the 16 of a Hubbard two-site block.  Every execution first runs a
+// pack kernel that finds, per psi block, the rows p holding a nonzero and
+// copies them, 4 rows to a chunk, into matrix-core panels; the fused kernel
+// then multiplies only those rows, and skips a (pair, link pair) item whose
+// MPO block W12[wl,wr][:, rows] is zero (an output block of forbidden charge
+// is then only written with zeros).  Nothing is assumed about psi's values:
+// the rows are found from them at every execution.
+//
+// Per output block, per sector pair, per chunk of 4 rows: the psi chunk
+// (l x (p_c, r)) and the pair's L and R link slices are copied from their
+// panels into LDS (16-byte copies, strides chosen so that every matrix-core
+// operand read is free of bank conflicts); A_wl = L_wl^T psi (l' x (p_c, r))
+// on the matrix cores into LDS; per link pair the wave's output tiles take
+// Z = A R_wr^T ((p_c, l') x r') and then W12[:, rows] Z, with Z fed to the
+// matrix cores straight from its accumulator registers.  No atomics: each
+// output element is written once.  The environment panels are built once, at
+// planning (L, R and the MPO must keep their values, which the plan checks).
 //
 // Tensors (the layouts of the four-call chain's operands):
 //   psi [Ql, Qr | s1, s2, l, r]   L [Ql', Ql | wl, l', l]   R [Qr', Qr | wr, r', r]
@@ -27,6 +42,7 @@
 // reached by a pair, ids in the order its merge walk first meets them.
 #include <algorithm>
 #include <array>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -36,11 +52,8 @@ namespace sb {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-constexpr int kTilesPerWave = 4;                    // 16 x 16 output tiles a wave accumulates
-constexpr int kMaxTiles = kWaves * kTilesPerWave;   // of one output block
-constexpr size_t kMaxLds = 96 * 1024;               // dynamic LDS of one workgroup
+constexpr size_t kMaxLds = 160 * 1024;  // dynamic LDS of one workgroup (gfx950)
+constexpr size_t kLdsBudget = 80 * 1024;  // above this, A panels are computed one at a time
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -77,29 +90,68 @@ __global__ __launch_bounds__(256) void env_mask_kernel(const double* __restrict_
   }
 }
 
-struct HeffPair {
-  i64 psi, L, R;  // payload offsets of the psi, L and R blocks
-  int dl, dr;     // extents of l and r
-  int wlm, wrm;   // left links to apply, right links to stage from R
-};
-
-struct HeffK {
-  const double *psi, *L, *R;
-  double* out;
-  const int2* items;  // (output block, first tile) of this launch's workgroups
-  int tpw;            // tiles per wave of one workgroup (at most kTilesPerWave)
-  const i32 *o_dlp, *o_drp, *o_pb;
-  const i64* o_off;
-  const HeffPair* pairs;
-  const i32* lidx;      // link pair (wl, wr) -> its W12 tile, or -1
-  const double* w16;    // per link pair a 16 x 16 tile: W12 on the diagonal blocks of P x P
-  int lp, w, Dl, Dr;  // lp = log2(d1 d2)
-  int sPs, sAs, sLs, sRs;  // LDS elements of the psi, A, L and R tiles (this launch)
-};
-
 __host__ __device__ __forceinline__ int up4(int x) { return (x + 3) & ~3; }
 __host__ __device__ __forceinline__ int up16(int x) { return (x + 15) & ~15; }
-__device__ __forceinline__ size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// row stride (elements) of an LDS panel of n columns (n a multiple of 16):
+// 16 modulo 32 doubles or 64 floats, so that the 4 k-rows x 16 columns a
+// matrix-core operand read touches fall on distinct banks
+template <class T>
+__host__ __device__ __forceinline__ int lds_stride(int n) {
+  constexpr int mod = sizeof(T) == 8 ? 32 : 64;
+  return n + ((16 - n % mod) % mod + mod) % mod;
+}
+
+// one sector pair (Ql, Qr) of an output block
+struct HPair {
+  i64 pp;          // packed psi: the first chunk panel of the psi block
+  i64 lp, rp;      // packed L / R: the first link panel of the environment block
+  i32 blk;         // the psi block's row in the pack tables
+  i32 item0;       // first link item: (iL | iR << 5 | lk << 10, W12 columns of lk), sorted by iL
+  int16_t nitem;
+  uint8_t nl, nr;  // link panels of the L and R blocks
+  uint8_t dl, dr;  // reach of l and r
+  uint16_t pc[4];  // per L panel (nl <= 4): the W12 columns its items reach
+};
+
+// one psi block to pack: payload offset, panel base, reach
+struct PsiJob {
+  i64 src, dst;
+  i32 blk, dl, dr, pstr;
+};
+
+// one environment link panel to build: payload offset, panel base, link,
+// rows (inner bond index) and columns (outer bond index) inside the reach
+struct EnvJob {
+  i64 src, dst;
+  i32 link, rows, cols, stride;
+};
+
+template <class T>
+struct HK {
+  const T *pp, *lp, *rp, *w12;  // psi chunks, L and R panels, W12 [lk][p'][p]
+  const i32* pmask;             // per psi block: row mask | row count << 16
+  double* out;
+  const int4* jobs;   // per workgroup: one block (y == -2) or up to 4 blocks, one per wave (-1: none)
+  const i32 *o_dlp, *o_drp, *o_pb;
+  const i64* o_off;
+  const HPair* pairs;
+  const int2* items;
+  int Dl;
+  int apb;  // L panels whose A products LDS holds at once
+  // LDS region of a block (bytes; the entry list at 0): its link items (int2,
+  // copied at the start) and the A products; a wave on its own has its own
+  // region, wave_lds bytes apart
+  int off_it, off_as, wave_lds;
+  const i32* o_ib;     // per output block: its first link item (the items of a block are contiguous)
+};
+
+// one active (sector pair, row chunk) of an output block, listed in LDS at
+// the start of its team's work: the pair, the chunk, its rows p and the L
+// panels an active item uses
+struct HEnt {
+  int q, ch, cm, amask;
+};
 
 template <class T>
 struct Mfma;
@@ -112,6 +164,10 @@ struct Mfma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
+  // rows m of a Z tile as (row chunk position, l' offset), ordered so that
+  // accumulator register v holds l' offset v with the k index = position
+  static __device__ __forceinline__ int zpos(int m) { return m & 3; }
+  static __device__ __forceinline__ int zoff(int m) { return m >> 2; }
 };
 template <>
 struct Mfma<float> {
@@ -121,229 +177,384 @@ struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+  static __device__ __forceinline__ int zpos(int m) { return m >> 2; }
+  static __device__ __forceinline__ int zoff(int m) { return m & 3; }
 };
 
-// One workgroup per output block (Ql', Qr'), its elements as rows (p', l')
-// by columns r' in 16 x 16 tiles on the matrix cores, up to 4 tiles per wave
-// held in registers over all of the block's work.  Per sector pair: the psi
-// block as a (l x (p, r)) matrix, and the L and R slices of the pair's links,
-// staged in LDS (zero padded) with one wait for all their loads; per left
-// link A = L_wl (l' x l) * psi (l x (p, r)) on the matrix cores into LDS; per
-// right link the output tiles accumulate V (r) * R_wr^T with
-// V[(p', l'), r] = sum_p W12[p', p] A[l', (p, r)] formed as each lane loads
-// its operand.  No atomics: each output element is written once.
-template <class T>
-__global__ __launch_bounds__(kThreads) void heff_kernel(HeffK k) {
-  using M = Mfma<T>;
-  extern __shared__ __align__(16) unsigned char lds[];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lp = k.lp, P = 1 << lp, w = k.w;
-  T* Ps = reinterpret_cast<T*>(lds);
-  T* As = Ps + k.sPs;
-  T* Ls = As + k.sAs;
-  T* Rs = Ls + k.sLs;
-  const int2 item = k.items[blockIdx.x];
-  const int o = item.x, t0 = item.y, tpw = k.tpw;
-  const int dlp = k.o_dlp[o], drp = k.o_drp[o];
-  const int rows = dlp << lp;
-  const int RT = up16(rows) >> 4;
-  const int lLM = dlp > 16 ? 5 : 4, LM = 1 << lLM, lML = lLM - 4;  // l' padded to 16 or 32
-  const int lNC = drp > 16 ? 5 : 4, NC = 1 << lNC;                    // r' padded to 16 or 32
-  const int T16 = min(RT << (lNC - 4), t0 + kWaves * tpw);           // this workgroup's tiles: t0 + wave + kWaves i
-  // this wave's output tiles: (row tile, column tile) and the lane's row (p', l')
-  int tci[kTilesPerWave], tlq[kTilesPerWave], tpq[kTilesPerWave];
-  bool tok[kTilesPerWave];
-  typename M::acc acc[kTilesPerWave];
+// a value every lane of the wave holds alike, in a scalar register
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ i64 uni(i64 v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v), hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (i64)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ int wave_or(int v) {
 #pragma unroll
-  for (int i = 0; i < kTilesPerWave; ++i) {
-    const int t = t0 + wave + kWaves * i;
-    tok[i] = i < tpw && t < T16;
-    const int ri = t % RT;
-    tci[i] = t / RT;
-    const int row = ri * 16 + (lane & 15);
-    tlq[i] = row >> lp;
-    tpq[i] = row & (P - 1);
-    acc[i] = typename M::acc{0, 0, 0, 0};
-  }
-  const int Dl = k.Dl, Dr = k.Dr;
-  // staging of one sector pair: Ps[l][c] (c = p + P r), Ls[s][l][l'] for the
-  // s-th left link and Rs[s][r][r'] for the s-th right link.  The first batch
-  // of entries (BP, BL, BR per thread) goes through registers: its loads are
-  // issued one pair ahead, while the previous pair computes, and stored at the
-  // start of the pair; entries beyond it (large pairs) load at the start.
-  constexpr int BP = 4, BL = 2, BR = 2;
-  struct Geo {
-    int dl, dr, K1, N1, K2, PDR, SL, SR, nP, nL, nR;
-  };
-  auto geo = [&](const HeffPair& pr) {
-    Geo g;
-    g.dl = pr.dl;
-    g.dr = pr.dr;
-    g.K1 = up4(g.dl);
-    g.N1 = up16(g.dr << lp);
-    g.K2 = up4(g.dr);
-    g.PDR = g.dr << lp;
-    g.SL = LM * g.K1;
-    g.SR = g.K2 * NC;
-    g.nP = g.K1 * g.N1;
-    g.nL = __popc(pr.wlm) * g.SL;
-    g.nR = __popc(pr.wrm) * g.SR;
-    return g;
-  };
-  auto ps_at = [&](const HeffPair& pr, const Geo& g, int t) -> T {  // t = c + N1 l
-    const int l = t / g.N1, c = t - l * g.N1;
-    return (l < g.dl && c < g.PDR) ? (T)k.psi[pr.psi + (c & (P - 1)) + (((i64)l + (i64)Dl * (c >> lp)) << lp)]
-                                   : (T)0;
-  };
-  auto ls_at = [&](const HeffPair& pr, const Geo& g, int t) -> T {
-    int s = 0, u = t;
-    while (u >= g.SL) {
-      u -= g.SL;
-      ++s;
-    }
-    int bits = pr.wlm;
-    for (int j = 0; j < s; ++j) bits &= bits - 1;
-    const int m = u & (LM - 1), l = u >> lLM;
-    return (m < dlp && l < g.dl) ? (T)k.L[pr.L + (__ffs(bits) - 1) + (i64)w * (m + (i64)Dl * l)] : (T)0;
-  };
-  auto rs_at = [&](const HeffPair& pr, const Geo& g, int t) -> T {
-    int s = 0, u = t;
-    while (u >= g.SR) {
-      u -= g.SR;
-      ++s;
-    }
-    int bits = pr.wrm;
-    for (int j = 0; j < s; ++j) bits &= bits - 1;
-    const int n = u & (NC - 1), r = u >> lNC;
-    return (n < drp && r < g.dr) ? (T)k.R[pr.R + (__ffs(bits) - 1) + (i64)w * (n + (i64)Dr * r)] : (T)0;
-  };
-  T vp[BP], vl[BL], vr[BR];
-  auto stage_load = [&](const HeffPair& pr, const Geo& g) {
+  for (int o = 32; o; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// the position of the j-th set bit of a row list packed as 4-bit fields
+__device__ __forceinline__ int nib(uint64_t lst, int j) { return (int)((lst >> (4 * j)) & 15); }
+
+__device__ __forceinline__ uint64_t row_list(int msk, int P) {
+  uint64_t lst = 0;
+  int c = 0;
+  for (int b = 0; b < P; ++b)
+    if (msk >> b & 1) lst |= (uint64_t)b << (4 * c++);
+  return lst;
+}
+
+// ---- pack kernels
+
+// one workgroup per psi block: find the rows p holding a nonzero inside the
+// reach, then copy them 4 to a chunk into panels [l][p_c + 4 r] (row stride
+// pstr); entries beyond the reach are zero from planning and never written
+template <class T, int P>
+__global__ __launch_bounds__(256) void heff_pack_psi(const double* __restrict__ psi, const PsiJob* __restrict__ jobs,
+                                                     int Dl, T* __restrict__ pp, i32* __restrict__ pmask) {
+  __shared__ int sm;
+  const PsiJob jb = jobs[blockIdx.x];
+  if (threadIdx.x == 0) sm = 0;
+  __syncthreads();
+  const int n = jb.dl * jb.dr;
+  int bits = 0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int r = t / jb.dl, l = t - r * jb.dl;
+    // P consecutive values, 16-byte aligned (block payloads and rows are multiples of P >= 4 values)
+    const double2* s = reinterpret_cast<const double2*>(psi + jb.src + (i64)P * (l + (i64)Dl * r));
+    double2 v[P / 2];
 #pragma unroll
-    for (int u = 0; u < BP; ++u) {
-      const int t = tid + kThreads * u;
-      vp[u] = t < g.nP ? ps_at(pr, g, t) : (T)0;
-    }
+    for (int h = 0; h < P / 2; ++h) v[h] = s[h];
 #pragma unroll
-    for (int u = 0; u < BL; ++u) {
-      const int t = tid + kThreads * u;
-      vl[u] = t < g.nL ? ls_at(pr, g, t) : (T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < BR; ++u) {
-      const int t = tid + kThreads * u;
-      vr[u] = t < g.nR ? rs_at(pr, g, t) : (T)0;
-    }
-  };
-  auto stage_store = [&](const HeffPair& pr, const Geo& g) {
-#pragma unroll
-    for (int u = 0; u < BP; ++u) {
-      const int t = tid + kThreads * u;
-      if (t < g.nP) Ps[t] = vp[u];
-    }
-#pragma unroll
-    for (int u = 0; u < BL; ++u) {
-      const int t = tid + kThreads * u;
-      if (t < g.nL) Ls[t] = vl[u];
-    }
-#pragma unroll
-    for (int u = 0; u < BR; ++u) {
-      const int t = tid + kThreads * u;
-      if (t < g.nR) Rs[t] = vr[u];
-    }
-    for (int t = tid + kThreads * BP; t < g.nP; t += kThreads) Ps[t] = ps_at(pr, g, t);
-    for (int t = tid + kThreads * BL; t < g.nL; t += kThreads) Ls[t] = ls_at(pr, g, t);
-    for (int t = tid + kThreads * BR; t < g.nR; t += kThreads) Rs[t] = rs_at(pr, g, t);
-  };
-  const int qe = k.o_pb[o + 1];
-  int q = k.o_pb[o];
-  HeffPair pr{};
-  Geo gq{};
-  if (q < qe) {
-    pr = k.pairs[q];
-    gq = geo(pr);
-    stage_load(pr, gq);
-  }
-  for (; q < qe; ++q) {
-    __syncthreads();  // the previous pair is done with the tiles
-    stage_store(pr, gq);
-    __syncthreads();
-    const HeffPair cur = pr;
-    const Geo g = gq;
-    if (q + 1 < qe) {  // the next pair's loads fly while this one computes
-      pr = k.pairs[q + 1];
-      gq = geo(pr);
-      stage_load(pr, gq);
-    }
-    const int dr = g.dr, K1 = g.K1, N1 = g.N1, K2 = g.K2;
-    int sl = 0;
-    for (int wlm = cur.wlm; wlm; wlm &= wlm - 1, ++sl) {
-      const int wl = __ffs(wlm) - 1;
-      // A[l'][c] = sum_l L_wl[l'][l] Ps[l][c]
-      const T* Lt = Ls + sl * LM * K1;
-      const int nt = N1 >> (4 - lML);  // (LM / 16) * (N1 / 16) tiles
-      for (int tt = wave; tt < nt; tt += kWaves) {
-        const int mi = tt & ((1 << lML) - 1), ni = tt >> lML;
-        typename M::acc c = typename M::acc{0, 0, 0, 0};
-        for (int k0 = 0; k0 < K1; k0 += 4) {
-          const int kk = k0 + (lane >> 4);
-          c = M::run(Lt[kk * LM + mi * 16 + (lane & 15)], Ps[kk * N1 + ni * 16 + (lane & 15)], c);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) As[(mi * 16 + M::row(v, lane)) * N1 + ni * 16 + (lane & 15)] = c[v];
-      }
-      __syncthreads();
-      int sr = 0;
-      for (int wrm = cur.wrm; wrm; wrm &= wrm - 1, ++sr) {
-        const int wr = __ffs(wrm) - 1;
-        const int lk = k.lidx[wl * w + wr];
-        if (lk < 0) continue;
-        const T* Rt = Rs + sr * K2 * NC;
-        // the 16 x 16 W12 tile as the A operand of four k-steps whose B
-        // operands are the accumulator registers of Z = A R^T (the k index
-        // of step s in lane group lane / 16 is the accumulator row of
-        // register s, so Z feeds the matrix cores without leaving registers)
-        const double* Wt = k.w16 + lk * 256 + (lane & 15) * 16;
-        T wa[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) wa[v] = (T)Wt[M::row(v, lane)];
-#pragma unroll
-        for (int i = 0; i < kTilesPerWave; ++i) {
-          if (tok[i]) {
-            const T* Ar = As + tlq[i] * N1 + tpq[i];
-            const T* Rc = Rt + tci[i] * 16 + (lane & 15);
-            typename M::acc z = typename M::acc{0, 0, 0, 0};
-            for (int k0 = 0; k0 < K2; k0 += 4) {
-              const int kk = k0 + (lane >> 4);
-              const T a = kk < dr ? Ar[kk << lp] : (T)0;
-              z = M::run(a, Rc[kk * NC], z);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[i] = M::run(wa[v], z[v], acc[i]);
-          }
-        }
-      }
-      __syncthreads();  // As is rewritten for the next left link
+    for (int h = 0; h < P / 2; ++h) {
+      if (v[h].x != 0.0) bits |= 1 << (2 * h);  // NaN counts as nonzero
+      if (v[h].y != 0.0) bits |= 1 << (2 * h + 1);
     }
   }
-  double* ob = k.out + k.o_off[o];
+  bits = wave_or(bits);
+  if ((threadIdx.x & 63) == 0 && bits) atomicOr(&sm, bits);
+  __syncthreads();
+  const int msk = sm, np = __popc(msk);
+  if (threadIdx.x == 0) pmask[jb.blk] = msk | (np << 16);
+  const uint64_t lst = row_list(msk, P);
+  const int K1 = up4(jb.dl);
+  const i64 csz = (i64)K1 * jb.pstr;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int r = t / jb.dl, l = t - r * jb.dl;
+    const double* s = psi + jb.src + (i64)P * (l + (i64)Dl * r);
+    T* d = pp + jb.dst + (i64)l * jb.pstr + 4 * r;
+    for (int ch = 0; 4 * ch < np; ++ch) {
+      T v[4];
 #pragma unroll
-  for (int i = 0; i < kTilesPerWave; ++i) {
-    if (tok[i]) {
-      const int col = tci[i] * 16 + (lane & 15);
+      for (int j = 0; j < 4; ++j) v[j] = 4 * ch + j < np ? (T)s[nib(lst, 4 * ch + j)] : (T)0;
+      T* dc = d + ch * csz;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = (tlq[i] << lp) + tpq[i] - (lane & 15) + M::row(v, lane);  // the tile's row of register v
-        if (row < rows && col < drp) ob[(row & (P - 1)) + ((i64)((row >> lp) + (i64)Dl * col) << lp)] = (double)acc[i][v];
-      }
+      for (int j = 0; j < 4; ++j) dc[j] = v[j];
     }
   }
 }
 
+// one workgroup per environment link panel: panel[k][m] = X[link, m, k]
+// (k the inner bond index, m the outer) inside the reach
 template <class T>
-size_t heff_lds_bytes(int sPs, int sAs, int sLs, int sRs) {
-  return sizeof(T) * ((size_t)sPs + (size_t)sAs + (size_t)sLs + (size_t)sRs);
+__global__ __launch_bounds__(256) void heff_pack_env(const double* __restrict__ src, const EnvJob* __restrict__ jobs,
+                                                     int w, i64 D, T* __restrict__ dst) {
+  const EnvJob jb = jobs[blockIdx.x];
+  const int n = jb.rows * jb.cols;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int k = t / jb.cols, m = t - k * jb.cols;
+    dst[jb.dst + (i64)k * jb.stride + m] = (T)src[jb.src + jb.link + (i64)w * (m + D * k)];
+  }
+}
+
+// copy n16 16-byte units global -> LDS, all loads of a thread's batch first
+__device__ __forceinline__ void copy16(void* __restrict__ dst, const void* __restrict__ src, int n16, int tid,
+                                       int nt) {
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  int t = tid;
+  for (; t + 3 * nt < n16; t += 4 * nt) {
+    const uint4 a = s[t], b = s[t + nt], c = s[t + 2 * nt], e = s[t + 3 * nt];
+    d[t] = a;
+    d[t + nt] = b;
+    d[t + 2 * nt] = c;
+    d[t + 3 * nt] = e;
+  }
+  for (; t < n16; t += nt) d[t] = s[t];
+}
+
+// ---- the fused kernel
+//
+// A workgroup of 4 waves takes either one output block of more than 4 rows l'
+// (a team of 4 waves: A products split over the waves, output row groups of
+// 4 l' owned by waves w, w + 4, ...; workgroup barriers) or up to 4 blocks of
+// at most 4 rows l', one per wave, each wave on its own in its own LDS region
+// (no workgroup barriers).  Output rows (p', l') come in groups of 4 l'; per
+// group P / 4 accumulator tiles of 16 rows (p' fastest), and column tiles of
+// 16 r' (at most CT), held in registers over the block's whole work.
+//
+// A block's team first lists its active (pair, chunk) entries in LDS (a chunk
+// is active when an item's W12 columns meet its rows) and copies its link
+// items there.  Per entry, A_i = L_i^T psi_chunk on the matrix cores with
+// the operands read straight from the panels (L1/L2-resident, 16 consecutive
+// values per 16 lanes) goes to LDS, its live rows only; per link item the
+// group's tiles take Z = A R^T and then W12[:, rows] Z, Z fed to the matrix
+// cores straight from its accumulator registers.
+template <class T, int P, int GPW, int CT, int KS>
+__global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
+  using M = Mfma<T>;
+  constexpr int TPG = P / 4;   // accumulator tiles per group of 4 l'
+  constexpr int LPT = 16 / P;  // l' per accumulator tile
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int sm_ne[4];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m = lane & 15, kq = lane >> 4;
+  const int4 job = k.jobs[blockIdx.x];
+  const bool team4 = job.y == -2;  // the same for the whole workgroup
+  const int NW = team4 ? 4 : 1;     // waves on this block
+  const int tw = team4 ? wave : 0;  // this wave's place in its team
+  const int o = team4 ? job.x : (wave == 0 ? job.x : wave == 1 ? job.y : wave == 2 ? job.z : job.w);
+  if (o < 0) return;  // a wave without a block (never in a team of 4)
+  unsigned char* reg = lds_raw + (team4 ? 0 : wave * k.wave_lds);
+  HEnt* ents = reinterpret_cast<HEnt*>(reg);
+  int2* its = reinterpret_cast<int2*>(reg + k.off_it);
+  T* As = reinterpret_cast<T*>(reg + k.off_as);
+  // ordering of the team's LDS traffic: a workgroup barrier for a team of 4,
+  // for a wave on its own the wave's LDS operations in program order
+  auto bar = [&]() {
+    if (team4) {
+      __syncthreads();
+    } else {
+      __threadfence_block();
+      __builtin_amdgcn_wave_barrier();
+    }
+  };
+  const int dlp = k.o_dlp[o], drp = k.o_drp[o];
+  const int ng = (dlp + 3) >> 2, nct = (drp + 15) >> 4;
+  const int LM = up16(dlp), LSTR = lds_stride<T>(LM), RSTR = lds_stride<T>(nct * 16);
+  const int q0 = k.o_pb[o], qe = k.o_pb[o + 1];
+  const int ib0 = k.o_ib[o], nib_ = k.o_ib[o + 1] - ib0;
+  const int nth = 64 * NW, ttid = 64 * tw + lane;
+  for (int t = ttid; t < nib_; t += nth) its[t] = k.items[ib0 + t];
+
+  // ---- the entry list (the team's first wave; a deterministic order: pairs, then chunks)
+  if (tw == 0) {
+    int base = 0;
+    for (int qb = q0; qb < qe; qb += 64) {
+      const int q = qb + lane;
+      int cnt = 0, cms = 0, ams = 0;  // per chunk: rows (16 bits, chunks 0 and 1), L panels (8 bits)
+      HPair pr{};
+      if (q < qe) {
+        pr = k.pairs[q];
+        const int pm = k.pmask[pr.blk];
+        const int np = pm >> 16;
+        const uint64_t lst = row_list(pm & 0xffff, P);
+        for (int ch = 0; 4 * ch < np; ++ch) {
+          int cm = 0;
+          for (int j = 4 * ch; j < min(np, 4 * ch + 4); ++j) cm |= 1 << nib(lst, j);
+          int am = 0;
+          if (pr.nl <= 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (i < pr.nl && (pr.pc[i] & cm)) am |= 1 << i;
+          } else {
+            for (int it = 0; it < pr.nitem; ++it) {
+              const int2 c2 = k.items[pr.item0 + it];
+              if (c2.y & cm) am |= 1 << (c2.x & 31);
+            }
+          }
+          if (am) {
+            if (ch < 2) cms |= cm << (16 * ch);
+            ams |= (int)((unsigned)(am & 255) << (8 * ch));
+            cnt |= 1 << ch;
+          }
+        }
+      }
+      const int ncnt = __popc(cnt);
+      int inc = ncnt;  // inclusive scan of the counts over the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d);
+        if (lane >= d) inc += y;
+      }
+      int pos = base + inc - ncnt;
+      if (cnt) {
+        for (int ch = 0; ch < 4; ++ch) {
+          if (!(cnt >> ch & 1)) continue;
+          int cm;
+          if (ch < 2) {
+            cm = (cms >> (16 * ch)) & 0xffff;
+          } else {  // chunks 2 and 3 (16 rows): the rows again
+            const int pm = k.pmask[pr.blk], np = pm >> 16;
+            const uint64_t lst = row_list(pm & 0xffff, P);
+            cm = 0;
+            for (int j = 4 * ch; j < min(np, 4 * ch + 4); ++j) cm |= 1 << nib(lst, j);
+          }
+          int am = (int)(((unsigned)ams >> (8 * ch)) & 255u);
+          if (pr.nl > 8) {  // more L panels than the packed mask holds
+            am = 0;
+            for (int it = 0; it < pr.nitem; ++it) {
+              const int2 c2 = k.items[pr.item0 + it];
+              if (c2.y & cm) am |= 1 << (c2.x & 31);
+            }
+          }
+          HEnt* en = ents + pos++;
+          en->q = q;
+          en->ch = ch;
+          en->cm = cm;
+          en->amask = am;
+        }
+      }
+      base += __shfl(inc, 63);
+    }
+    if (lane == 0) sm_ne[wave] = base;
+  }
+  bar();
+  const int nE = sm_ne[team4 ? 0 : wave];
+
+  typename M::acc acc[GPW][CT][TPG];
+#pragma unroll
+  for (int a = 0; a < GPW; ++a)
+#pragma unroll
+    for (int b = 0; b < CT; ++b)
+#pragma unroll
+      for (int c = 0; c < TPG; ++c) acc[a][b][c] = typename M::acc{0, 0, 0, 0};
+
+  double* ob = k.out + k.o_off[o];
+  const i64 Dl = k.Dl;
+  if (nE == 0) {
+    // nothing to multiply: zeros over the reach, (P dlp) contiguous values per r'
+    const int run = P * dlp / 2;  // 16-byte units per r'
+    for (int t = ttid; t < run * drp; t += nth) {
+      const int rq = t / run, u = t - rq * run;
+      reinterpret_cast<uint4*>(ob + (i64)P * Dl * rq)[u] = make_uint4(0, 0, 0, 0);
+    }
+    return;
+  }
+
+  const int AR = 4 * ng;  // live rows of the A products
+  const int zp = M::zpos(m), zl = M::zoff(m);
+  const T* const gpp = k.pp;
+  const T* const glp = k.lp;
+  const T* const grp = k.rp;
+  const T* const gw12 = k.w12;
+  // this lane's row of an entry's chunk (the k index of the W12 operand), -1 past the chunk's rows
+  auto row_of = [=](int cm) {
+    int prow = -1, c = 0;
+    for (int b = 0; b < P; ++b)
+      if (cm >> b & 1) {
+        if (c == kq) prow = b;
+        ++c;
+      }
+    return prow;
+  };
+  const int lm_sh = LM > 16 ? 1 : 0;  // row tiles of A: LM / 16 = 1 or 2
+  for (int e = 0; e < nE; ++e) {
+    const HEnt en = ents[e];
+    const int q = uni(en.q), ch = uni(en.ch), cm = uni(en.cm), amask = uni(en.amask);
+    const HPair pr = k.pairs[q];
+    const int dl = pr.dl, dr = pr.dr, nl = pr.nl;
+    const int K1 = up4(dl), K2 = up4(dr), N1 = 4 * K2, PSTR = lds_stride<T>(N1);
+    const int prow = row_of(cm);
+    const T* Pg = gpp + pr.pp + (i64)ch * K1 * PSTR + m;
+    for (int b0 = 0; b0 < nl; b0 += k.apb) {
+      const int b1 = min(nl, b0 + k.apb);
+      if (!(((unsigned)amask >> b0) & ((1u << (b1 - b0)) - 1u))) continue;
+      bar();  // the previous user of As is done
+      // A_i[l'][c] = sum_l L_i[l][l'] psi[l][c], c = p_c + 4 r: operands straight from the panels
+      const int ntl = N1 >> 4, per = ntl << lm_sh;
+      for (int t = tw; t < (b1 - b0) * per; t += NW) {
+        const int ii = t / per, rem = t - ii * per, i = b0 + ii;
+        if (!(amask >> i & 1)) continue;
+        const int mi = rem & lm_sh, ni = rem >> lm_sh;
+        const T* Lg = glp + pr.lp + (i64)i * K1 * LSTR + mi * 16 + m;
+        const T* Pt = Pg + ni * 16;
+        T la[KS], pa[KS];
+#pragma unroll
+        for (int st = 0; st < KS; ++st) {
+          const int kk = 4 * st + kq;
+          la[st] = 4 * st < K1 ? Lg[kk * LSTR] : (T)0;
+          pa[st] = 4 * st < K1 ? Pt[kk * PSTR] : (T)0;
+        }
+        typename M::acc c = typename M::acc{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < KS; ++st)
+          if (4 * st < K1) c = M::run(la[st], pa[st], c);
+        T* At = As + (i64)ii * AR * PSTR + ni * 16 + m;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int lq = mi * 16 + M::row(v, lane);
+          if (lq < AR) At[lq * PSTR] = c[v];
+        }
+      }
+      bar();
+      for (int it = 0; it < pr.nitem; ++it) {
+        const int2 c2 = its[pr.item0 - ib0 + it];
+        const int code = uni(c2.x), sup = uni(c2.y);
+        const int iL = code & 31, iR = (code >> 5) & 31, lk = code >> 10;
+        if (iL < b0 || iL >= b1 || !(sup & cm)) continue;
+        const T wv = prow >= 0 ? gw12[lk * P * P + (m % P) * P + prow] : (T)0;
+        const T* Ai = As + (i64)(iL - b0) * AR * PSTR + zp;
+        const T* Rg = grp + pr.rp + (i64)iR * K2 * RSTR + m;
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          if (ct >= nct) break;
+          T rb[KS];
+#pragma unroll
+          for (int st = 0; st < KS; ++st) rb[st] = 4 * st < K2 ? Rg[(4 * st + kq) * RSTR + ct * 16] : (T)0;
+#pragma unroll
+          for (int gi = 0; gi < GPW; ++gi) {
+            const int gq = tw + NW * gi;
+            if (gq >= ng) break;
+            const T* Ag = Ai + (4 * gq + zl) * PSTR;
+            typename M::acc z = typename M::acc{0, 0, 0, 0};
+#pragma unroll
+            for (int st = 0; st < KS; ++st)
+              if (4 * st < K2) z = M::run(Ag[4 * (4 * st + kq)], rb[st], z);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const T wa = (m / P) == (v % LPT) ? wv : (T)0;
+              acc[gi][ct][v / LPT] = M::run(wa, z[v], acc[gi][ct][v / LPT]);
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int gi = 0; gi < GPW; ++gi) {
+    const int gq = tw + NW * gi;
+    if (gq >= ng) break;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      if (ct >= nct) break;
+      const int rq = ct * 16 + m;
+#pragma unroll
+      for (int t = 0; t < TPG; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = M::row(v, lane);
+          const int lq = 4 * gq + t * LPT + row / P, pq = row % P;
+          if (lq < dlp && rq < drp) ob[pq + (i64)P * (lq + Dl * rq)] = (double)acc[gi][ct][t][v];
+        }
+    }
+  }
+}
+
+// kernels by (precision, P, groups per wave, column tiles)
+// (big: sectors of up to 32 states; else up to 16, with fewer registers)
+template <class T>
+const void* heff_fn(int P, int big) {
+  if (P == 4) return big ? (const void*)heff_kernel<T, 4, 2, 2, 8> : (const void*)heff_kernel<T, 4, 1, 1, 4>;
+  if (P == 8) return big ? (const void*)heff_kernel<T, 8, 2, 2, 8> : (const void*)heff_kernel<T, 8, 1, 1, 4>;
+  return big ? (const void*)heff_kernel<T, 16, 2, 2, 8> : (const void*)heff_kernel<T, 16, 1, 1, 4>;
+}
+template <class T>
+const void* pack_psi_fn(int P) {
+  if (P == 4) return (const void*)heff_pack_psi<T, 4>;
+  if (P == 8) return (const void*)heff_pack_psi<T, 8>;
+  return (const void*)heff_pack_psi<T, 16>;
 }
 
 // everything the fused plan keeps on the device, and the host-side checks of its contract
@@ -351,30 +562,25 @@ struct HeffState {
   const Tensor *psi, *L, *R, *W1, *W2;
   Tensor* out;
   i64 vL, vR, vW1, vW2;  // versions of the operator tensors at planning
-  DBuf<int2> items;  // (output block, first tile), launch class by launch class
-  DBuf<i32> o_dlp, o_drp, o_pb, lidx;
-  DBuf<i64> o_off;
-  DBuf<HeffPair> pairs;
-  DBuf<double> w16;
-  struct Launch {
-    HeffK k{};
-    size_t lds = 0;
-    i64 n = 0;
-    hipStream_t stream = nullptr;  // the side stream of a launch other than the first (nullptr: the context's)
-    hipEvent_t done = nullptr;
-  };
-  std::vector<Launch> launches;  // one per size class with work
-  hipEvent_t fork = nullptr;
-  ~HeffState() {
-    for (auto& la : launches) {
-      if (la.stream) (void)hipStreamSynchronize(la.stream);
-      if (la.done) (void)hipEventDestroy(la.done);
-      if (la.stream) (void)hipStreamDestroy(la.stream);
-    }
-    if (fork) (void)hipEventDestroy(fork);
-  }
   bool fp32 = false;
+  int P = 16;
   i64 nout = 0;
+  DBuf<i32> o_dlp, o_drp, o_pb, o_ib, pmask;
+  DBuf<int4> jobs;
+  DBuf<int2> items;
+  DBuf<i64> o_off;
+  DBuf<HPair> pairs;
+  DBuf<PsiJob> pjobs;
+  DBuf<unsigned char> pp, lp, rp, w12;  // panels and W12 in the plan's precision
+  i64 njobs = 0;
+  struct Launch {
+    const void* fn = nullptr;
+    i64 first = 0, n = 0;  // workgroups: jobs [first, first + n)
+    int apb = 1;           // L panels whose A products LDS holds at once
+    int off_it = 0, off_as = 0, wave_lds = 0;  // LDS region of a block (bytes)
+    size_t lds = 0;
+  };
+  std::vector<Launch> launches;  // one per kernel shape with work, in order on the context's stream
 };
 
 template <class T>
@@ -387,6 +593,58 @@ template <class T>
 void up(Ctx* ctx, DBuf<T>& d, const std::vector<T>& h) {
   d.alloc(ctx, h.empty() ? 1 : h.size());
   if (!h.empty()) HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+}
+
+void launch_pack_psi(HeffState* s, hipStream_t st) {
+  if (!s->njobs) return;
+  void* pp = s->pp.get();
+  const double* psi = s->psi->dptr();
+  const PsiJob* jobs = s->pjobs.get();
+  i32* pm = s->pmask.get();
+  const int Dl = (int)s->psi->dims[4];
+  const dim3 grid((unsigned)s->njobs), block(256);
+  if (s->fp32) {
+    if (s->P == 4)
+      hipLaunchKernelGGL((heff_pack_psi<float, 4>), grid, block, 0, st, psi, jobs, Dl, (float*)pp, pm);
+    else if (s->P == 8)
+      hipLaunchKernelGGL((heff_pack_psi<float, 8>), grid, block, 0, st, psi, jobs, Dl, (float*)pp, pm);
+    else
+      hipLaunchKernelGGL((heff_pack_psi<float, 16>), grid, block, 0, st, psi, jobs, Dl, (float*)pp, pm);
+  } else {
+    if (s->P == 4)
+      hipLaunchKernelGGL((heff_pack_psi<double, 4>), grid, block, 0, st, psi, jobs, Dl, (double*)pp, pm);
+    else if (s->P == 8)
+      hipLaunchKernelGGL((heff_pack_psi<double, 8>), grid, block, 0, st, psi, jobs, Dl, (double*)pp, pm);
+    else
+      hipLaunchKernelGGL((heff_pack_psi<double, 16>), grid, block, 0, st, psi, jobs, Dl, (double*)pp, pm);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+void launch_main(HeffState* s, const HeffState::Launch& la, hipStream_t st) {
+  HK<T> k{};
+  k.pp = (const T*)s->pp.get();
+  k.lp = (const T*)s->lp.get();
+  k.rp = (const T*)s->rp.get();
+  k.w12 = (const T*)s->w12.get();
+  k.pmask = s->pmask.get();
+  k.out = s->out->dptr();
+  k.jobs = s->jobs.get() + la.first;
+  k.o_dlp = s->o_dlp.get();
+  k.o_drp = s->o_drp.get();
+  k.o_pb = s->o_pb.get();
+  k.o_off = s->o_off.get();
+  k.pairs = s->pairs.get();
+  k.items = s->items.get();
+  k.Dl = (int)s->psi->dims[4];
+  k.apb = la.apb;
+  k.off_it = la.off_it;
+  k.off_as = la.off_as;
+  k.wave_lds = la.wave_lds;
+  k.o_ib = s->o_ib.get();
+  void* args[] = {&k};
+  HIP_CHECK(hipLaunchKernel(la.fn, dim3((unsigned)la.n), dim3(256), args, la.lds, st));
 }
 
 }  // namespace
@@ -412,10 +670,13 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   want(W1.dims[1] == d1 && W1.dims[2] == d1 && W2.dims[1] == d2 && W2.dims[2] == d2, "site dims must agree");
   want(out.dims == psi.dims && out.N2 == 4, "Hpsi must have psi's dims and N2 = 4");
   const i64 P = d1 * d2;
-  if (P > 16 || w > 31) fail(SB_E_LIMIT, "H_eff plan: d1*d2 above 16 or MPO link dimension above 31");
+  if (!(P == 4 || P == 8 || P == 16) || w > 31)
+    fail(SB_E_LIMIT, "H_eff plan: d1*d2 must be 4, 8 or 16 and the MPO link dimension at most 31");
   const bool fp32 = (flags & SB_PREC_MASK) == SB_PREC_FP32;
   if ((flags & SB_PREC_MASK) != SB_PREC_FP64 && !fp32)
     fail(SB_E_ARGUMENT, "ArgumentError: the fused H_eff plan computes in FP64 or FP32");
+  const size_t esz = fp32 ? sizeof(float) : sizeof(double);
+  auto stride = [&](int n) { return fp32 ? lds_stride<float>(n) : lds_stride<double>(n); };
 
   // ---- structure from the key lists, link masks and extents of the environments
   auto kpsi = down(ctx, psi.keys.get(), (size_t)psi.n * 2), kL = down(ctx, L.keys.get(), (size_t)L.n * 2),
@@ -435,9 +696,9 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
        hW2 = down(ctx, W2.dptr(), (size_t)(W2.n ? W2.blksize : 0));
   HIP_CHECK(hipStreamSynchronize(ctx->stream));
 
-  // W12[wl, wr][p', p] = sum_wm W1[wl, s1', s1, wm] W2[wm, s2', s2, wr], rows p' in CSR per link pair
-  std::vector<i32> lidx((size_t)(w * w), -1), wptr(1, 0), wcol;
-  std::vector<double> wval;
+  // W12[wl, wr][p', p] = sum_wm W1[wl, s1', s1, wm] W2[wm, s2', s2, wr], P x P per nonzero link pair
+  std::vector<i32> lidx((size_t)(w * w), -1), csupp;
+  std::vector<double> w12;
   int nlink = 0;
   for (i64 wl = 0; wl < w; ++wl)
     for (i64 wr = 0; wr < w; ++wr) {
@@ -453,35 +714,19 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
                 for (i64 s2 = 0; s2 < d2; ++s2) {
                   const double b = hW2[(size_t)(m + wm * (s2p + d2 * (s2 + d2 * wr)))];
                   if (b == 0.0) continue;
-                  blk[(size_t)((s1p + d1 * s2p) + P * (s1 + d1 * s2))] += a * b;
+                  blk[(size_t)((s1p + d1 * s2p) * P + (s1 + d1 * s2))] += a * b;
                   any = true;
                 }
             }
       if (!any) continue;
       lidx[(size_t)(wl * w + wr)] = nlink++;
-      for (i64 pp = 0; pp < P; ++pp) {
-        for (i64 p = 0; p < P; ++p) {
-          const double c = blk[(size_t)(pp + P * p)];
-          if (c != 0.0) {
-            wcol.push_back((i32)p);
-            wval.push_back(c);
-          }
-        }
-        wptr.push_back((i32)wcol.size());
-      }
+      int cs = 0;
+      for (i64 pp = 0; pp < P; ++pp)
+        for (i64 p = 0; p < P; ++p)
+          if (blk[(size_t)(pp * P + p)] != 0.0) cs |= 1 << p;
+      csupp.push_back(cs);
+      w12.insert(w12.end(), blk.begin(), blk.end());
     }
-  // the W12 tiles of the matrix cores: rows p' + P a, columns p + P a (a < 16 / P)
-  if (16 % P) fail(SB_E_LIMIT, "H_eff plan: d1*d2 must divide 16 for the fused kernel");
-  std::vector<double> w16((size_t)std::max(nlink, 1) * 256, 0.0);
-  for (int lk = 0; lk < nlink; ++lk)
-    for (i64 pp = 0; pp < P; ++pp)
-      for (int x = wptr[(size_t)(lk * P + pp)]; x < wptr[(size_t)(lk * P + pp + 1)]; ++x)
-        for (i64 a = 0; a < 16 / P; ++a)
-          w16[(size_t)(lk * 256 + (pp + P * a) * 16 + (wcol[(size_t)x] + P * a))] = wval[(size_t)x];
-  std::vector<int> wl_useful((size_t)w, 0);  // right links each left link pairs with
-  for (i64 wl = 0; wl < w; ++wl)
-    for (i64 wr = 0; wr < w; ++wr)
-      if (lidx[(size_t)(wl * w + wr)] >= 0) wl_useful[(size_t)wl] |= 1 << wr;
 
   // extents of the bond sectors: the environments' nonzero reach
   std::vector<int> elp((size_t)nl, 0), el((size_t)nl, 0), erp((size_t)nr, 0), er((size_t)nr, 0);
@@ -542,144 +787,193 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
     oids[(size_t)i] = created[sorted_keys[(size_t)i]];
   }
 
-  // ---- work list: per output block (by id) its sector pairs
+  // ---- panels: per used L / R block its link panels, per used psi block its chunk panels
+  std::vector<i64> lbase((size_t)std::max<i64>(L.n, 1), -1), rbase((size_t)std::max<i64>(R.n, 1), -1),
+      pbase((size_t)std::max<i64>(psi.n, 1), -1);
+  std::vector<i32> pblk((size_t)std::max<i64>(psi.n, 1), -1);
+  std::vector<EnvJob> ljobs, rjobs;
+  std::vector<PsiJob> pjobs;
+  i64 lsize = 0, rsize = 0, psize = 0;
+  auto env_panels = [&](i64 i, const std::vector<i32>& key, const std::vector<i32>& ids, const std::vector<i32>& hm,
+                        const std::vector<int>& eo, const std::vector<int>& ei, const Tensor& X, i64& size,
+                        std::vector<i64>& base, std::vector<EnvJob>& jobs) {
+    if (base[(size_t)i] >= 0) return;
+    const int a = key[(size_t)(2 * i)] - 1, b = key[(size_t)(2 * i + 1)] - 1;
+    const int rows = up4(ei[(size_t)b]), str = stride(up16(eo[(size_t)a]));
+    base[(size_t)i] = size;
+    int j = 0;
+    for (int lk = 0; lk < (int)w; ++lk) {
+      if (!(hm[(size_t)(3 * i)] >> lk & 1)) continue;
+      EnvJob jb;
+      jb.src = (i64)(ids[(size_t)i] - 1) * X.blksize;
+      jb.dst = size + (i64)j * rows * str;
+      jb.link = lk;
+      jb.rows = ei[(size_t)b];
+      jb.cols = eo[(size_t)a];
+      jb.stride = str;
+      jobs.push_back(jb);
+      ++j;
+    }
+    size += (i64)j * rows * str;
+  };
+
+  // ---- work list: per output block (by id) its sector pairs and their link items
   const i64 obs = out.blksize;
-  std::vector<i32> o_dlp((size_t)nout), o_drp((size_t)nout), o_pb((size_t)nout + 1, 0);
+  std::vector<i32> o_dlp((size_t)nout), o_drp((size_t)nout), o_pb((size_t)nout + 1, 0), o_ib((size_t)nout + 1, 0);
+  std::vector<int2> items;
   std::vector<i64> o_off((size_t)nout);
-  std::vector<HeffPair> pairs;
-  // per output block: its LDS tile needs (elements) and tile count
-  std::vector<std::array<int, 4>> need((size_t)nout, std::array<int, 4>{1, 1, 1, 1});
-  int max_tiles = 0;
-  double flops = 0, bytes_out = 0;
-  std::vector<char> psi_used((size_t)std::max<i64>(psi.n, 1), 0);
+  std::vector<HPair> pairs;
+  std::vector<std::array<int, 4>> pgeo;  // per pair: K1, N1 (psi chunk), nl, nr, for the LDS need
   for (i64 o = 0; o < nout; ++o) {
     const int a = by_id[(size_t)o].first, c = by_id[(size_t)o].second;
     const int dlp = elp[(size_t)a], drp = erp[(size_t)c];
     o_dlp[(size_t)o] = dlp;
     o_drp[(size_t)o] = drp;
     o_off[(size_t)o] = o * obs;  // ids 1..n: block id o+1 at o*blksize
-    bytes_out += 8.0 * P * dlp * drp;
     if (dlp > 32 || drp > 32)
       fail(SB_E_LIMIT, "H_eff plan: a bond sector above 32 states (use the contract! chain)");
-    const int LM = up16(dlp), NC = up16(drp);
-    max_tiles = std::max<int>(max_tiles, (up16((int)(P * dlp)) / 16) * (NC / 16));
     for (auto& lb : Lrow[(size_t)a]) {
       const int il = lb.first;
       const int lmask = hmL[(size_t)(3 * lb.second)];
       for (auto& rb : Rrow[(size_t)c]) {
         const int ir = rb.first;
-        auto it = psi_at.find({il, ir});
-        if (it == psi_at.end()) continue;
+        auto itp = psi_at.find({il, ir});
+        if (itp == psi_at.end()) continue;
         const int rmask = hmR[(size_t)(3 * rb.second)];
-        int wlm = 0;
-        for (i64 wl = 0; wl < w; ++wl)
-          if ((lmask >> wl & 1) && (wl_useful[(size_t)wl] & rmask)) wlm |= 1 << wl;
         const int dl = el[(size_t)il], dr = er[(size_t)ir];
-        if (!wlm || !dl || !dr || !dlp || !drp) continue;
-        HeffPair hp;
-        hp.psi = (i64)(ipsi[(size_t)it->second] - 1) * psi.blksize;
-        hp.L = (i64)(iL[(size_t)lb.second] - 1) * L.blksize;
-        hp.R = (i64)(iR[(size_t)rb.second] - 1) * R.blksize;
-        hp.dl = dl;
-        hp.dr = dr;
-        hp.wlm = wlm;
-        hp.wrm = rmask;
-        pairs.push_back(hp);
-        psi_used[(size_t)it->second] = 1;
-        {
-          const int K1 = up4(dl), N1 = up16((int)(P * dr)), K2 = up4(dr);
-          auto& nd = need[(size_t)o];
-          nd[0] = std::max(nd[0], K1 * N1);
-          nd[1] = std::max(nd[1], LM * N1);
-          nd[2] = std::max(nd[2], __builtin_popcount(wlm) * LM * K1);
-          nd[3] = std::max(nd[3], __builtin_popcount(rmask) * K2 * NC);
-        }
-        for (i64 wl = 0; wl < w; ++wl) {
-          if (!(wlm >> wl & 1)) continue;
-          flops += 2.0 * P * dlp * dl * dr;  // A = L_wl psi
-          for (i64 wr = 0; wr < w; ++wr) {
+        if (!dl || !dr || !dlp || !drp) continue;
+        std::vector<int2> its;
+        for (int wl = 0; wl < (int)w; ++wl) {
+          if (!(lmask >> wl & 1)) continue;
+          for (int wr = 0; wr < (int)w; ++wr) {
+            if (!(rmask >> wr & 1)) continue;
             const int lk = lidx[(size_t)(wl * w + wr)];
-            if (lk < 0 || !(rmask >> wr & 1)) continue;
-            double rowsum = 0;
-            for (i64 pp = 0; pp < P; ++pp) {
-              const int n = wptr[(size_t)(lk * P + pp + 1)] - wptr[(size_t)(lk * P + pp)];
-              if (n) rowsum += 2.0 * n + 2.0;  // W12 row . A, then times R
-            }
-            flops += rowsum * dlp * drp * dr;
+            if (lk < 0) continue;
+            const int xl = __builtin_popcount(lmask & ((1 << wl) - 1)), xr = __builtin_popcount(rmask & ((1 << wr) - 1));
+            its.push_back(make_int2(xl | (xr << 5) | (lk << 10), csupp[(size_t)lk]));
           }
         }
+        if (its.empty()) continue;
+        const i64 ib = itp->second;
+        env_panels(lb.second, kL, iL, hmL, elp, el, L, lsize, lbase, ljobs);
+        env_panels(rb.second, kR, iR, hmR, erp, er, R, rsize, rbase, rjobs);
+        const int K1 = up4(dl), N1 = 4 * up4(dr), pstr = stride(N1);
+        if (pbase[(size_t)ib] < 0) {
+          pbase[(size_t)ib] = psize;
+          pblk[(size_t)ib] = (i32)pjobs.size();
+          PsiJob jb;
+          jb.src = (i64)(ipsi[(size_t)ib] - 1) * psi.blksize;
+          jb.dst = psize;
+          jb.blk = (i32)pjobs.size();
+          jb.dl = dl;
+          jb.dr = dr;
+          jb.pstr = pstr;
+          pjobs.push_back(jb);
+          psize += (P / 4) * (i64)K1 * pstr;
+        }
+        HPair hp;
+        hp.pp = pbase[(size_t)ib];
+        hp.lp = lbase[(size_t)lb.second];
+        hp.rp = rbase[(size_t)rb.second];
+        hp.blk = pblk[(size_t)ib];
+        hp.item0 = (i32)items.size();
+        hp.nitem = (int16_t)its.size();
+        hp.nl = (uint8_t)__builtin_popcount(lmask);
+        hp.nr = (uint8_t)__builtin_popcount(rmask);
+        hp.dl = (uint8_t)dl;
+        hp.dr = (uint8_t)dr;
+        for (int i = 0; i < 4; ++i) hp.pc[i] = 0;
+        for (const int2& c2 : its)
+          if ((c2.x & 31) < 4) hp.pc[c2.x & 31] |= (uint16_t)c2.y;
+        items.insert(items.end(), its.begin(), its.end());
+        pairs.push_back(hp);
+        pgeo.push_back({K1, N1, hp.nl, hp.nr});
       }
     }
     o_pb[(size_t)o + 1] = (i32)pairs.size();
+    o_ib[(size_t)o + 1] = (i32)items.size();
   }
-  if (max_tiles > kMaxTiles)
-    fail(SB_E_LIMIT, "H_eff plan: an output block needs more than 16 tiles of 16 x 16 (use the contract! chain)");
-  if ((i64)pairs.size() >= (1ll << 31)) fail(SB_E_LIMIT, "H_eff plan: too many sector pairs");
-  auto lds_of = [&](const std::array<int, 4>& nd) {
-    return fp32 ? heff_lds_bytes<float>(nd[0], nd[1], nd[2], nd[3]) : heff_lds_bytes<double>(nd[0], nd[1], nd[2], nd[3]);
+  if ((i64)pairs.size() >= (1ll << 31) || (i64)items.size() >= (1ll << 31))
+    fail(SB_E_LIMIT, "H_eff plan: too many sector pairs");
+
+  // ---- work: a block of more than 4 rows l' takes a workgroup (a team of 4
+  // waves), blocks of at most 4 rows go 4 to a workgroup, one per wave;
+  // blocks with sectors above 16 states or more than 16 l' per wave run a
+  // kernel with more registers.  Per kernel shape the largest blocks first.
+  auto st = std::make_shared<HeffState>();
+  st->fp32 = fp32;
+  st->P = (int)P;
+  auto a16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+  struct Shape {
+    std::vector<i32> team, solo;
+    size_t list = 16, its = 16, a1_team = 0, aall_team = 0, a1_solo = 0, aall_solo = 0;  // bytes
   };
-  // launch classes by LDS need (a launch reserves its largest block's LDS for
-  // every workgroup, so small blocks go together to keep occupancy up); in a
-  // class, blocks with the most work first
-  const size_t class_cap[3] = {24 * 1024, 48 * 1024, kMaxLds};
-  std::vector<std::vector<i32>> cls(3);
+  Shape shapes[2];
   std::vector<i64> work((size_t)nout, 0);
   for (i64 o = 0; o < nout; ++o) {
-    if (!o_dlp[(size_t)o] || !o_drp[(size_t)o]) continue;  // no entry to write
-    const size_t b = lds_of(need[(size_t)o]);
-    if (b > kMaxLds) fail(SB_E_LIMIT, "H_eff plan: sector blocks too large for the fused kernel's LDS tiles");
-    int c = 0;
-    while (b > class_cap[c]) ++c;
-    cls[(size_t)c].push_back((i32)o);
-    work[(size_t)o] = (i64)(o_pb[(size_t)o + 1] - o_pb[(size_t)o] + 1) * o_dlp[(size_t)o] * o_drp[(size_t)o];
-  }
-  // workgroups: small blocks whole (4 tiles a wave); in the largest class a
-  // block's tiles spread over several workgroups (1 tile a wave), each
-  // recomputing the pair products the tiles need, for more parallel work
-  std::vector<int2> items;
-  struct Span {
-    size_t count;
-    std::array<int, 4> mx;
-    int tpw;
-  };
-  std::vector<Span> cls_span;  // per class with work
-  for (size_t c = 0; c < cls.size(); ++c) {
-    auto& v = cls[c];
-    if (v.empty()) continue;
-    std::stable_sort(v.begin(), v.end(), [&](i32 x, i32 y) { return work[(size_t)x] > work[(size_t)y]; });
-    std::array<int, 4> mx{1, 1, 1, 1};
-    for (i32 o : v)
-      for (int j = 0; j < 4; ++j) mx[(size_t)j] = std::max(mx[(size_t)j], need[(size_t)o][(size_t)j]);
-    const int tpw = c + 1 == cls.size() ? 1 : kTilesPerWave;
-    size_t n = 0;
-    for (i32 o : v) {
-      const int tiles = (up16((int)(P * o_dlp[(size_t)o])) / 16) * (up16(o_drp[(size_t)o]) / 16);
-      for (int t0 = 0; t0 < tiles; t0 += kWaves * tpw, ++n) items.push_back(make_int2(o, t0));
+    const int dlp = o_dlp[(size_t)o], drp = o_drp[(size_t)o];
+    if (!dlp || !drp) continue;  // no entry to write
+    const int ng = (dlp + 3) / 4, nct = (drp + 15) / 16;
+    int kmax = 0;  // the largest inner sector of the block's pairs
+    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q)
+      kmax = std::max<int>(kmax, std::max(pairs[(size_t)q].dl, pairs[(size_t)q].dr));
+    const int big = (ng > 4 || nct > 1 || kmax > 16) ? 1 : 0;
+    Shape& sh = shapes[big];
+    const bool team = ng > 1;
+    (team ? sh.team : sh.solo).push_back((i32)o);
+    sh.list = std::max(sh.list, a16(sizeof(HEnt) * (size_t)(P / 4) * (size_t)(o_pb[(size_t)o + 1] - o_pb[(size_t)o])));
+    sh.its = std::max(sh.its, a16(sizeof(int2) * (size_t)(o_ib[(size_t)o + 1] - o_ib[(size_t)o])));
+    const int AR = 4 * ng;  // live rows of the A products
+    size_t a1 = 0, aall = 0;
+    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q) {
+      const auto& gq = pgeo[(size_t)q];
+      const int K1 = gq[0], N1 = gq[1], pstr = stride(N1);
+      a1 = std::max(a1, esz * AR * pstr);
+      aall = std::max(aall, esz * gq[2] * AR * pstr);
+      work[(size_t)o] += (i64)(gq[2] + pairs[(size_t)q].nitem) * K1 * N1;
     }
-    cls_span.push_back({n, mx, tpw});
+    if (team) {
+      sh.a1_team = std::max(sh.a1_team, a1);
+      sh.aall_team = std::max(sh.aall_team, aall);
+    } else {
+      sh.a1_solo = std::max(sh.a1_solo, a1);
+      sh.aall_solo = std::max(sh.aall_solo, aall);
+    }
   }
-  if (items.size() >= (size_t)1 << 31) fail(SB_E_LIMIT, "H_eff plan: too many workgroups");
-  // matrix-core work one execution issues (16 x 16 x 4 products, 2048 FLOPs each)
-  double mfma = 0;
-  {
-    size_t it = 0;
-    for (auto& cs : cls_span)
-      for (size_t j = 0; j < cs.count; ++j, ++it) {
-        const int o = items[it].x, t0 = items[it].y;
-        const int dlp = o_dlp[(size_t)o], drp = o_drp[(size_t)o];
-        const int RT = up16((int)(P * dlp)) / 16, ML = up16(dlp) / 16, CT = up16(drp) / 16;
-        const int ntile = std::min(RT * CT - t0, kWaves * cs.tpw);
-        for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q) {
-          const HeffPair& hp = pairs[(size_t)q];
-          const int K1 = up4(hp.dl), N1 = up16((int)(P * hp.dr)), K2 = up4(hp.dr);
-          for (int wl = 0; wl < w; ++wl) {
-            if (!(hp.wlm >> wl & 1)) continue;
-            mfma += (double)ML * (N1 / 16) * (K1 / 4);
-            for (int wr = 0; wr < w; ++wr)
-              if ((hp.wrm >> wr & 1) && lidx[(size_t)(wl * w + wr)] >= 0) mfma += (double)ntile * (K2 / 4 + 4);
-          }
-        }
-      }
+  std::vector<int4> jobs;
+  for (int big = 0; big < 2; ++big) {
+    Shape& sh = shapes[big];
+    if (sh.team.empty() && sh.solo.empty()) continue;
+    auto by_work = [&](i32 x, i32 y) { return work[(size_t)x] > work[(size_t)y]; };
+    std::stable_sort(sh.team.begin(), sh.team.end(), by_work);
+    std::stable_sort(sh.solo.begin(), sh.solo.end(), by_work);
+    HeffState::Launch la;
+    la.first = (i64)jobs.size();
+    for (i32 o : sh.team) jobs.push_back(make_int4(o, -2, -2, -2));
+    for (size_t i = 0; i < sh.solo.size(); i += 4) {
+      auto at = [&](size_t j) { return j < sh.solo.size() ? sh.solo[j] : -1; };
+      jobs.push_back(make_int4(at(i), at(i + 1), at(i + 2), at(i + 3)));
+    }
+    la.n = (i64)jobs.size() - la.first;
+    la.off_it = (int)sh.list;
+    la.off_as = (int)(sh.list + sh.its);
+    auto lds_with = [&](size_t at, size_t as_) {
+      const size_t wl = a16(la.off_as + as_);
+      return std::make_pair(wl, std::max(sh.solo.empty() ? 0 : 4 * wl, (size_t)la.off_as + at));
+    };
+    la.apb = 31;
+    auto l = lds_with(sh.aall_team, sh.aall_solo);
+    if (l.second > kLdsBudget) {
+      la.apb = 1;
+      l = lds_with(sh.a1_team, sh.a1_solo);
+    }
+    la.wave_lds = (int)l.first;
+    la.lds = std::max<size_t>(l.second, 16);
+    if (la.lds + 64 > kMaxLds)
+      fail(SB_E_LIMIT, "H_eff plan: sector blocks too large for the fused kernel's LDS tiles");
+    la.fn = fp32 ? heff_fn<float>((int)P, big) : heff_fn<double>((int)P, big);
+    HIP_CHECK(hipFuncSetAttribute(la.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la.lds));
+    st->launches.push_back(la);
   }
 
   // ---- install Hpsi's block list (overwrite semantics), payload zero
@@ -705,7 +999,6 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   out.external = false;
   out.touch();
 
-  auto st = std::make_shared<HeffState>();
   st->psi = &psi;
   st->L = &L;
   st->R = &R;
@@ -716,64 +1009,103 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   st->vR = R.version;
   st->vW1 = W1.version;
   st->vW2 = W2.version;
-  st->fp32 = fp32;
   st->nout = nout;
-  up(ctx, st->items, items);
+  st->njobs = (i64)pjobs.size();
+  up(ctx, st->jobs, jobs);
   up(ctx, st->o_dlp, o_dlp);
   up(ctx, st->o_drp, o_drp);
   up(ctx, st->o_pb, o_pb);
+  up(ctx, st->o_ib, o_ib);
   up(ctx, st->o_off, o_off);
   up(ctx, st->pairs, pairs);
-  up(ctx, st->lidx, lidx);
-  up(ctx, st->w16, w16);
-  HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
-  HeffK k{};
-  k.o_dlp = st->o_dlp.get();
-  k.o_drp = st->o_drp.get();
-  k.o_pb = st->o_pb.get();
-  k.o_off = st->o_off.get();
-  k.pairs = st->pairs.get();
-  k.lidx = st->lidx.get();
-  k.w16 = st->w16.get();
-  k.lp = __builtin_ctz((unsigned)P);
-  k.w = (int)w;
-  k.Dl = (int)Dl;
-  k.Dr = (int)Dr;
-  size_t lds_max = 0;
-  // the class of the largest blocks first, on the context's stream; the others on
-  // side streams of their own, forked from and joined to it at every execution
-  std::vector<size_t> corder(cls_span.size());
-  for (size_t i = 0; i < corder.size(); ++i) corder[i] = corder.size() - 1 - i;  // largest blocks start first
-  std::vector<size_t> cfirst(cls_span.size(), 0);
-  for (size_t i = 1; i < cls_span.size(); ++i) cfirst[i] = cfirst[i - 1] + cls_span[i - 1].count;
-  for (size_t ci : corder) {
-    const Span& cs = cls_span[ci];
-    HeffState::Launch la;
-    la.k = k;
-    la.k.items = st->items.get() + cfirst[ci];
-    la.k.tpw = cs.tpw;
-    la.k.sPs = cs.mx[0];
-    la.k.sAs = cs.mx[1];
-    la.k.sLs = cs.mx[2];
-    la.k.sRs = cs.mx[3];
-    la.lds = lds_of(cs.mx);
-    la.n = (i64)cs.count;
-    lds_max = std::max(lds_max, la.lds);
-    st->launches.push_back(la);
+  up(ctx, st->items, items);
+  up(ctx, st->pjobs, pjobs);
+  st->pmask.alloc(ctx, (size_t)std::max<i64>(st->njobs, 1));
+  // panels: zero (the padding stays zero), then filled from the operands
+  auto zero_buf = [&](DBuf<unsigned char>& b, i64 elems) {
+    const size_t bytes = (size_t)std::max<i64>(elems, 4) * esz;
+    b.alloc(ctx, (bytes + 15) & ~(size_t)15);
+    k_zero(ctx, b.get(), (bytes + 15) & ~(size_t)15);
+  };
+  zero_buf(st->pp, psize);
+  zero_buf(st->lp, lsize);
+  zero_buf(st->rp, rsize);
+  {
+    std::vector<unsigned char> hw(std::max<size_t>(w12.size(), 1) * esz, 0);
+    for (size_t i = 0; i < w12.size(); ++i) {
+      if (fp32) {
+        const float f = (float)w12[i];
+        std::memcpy(hw.data() + i * esz, &f, esz);
+      } else {
+        std::memcpy(hw.data() + i * esz, &w12[i], esz);
+      }
+    }
+    up(ctx, st->w12, hw);
   }
-  for (size_t i = 1; i < st->launches.size(); ++i) {
-    auto& la = st->launches[i];
-    HIP_CHECK(hipStreamCreateWithFlags(&la.stream, hipStreamNonBlocking));
-    HIP_CHECK(hipEventCreateWithFlags(&la.done, hipEventDisableTiming));
+  {
+    DBuf<EnvJob> lj, rj;
+    up(ctx, lj, ljobs);
+    up(ctx, rj, rjobs);
+    if (!ljobs.empty()) {
+      if (fp32)
+        hipLaunchKernelGGL(heff_pack_env<float>, dim3((unsigned)ljobs.size()), dim3(256), 0, ctx->stream, L.dptr(),
+                           lj.get(), (int)w, Dl, (float*)st->lp.get());
+      else
+        hipLaunchKernelGGL(heff_pack_env<double>, dim3((unsigned)ljobs.size()), dim3(256), 0, ctx->stream, L.dptr(),
+                           lj.get(), (int)w, Dl, (double*)st->lp.get());
+    }
+    if (!rjobs.empty()) {
+      if (fp32)
+        hipLaunchKernelGGL(heff_pack_env<float>, dim3((unsigned)rjobs.size()), dim3(256), 0, ctx->stream, R.dptr(),
+                           rj.get(), (int)w, Dr, (float*)st->rp.get());
+      else
+        hipLaunchKernelGGL(heff_pack_env<double>, dim3((unsigned)rjobs.size()), dim3(256), 0, ctx->stream, R.dptr(),
+                           rj.get(), (int)w, Dr, (double*)st->rp.get());
+    }
+    HIP_CHECK(hipGetLastError());
+    // psi's rows at planning: the accounting below (every execution finds them anew)
+    launch_pack_psi(st.get(), ctx->stream);
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host vectors and job lists go out of scope
   }
-  if (st->launches.size() > 1) HIP_CHECK(hipEventCreateWithFlags(&st->fork, hipEventDisableTiming));
-  if (lds_max) {
-    if (fp32)
-      HIP_CHECK(hipFuncSetAttribute((const void*)heff_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_max));
-    else
-      HIP_CHECK(hipFuncSetAttribute((const void*)heff_kernel<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_max));
+  std::vector<i32> hpm = down(ctx, st->pmask.get(), (size_t)st->njobs);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+  // ---- accounting, for psi's rows at planning.  Useful FLOPs: per active
+  // chunk the A products of its L panels (2 dlp dl (rows dr)) and per active
+  // item Z (2 rows dlp dr drp) and the W12 step (2 nnz(W12[:, rows]) dlp drp);
+  // matrix-core FLOPs: the 16 x 16 x 4 products those issue, padding included
+  double flops = 0, mfma = 0;
+  for (i64 o = 0; o < nout; ++o) {
+    const int dlp = o_dlp[(size_t)o], drp = o_drp[(size_t)o];
+    if (!dlp || !drp) continue;
+    const int ng = (dlp + 3) / 4, nct = (drp + 15) / 16, LM = up16(dlp);
+    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q) {
+      const HPair& hp = pairs[(size_t)q];
+      const int pm = hpm[(size_t)hp.blk], np = pm >> 16;
+      std::vector<int> rowsl;
+      for (int b = 0; b < P; ++b)
+        if (pm >> b & 1) rowsl.push_back(b);
+      const int K1 = up4(hp.dl), K2 = up4(hp.dr);
+      for (int ch = 0; 4 * ch < np; ++ch) {
+        int cm = 0, nrow = 0;
+        for (int j = 4 * ch; j < std::min(np, 4 * ch + 4); ++j) cm |= 1 << rowsl[(size_t)j], ++nrow;
+        int amask = 0;
+        for (int it = 0; it < hp.nitem; ++it) {
+          const int code = items[(size_t)(hp.item0 + it)].x, lk = code >> 10;
+          if (!(csupp[(size_t)lk] & cm)) continue;
+          amask |= 1 << (code & 31);
+          int nnz = 0;
+          for (int pp = 0; pp < P; ++pp)
+            for (int b = 0; b < P; ++b)
+              if ((cm >> b & 1) && w12[(size_t)(lk * P * P + pp * P + b)] != 0.0) ++nnz;
+          flops += 2.0 * nrow * dlp * hp.dr * drp + 2.0 * nnz * dlp * drp;
+          mfma += (double)ng * nct * (K2 / 4 + 4);
+        }
+        const int na = __builtin_popcount(amask);
+        flops += 2.0 * na * dlp * hp.dl * nrow * hp.dr;
+        mfma += (double)na * (LM / 16) * (4 * K2 / 16) * (K1 / 4);
+      }
+    }
   }
 
   auto pl = std::make_unique<Plan>();
@@ -785,37 +1117,23 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
                          if (s->L->version != s->vL || s->R->version != s->vR || s->W1->version != s->vW1 ||
                              s->W2->version != s->vW2)
                            fail(SB_E_ERROR, "H_eff plan: L, R, W1 or W2 changed since the plan was made");
-                         if (s->fork) HIP_CHECK(hipEventRecord(s->fork, ctx->stream));
+                         launch_pack_psi(s, ctx->stream);
                          for (auto& la : s->launches) {
-                           HeffK k = la.k;
-                           k.psi = s->psi->dptr();
-                           k.L = s->L->dptr();
-                           k.R = s->R->dptr();
-                           k.out = s->out->dptr();
-                           hipStream_t st = la.stream ? la.stream : ctx->stream;
-                           if (la.stream) HIP_CHECK(hipStreamWaitEvent(la.stream, s->fork, 0));
                            if (s->fp32)
-                             hipLaunchKernelGGL(heff_kernel<float>, dim3((unsigned)la.n), dim3(kThreads), la.lds,
-                                                st, k);
+                             launch_main<float>(s, la, ctx->stream);
                            else
-                             hipLaunchKernelGGL(heff_kernel<double>, dim3((unsigned)la.n), dim3(kThreads), la.lds,
-                                                st, k);
-                           HIP_CHECK(hipGetLastError());
-                           if (la.done) HIP_CHECK(hipEventRecord(la.done, la.stream));
+                             launch_main<double>(s, la, ctx->stream);
                          }
-                         for (auto& la : s->launches)
-                           if (la.done) HIP_CHECK(hipStreamWaitEvent(ctx->stream, la.done, 0));
                        }});
   pl->flops = flops;
   // algorithmic bytes: the psi blocks the pairs read, L and R inside their reach, Hpsi written
-  double bpsi = 0, benv = 0;
-  for (i64 i = 0; i < psi.n; ++i)
-    if (psi_used[(size_t)i])
-      bpsi += 8.0 * P * el[(size_t)(kpsi[(size_t)(2 * i)] - 1)] * er[(size_t)(kpsi[(size_t)(2 * i + 1)] - 1)];
+  double bpsi = 0, benv = 0, bytes_out = 0;
+  for (const PsiJob& jb : pjobs) bpsi += 8.0 * P * jb.dl * jb.dr;
   for (i64 i = 0; i < L.n; ++i)
     benv += 8.0 * w * hmL[(size_t)(3 * i + 1)] * hmL[(size_t)(3 * i + 2)];
   for (i64 i = 0; i < R.n; ++i)
     benv += 8.0 * w * hmR[(size_t)(3 * i + 1)] * hmR[(size_t)(3 * i + 2)];
+  for (i64 o = 0; o < nout; ++o) bytes_out += 8.0 * P * o_dlp[(size_t)o] * o_drp[(size_t)o];
   pl->gemm_bytes = bpsi + benv + bytes_out;
   pl->a_bytes = bpsi;
   pl->b_bytes = benv;
