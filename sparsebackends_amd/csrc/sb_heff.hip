@@ -23,15 +23,19 @@
 // is then only written with zeros).  Nothing is assumed about psi's values:
 // the rows are found from them at every execution.
 //
-// Per output block, per sector pair, per chunk of 4 rows: the psi chunk
-// (l x (p_c, r)) and the pair's L and R link slices are copied from their
-// panels into LDS (16-byte copies, strides chosen so that every matrix-core
-// operand read is free of bank conflicts); A_wl = L_wl^T psi (l' x (p_c, r))
-// on the matrix cores into LDS; per link pair the wave's output tiles take
-// Z = A R_wr^T ((p_c, l') x r') and then W12[:, rows] Z, with Z fed to the
-// matrix cores straight from its accumulator registers.  No atomics: each
-// output element is written once.  The environment panels are built once, at
-// planning (L, R and the MPO must keep their values, which the plan checks).
+// One launch of 4-wave workgroups covers every output block: a block of more
+// than 4 rows l' takes a workgroup, smaller blocks go four to a workgroup, one
+// per wave.  Per block, its active (sector pair, chunk of 4 rows) entries are
+// listed in LDS; per entry A_wl = L_wl^T psi (l' x (p_c, r)) runs on the
+// matrix cores with its operands read straight from the panels (row strides
+// chosen so that 16 lanes read 16 consecutive values), only its live rows
+// kept in LDS; per link pair the wave's output tiles take Z = A R_wr^T
+// ((p_c, l') x r') and then W12[:, rows] Z, with Z fed to the matrix cores
+// straight from its accumulator registers.  No atomics: each output element
+// is written once.  A block with nothing to multiply is written with zeros,
+// unless it is zero from the plan's previous execution already.  The
+// environment panels are built once, at planning (L, R and the MPO must keep
+// their values, which the plan checks).
 //
 // Tensors (the layouts of the four-call chain's operands):
 //   psi [Ql, Qr | s1, s2, l, r]   L [Ql', Ql | wl, l', l]   R [Qr', Qr | wr, r', r]
@@ -143,6 +147,7 @@ struct HK {
   // copied at the start) and the A products; a wave on its own has its own
   // region, wave_lds bytes apart
   int off_it, off_as, wave_lds;
+  i32* dirty;          // per output block: 0 when its payload is known to be zero
   const i32* o_ib;     // per output block: its first link item (the items of a block are contiguous)
 };
 
@@ -424,14 +429,18 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
   double* ob = k.out + k.o_off[o];
   const i64 Dl = k.Dl;
   if (nE == 0) {
-    // nothing to multiply: zeros over the reach, (P dlp) contiguous values per r'
+    // nothing to multiply: zeros over the reach, (P dlp) contiguous values per
+    // r', unless the block is zero from the previous execution already
+    if (!k.dirty[o]) return;
     const int run = P * dlp / 2;  // 16-byte units per r'
     for (int t = ttid; t < run * drp; t += nth) {
       const int rq = t / run, u = t - rq * run;
       reinterpret_cast<uint4*>(ob + (i64)P * Dl * rq)[u] = make_uint4(0, 0, 0, 0);
     }
+    if (ttid == 0) k.dirty[o] = 0;
     return;
   }
+  if (ttid == 0) k.dirty[o] = 1;
 
   const int AR = 4 * ng;  // live rows of the A products
   const int zp = M::zpos(m), zl = M::zoff(m);
@@ -565,8 +574,9 @@ struct HeffState {
   bool fp32 = false;
   int P = 16;
   i64 nout = 0;
-  DBuf<i32> o_dlp, o_drp, o_pb, o_ib, pmask;
+  DBuf<i32> o_dlp, o_drp, o_pb, o_ib, pmask, dirty;
   DBuf<int4> jobs;
+  i64 vout = -1;  // Hpsi's version after this plan's last execution (another value: Hpsi written outside)
   DBuf<int2> items;
   DBuf<i64> o_off;
   DBuf<HPair> pairs;
@@ -643,6 +653,7 @@ void launch_main(HeffState* s, const HeffState::Launch& la, hipStream_t st) {
   k.off_as = la.off_as;
   k.wave_lds = la.wave_lds;
   k.o_ib = s->o_ib.get();
+  k.dirty = s->dirty.get();
   void* args[] = {&k};
   HIP_CHECK(hipLaunchKernel(la.fn, dim3((unsigned)la.n), dim3(256), args, la.lds, st));
 }
@@ -896,85 +907,9 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   if ((i64)pairs.size() >= (1ll << 31) || (i64)items.size() >= (1ll << 31))
     fail(SB_E_LIMIT, "H_eff plan: too many sector pairs");
 
-  // ---- work: a block of more than 4 rows l' takes a workgroup (a team of 4
-  // waves), blocks of at most 4 rows go 4 to a workgroup, one per wave;
-  // blocks with sectors above 16 states or more than 16 l' per wave run a
-  // kernel with more registers.  Per kernel shape the largest blocks first.
   auto st = std::make_shared<HeffState>();
   st->fp32 = fp32;
   st->P = (int)P;
-  auto a16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  struct Shape {
-    std::vector<i32> team, solo;
-    size_t list = 16, its = 16, a1_team = 0, aall_team = 0, a1_solo = 0, aall_solo = 0;  // bytes
-  };
-  Shape shapes[2];
-  std::vector<i64> work((size_t)nout, 0);
-  for (i64 o = 0; o < nout; ++o) {
-    const int dlp = o_dlp[(size_t)o], drp = o_drp[(size_t)o];
-    if (!dlp || !drp) continue;  // no entry to write
-    const int ng = (dlp + 3) / 4, nct = (drp + 15) / 16;
-    int kmax = 0;  // the largest inner sector of the block's pairs
-    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q)
-      kmax = std::max<int>(kmax, std::max(pairs[(size_t)q].dl, pairs[(size_t)q].dr));
-    const int big = (ng > 4 || nct > 1 || kmax > 16) ? 1 : 0;
-    Shape& sh = shapes[big];
-    const bool team = ng > 1;
-    (team ? sh.team : sh.solo).push_back((i32)o);
-    sh.list = std::max(sh.list, a16(sizeof(HEnt) * (size_t)(P / 4) * (size_t)(o_pb[(size_t)o + 1] - o_pb[(size_t)o])));
-    sh.its = std::max(sh.its, a16(sizeof(int2) * (size_t)(o_ib[(size_t)o + 1] - o_ib[(size_t)o])));
-    const int AR = 4 * ng;  // live rows of the A products
-    size_t a1 = 0, aall = 0;
-    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q) {
-      const auto& gq = pgeo[(size_t)q];
-      const int K1 = gq[0], N1 = gq[1], pstr = stride(N1);
-      a1 = std::max(a1, esz * AR * pstr);
-      aall = std::max(aall, esz * gq[2] * AR * pstr);
-      work[(size_t)o] += (i64)(gq[2] + pairs[(size_t)q].nitem) * K1 * N1;
-    }
-    if (team) {
-      sh.a1_team = std::max(sh.a1_team, a1);
-      sh.aall_team = std::max(sh.aall_team, aall);
-    } else {
-      sh.a1_solo = std::max(sh.a1_solo, a1);
-      sh.aall_solo = std::max(sh.aall_solo, aall);
-    }
-  }
-  std::vector<int4> jobs;
-  for (int big = 0; big < 2; ++big) {
-    Shape& sh = shapes[big];
-    if (sh.team.empty() && sh.solo.empty()) continue;
-    auto by_work = [&](i32 x, i32 y) { return work[(size_t)x] > work[(size_t)y]; };
-    std::stable_sort(sh.team.begin(), sh.team.end(), by_work);
-    std::stable_sort(sh.solo.begin(), sh.solo.end(), by_work);
-    HeffState::Launch la;
-    la.first = (i64)jobs.size();
-    for (i32 o : sh.team) jobs.push_back(make_int4(o, -2, -2, -2));
-    for (size_t i = 0; i < sh.solo.size(); i += 4) {
-      auto at = [&](size_t j) { return j < sh.solo.size() ? sh.solo[j] : -1; };
-      jobs.push_back(make_int4(at(i), at(i + 1), at(i + 2), at(i + 3)));
-    }
-    la.n = (i64)jobs.size() - la.first;
-    la.off_it = (int)sh.list;
-    la.off_as = (int)(sh.list + sh.its);
-    auto lds_with = [&](size_t at, size_t as_) {
-      const size_t wl = a16(la.off_as + as_);
-      return std::make_pair(wl, std::max(sh.solo.empty() ? 0 : 4 * wl, (size_t)la.off_as + at));
-    };
-    la.apb = 31;
-    auto l = lds_with(sh.aall_team, sh.aall_solo);
-    if (l.second > kLdsBudget) {
-      la.apb = 1;
-      l = lds_with(sh.a1_team, sh.a1_solo);
-    }
-    la.wave_lds = (int)l.first;
-    la.lds = std::max<size_t>(l.second, 16);
-    if (la.lds + 64 > kMaxLds)
-      fail(SB_E_LIMIT, "H_eff plan: sector blocks too large for the fused kernel's LDS tiles");
-    la.fn = fp32 ? heff_fn<float>((int)P, big) : heff_fn<double>((int)P, big);
-    HIP_CHECK(hipFuncSetAttribute(la.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la.lds));
-    st->launches.push_back(la);
-  }
 
   // ---- install Hpsi's block list (overwrite semantics), payload zero
   out.n = nout;
@@ -1010,8 +945,9 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   st->vW1 = W1.version;
   st->vW2 = W2.version;
   st->nout = nout;
+  up(ctx, st->dirty, std::vector<i32>((size_t)std::max<i64>(nout, 1), 0));  // Hpsi starts zero
+  st->vout = out.version;
   st->njobs = (i64)pjobs.size();
-  up(ctx, st->jobs, jobs);
   up(ctx, st->o_dlp, o_dlp);
   up(ctx, st->o_drp, o_drp);
   up(ctx, st->o_pb, o_pb);
@@ -1070,6 +1006,109 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   std::vector<i32> hpm = down(ctx, st->pmask.get(), (size_t)st->njobs);
   HIP_CHECK(hipStreamSynchronize(ctx->stream));
 
+  // ---- work: a block of more than 4 rows l' takes a workgroup (a team of 4
+  // waves), blocks of at most 4 rows go 4 to a workgroup, one per wave;
+  // blocks with sectors above 16 states or more than 16 l' per wave run a
+  // kernel with more registers.  Per kernel shape the largest blocks first.
+  auto a16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+  struct Shape {
+    std::vector<i32> team, solo;
+    size_t list = 16, its = 16, a1_team = 0, aall_team = 0, a1_solo = 0, aall_solo = 0;  // bytes
+  };
+  Shape shapes[2];
+  std::vector<i64> work((size_t)nout, 0);
+  for (i64 o = 0; o < nout; ++o) {
+    const int dlp = o_dlp[(size_t)o], drp = o_drp[(size_t)o];
+    if (!dlp || !drp) continue;  // no entry to write
+    const int ng = (dlp + 3) / 4, nct = (drp + 15) / 16;
+    int kmax = 0;  // the largest inner sector of the block's pairs
+    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q)
+      kmax = std::max<int>(kmax, std::max(pairs[(size_t)q].dl, pairs[(size_t)q].dr));
+    const int big = (ng > 4 || nct > 1 || kmax > 16) ? 1 : 0;
+    Shape& sh = shapes[big];
+    const bool team = ng > 1;
+    (team ? sh.team : sh.solo).push_back((i32)o);
+    sh.list = std::max(sh.list, a16(sizeof(HEnt) * (size_t)(P / 4) * (size_t)(o_pb[(size_t)o + 1] - o_pb[(size_t)o])));
+    sh.its = std::max(sh.its, a16(sizeof(int2) * (size_t)(o_ib[(size_t)o + 1] - o_ib[(size_t)o])));
+    const int AR = 4 * ng;  // live rows of the A products
+    size_t a1 = 0, aall = 0;
+    for (int q = o_pb[(size_t)o]; q < o_pb[(size_t)o + 1]; ++q) {
+      const auto& gq = pgeo[(size_t)q];
+      const int K1 = gq[0], N1 = gq[1], pstr = stride(N1);
+      a1 = std::max(a1, esz * AR * pstr);
+      aall = std::max(aall, esz * gq[2] * AR * pstr);
+      // work with psi's rows at planning (the order of the jobs only)
+      const HPair& hp = pairs[(size_t)q];
+      const int pm = hpm[(size_t)hp.blk], np = pm >> 16;
+      int nch = 0;
+      for (int ch = 0; 4 * ch < np; ++ch) {
+        int cm = 0;
+        for (int b = 0, c = 0; b < P; ++b)
+          if (pm >> b & 1) {
+            if (c >= 4 * ch && c < 4 * ch + 4) cm |= 1 << b;
+            ++c;
+          }
+        for (int it = 0; it < hp.nitem; ++it)
+          if (items[(size_t)(hp.item0 + it)].y & cm) {
+            ++nch;
+            break;
+          }
+      }
+      work[(size_t)o] += (i64)nch * (gq[2] + hp.nitem) * K1 * N1;
+    }
+    if (team) {
+      sh.a1_team = std::max(sh.a1_team, a1);
+      sh.aall_team = std::max(sh.aall_team, aall);
+    } else {
+      sh.a1_solo = std::max(sh.a1_solo, a1);
+      sh.aall_solo = std::max(sh.aall_solo, aall);
+    }
+  }
+  std::vector<int4> jobs;
+  for (int big = 0; big < 2; ++big) {
+    Shape& sh = shapes[big];
+    if (sh.team.empty() && sh.solo.empty()) continue;
+    auto by_work = [&](i32 x, i32 y) { return work[(size_t)x] > work[(size_t)y]; };
+    std::stable_sort(sh.team.begin(), sh.team.end(), by_work);
+    std::stable_sort(sh.solo.begin(), sh.solo.end(), by_work);
+    HeffState::Launch la;
+    la.first = (i64)jobs.size();
+    // teams and quads of solo blocks in one list, the most work first
+    std::vector<std::pair<i64, int4>> js;
+    for (i32 o : sh.team) js.push_back({work[(size_t)o], make_int4(o, -2, -2, -2)});
+    for (size_t i = 0; i < sh.solo.size(); i += 4) {
+      auto at = [&](size_t j) { return j < sh.solo.size() ? sh.solo[j] : -1; };
+      js.push_back({work[(size_t)sh.solo[i]], make_int4(at(i), at(i + 1), at(i + 2), at(i + 3))});
+    }
+    std::stable_sort(js.begin(), js.end(),
+                     [](const std::pair<i64, int4>& x, const std::pair<i64, int4>& y) { return x.first > y.first; });
+    for (auto& j : js) jobs.push_back(j.second);
+    la.n = (i64)jobs.size() - la.first;
+    la.off_it = (int)sh.list;
+    la.off_as = (int)(sh.list + sh.its);
+    auto lds_with = [&](size_t at, size_t as_) {
+      const size_t wl = a16(la.off_as + as_);
+      return std::make_pair(wl, std::max(sh.solo.empty() ? 0 : 4 * wl, (size_t)la.off_as + at));
+    };
+    la.apb = 31;
+    auto l = lds_with(sh.aall_team, sh.aall_solo);
+    if (l.second > kLdsBudget) {
+      la.apb = 1;
+      l = lds_with(sh.a1_team, sh.a1_solo);
+    }
+    la.wave_lds = (int)l.first;
+    la.lds = std::max<size_t>(l.second, 16);
+    if (la.lds + 64 > kMaxLds)
+      fail(SB_E_LIMIT, "H_eff plan: sector blocks too large for the fused kernel's LDS tiles");
+    la.fn = fp32 ? heff_fn<float>((int)P, big) : heff_fn<double>((int)P, big);
+    HIP_CHECK(hipFuncSetAttribute(la.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)la.lds));
+    st->launches.push_back(la);
+  }
+
+  up(ctx, st->jobs, jobs);
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+
   // ---- accounting, for psi's rows at planning.  Useful FLOPs: per active
   // chunk the A products of its L panels (2 dlp dl (rows dr)) and per active
   // item Z (2 rows dlp dr drp) and the W12 step (2 nnz(W12[:, rows]) dlp drp);
@@ -1118,6 +1157,13 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
                              s->W2->version != s->vW2)
                            fail(SB_E_ERROR, "H_eff plan: L, R, W1 or W2 changed since the plan was made");
                          launch_pack_psi(s, ctx->stream);
+                         // Hpsi written outside this plan: its payload is cleared, padding
+                         // included (the kernel writes the blocks' reach only), every block zero
+                         if (s->out->version != s->vout && s->nout) {
+                           k_zero(ctx, s->out->dptr(), (size_t)(s->nout * s->out->blksize) * sizeof(double));
+                           k_zero(ctx, s->dirty.get(), (size_t)s->nout * sizeof(i32));
+                         }
+                         s->vout = s->out->version + 1;  // the plan's finish() marks Hpsi written
                          for (auto& la : s->launches) {
                            if (s->fp32)
                              launch_main<float>(s, la, ctx->stream);

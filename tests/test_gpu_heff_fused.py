@@ -58,6 +58,55 @@ def test_fused_replay_new_psi(name):
     assert not np.array_equal(got["data"], first["data"])
 
 
+def _oracle_for(prob, data):
+    p2 = copy.copy(prob)
+    p2.psi_parts = dict(prob.psi_parts, data=data)
+    return HeffOracle(p2).run().parts()
+
+
+@pytest.mark.parametrize("name", ["hubbard", "heisenberg"])
+def test_fused_rows_change_between_executions(name):
+    """The rows of psi that hold nonzeros are found at every execution:
+    psi with blocks cleared (their output blocks must return to zero), then
+    psi with values in every row of the reach, charge-forbidden rows too (the
+    blocks of forbidden charge become nonzero), then the first psi again"""
+    prob = small(name)
+    print(f"[heff fused rows] {name}", flush=True)
+    prob.upload()
+    prob.make_fused_plan(0)
+    pp = prob.psi_parts
+    d, Dl, Dr = prob.m.d, prob.Dl, prob.Dr
+    bs = d * d * Dl * Dr
+    rng = np.random.default_rng(23)
+    half = pp["data"].copy()
+    for i in range(0, len(pp["ids"]), 2):
+        half[i * bs:(i + 1) * bs] = 0.0
+    full = pp["data"].copy()
+    for i, (a, b) in enumerate(pp["keys"]):
+        blk = np.zeros((d, d, Dl, Dr), order="F")
+        blk[:, :, :prob.Bl.dims[a - 1], :prob.Br.dims[b - 1]] = rng.standard_normal(
+            (d, d, prob.Bl.dims[a - 1], prob.Br.dims[b - 1]))
+        full[i * bs:(i + 1) * bs] = blk.reshape(-1, order="F")
+    for data in (half, full, pp["data"], half):
+        prob.psi.set_data_(data)
+        prob.fused.execute()
+        got = prob.Hpsi_fused.parts()
+        q = _oracle_for(prob, data)
+        same_blocks(got, q)
+        assert rel_frobenius(got["data"], q["data"]) <= FP64_TOL
+
+
+def test_fused_after_output_written_outside():
+    """Hpsi written between executions: every block is recomputed or cleared"""
+    prob = small("hubbard")
+    prob.upload()
+    prob.make_fused_plan(0)
+    first = prob.Hpsi_fused.parts()
+    prob.Hpsi_fused.set_data_(np.random.default_rng(5).standard_normal(first["data"].shape))
+    prob.fused.execute()
+    assert np.array_equal(prob.Hpsi_fused.parts()["data"], first["data"])
+
+
 @pytest.mark.parametrize("name", ["hubbard", "hubbard-chi256"])
 def test_fused_fp32(name):
     prob = small(name)
