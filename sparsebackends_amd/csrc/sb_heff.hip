@@ -122,6 +122,7 @@ struct HPair {
 struct PsiJob {
   i64 src, dst;
   i32 blk, dl, dr, pstr;
+  i32 u0, nu;  // the pairs that read this block: uses[u0, u0 + nu) = (pair, output block)
 };
 
 // one environment link panel to build: payload offset, panel base, link,
@@ -148,6 +149,8 @@ struct HK {
   // region, wave_lds bytes apart
   int off_it, off_as, wave_lds;
   i32* dirty;          // per output block: 0 when its payload is known to be zero
+  const i32* active;   // per output block: == epoch when a pair of it has work this execution
+  int epoch;
   const i32* o_ib;     // per output block: its first link item (the items of a block are contiguous)
 };
 
@@ -211,22 +214,38 @@ __device__ __forceinline__ uint64_t row_list(int msk, int P) {
 
 // ---- pack kernels
 
+template <class T>
+struct PackK {
+  const double* psi;
+  const PsiJob* jobs;
+  int Dl;
+  T* pp;
+  i32* pmask;
+  const int2* uses;  // (pair, output block) of the pairs reading each psi block
+  const HPair* pairs;
+  const int2* items;
+  i32* active;  // per output block: the execution's epoch when a pair of it has work
+  int epoch;
+};
+
 // one workgroup per psi block: find the rows p holding a nonzero inside the
-// reach, then copy them 4 to a chunk into panels [l][p_c + 4 r] (row stride
-// pstr); entries beyond the reach are zero from planning and never written
+// reach, copy them 4 to a chunk into panels [l][p_c + 4 r] (row stride pstr;
+// entries beyond the reach are zero from planning and never written), and
+// mark the output blocks of the pairs that have work with these rows (a
+// chunk of rows meets the W12 columns of one of the pair's link items)
 template <class T, int P>
-__global__ __launch_bounds__(256) void heff_pack_psi(const double* __restrict__ psi, const PsiJob* __restrict__ jobs,
-                                                     int Dl, T* __restrict__ pp, i32* __restrict__ pmask) {
+__global__ __launch_bounds__(256) void heff_pack_psi(PackK<T> k) {
   __shared__ int sm;
-  const PsiJob jb = jobs[blockIdx.x];
+  const PsiJob jb = k.jobs[blockIdx.x];
   if (threadIdx.x == 0) sm = 0;
   __syncthreads();
   const int n = jb.dl * jb.dr;
+  const i64 Dl = k.Dl;
   int bits = 0;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int r = t / jb.dl, l = t - r * jb.dl;
     // P consecutive values, 16-byte aligned (block payloads and rows are multiples of P >= 4 values)
-    const double2* s = reinterpret_cast<const double2*>(psi + jb.src + (i64)P * (l + (i64)Dl * r));
+    const double2* s = reinterpret_cast<const double2*>(k.psi + jb.src + (i64)P * (l + Dl * r));
     double2 v[P / 2];
 #pragma unroll
     for (int h = 0; h < P / 2; ++h) v[h] = s[h];
@@ -240,14 +259,14 @@ __global__ __launch_bounds__(256) void heff_pack_psi(const double* __restrict__ 
   if ((threadIdx.x & 63) == 0 && bits) atomicOr(&sm, bits);
   __syncthreads();
   const int msk = sm, np = __popc(msk);
-  if (threadIdx.x == 0) pmask[jb.blk] = msk | (np << 16);
+  if (threadIdx.x == 0) k.pmask[jb.blk] = msk | (np << 16);
   const uint64_t lst = row_list(msk, P);
   const int K1 = up4(jb.dl);
   const i64 csz = (i64)K1 * jb.pstr;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int r = t / jb.dl, l = t - r * jb.dl;
-    const double* s = psi + jb.src + (i64)P * (l + (i64)Dl * r);
-    T* d = pp + jb.dst + (i64)l * jb.pstr + 4 * r;
+    const double* s = k.psi + jb.src + (i64)P * (l + Dl * r);
+    T* d = k.pp + jb.dst + (i64)l * jb.pstr + 4 * r;
     for (int ch = 0; 4 * ch < np; ++ch) {
       T v[4];
 #pragma unroll
@@ -256,6 +275,24 @@ __global__ __launch_bounds__(256) void heff_pack_psi(const double* __restrict__ 
 #pragma unroll
       for (int j = 0; j < 4; ++j) dc[j] = v[j];
     }
+  }
+  for (int t = threadIdx.x; t < jb.nu; t += blockDim.x) {
+    const int2 u = k.uses[jb.u0 + t];
+    const HPair pr = k.pairs[u.x];
+    bool act = false;
+    for (int ch = 0; 4 * ch < np && !act; ++ch) {
+      int cm = 0;
+      for (int j = 4 * ch; j < min(np, 4 * ch + 4); ++j) cm |= 1 << nib(lst, j);
+      if (pr.nl <= 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i < pr.nl && (pr.pc[i] & cm)) act = true;
+      } else {
+        for (int it = 0; it < pr.nitem; ++it)
+          if (k.items[pr.item0 + it].y & cm) act = true;
+      }
+    }
+    if (act) k.active[u.y] = k.epoch;
   }
 }
 
@@ -338,9 +375,26 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
   const int dlp = k.o_dlp[o], drp = k.o_drp[o];
   const int ng = (dlp + 3) >> 2, nct = (drp + 15) >> 4;
   const int LM = up16(dlp), LSTR = lds_stride<T>(LM), RSTR = lds_stride<T>(nct * 16);
+  const int nth = 64 * NW, ttid = 64 * tw + lane;
+  double* ob = k.out + k.o_off[o];
+  const i64 Dl = k.Dl;
+  // nothing to multiply: zeros over the reach, (P dlp) contiguous values per
+  // r', unless the block is zero from the previous execution already
+  auto zero_block = [&]() {
+    if (!k.dirty[o]) return;
+    const int run = P * dlp / 2;  // 16-byte units per r'
+    for (int t = ttid; t < run * drp; t += nth) {
+      const int rq = t / run, u = t - rq * run;
+      reinterpret_cast<uint4*>(ob + (i64)P * Dl * rq)[u] = make_uint4(0, 0, 0, 0);
+    }
+    if (ttid == 0) k.dirty[o] = 0;
+  };
+  if (k.active[o] != k.epoch) {  // no pair of the block has work with psi's rows
+    zero_block();
+    return;
+  }
   const int q0 = k.o_pb[o], qe = k.o_pb[o + 1];
   const int ib0 = k.o_ib[o], nib_ = k.o_ib[o + 1] - ib0;
-  const int nth = 64 * NW, ttid = 64 * tw + lane;
   for (int t = ttid; t < nib_; t += nth) its[t] = k.items[ib0 + t];
 
   // ---- the entry list (the team's first wave; a deterministic order: pairs, then chunks)
@@ -426,18 +480,8 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
 #pragma unroll
       for (int c = 0; c < TPG; ++c) acc[a][b][c] = typename M::acc{0, 0, 0, 0};
 
-  double* ob = k.out + k.o_off[o];
-  const i64 Dl = k.Dl;
   if (nE == 0) {
-    // nothing to multiply: zeros over the reach, (P dlp) contiguous values per
-    // r', unless the block is zero from the previous execution already
-    if (!k.dirty[o]) return;
-    const int run = P * dlp / 2;  // 16-byte units per r'
-    for (int t = ttid; t < run * drp; t += nth) {
-      const int rq = t / run, u = t - rq * run;
-      reinterpret_cast<uint4*>(ob + (i64)P * Dl * rq)[u] = make_uint4(0, 0, 0, 0);
-    }
-    if (ttid == 0) k.dirty[o] = 0;
+    zero_block();
     return;
   }
   if (ttid == 0) k.dirty[o] = 1;
@@ -559,12 +603,6 @@ const void* heff_fn(int P, int big) {
   if (P == 8) return big ? (const void*)heff_kernel<T, 8, 2, 2, 8> : (const void*)heff_kernel<T, 8, 1, 1, 4>;
   return big ? (const void*)heff_kernel<T, 16, 2, 2, 8> : (const void*)heff_kernel<T, 16, 1, 1, 4>;
 }
-template <class T>
-const void* pack_psi_fn(int P) {
-  if (P == 4) return (const void*)heff_pack_psi<T, 4>;
-  if (P == 8) return (const void*)heff_pack_psi<T, 8>;
-  return (const void*)heff_pack_psi<T, 16>;
-}
 
 // everything the fused plan keeps on the device, and the host-side checks of its contract
 struct HeffState {
@@ -574,7 +612,9 @@ struct HeffState {
   bool fp32 = false;
   int P = 16;
   i64 nout = 0;
-  DBuf<i32> o_dlp, o_drp, o_pb, o_ib, pmask, dirty;
+  DBuf<i32> o_dlp, o_drp, o_pb, o_ib, pmask, dirty, active;
+  DBuf<int2> uses;
+  int epoch = 0;  // executions so far: the pack kernel marks the blocks with work with it
   DBuf<int4> jobs;
   i64 vout = -1;  // Hpsi's version after this plan's last execution (another value: Hpsi written outside)
   DBuf<int2> items;
@@ -605,30 +645,35 @@ void up(Ctx* ctx, DBuf<T>& d, const std::vector<T>& h) {
   if (!h.empty()) HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
 }
 
+template <class T>
+void launch_pack_psi_t(HeffState* s, hipStream_t st) {
+  PackK<T> k{};
+  k.psi = s->psi->dptr();
+  k.jobs = s->pjobs.get();
+  k.Dl = (int)s->psi->dims[4];
+  k.pp = (T*)s->pp.get();
+  k.pmask = s->pmask.get();
+  k.uses = s->uses.get();
+  k.pairs = s->pairs.get();
+  k.items = s->items.get();
+  k.active = s->active.get();
+  k.epoch = s->epoch;
+  const dim3 grid((unsigned)s->njobs), block(256);
+  if (s->P == 4)
+    hipLaunchKernelGGL((heff_pack_psi<T, 4>), grid, block, 0, st, k);
+  else if (s->P == 8)
+    hipLaunchKernelGGL((heff_pack_psi<T, 8>), grid, block, 0, st, k);
+  else
+    hipLaunchKernelGGL((heff_pack_psi<T, 16>), grid, block, 0, st, k);
+  HIP_CHECK(hipGetLastError());
+}
+
 void launch_pack_psi(HeffState* s, hipStream_t st) {
   if (!s->njobs) return;
-  void* pp = s->pp.get();
-  const double* psi = s->psi->dptr();
-  const PsiJob* jobs = s->pjobs.get();
-  i32* pm = s->pmask.get();
-  const int Dl = (int)s->psi->dims[4];
-  const dim3 grid((unsigned)s->njobs), block(256);
-  if (s->fp32) {
-    if (s->P == 4)
-      hipLaunchKernelGGL((heff_pack_psi<float, 4>), grid, block, 0, st, psi, jobs, Dl, (float*)pp, pm);
-    else if (s->P == 8)
-      hipLaunchKernelGGL((heff_pack_psi<float, 8>), grid, block, 0, st, psi, jobs, Dl, (float*)pp, pm);
-    else
-      hipLaunchKernelGGL((heff_pack_psi<float, 16>), grid, block, 0, st, psi, jobs, Dl, (float*)pp, pm);
-  } else {
-    if (s->P == 4)
-      hipLaunchKernelGGL((heff_pack_psi<double, 4>), grid, block, 0, st, psi, jobs, Dl, (double*)pp, pm);
-    else if (s->P == 8)
-      hipLaunchKernelGGL((heff_pack_psi<double, 8>), grid, block, 0, st, psi, jobs, Dl, (double*)pp, pm);
-    else
-      hipLaunchKernelGGL((heff_pack_psi<double, 16>), grid, block, 0, st, psi, jobs, Dl, (double*)pp, pm);
-  }
-  HIP_CHECK(hipGetLastError());
+  if (s->fp32)
+    launch_pack_psi_t<float>(s, st);
+  else
+    launch_pack_psi_t<double>(s, st);
 }
 
 template <class T>
@@ -654,6 +699,8 @@ void launch_main(HeffState* s, const HeffState::Launch& la, hipStream_t st) {
   k.wave_lds = la.wave_lds;
   k.o_ib = s->o_ib.get();
   k.dirty = s->dirty.get();
+  k.active = s->active.get();
+  k.epoch = s->epoch;
   void* args[] = {&k};
   HIP_CHECK(hipLaunchKernel(la.fn, dim3((unsigned)la.n), dim3(256), args, la.lds, st));
 }
@@ -834,6 +881,7 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   std::vector<int2> items;
   std::vector<i64> o_off((size_t)nout);
   std::vector<HPair> pairs;
+  std::vector<std::vector<int2>> uses_by_job;  // per psi job: (pair, output block) of the pairs reading it
   std::vector<std::array<int, 4>> pgeo;  // per pair: K1, N1 (psi chunk), nl, nr, for the LDS need
   for (i64 o = 0; o < nout; ++o) {
     const int a = by_id[(size_t)o].first, c = by_id[(size_t)o].second;
@@ -898,6 +946,8 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
           if ((c2.x & 31) < 4) hp.pc[c2.x & 31] |= (uint16_t)c2.y;
         items.insert(items.end(), its.begin(), its.end());
         pairs.push_back(hp);
+        if (uses_by_job.size() < pjobs.size()) uses_by_job.resize(pjobs.size());
+        uses_by_job[(size_t)hp.blk].push_back(make_int2((int)pairs.size() - 1, (int)o));
         pgeo.push_back({K1, N1, hp.nl, hp.nr});
       }
     }
@@ -955,7 +1005,18 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   up(ctx, st->o_off, o_off);
   up(ctx, st->pairs, pairs);
   up(ctx, st->items, items);
+  {
+    std::vector<int2> uses;
+    uses_by_job.resize(pjobs.size());
+    for (size_t j = 0; j < pjobs.size(); ++j) {
+      pjobs[j].u0 = (i32)uses.size();
+      pjobs[j].nu = (i32)uses_by_job[j].size();
+      uses.insert(uses.end(), uses_by_job[j].begin(), uses_by_job[j].end());
+    }
+    up(ctx, st->uses, uses);
+  }
   up(ctx, st->pjobs, pjobs);
+  up(ctx, st->active, std::vector<i32>((size_t)std::max<i64>(nout, 1), -1));
   st->pmask.alloc(ctx, (size_t)std::max<i64>(st->njobs, 1));
   // panels: zero (the padding stays zero), then filled from the operands
   auto zero_buf = [&](DBuf<unsigned char>& b, i64 elems) {
@@ -1156,6 +1217,7 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
                          if (s->L->version != s->vL || s->R->version != s->vR || s->W1->version != s->vW1 ||
                              s->W2->version != s->vW2)
                            fail(SB_E_ERROR, "H_eff plan: L, R, W1 or W2 changed since the plan was made");
+                         ++s->epoch;
                          launch_pack_psi(s, ctx->stream);
                          // Hpsi written outside this plan: its payload is cleared, padding
                          // included (the kernel writes the blocks' reach only), every block zero
