@@ -102,6 +102,9 @@ def parse():
     ap.add_argument("--serve5-plain-json", default="",
                     help="reuse the plain run of an earlier --serve-config5 output (same requests) instead of "
                          "running it")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="internal: three fused applications and a 256 MB copy, for the profiled child runs")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the in-run rocprofv3 traffic measurement")
     ap.add_argument("--only", default="", choices=["", "batch"],
                     help="run one section alone (batch: the batching configuration, for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -740,6 +743,66 @@ def coo_section(sb, a):
                 cpu_oracle_us=tcpu * 1e6, cpu_note="oracle C++ restatement, one host core")
 
 
+def traffic_probe(sb, a):
+    """the profiled child of measure_traffic: the fused plan applied three
+    times on the workload, then a plain 256 MB copy (the counters' scale)"""
+    prob = make_problem(a.workload)
+    prob.upload().outputs()
+    plan = prob.make_fused_plan(sb.PREC_FP64)
+    for _ in range(3):
+        plan.execute()
+    sb.synchronize()
+    calibration_copy(sb, 256)
+
+
+def measure_traffic():
+    """HBM bytes of one fused application, measured in this run: two
+    rocprofv3 --pmc runs of this script in probe mode (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), summed over the last application's
+    dispatches (the pack kernel and the fused kernel), each scaled by what the
+    same counter reads for a plain copy of 256 MB.  None when rocprofv3 is
+    missing or a run fails (the reason in the second value)."""
+    import glob
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None, "rocprofv3 not found"
+    out = {}
+    for cn in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="sb_traffic_")
+        cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", cn, "-d", d, "-o", "probe", "--", sys.executable,
+               os.path.abspath(__file__), "--traffic-probe"]
+        r = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, TMPDIR=d))
+        dbs = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+        if r.returncode != 0 or not dbs:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"rocprofv3 --pmc {cn} failed (exit {r.returncode})"
+        rows = []
+        for db in dbs:
+            c = sqlite3.connect(db)
+            rows += list(c.execute("select dispatch_id, kernel_name, value from counters_collection "
+                                   "where counter_name = ?", (cn,)))
+            c.close()
+        shutil.rmtree(d, ignore_errors=True)
+        per = {}
+        for did, kn, v in rows:
+            k = per.setdefault(did, [kn, 0.0])
+            k[1] += v
+        disp = [per[i] for i in sorted(per)]
+        copy = [v for kn, v in disp if "gather_blocks" in kn]
+        packs = [i for i, (kn, v) in enumerate(disp) if "heff_pack_psi" in kn]
+        if not copy or not packs:
+            return None, f"rocprofv3 --pmc {cn}: dispatches not found"
+        last = packs[-1]
+        kib = disp[last][1] + sum(v for kn, v in disp[last + 1:] if "heff_kernel" in kn)
+        scale = 256e6 / (copy[-1] * 1024.0)
+        out[cn] = dict(kib=kib, scale=scale, bytes=kib * 1024.0 * scale)
+    return out, None
+
+
 def calibration_copy(sb, mb):
     """one plain copy kernel of mb MB each way (an identity permute), the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters"""
@@ -759,6 +822,9 @@ def main():
         sys.exit(2)
     import sparsebackends_amd as sb
 
+    if a.traffic_probe:
+        traffic_probe(sb, a)
+        return
     if a.serve_config5 > 0:
         print(json.dumps(dict(serving_config5=serve5_section(sb, a))), flush=True)
         return
@@ -842,41 +908,46 @@ def main():
         return r
 
     if fused is not None:
-        # dominant kernel: the fused H_eff kernel; one application is one plan
-        # step (its size-class launches run concurrently, forked from and
-        # joined to the context's stream), timed by the events around it
+        # dominant kernel: the fused application (the psi pack kernel and the
+        # fused kernel, one plan step), timed by the events around it.  The
+        # fused algorithm multiplies only psi's nonzero physical rows: its
+        # useful FLOPs over its algorithmic bytes are far below the FP64 ridge
+        # (78.6 TFLOP/s / 8 TB/s = 9.8 FLOP/B), so its roofline is HBM.
         fst = fused.stats()
         mf = fused.mfma_flops()
         rec = prof["heff_fused"]
         avg_ms = rec["total_ms"] / rec["count"]
-        roof = dict(bound="mfma", achieved=mf / (avg_ms * 1e-3) / 1e12, peak=PEAK[a.prec], unit="TFLOP/s",
-                    kernel="heff_fused (heff_kernel launches of one application)", avg_launch_ms=avg_ms,
-                    note=(f"matrix-core FLOPs the launches execute per application ({mf / 1e9:.3f} GFLOP, tiles "
-                          f"padded to 16) over the application's time; useful FLOPs of the fused algorithm "
-                          f"{fst['flops'] / 1e9:.3f} GFLOP ({fst['flops'] / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s); "
-                          f"the four-call chain's algorithmic {alg / 1e9:.3f} GFLOP "
-                          f"({alg / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s); {fst['gemm_bytes'] / 1e6:.1f} MB "
-                          f"algorithmic traffic ({fst['gemm_bytes'] / (avg_ms * 1e-3) / 1e9:.0f} GB/s)"),
-                    useful_gflop=fst["flops"] / 1e9, mfma_gflop=mf / 1e9, algorithmic_bytes=fst["gemm_bytes"])
+        ab = fst["gemm_bytes"]
+        ai = fst["flops"] / ab
+        roof = dict(bound="hbm", achieved=ab / (avg_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                    kernel="heff_fused (heff_pack_psi + heff_kernel: one application)", avg_launch_ms=avg_ms,
+                    algorithmic_bytes=ab, arithmetic_intensity_flop_per_byte=ai,
+                    useful_gflop=fst["flops"] / 1e9, mfma_gflop=mf / 1e9,
+                    note=(f"algorithmic bytes per application: psi read inside the environments' reach, L and R "
+                          f"inside their reach, Hpsi written over every block's reach ({ab / 1e6:.1f} MB); useful "
+                          f"FLOPs of the fused algorithm {fst['flops'] / 1e9:.4f} GFLOP "
+                          f"({fst['flops'] / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s, {ai:.2f} FLOP/B), matrix-core "
+                          f"FLOPs issued {mf / 1e9:.3f} GFLOP ({mf / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s, "
+                          f"{mf / (avg_ms * 1e-3) / 1e12 / PEAK['fp64'] * 100:.1f} % of the FP64 matrix-core peak); "
+                          f"the four-call chain's algorithmic {alg / 1e9:.3f} GFLOP per application "
+                          f"({alg / (avg_ms * 1e-3) / 1e12:.2f} TFLOP/s, the work unit of value)"))
         roof["frac"] = roof["achieved"] / roof["peak"]
+        roof["traffic"] = None
+        if a.prec == "fp64" and not a.no_traffic:
+            tr, why = measure_traffic()
+            if tr is not None:
+                roof["traffic"] = tr["FETCH_SIZE"]["bytes"] + tr["WRITE_SIZE"]["bytes"]
+                roof["traffic_detail"] = dict(
+                    fetch_bytes=tr["FETCH_SIZE"]["bytes"], write_bytes=tr["WRITE_SIZE"]["bytes"],
+                    fetch_scale=tr["FETCH_SIZE"]["scale"], write_scale=tr["WRITE_SIZE"]["scale"],
+                    source="measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py "
+                           "--traffic-probe (child processes), the last application's dispatches, each counter "
+                           "scaled by a plain 256 MB copy", traffic_over_algorithmic=roof["traffic"] / ab)
+            else:
+                roof["traffic_note"] = why
     else:
         roof = chain_roofline(prof, dt)
-    roof["traffic"] = None
-    # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-    # workload, kept beside the profile summary; taken only when it was
-    # measured on this very library build (its sha256), else null
-    tf = os.path.join(ROOT, "profiles", "heff_fused_hubbard_chi2048.traffic.json")
-    if fused is not None and a.prec == "fp64" and os.path.exists(tf):
-        import hashlib
-        with open(tf) as fh:
-            t = json.load(fh)
-        with open(sb.LIB_PATH, "rb") as fh:
-            lib_sha = hashlib.sha256(fh.read()).hexdigest()
-        if t.get("workload") == prob_desc and t.get("library_sha256") == lib_sha:
-            roof["traffic"] = t["fetch_bytes"] + t["write_bytes"]
-            roof["traffic_unit"] = "bytes per application (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/)"
-        else:
-            roof["traffic_note"] = "profiles/ traffic was measured on another build: not used"
+        roof["traffic"] = None
 
     out = {
         "metric": "H_eff·ψ TFLOP/s FP64 vs mixed at |ΔE|≤1e-10; DMRG solves/s + p99 latency",
