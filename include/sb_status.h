@@ -32,6 +32,25 @@
 #define SB_KIND_DENSE       3  /* plain Array (column-major) */
 #define SB_KIND_ALIASED     4  /* AliasedBlockSparse    (aliased/storage.jl:46-59) */
 
+/* declared key integer types (rekey) and payload element types (retypeval):
+ * keys are stored as int32 and payloads as float64 on the device whatever
+ * the declared type; sb_tensor_types reports the declaration */
+#define SB_KEY_INT64   0  /* the reference's default K = Int */
+#define SB_KEY_INT32   1
+#define SB_KEY_INT16   2
+#define SB_KEY_INT8    3
+#define SB_KEY_UINT64  4
+#define SB_KEY_UINT32  5
+#define SB_KEY_UINT16  6
+#define SB_KEY_UINT8   7
+#define SB_VAL_FLOAT64 0  /* the default T = Float64 */
+#define SB_VAL_FLOAT32 1
+#define SB_VAL_FLOAT16 2
+#define SB_VAL_INT64   3
+#define SB_VAL_INT32   4
+#define SB_VAL_INT16   5
+#define SB_VAL_INT8    6
+
 /* contraction flags.  With flags == 0 every kernel follows the reference
  * code, including its documented divergences from the dense mathematics. */
 /* multi-label BS x BS merge-join: join on the order the keys are sorted in,

@@ -272,6 +272,23 @@ int sb_batch_autotune(sb_ctx_t ctx, sb_batch_t batch);
 int sb_batch_stage_tile(sb_batch_t batch, int stage, int* tile, double* tuned_ms);
 int sb_batch_free(sb_batch_t batch);
 
+/* ------------------------------------------------------------ key and value type conversions
+ * The device stores keys as int32 and payloads as float64 whatever the
+ * declared types; these check the reference's conditions on the device and
+ * return a copy declaring the new type (read it back with sb_tensor_types).
+ * A failed check returns SB_E_ERROR with the reference's message. */
+/* rekey(A, K2; check) (blocksparse/conversions.jl:327-342 for NewBlockSparseSorted,
+ * coo/conversions.jl:41-53 for COOTensor, aliased/conversions.jl:76-93 for
+ * AliasedBlockSparse): key_type one of SB_KEY_*; with check, every prefix
+ * dimension (block-sparse, aliased) and key coordinate must fit K2 */
+int sb_rekey(sb_ctx_t ctx, sb_tensor_t A, int key_type, int check, sb_tensor_t* out);
+/* retypeval(A::COOTensor, T2) (coo/conversions.jl:65-82): val_type one of
+ * SB_VAL_*; every value must survive the round trip Float64(T2(v)) == v
+ * (integer T2: v integral and inside T2's range) */
+int sb_coo_retypeval(sb_ctx_t ctx, sb_tensor_t A, int val_type, sb_tensor_t* out);
+/* the declared key and payload types of a tensor (SB_KEY_*, SB_VAL_*) */
+int sb_tensor_types(sb_tensor_t t, int* key_type, int* val_type);
+
 /* ------------------------------------------------------------ Krylov vector operations
  * The storage hooks a Krylov solver reaches through ITensors' ExternalStorage
  * (SparseBackends/src/tensor_wrappers.jl:792-923), on block-sparse tensors. */

@@ -36,6 +36,9 @@ SIGNATURES = {
     "sb_ctx_profile_read": [P, C.c_char_p, C.c_int64, I64P],
     "sb_tensor_free": [P],
     "sb_tensor_kind": [P, I32P],
+    "sb_tensor_types": [P, I32P, I32P],
+    "sb_rekey": [P, P, C.c_int, C.c_int, PP],
+    "sb_coo_retypeval": [P, P, C.c_int, PP],
     "sb_bs_new": [P, C.c_int, C.c_int, I64P, PP],
     "sb_bs_from_host": [P, C.c_int, C.c_int, I64P, C.c_int64, I64P, I64P, F64P, PP],
     "sb_bs_info": [P, I32P, I32P, I64P, I64P, I64P],

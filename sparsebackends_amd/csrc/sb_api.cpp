@@ -6,6 +6,7 @@
 
 #include "../../include/sparsebackends_amd.h"
 #include "sb_batch.h"
+#include "sb_convert.h"
 #include "sb_factorize.h"
 #include "sb_graph.h"
 #include "sb_krylov.h"
@@ -150,6 +151,18 @@ int sb_ctx_profile_read(sb_ctx_t c, char* buf, int64_t cap, int64_t* needed) {
 
 int sb_tensor_free(sb_tensor_t t) { SB_TRY(delete T_(t)) }
 int sb_tensor_kind(sb_tensor_t t, int* kind) { SB_TRY(*kind = T_(t)->kind) }
+int sb_tensor_types(sb_tensor_t t, int* key_type, int* val_type) {
+  SB_TRY({
+    if (key_type) *key_type = T_(t)->key_type;
+    if (val_type) *val_type = T_(t)->val_type;
+  })
+}
+int sb_rekey(sb_ctx_t c, sb_tensor_t A, int key_type, int check, sb_tensor_t* out) {
+  SB_TRY(*out = H_(tensor_rekey(C_(c), *T_(A), key_type, check != 0)))
+}
+int sb_coo_retypeval(sb_ctx_t c, sb_tensor_t A, int val_type, sb_tensor_t* out) {
+  SB_TRY(*out = H_(coo_retypeval(C_(c), *T_(A, COO), val_type)))
+}
 
 // ---------------------------------------------------------------- block-sparse
 int sb_bs_new(sb_ctx_t c, int N, int N2, const int64_t* dims, sb_tensor_t* out) {

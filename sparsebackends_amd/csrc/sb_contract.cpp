@@ -4,6 +4,8 @@
 // block allocation; all integer kernels on the GPU, bit-exact) and a numeric
 // phase (operand permutes + grouped block products).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -2438,6 +2440,45 @@ static void coo_dense_aliased(Plan& pl, Tensor& C, const Labels& labelsC, CooOpe
 }
 
 // ================================================================ dispatcher
+
+// SB_TRACE=1 in the environment: one line per contract! call naming the
+// storage kinds, block counts and the kernel family taken, as the reference
+// prints its dispatch (contract_bs_bs.jl:138-143, contract_bs_dense.jl:225-230,
+// contract_coo_dense.jl:59-63, contract_coo_bs.jl:29-34), followed by what the
+// plan holds: matched block pairs, output blocks, FLOPs
+bool trace_on() {
+  static const bool on = [] {
+    const char* v = std::getenv("SB_TRACE");
+    return v && std::string(v) == "1";
+  }();
+  return on;
+}
+
+namespace {
+
+std::string trace_operand(const Tensor& T) {
+  char b[160];
+  if (T.kind == BS)
+    std::snprintf(b, sizeof b, "BS(blocks=%lld, P=%d, N2=%d)", (long long)T.n, T.P(), T.N2);
+  else if (T.kind == COO)
+    std::snprintf(b, sizeof b, "COO(nnz=%lld, N=%d)", (long long)T.n, (int)T.dims.size());
+  else if (T.kind == ALIASED)
+    std::snprintf(b, sizeof b, "AliasedBS(blocks=%lld, P=%d, N2=%d)", (long long)T.n, T.P(), T.N2);
+  else
+    std::snprintf(b, sizeof b, "Dense(N=%d)", (int)T.dims.size());
+  return b;
+}
+
+void trace_dispatch(const char* entry, const Tensor& A, const Tensor& B, int axis, const char* family,
+                    const Plan& pl, const Tensor& C) {
+  std::printf("[SB_TRACE] %s  %s x %s  axis=%d  -> %s  pairs=%lld  C.blocks=%lld  flops=%.6g\n", entry,
+              trace_operand(A).c_str(), trace_operand(B).c_str(), axis, family, (long long)pl.npairs,
+              (long long)C.n, pl.flops);
+  std::fflush(stdout);
+}
+
+}  // namespace
+
 std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, const Tensor& A,
                                     const Labels& labelsA, const Tensor& B, const Labels& labelsB, unsigned flags) {
   SB_ASSERT(labelsA.size() == A.dims.size(), "labelsA length must match A rank");
@@ -2456,6 +2497,9 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
   for (auto& l : shared)
     if (!contains(labelsC, l)) reduced.push_back(l);
   Oper Ao{&A, labelsA, mapA}, Bo{&B, labelsB, mapB};
+  const char* entry = "contract!";
+  const char* family = "";
+  int axis = 0;
   if (reduced.size() == 1) {
     SB_ASSERT(shared.size() == 1, "Expected exactly 1 shared label");
     const Label& r = reduced[0];
@@ -2465,61 +2509,91 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
     if (C.kind == BS && A.kind == BS && B.kind == BS) {
       // contract! BS x BS (contract_bs_bs.jl:126-151)
       int axisBr = at(mapB, r);
+      entry = "contract_bs_bs.contract!";
+      axis = axisBr;
       if (axisBr <= B.P()) {
+        family = "prefix_outer";
         permute_r_to_last_prefix(*pl, Ao, r);
         permute_r_to_last_prefix(*pl, Bo, r);
         prefix_outer_bb(*pl, C, labelsC, Ao, Bo, r);
       } else {
+        family = "dense_bb";
         dense_bb(*pl, C, labelsC, Ao, Bo, r, flags);
       }
     } else if (C.kind == BS && A.kind == BS && B.kind == DENSE) {
       // contract! BS x Dense (contract_bs_dense.jl:207-245)
       SB_ASSERT(has(mapA, r), "rlab not found in mapA");
       SB_ASSERT(has(mapB, r), "rlab not found in mapB");
+      entry = "contract_bs_dense.contract!";
+      axis = at(mapA, r);
       if (at(mapA, r) <= A.P()) {
+        family = "prefix_outer_bd";
         permute_r_to_last_prefix(*pl, Ao, r);
         prefix_outer_bd(*pl, C, labelsC, Ao, Bo, r);
       } else {
+        family = "dense_bd";
         dense_bd(*pl, C, labelsC, Ao, Bo, r, flags);
       }
     } else if (C.kind == COO && A.kind == COO && B.kind == COO) {
       CooOper Ac{&A, labelsA, mapA}, Bc{&B, labelsB, mapB};
+      entry = "contract_coo_coo.contract!";
+      family = "coo_coo";
       contract_coo_coo(*pl, C, labelsC, Ac, Bc, r);
     } else if (C.kind == BS && A.kind == COO && B.kind == BS) {
       // contract! COO x BS (contract_coo_bs.jl:14-46)
       CooOper Ac{&A, labelsA, mapA};
       coo_rlab_last_sorted(*pl, Ac, r);
+      entry = "contract_coo_bs.contract!";
+      axis = at(mapB, r);
       if (at(mapB, r) <= B.P()) {
+        family = "prefix";
         permute_r_to_last_prefix(*pl, Bo, r);
         coo_bs_prefix(*pl, C, labelsC, Ac, Bo, r);
       } else {
+        family = "dense_index";
         coo_bs_dense_index(*pl, C, labelsC, Ac, Bo, r);
       }
     } else if (C.kind == BS && A.kind == COO && B.kind == DENSE) {
       CooOper Ac{&A, labelsA, mapA};
+      entry = "contract_coo_dense.contract!";
+      family = "coo_dense";
       contract_coo_dense(*pl, C, labelsC, Ac, Bo, r, flags);
     } else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE) {
+      entry = "contract_aliased!";
+      family = "aliased_dense";
       contract_aliased_ad(*pl, C, labelsC, Ao, Bo, r, flags);  // contract_aliased_shared.jl:535-547
     } else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED) {
+      entry = "contract_aliased!";
+      family = "aliased_aliased";
       contract_aliased_aa(*pl, C, labelsC, Ao, Bo, r, flags);  // contract_aliased_shared.jl:550-562
     } else if (C.kind == ALIASED && A.kind == COO && B.kind == DENSE) {
       CooOper Ac{&A, labelsA, mapA};
+      entry = "contract_aliased_coo_dense.contract_aliased!";
+      family = "coo_dense_aliased";
       coo_dense_aliased(*pl, C, labelsC, Ac, Bo, r);  // tensor_wrappers_aliased.jl:263-275
     } else {
       fail(SB_E_METHOD, "MethodError: no single-label contract! method for these storage kinds");
     }
   } else {
-    if (C.kind == BS && A.kind == BS && B.kind == BS)
+    entry = "contract_shared!";
+    axis = (int)shared.size();
+    if (C.kind == BS && A.kind == BS && B.kind == BS) {
+      family = "shared_bs_bs";
       shared_bs_bs(*pl, C, labelsC, Ao, Bo, shared, flags);
-    else if (C.kind == BS && A.kind == BS && B.kind == DENSE)
+    } else if (C.kind == BS && A.kind == BS && B.kind == DENSE) {
+      family = "shared_bs_dense";
       shared_bs_dense(*pl, C, labelsC, Ao, Bo, shared, flags);
-    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE)
+    } else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == DENSE) {
+      family = "shared_aliased_dense";
       shared_al_dense(*pl, C, labelsC, Ao, Bo, shared, flags);
-    else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED)
+    } else if (C.kind == ALIASED && A.kind == ALIASED && B.kind == ALIASED) {
+      family = "shared_aliased_aliased";
       shared_al_al(*pl, C, labelsC, Ao, Bo, shared, flags);
-    else
+    } else {
       fail(SB_E_METHOD, "MethodError: no contract_shared! method for these storage kinds");
+    }
   }
+  if (trace_on()) trace_dispatch(entry, A, B, axis, family, *pl, C);
   return pl;
 }
 

@@ -130,6 +130,9 @@ std::unique_ptr<Plan> contract_plan(Ctx* ctx, Tensor& C, const Labels& labelsC, 
 // The two-site H_eff*psi (the four-call chain psi*L*W1*W2*R of two-site DMRG)
 // as one plan with one fused kernel (sb_heff.hip); out gets the chain's block
 // list, and every execution psi's current H_eff image.
+// SB_TRACE=1 in the environment: planning prints its dispatch decisions
+bool trace_on();
+
 std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const Tensor& L, const Tensor& W1,
                                 const Tensor& W2, const Tensor& R, unsigned flags);
 

@@ -46,6 +46,7 @@
 // reached by a pair, ids in the order its merge walk first meets them.
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -1253,6 +1254,13 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   pl->out_box = true;
   pl->in_a = &psi;
   pl->in_b = &L;
+  if (trace_on()) {
+    std::printf("[SB_TRACE] heff_plan  psi BS(blocks=%lld) L BS(blocks=%lld) R BS(blocks=%lld)  -> fused H_eff*psi  "
+                "pairs=%lld  items=%lld  Hpsi.blocks=%lld  launches=%zu  flops=%.6g (psi's rows at planning)\n",
+                (long long)psi.n, (long long)L.n, (long long)R.n, (long long)pairs.size(), (long long)items.size(),
+                (long long)nout, st->launches.size(), flops);
+    std::fflush(stdout);
+  }
   return pl;
 }
 

@@ -169,6 +169,8 @@ struct Tensor {
   DBuf<double> data;    // payload
   double* borrowed = nullptr;  // dense tensor viewing caller memory
   bool dirty = false;   // COO
+  int key_type = 0;     // declared key type (SB_KEY_*; the device keys are int32)
+  int val_type = 0;     // declared payload type (SB_VAL_*; the device payload is float64)
   // aliased block-sparse (aliased/storage.jl:46-59)
   DBuf<double> scalars;
   i64 ntemplates = 0;

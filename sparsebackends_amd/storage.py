@@ -159,6 +159,14 @@ def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
+_status = check  # the status check, for functions whose keyword argument is named check
+
+# declared key and payload types (include/sb_status.h SB_KEY_*, SB_VAL_*): the
+# device holds int32 keys and float64 payloads; parts() returns the declared ones
+KEY_TYPES = [np.int64, np.int32, np.int16, np.int8, np.uint64, np.uint32, np.uint16, np.uint8]
+VAL_TYPES = [np.float64, np.float32, np.float16, np.int64, np.int32, np.int16, np.int8]
+
+
 class _Tensor:
     kind = 0
 
@@ -195,6 +203,21 @@ class _Tensor:
         p = _i64(perm)
         check(load().sb_permutedims_inplace(self.ctx, self._h, _ptr(p, C.c_int64)))
         return self
+
+    def types(self):
+        """(key dtype, value dtype) the tensor declares (rekey / retypeval change them)"""
+        k, v = C.c_int(), C.c_int()
+        check(load().sb_tensor_types(self._h, C.byref(k), C.byref(v)))
+        return KEY_TYPES[k.value], VAL_TYPES[v.value]
+
+    def _declared(self, d):
+        kt, vt = self.types()
+        if "keys" in d:
+            d["keys"] = d["keys"].astype(kt)
+        for f in ("data", "vals", "templates", "scalars"):
+            if f in d and vt is not np.float64:
+                d[f] = d[f].astype(vt)
+        return d
 
 
 class DenseTensor(_Tensor):
@@ -295,7 +318,7 @@ class NewBlockSparseSorted(_Tensor):
         data = np.zeros(nb * bs, dtype=np.float64)
         check(load().sb_bs_to_host(self.ctx, self._h, _ptr(keys, C.c_int64), _ptr(ids, C.c_int64),
                                    _ptr(data, C.c_double)))
-        return dict(dims=dims, N2=N2, blksize=bs, keys=keys, ids=ids, data=data)
+        return self._declared(dict(dims=dims, N2=N2, blksize=bs, keys=keys, ids=ids, data=data))
 
     @property
     def keys(self):
@@ -468,7 +491,8 @@ class AliasedBlockSparse(_Tensor):
         tm = np.zeros(nt * bs, dtype=np.float64)
         check(load().sb_al_to_host(self.ctx, self._h, _ptr(keys, C.c_int64), _ptr(ids, C.c_int64),
                                    _ptr(sc, C.c_double), _ptr(tm, C.c_double)))
-        return dict(dims=dims, N2=N2, blksize=bs, ntemplates=nt, keys=keys, alias_ids=ids, scalars=sc, templates=tm)
+        return self._declared(dict(dims=dims, N2=N2, blksize=bs, ntemplates=nt, keys=keys, alias_ids=ids,
+                                   scalars=sc, templates=tm))
 
     def __getitem__(self, I):
         a = _i64(I)
@@ -523,7 +547,8 @@ class COOTensor(_Tensor):
         keys = np.zeros((nnz.value, N.value), dtype=np.int64)
         vals = np.zeros(nnz.value, dtype=np.float64)
         check(load().sb_coo_to_host(self.ctx, self._h, _ptr(keys, C.c_int64), _ptr(vals, C.c_double)))
-        return dict(dims=tuple(int(x) for x in d[:N.value]), keys=keys, vals=vals, dirty=bool(dirty.value))
+        return self._declared(dict(dims=tuple(int(x) for x in d[:N.value]), keys=keys, vals=vals,
+                                   dirty=bool(dirty.value)))
 
     @property
     def dims(self):
@@ -570,6 +595,27 @@ class COOTensor(_Tensor):
         out = C.c_void_p()
         check(load().sb_coo_fuse_axes(self.ctx, self._h, ax1, ax2, C.byref(out)))
         return COOTensor(out, self.device)
+
+
+def rekey(A, K2, check=True):
+    """rekey(A, K2; check=true) (blocksparse/conversions.jl:327-342, coo/conversions.jl:41-53,
+    aliased/conversions.jl:76-93): a copy whose keys have the integer type K2 (a numpy
+    integer type); with check, an ErrorException (SBError) when a prefix dimension or a
+    key coordinate does not fit K2."""
+    code = [np.dtype(t) for t in KEY_TYPES].index(np.dtype(K2))
+    out = C.c_void_p()
+    _status(load().sb_rekey(A.ctx, A.handle, code, 1 if check else 0, C.byref(out)))
+    return type(A)(out, A.device)
+
+
+def retypeval(A, T2):
+    """retypeval(A::COOTensor, T2) (coo/conversions.jl:65-82): a copy whose values have
+    the type T2 (a numpy type); ErrorException (SBError) when a value does not survive
+    the round trip Float64(T2(v)) == v."""
+    code = [np.dtype(t) for t in VAL_TYPES].index(np.dtype(T2))
+    out = C.c_void_p()
+    check(load().sb_coo_retypeval(A.ctx, A.handle, code, C.byref(out)))
+    return COOTensor(out, A.device)
 
 
 def blocksparse_from_dense(arr, N2, atol=0.0, rtol=0.0, dropzeros=True, device=0):
