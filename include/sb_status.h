@@ -25,6 +25,8 @@
 #define SB_E_NOMEM       8  /* device or host allocation failed */
 #define SB_E_HANDLE      9  /* null handle or handle of the wrong kind */
 #define SB_E_LIMIT      10  /* a size beyond what the library supports */
+#define SB_E_NOCONV     11  /* LAPACKException: an SVD did not converge (the sweep cap of the
+                               one-sided Jacobi, sb_ctx_set_svd_sweeps, was reached) */
 
 /* tensor kinds (the reference's storage types) */
 #define SB_KIND_BLOCKSPARSE 1  /* NewBlockSparseSorted  (blocksparse/storage.jl:3-9) */

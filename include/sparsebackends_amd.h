@@ -59,6 +59,10 @@ int sb_ctx_trim(sb_ctx_t ctx);
  * timing then disturbs the stream least.  Plan chains run eagerly (no graph
  * replays) while profiling is on. */
 int sb_ctx_profile(sb_ctx_t ctx, int enable);
+/* the sweep cap of the one-sided Jacobi SVDs (blocksparse_svd, the sector SVD;
+ * default 40): a factorisation still rotating after it returns SB_E_NOCONV,
+ * as the reference's LAPACK SVD raises LAPACKException when it does not converge */
+int sb_ctx_set_svd_sweeps(sb_ctx_t ctx, int sweeps);
 int sb_ctx_profile_reset(sb_ctx_t ctx);
 /* synchronises; writes lines "name count total_ms min_ms max_ms" into buf
  * (NUL-terminated, at most cap bytes); *needed = bytes the full text takes */

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libsparsebackends_amd.so")
 
 ERROR_NAMES = {1: "AssertionError", 2: "ErrorException", 3: "ArgumentError", 4: "KeyError",
                5: "MethodError", 6: "BoundsError", 7: "DeviceError", 8: "OutOfMemory",
-               9: "InvalidHandle", 10: "LimitExceeded"}
+               9: "InvalidHandle", 10: "LimitExceeded", 11: "LAPACKException"}
 
 # every entry point the header declares: (name, argtypes)
 P = C.c_void_p
@@ -32,6 +32,7 @@ SIGNATURES = {
     "sb_ctx_memory": [P, I64P, I64P],
     "sb_ctx_trim": [P],
     "sb_ctx_profile": [P, C.c_int],
+    "sb_ctx_set_svd_sweeps": [P, C.c_int],
     "sb_ctx_profile_reset": [P],
     "sb_ctx_profile_read": [P, C.c_char_p, C.c_int64, I64P],
     "sb_tensor_free": [P],

@@ -150,6 +150,12 @@ int sb_ctx_profile_read(sb_ctx_t c, char* buf, int64_t cap, int64_t* needed) {
 }
 
 int sb_tensor_free(sb_tensor_t t) { SB_TRY(delete T_(t)) }
+int sb_ctx_set_svd_sweeps(sb_ctx_t c, int sweeps) {
+  SB_TRY({
+    if (sweeps < 1 || sweeps > 10000) fail(SB_E_ARGUMENT, "ArgumentError: the SVD sweep cap must be in 1..10000");
+    C_(c)->svd_sweeps = sweeps;
+  })
+}
 int sb_tensor_kind(sb_tensor_t t, int* kind) { SB_TRY(*kind = T_(t)->kind) }
 int sb_tensor_types(sb_tensor_t t, int* key_type, int* val_type) {
   SB_TRY({

@@ -10,8 +10,16 @@
 
 namespace sb {
 
+// the sweep cap of a one-sided Jacobi SVD was reached with rows still
+// rotating: the reference's LAPACK SVD raises LAPACKException when it fails
+// to converge
+[[noreturn]] void no_convergence(Ctx* ctx) {
+  fail(SB_E_NOCONV, "LAPACKException: the one-sided Jacobi SVD did not converge in " +
+                        std::to_string(ctx->svd_sweeps) + " sweeps");
+}
+
 void k_svd_jacobi(Ctx*, const double* A, int d1, int d2, i64 ng, const i64* gstart, const i64* bsrc, const i64* woff,
-                  double* W, double* sig, double* Uout, int* perm);
+                  double* W, double* sig, double* Uout, int* perm, int* noconv);
 void k_svd_emit(Ctx*, i64 k, int d1, const i32* kg, const i32* kc, const i64* soff, const i64* gstart, int d2,
                 const i64* woff, const double* W, const double* sig, const double* Uout, const int* perm, bool right,
                 double* Ud, double* SVd);
@@ -21,8 +29,8 @@ void k_sect_gather(Ctx*, const double* psi, int d1, int d2, i64 Dl, i64 ntask, c
 size_t sect_svd_lds(int n, int len);
 void k_sect_svd(Ctx*, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                double* SVo, const int* glist);
-void k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& Rs, const std::vector<int>& Cs,
+                double* SVo, const int* glist, int* noconv);
+bool k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& Rs, const std::vector<int>& Cs,
                      double* X, const i64* xoff, const i64* uoff, const i64* soff, const i64* sgoff, double* sig,
                      double* Uo, double* SVo);
 void k_sect_emit(Ctx*, bool isA, i64 nblk, int d, i64 D1, i64 D2, i64 nside, const i64* blk, const i64* sec,
@@ -83,12 +91,16 @@ SvdResult bs_svd(Ctx* ctx, const Tensor& A, bool right, i64 maxdim, i64 mindim, 
   up(ctx, d_bsrc, bsrc);
   up(ctx, d_woff, woff);
   DBuf<double> W(ctx, n * bs ? n * bs : 1), sig(ctx, ng * d1 ? ng * d1 : 1), Uo(ctx, ng * d1 * d1 ? ng * d1 * d1 : 1);
-  DBuf<int> perm(ctx, ng * d1 ? ng * d1 : 1);
+  DBuf<int> perm(ctx, ng * d1 ? ng * d1 : 1), noconv(ctx, 1);
+  k_zero(ctx, noconv.get(), sizeof(int));
   k_svd_jacobi(ctx, A.dptr(), d1, d2, ng, d_gstart.get(), d_bsrc.get(), d_woff.get(), W.get(), sig.get(), Uo.get(),
-               perm.get());
+               perm.get(), noconv.get());
   std::vector<double> hs;
   down(ctx, hs, sig.get(), (size_t)(ng * d1));
+  int hnc = 0;
+  HIP_CHECK(hipMemcpyAsync(&hnc, noconv.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (hnc) no_convergence(ctx);
 
   // global order and truncation (ops_factorize.jl:86-104)
   struct SV {
@@ -289,15 +301,19 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   k_sect_gather(ctx, psi.dptr(), d1, d2, Dl, ntask, d_task.get(), d_xoff.get(), d_R.get(), d_C.get(), d_gtr.get(),
                 X.get());
   {
-    DBuf<int> d_lg;
+    DBuf<int> d_lg, noconv(ctx, 1);
+    k_zero(ctx, noconv.get(), sizeof(int));
     if (!lds_g.empty()) {
       up(ctx, d_lg, lds_g);
       k_sect_svd(ctx, (i64)lds_g.size(), lds, X.get(), d_xoff.get(), d_R.get(), d_C.get(), d_tr.get(), d_uoff.get(),
-                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), d_lg.get());
+                 d_soff.get(), d_sgoff.get(), sig.get(), Uo.get(), SVo.get(), d_lg.get(), noconv.get());
     }
-    k_sect_svd_qrcp(ctx, big_g, qp_R, qp_C, X.get(), d_xoff.get(), d_uoff.get(), d_soff.get(), d_sgoff.get(),
-                    sig.get(), Uo.get(), SVo.get());
+    const bool nc_big = k_sect_svd_qrcp(ctx, big_g, qp_R, qp_C, X.get(), d_xoff.get(), d_uoff.get(), d_soff.get(),
+                                        d_sgoff.get(), sig.get(), Uo.get(), SVo.get());
+    int hnc = 0;
+    HIP_CHECK(hipMemcpyAsync(&hnc, noconv.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host list goes out of scope
+    if (hnc || nc_big) no_convergence(ctx);
   }
   std::vector<double> hs;
   down(ctx, hs, sig.get(), (size_t)sgoff[ng]);

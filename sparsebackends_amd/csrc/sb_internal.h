@@ -107,6 +107,7 @@ struct Ctx {
   // kernel timing with HIP events on the stream: 0 off, 1 every timed step,
   // 2 only the block products (grouped_gemm), the fewest events
   int profile = 0;
+  int svd_sweeps = 40;  // sweep cap of the one-sided Jacobi SVDs (SB_E_NOCONV when reached)
   int graphs = -1;  // eigensolves replay their operator as a HIP graph: 1 always, 0 never, -1 for long solves
   bool profiled(const char* name) const;
   // event pool and profiling records: several host threads may share a

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict
                                                        const i64* __restrict__ gstart, const i64* __restrict__ bsrc,
                                                        const i64* __restrict__ woff, double* __restrict__ W,
                                                        double* __restrict__ sig, double* __restrict__ Uout,
-                                                       int* __restrict__ perm) {
+                                                       int* __restrict__ perm, int max_sweeps,
+                                                       int* __restrict__ noconv) {
   __shared__ double U[SVD_MAXD * SVD_MAXD];
   __shared__ double s_sig[SVD_MAXD];
   const int g = blockIdx.x, lane = threadIdx.x;
@@ -117,7 +118,8 @@ __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict
   double fro2 = 0;
   for (i64 e = lane; e < (i64)d1 * nc; e += 64) fro2 = fma(Wg[e], Wg[e], fro2);
   const double null2 = jacobi_null2(wave_sum(fro2));
-  for (int sweep = 0; sweep < 40; ++sweep) {
+  bool conv = false;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     bool rotated = false;
     for (int p = 0; p < d1 - 1; ++p)
       for (int q = p + 1; q < d1; ++q) {
@@ -150,8 +152,12 @@ __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict
         rotated = true;
         __syncthreads();  // row and U updates visible before the next pair
       }
-    if (!rotated) break;
+    if (!rotated) {
+      conv = true;
+      break;
+    }
   }
+  if (!conv && d1 > 1 && lane == 0) atomicOr(noconv, 1);
   // row norms = singular values
   for (int r = 0; r < d1; ++r) {
     const double* rr = Wg + (i64)r * nc;
@@ -186,11 +192,11 @@ __global__ __launch_bounds__(64) void svd_jacobi_kernel(const double* __restrict
 }
 
 void k_svd_jacobi(Ctx* ctx, const double* A, int d1, int d2, i64 ng, const i64* gstart, const i64* bsrc,
-                  const i64* woff, double* W, double* sig, double* Uout, int* perm) {
+                  const i64* woff, double* W, double* sig, double* Uout, int* perm, int* noconv) {
   if (!ng) return;
   SB_ASSERT(d1 >= 1 && d1 <= SVD_MAXD, "blocksparse_svd: first dense dim above 32");
   hipLaunchKernelGGL(svd_jacobi_kernel, dim3((unsigned)ng), dim3(64), 0, ctx->stream, A, d1, d2, gstart, bsrc, woff,
-                     W, sig, Uout, perm);
+                     W, sig, Uout, perm, ctx->svd_sweeps, noconv);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -333,7 +339,7 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
                                                              const i64* __restrict__ uoff, const i64* __restrict__ soff,
                                                              const i64* __restrict__ sgoff, double* __restrict__ sig,
                                                              double* __restrict__ Uo, double* __restrict__ SVo,
-                                                             const int* __restrict__ glist) {
+                                                             const int* __restrict__ glist, int max_sweeps, int* __restrict__ noconv) {
   extern __shared__ double lds[];
   const int g = glist[blockIdx.x];
   const bool tr = trans[g] != 0;
@@ -352,7 +358,8 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
     for (int r = lane; r < n; r += 64) Q[r + (i64)n * c] = r == c ? 1.0 : 0.0;
   const double null2 = jacobi_null2(sect_fro2(Xg, (i64)n * len));
   const double ctol = jacobi_tol(len);
-  for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
+  bool conv = n <= 1;
+  for (int sweep = 0; sweep < max_sweeps && n > 1; ++sweep) {
     if (tid == 0) *flag = 0;
     __syncthreads();
     for (int rd = 0; rd < m; ++rd) {
@@ -369,8 +376,12 @@ __global__ __launch_bounds__(SS_THREADS) void sect_svd_kernel(const double* __re
     }
     const int f = *flag;
     __syncthreads();
-    if (!f) break;
+    if (!f) {
+      conv = true;
+      break;
+    }
   }
+  if (!conv && tid == 0) atomicOr(noconv, 1);
   sect_emit(Xs, Q, n, len, tr, Rg, Cg, nrm, ord, sig + sgoff[g], Uo + uoff[g], SVo + soff[g]);
 }
 
@@ -380,11 +391,11 @@ size_t sect_svd_lds(int n, int len) {  // LDS bytes of a sector in sect_svd_kern
 
 void k_sect_svd(Ctx* ctx, i64 ng, size_t lds, const double* X, const i64* xoff, const int* R, const int* Cc,
                 const int* trans, const i64* uoff, const i64* soff, const i64* sgoff, double* sig, double* Uo,
-                double* SVo, const int* glist) {
+                double* SVo, const int* glist, int* noconv) {
   if (!ng) return;
   HIP_CHECK(hipFuncSetAttribute((const void*)sect_svd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(sect_svd_kernel, dim3((unsigned)ng), dim3(SS_THREADS), lds, ctx->stream, X, xoff, R, Cc, trans,
-                     uoff, soff, sgoff, sig, Uo, SVo, glist);
+                     uoff, soff, sgoff, sig, Uo, SVo, glist, ctx->svd_sweeps, noconv);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -425,6 +436,7 @@ struct BjRun {
   std::vector<BjSector> hs;
   DBuf<BjSector> d_sec;
   DBuf<double> Qw;
+  bool unconverged = false;  // the sweep cap was reached with rows still rotating
 };
 
 constexpr int BJ_MAXBS = 8;  // rows per block: eight waves, one row pair each per step
@@ -830,7 +842,8 @@ void bj_jacobi(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn,
   if (ch == 16) HIP_CHECK(hipFuncSetAttribute((const void*)sect_bj_cross_kernel<16>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
   std::vector<int> hf(nb);
-  for (int sweep = 0; sweep < 40; ++sweep) {
+  run.unconverged = true;
+  for (int sweep = 0; sweep < ctx->svd_sweeps; ++sweep) {
     // every (round, pair) flag of the sweep is written by its round launch
     for (int rd = 0; rd < maxrounds; ++rd) {
       const dim3 grid((unsigned)(nb * maxpairs)), block(BJ_THREADS);
@@ -855,7 +868,10 @@ void bj_jacobi(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& nn,
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
     bool any = false;
     for (int f : hf) any = any || f;
-    if (!any) break;
+    if (!any) {
+      run.unconverged = false;
+      break;
+    }
     k_zero(ctx, flag.get(), nb * sizeof(int));
   }
   HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the round workspaces go out of scope
@@ -1187,11 +1203,11 @@ __global__ __launch_bounds__(SS_THREADS) void sect_qp_emit_kernel(const QpSector
 
 // The sectors gl (sector numbers, R x C column-major at xoff): their triplets
 // like k_sect_svd, through the QR preconditioning and the block Jacobi.
-void k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& Rs, const std::vector<int>& Cs,
+bool k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int>& Rs, const std::vector<int>& Cs,
                      double* X, const i64* xoff, const i64* uoff, const i64* soff, const i64* sgoff, double* sig,
                      double* Uo, double* SVo) {
   const int nb = (int)gl.size();
-  if (!nb) return;
+  if (!nb) return false;
   std::vector<QpSector> hq(nb);
   i64 ttot = 0, ptot = 0, vtot = 0, uftot = 0;
   int maxk = 1, maxR = 1, wgs = 0;
@@ -1298,6 +1314,7 @@ void k_sect_svd_qrcp(Ctx* ctx, const std::vector<int>& gl, const std::vector<int
                      Uf.get(), perm.get(), uoff, soff, sgoff, sig, Uo, SVo);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the workspaces go out of scope
+  return !jb.empty() && run.unconverged;
 }
 
 // factors of the kept triplets.  A block b: (il, sector g) -> [s1, l, mm]

@@ -134,6 +134,13 @@ def profile(enable=True, device=0, products_only=False):
     check(load().sb_ctx_profile(context(device), (2 if products_only else 1) if enable else 0))
 
 
+def set_svd_sweeps(sweeps=40, device=0):
+    """the sweep cap of the one-sided Jacobi SVDs (default 40); a factorisation
+    still rotating after it raises SBError of kind "LAPACKException", as the
+    reference's LAPACK SVD does when it fails to converge"""
+    check(load().sb_ctx_set_svd_sweeps(context(device), int(sweeps)))
+
+
 def profile_reset(device=0):
     check(load().sb_ctx_profile_reset(context(device)))
 
