@@ -7,7 +7,8 @@ rekey(A, K2; check) (SparseBackends/src/blocksparse/conversions.jl:327-342,
 coo/conversions.jl:41-53, aliased/conversions.jl:76-93): with check, first
 every prefix dimension (block-sparse and aliased; not COO) must be <=
 typemax(K2), then every key coordinate must lie in [typemin(K2),
-typemax(K2)]; the first failure raises ErrorException.  retypeval(A::COO, T2)
+typemax(K2)], key by key in stored order and axis by axis within a key;
+the first failure raises ErrorException.  retypeval(A::COO, T2)
 (coo/conversions.jl:65-82): values in stored order; an integer T2 needs an
 integral value, then T2(v) must not throw (inside T2's range), then
 Float64(T2(v)) == v (NaN never passes).
@@ -29,12 +30,11 @@ def rekey_failure(dims, P, keys, K2, kind):
         for i in range(P):
             if dims[i] > info.max:
                 return f"Dimension {dims[i]} at prefix axis {i + 1} cannot be indexed by {name} (max={info.max})"
-    k = np.asarray(keys, dtype=np.int64).reshape(-1)
-    if k.size:
-        for v in (int(k.min()), int(k.max())):
-            if v < info.min or v > info.max:
-                what = "Coordinate" if kind == "coo" else "Key coordinate"
-                return f"{what} {v} is out of range for {name} ({info.min}..{info.max})"
+    # `for key in A.keys, v in key`: key by key in stored order, axis by axis
+    for v in np.asarray(keys, dtype=np.int64).reshape(-1):
+        if v < info.min or v > info.max:
+            what = "Coordinate" if kind == "coo" else "Key coordinate"
+            return f"{what} {int(v)} is out of range for {name} ({info.min}..{info.max})"
     return None
 
 

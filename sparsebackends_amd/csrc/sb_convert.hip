@@ -11,6 +11,7 @@
 // host mirror applies when it reads the tensor back.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 
 #include "sb_convert.h"
@@ -40,14 +41,14 @@ const char* kKeyMax[] = {"9223372036854775807", "2147483647", "32767", "127",
 // SB_VAL_* codes
 const char* kVals[] = {"Float64", "Float32", "Float16", "Int64", "Int32", "Int16", "Int8"};
 
-__global__ void key_range_kernel(const i32* __restrict__ keys, i64 n, int* __restrict__ out) {
-  int lo = INT_MAX, hi = INT_MIN;
+// the first key coordinate (in storage order: key by key, axis by axis, as
+// the reference's `for key in A.keys, v in key`) outside [lo, hi]
+__global__ void key_first_bad_kernel(const i32* __restrict__ keys, i64 n, double lo, double hi,
+                                     unsigned long long* __restrict__ first) {
   for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-    lo = min(lo, keys[i]);
-    hi = max(hi, keys[i]);
+    const double v = (double)keys[i];
+    if (v < lo || v > hi) atomicMin(first, (unsigned long long)i);
   }
-  atomicMin(&out[0], lo);
-  atomicMax(&out[1], hi);
 }
 
 // value v survives T2: 0, else the reason (1 non-integer for an integer T2,
@@ -76,15 +77,64 @@ __global__ void lossless_kernel(const double* __restrict__ v, i64 n, int code, u
     if (lossless_fail(v[i], code)) atomicMin(first, (unsigned long long)i);
 }
 
-std::string fmt_num(double v) {
-  if (v == std::floor(v) && std::fabs(v) < 1e15) {
-    char b[64];
-    std::snprintf(b, sizeof b, "%.1f", v);
-    return b;
-  }
+// Julia's shortest printing of a finite value: the fewest significant digits
+// that read back as v in its type (keep(r) says whether r does), plain when
+// 1e-4 <= |v| < 1e6 (with ".0" on integral values), else d.ddd and the
+// exponent after `ex` ("e" for Float64, "f" for Float32); plain Float32
+// values end in "f0"
+std::string julia_digits(double v, bool (*keep)(double, double), const char* ex, const char* plain_suffix) {
   char b[64];
-  std::snprintf(b, sizeof b, "%.17g", v);
-  return b;
+  for (int p = 1; p <= 17; ++p) {
+    std::snprintf(b, sizeof b, "%.*e", p - 1, v);
+    if (keep(std::strtod(b, nullptr), v)) break;
+  }
+  std::string s = b, sign;
+  if (s[0] == '-') sign = "-", s = s.substr(1);
+  const size_t e = s.find('e');
+  const int x = std::atoi(s.c_str() + e + 1);
+  std::string dig;
+  for (size_t i = 0; i < e; ++i)
+    if (s[i] != '.') dig += s[i];
+  while (dig.size() > 1 && dig.back() == '0') dig.pop_back();
+  std::string out;
+  if (x >= -4 && x <= 5) {
+    if (x >= 0) {
+      std::string ip = dig.substr(0, std::min<size_t>(dig.size(), (size_t)x + 1));
+      while ((int)ip.size() < x + 1) ip += '0';
+      const std::string fp = dig.size() > (size_t)x + 1 ? dig.substr((size_t)x + 1) : "0";
+      out = ip + "." + fp;
+    } else {
+      out = "0." + std::string((size_t)(-x - 1), '0') + dig;
+    }
+    return sign + out + plain_suffix;
+  }
+  out = dig.substr(0, 1) + "." + (dig.size() > 1 ? dig.substr(1) : "0");
+  return sign + out + ex + std::to_string(x);
+}
+
+bool same_f64(double r, double v) { return r == v; }
+bool same_f32(double r, double v) { return (double)(float)r == v; }
+bool same_f16(double r, double v) { return (double)(_Float16)r == v; }
+
+// v as Julia prints a Float64
+std::string fmt_num(double v) {
+  if (v != v) return "NaN";
+  if (std::isinf(v)) return v > 0 ? "Inf" : "-Inf";
+  if (v == 0) return std::signbit(v) ? "-0.0" : "0.0";
+  return julia_digits(v, same_f64, "e", "");
+}
+
+// c, a value of the narrower float type T2 held in a double, as Julia
+// prints that type: 0.1f0, 1.0f-5 (Float32); Float16(0.1) (Float16)
+std::string fmt_narrow(double c, int code) {
+  if (c != c || std::isinf(c)) {
+    const std::string s = c != c ? "NaN" : (c > 0 ? "Inf" : "-Inf");
+    return code == SB_VAL_FLOAT32 ? s + "32" : s + "16";
+  }
+  if (code == SB_VAL_FLOAT32)
+    return c == 0 ? (std::signbit(c) ? "-0.0f0" : "0.0f0") : julia_digits(c, same_f32, "f", "f0");
+  return "Float16(" + (c == 0 ? std::string(std::signbit(c) ? "-0.0" : "0.0") : julia_digits(c, same_f16, "e", "")) +
+         ")";
 }
 
 // a copy of every field and buffer of T (keys, ids, payload, scalars)
@@ -129,20 +179,22 @@ TP tensor_rekey(Ctx* ctx, const Tensor& A, int key_code, bool check) {
                                kKeyMax[key_code] + ")");
     const i64 nv = A.n * (A.kind == COO ? A.N() : A.P());
     if (nv > 0) {
-      DBuf<int> r(ctx, 2);
-      const int init[2] = {INT_MAX, INT_MIN};
-      HIP_CHECK(hipMemcpyAsync(r.get(), init, sizeof init, hipMemcpyHostToDevice, ctx->stream));
-      hipLaunchKernelGGL(key_range_kernel, dim3((unsigned)std::min<i64>((nv + 255) / 256, 1024)), dim3(256), 0,
-                         ctx->stream, A.keys.get(), nv, r.get());
+      DBuf<unsigned long long> f(ctx, 1);
+      const unsigned long long none = ~0ull;
+      HIP_CHECK(hipMemcpyAsync(f.get(), &none, sizeof none, hipMemcpyHostToDevice, ctx->stream));
+      hipLaunchKernelGGL(key_first_bad_kernel, dim3((unsigned)std::min<i64>((nv + 255) / 256, 1024)), dim3(256), 0,
+                         ctx->stream, A.keys.get(), nv, K.lo, K.hi, f.get());
       HIP_CHECK(hipGetLastError());
-      int h[2];
-      HIP_CHECK(hipMemcpyAsync(h, r.get(), sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+      unsigned long long first;
+      HIP_CHECK(hipMemcpyAsync(&first, f.get(), sizeof first, hipMemcpyDeviceToHost, ctx->stream));
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      for (int v : {h[0], h[1]})
-        if ((double)v < K.lo || (double)v > K.hi)
-          fail(SB_E_ERROR, std::string(A.kind == COO ? "Coordinate " : "Key coordinate ") + std::to_string(v) +
-                               " is out of range for " + K.name + " (" + kKeyMin[key_code] + ".." +
-                               kKeyMax[key_code] + ")");
+      if (first != none) {
+        i32 v;
+        HIP_CHECK(hipMemcpy(&v, A.keys.get() + first, sizeof v, hipMemcpyDeviceToHost));
+        fail(SB_E_ERROR, std::string(A.kind == COO ? "Coordinate " : "Key coordinate ") + std::to_string(v) +
+                             " is out of range for " + K.name + " (" + kKeyMin[key_code] + ".." +
+                             kKeyMax[key_code] + ")");
+      }
     }
   }
   TP R = copy_tensor(ctx, A);
@@ -169,11 +221,14 @@ TP coo_retypeval(Ctx* ctx, const Tensor& A, int val_code) {
       const std::string T2 = kVals[val_code];
       const int why = lossless_fail(v, val_code);
       if (why == 1) fail(SB_E_ERROR, "Value " + fmt_num(v) + " is non-integer; cannot convert to " + T2 + " without loss");
-      if (why == 2) fail(SB_E_ERROR, "Value " + fmt_num(v) + " cannot be converted to " + T2 + ": InexactError");
+      if (why == 2)  // the InexactError T2(v) throws, as Julia prints it
+        fail(SB_E_ERROR, "Value " + fmt_num(v) + " cannot be converted to " + T2 + ": InexactError(:" + T2 + ", " +
+                             T2 + ", " + fmt_num(v) + ")");
       const double c = val_code == SB_VAL_FLOAT32   ? (double)(float)v
                        : val_code == SB_VAL_FLOAT16 ? (double)(_Float16)v
                                                     : v;
-      fail(SB_E_ERROR, "Value " + fmt_num(v) + " → " + fmt_num(c) + " → " + fmt_num(c) +
+      const std::string cs = val_code == SB_VAL_FLOAT64 ? fmt_num(c) : fmt_narrow(c, val_code);
+      fail(SB_E_ERROR, "Value " + fmt_num(v) + " → " + cs + " → " + fmt_num(c) +
                            ": roundtrip failed; conversion to " + T2 + " is lossy");
     }
   }

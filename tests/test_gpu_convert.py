@@ -25,15 +25,17 @@ def _coo(dims, keys, vals):
 
 
 @pytest.mark.parametrize("K2", INTS)
-@pytest.mark.parametrize("case", ["small", "dim300", "key256"])
+@pytest.mark.parametrize("case", ["small", "dim300", "key256", "order"])
 def test_rekey_matches_oracle(K2, case):
     print(f"[rekey] {np.dtype(K2).name} {case}", flush=True)
     if case == "small":
         dims, keys = (5, 7, 3), [[1, 2], [4, 7], [5, 1]]
     elif case == "dim300":
         dims, keys = (300, 4, 3), [[1, 1], [120, 4]]
-    else:
+    elif case == "key256":
         dims, keys = (300, 4, 3), [[256, 2], [3, 1]]
+    else:  # two coordinates out of Int8's range: the first in stored order is named, not the largest
+        dims, keys = (300, 4, 3), [[200, 1], [250, 2]]
     for kind, A in (("bs", _bs(dims, 1, keys)),
                     ("coo", _coo(dims[:2], keys, [1.5] * len(keys))),
                     ("aliased", sb.AliasedBlockSparse.new(dims, 1))):
@@ -101,3 +103,17 @@ def test_retypeval_first_failure_in_stored_order():
         sb.retypeval(A, np.float32)
     assert retypeval_failure(A.parts()["vals"], np.float32)[0] == 3777
     assert "0.10000000000000001" in str(e.value) or "0.1" in str(e.value)
+
+
+@pytest.mark.parametrize("vals,T2,msg", [
+    ([0.1], np.float32, "Value 0.1 → 0.1f0 → 0.10000000149011612: roundtrip failed; conversion to Float32 is lossy"),
+    ([70000.0], np.float16, "Value 70000.0 → Inf16 → Inf: roundtrip failed; conversion to Float16 is lossy"),
+    ([1.0, 0.5], np.int8, "Value 0.5 is non-integer; cannot convert to Int8 without loss"),
+    ([300.0], np.int8, "Value 300.0 cannot be converted to Int8: InexactError(:Int8, Int8, 300.0)"),
+])
+def test_retypeval_messages(vals, T2, msg):
+    """the reference's error strings with the values as Julia prints them"""
+    A = _coo((10, 10), [[i + 1, 2] for i in range(len(vals))], vals)
+    with pytest.raises(sb.SBError) as e:
+        sb.retypeval(A, T2)
+    assert str(e.value).endswith(msg), str(e.value)

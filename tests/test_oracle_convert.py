@@ -25,6 +25,13 @@ def test_rekey_hand_cases():
     assert rekey_failure((2, 300), 2, [[1, 1]], np.uint8, "bs") == "Dimension 300 at prefix axis 2 cannot be indexed by UInt8 (max=255)"
     assert rekey_failure((300, 2), 2, [[256, 1]], np.uint8, "coo") == "Coordinate 256 is out of range for UInt8 (0..255)"
     assert rekey_failure((10,), 1, [], np.int8, "coo") is None
+    # two coordinates out of range: the first in stored order (key by key, axis by axis) is named,
+    # not the smallest or the largest (block-sparse keys never exceed their prefix dimensions, which
+    # are checked first, so only COO reaches this)
+    assert rekey_failure((300, 300), 2, [[1, 200], [250, 2]], np.int8, "coo") == \
+        "Coordinate 200 is out of range for Int8 (-128..127)"
+    assert rekey_failure((300, 300), 2, [[250, 1], [1, 200]], np.int8, "coo") == \
+        "Coordinate 250 is out of range for Int8 (-128..127)"
 
 
 def test_retypeval_float_round_trips():
