@@ -5,6 +5,7 @@
 // DMRG runs on H_eff, in the reference KrylovKit's through ITensorMPS).
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 
 #include "sb_krylov.h"
@@ -261,53 +262,101 @@ TP bs_restrict(Ctx* ctx, const Tensor& A, const Tensor& B) {
 
 // ---------------------------------------------------------------- symmetric eigenproblem (host)
 
-// Cyclic Jacobi (Rutishauser's rotation formulas) on an m x m symmetric
-// column-major matrix; eigenvalues ascending in w, eigenvectors in U's columns.
+// Symmetric eigenproblem of a small m x m column-major matrix: Householder
+// reduction to tridiagonal form, then the implicit QL iteration with
+// Wilkinson's shift on the tridiagonal (the method of Bowdler, Martin, Reinsch
+// and Wilkinson, 1968), every orthogonal transformation accumulated in U.
+// Eigenvalues ascending in w, eigenvectors in U's columns.
 void sym_eig(int m, const std::vector<double>& A_in, std::vector<double>& w, std::vector<double>& U) {
-  std::vector<double> A(A_in);
+  std::vector<double> a(A_in);
+  auto A = [&](int i, int j) -> double& { return a[i + (size_t)j * m]; };
   U.assign((size_t)m * m, 0.0);
-  for (int i = 0; i < m; ++i) U[i + (size_t)i * m] = 1.0;
-  auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * m]; };
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0, tot = 0;
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i) {
-        tot += a(i, j) * a(i, j);
-        if (i != j) off += a(i, j) * a(i, j);
-      }
-    if (off == 0.0 || off <= 1e-32 * tot) break;
-    for (int p = 0; p < m - 1; ++p)
-      for (int q = p + 1; q < m; ++q) {
-        const double apq = a(p, q);
-        if (apq == 0.0) continue;
-        const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(1.0 + theta * theta));
-        const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
-        for (int k = 0; k < m; ++k) {  // A <- A J (columns p, q)
-          const double akp = a(k, p), akq = a(k, q);
-          a(k, p) = c * akp - s * akq;
-          a(k, q) = s * akp + c * akq;
+  auto Z = [&](int i, int j) -> double& { return U[i + (size_t)j * m]; };
+  for (int i = 0; i < m; ++i) Z(i, i) = 1.0;
+  std::vector<double> d(m), e(m, 0.0), v(m), p(m);
+  // A <- H_k A H_k, H_k = I - 2 v v^T / |v|^2 zeroing column k below its subdiagonal
+  for (int k = 0; k + 2 < m; ++k) {
+    double sig = 0;
+    for (int i = k + 1; i < m; ++i) sig += A(i, k) * A(i, k);
+    sig = std::sqrt(sig);
+    if (sig == 0.0) continue;
+    const double alpha = A(k + 1, k) >= 0 ? -sig : sig;
+    double vv = 0;
+    for (int i = k + 1; i < m; ++i) {
+      v[i] = A(i, k) - (i == k + 1 ? alpha : 0.0);
+      vv += v[i] * v[i];
+    }
+    if (vv == 0.0) continue;
+    const double beta = 2.0 / vv;
+    double vp = 0;
+    for (int i = k + 1; i < m; ++i) {  // p = beta B v over the trailing block B
+      double s = 0;
+      for (int j = k + 1; j < m; ++j) s += A(i, j) * v[j];
+      p[i] = beta * s;
+      vp += v[i] * p[i];
+    }
+    const double K = 0.5 * beta * vp;
+    for (int i = k + 1; i < m; ++i) p[i] -= K * v[i];  // q = p - K v
+    for (int j = k + 1; j < m; ++j)
+      for (int i = k + 1; i < m; ++i) A(i, j) -= v[i] * p[j] + p[i] * v[j];
+    A(k + 1, k) = A(k, k + 1) = alpha;
+    for (int i = k + 2; i < m; ++i) A(i, k) = A(k, i) = 0.0;
+    for (int r = 0; r < m; ++r) {  // U <- U H_k
+      double t = 0;
+      for (int i = k + 1; i < m; ++i) t += Z(r, i) * v[i];
+      t *= beta;
+      for (int i = k + 1; i < m; ++i) Z(r, i) -= t * v[i];
+    }
+  }
+  for (int i = 0; i < m; ++i) d[i] = A(i, i);
+  for (int i = 0; i + 1 < m; ++i) e[i] = A(i + 1, i);  // e[i] couples rows i and i+1
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int l = 0; l < m; ++l) {
+    for (int it = 0; it < 64; ++it) {
+      int n = l;
+      while (n + 1 < m && std::fabs(e[n]) > eps * (std::fabs(d[n]) + std::fabs(d[n + 1]))) ++n;
+      if (n == l) break;
+      // Wilkinson's shift from the leading 2 x 2 of the unreduced block l..n
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[n] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+      double s = 1.0, c = 1.0, sh = 0.0;
+      int i = n - 1;
+      for (; i >= l; --i) {  // chase the bulge from the bottom of the block up
+        const double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {  // the block splits at i: deflate and restart
+          d[i + 1] -= sh;
+          e[n] = 0.0;
+          break;
         }
-        for (int k = 0; k < m; ++k) {  // A <- J^T A (rows p, q)
-          const double apk = a(p, k), aqk = a(q, k);
-          a(p, k) = c * apk - s * aqk;
-          a(q, k) = s * apk + c * aqk;
-        }
-        a(p, q) = a(q, p) = 0.0;
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - sh;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        sh = s * r;
+        d[i + 1] = g + sh;
+        g = c * r - b;
         for (int k = 0; k < m; ++k) {
-          const double ukp = U[k + (size_t)p * m], ukq = U[k + (size_t)q * m];
-          U[k + (size_t)p * m] = c * ukp - s * ukq;
-          U[k + (size_t)q * m] = s * ukp + c * ukq;
+          const double zk1 = Z(k, i + 1), zk = Z(k, i);
+          Z(k, i + 1) = s * zk + c * zk1;
+          Z(k, i) = c * zk - s * zk1;
         }
       }
+      if (r == 0.0 && i >= l) continue;
+      d[l] -= sh;
+      e[l] = g;
+      e[n] = 0.0;
+    }
   }
   std::vector<int> ord(m);
   for (int i = 0; i < m; ++i) ord[i] = i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return d[x] < d[y]; });
   w.resize(m);
   std::vector<double> Us((size_t)m * m);
   for (int j = 0; j < m; ++j) {
-    w[j] = a(ord[j], ord[j]);
+    w[j] = d[ord[j]];
     std::copy(&U[(size_t)ord[j] * m], &U[(size_t)ord[j] * m] + m, &Us[(size_t)j * m]);
   }
   U.swap(Us);

@@ -26,7 +26,7 @@ void noalloc_setup(int device);
 void bs_norm_noalloc(Ctx*, const Tensor& A, double* out);
 
 // eigenvalues (ascending) and eigenvectors (columns) of a symmetric
-// column-major m x m matrix, cyclic Jacobi
+// column-major m x m matrix (Householder tridiagonalisation + implicit QL)
 void sym_eig(int m, const std::vector<double>& A, std::vector<double>& w, std::vector<double>& U);
 
 struct EigResult {
