@@ -221,6 +221,19 @@ def eig_section(sb, a, prob, plans, fused64, fused32):
         vals, _, ilo, ihi = r
         res[name] = dict(energy=vals[0], abs_dE_vs_fp64=abs(vals[0] - e64), seconds=t, time_to_energy_vs_fp64=t64 / t,
                          numops_low=ilo.numops, numops_fp64=ihi.numops, converged=ihi.converged, tol_low=tol_lo)
+        if fplan is not None:
+            # the same two stages eigsolve_refined runs, timed apart: the FP64 stage alone bounds what
+            # any speed-up of the low-precision operator can give
+            pm.psi.axpby_(1.0, x0d, 0.0)
+            _, t_lo = timed(lambda: sb.eigsolve(plans_lo, pm.psi, krylovdim=a.krylovdim, maxiter=50, tol=tol_lo))
+            prob.psi.axpby_(1.0, pm.psi, 0.0)
+            (v2, _, i2), t_hi = timed(lambda: sb.eigsolve(op64, prob.psi, krylovdim=8, maxiter=10, tol=a.eig_tol))
+            res[name].update(low_stage_s=t_lo, fp64_stage_s=t_hi, stages_energy_equal=bool(v2[0] == vals[0]),
+                             fp64_iteration_ms=1e3 * t64 / info.numops, fp64_stage_iteration_ms=1e3 * t_hi / i2.numops,
+                             low_iteration_ms=1e3 * t_lo / ilo.numops,
+                             cap_if_low_stage_free=t64 / t_hi,
+                             cap_note="time-to-energy ratio if the low-precision stage took no time: FP64 solve / "
+                                      "its FP64 stage alone")
         del pm, plans_lo
     if not a.no_dropin:
         ed, td, nd = dropin_eigsolve(sb, prob, x0d, a.krylovdim, a.eig_tol)
@@ -595,8 +608,13 @@ def serve5_section(sb, a):
     S.run_plain([S.SolveRequest(10 ** 6, 8, 16, 1.0)])  # first-use costs out of the timed runs
     t_start = [time.perf_counter()]
 
+    def sweep_line(r, k, e, low):  # long solves print as they go
+        if r.sites * r.chi_max >= 128 * 256:
+            print(f"[serve5]     request {r.rid} sweep {k} ({'FP32' if low else 'FP64'} products) E={e:.12f} at "
+                  f"{time.perf_counter() - t_start[0]:.1f} s", flush=True)
+
     def solver(r, eig, dev):  # the library's solve, with a line per finished request
-        res = S.solve_dmrg(r, eig, dev)
+        res = S.solve_dmrg(r, eig, dev, on_sweep=sweep_line)
         print(f"[serve5]   request {r.rid} (N={r.sites}, chi_max={r.chi_max}) done at "
               f"{time.perf_counter() - t_start[0]:.1f} s, {len(res[1])} sweeps", flush=True)
         return res
@@ -638,7 +656,9 @@ def serve5_section(sb, a):
             row["solves_per_s_ratio"] = row["served"]["solves_per_s"] / row["plain"]["solves_per_s"]
             row["p99_ratio"] = row["served"]["p99_s"] / row["plain"]["p99_s"]
             print(f"[serve5] rate {rate:g}/s served: {row['served']['solves_per_s']:.4f} solves/s "
-                  f"(x{row['solves_per_s_ratio']:.2f}), p99 x{row['p99_ratio']:.2f}", flush=True)
+                  f"(x{row['solves_per_s_ratio']:.2f}), p50 {row['served']['p50_s']:.1f} s, p99 "
+                  f"{row['served']['p99_s']:.1f} s (x{row['p99_ratio']:.2f}), energies equal to plain: "
+                  f"{row['served']['identical_to_plain']}", flush=True)
         out["rates"][f"{rate:g}"] = row
     if "mixed" in parts:
         for rate in [float(x) for x in a.serve5_mixed_rates.split(",") if x]:

@@ -103,12 +103,14 @@ def neel(n):
     return [0, 1] * (n // 2) + [0] * (n % 2)
 
 
-def solve_dmrg(req: SolveRequest, eigsolver=None, device=0):
+def solve_dmrg(req: SolveRequest, eigsolver=None, device=0, on_sweep=None):
     """Runs one request to completion on the calling thread's context.
     Returns (energy, energies per sweep, converged, maxlinkdim, peak bytes,
     largest discarded weight of the last sweep):
     peak bytes = the context's device bytes in use after a sweep beyond what
-    it held before the solve, at most over the sweeps."""
+    it held before the solve, at most over the sweeps.  on_sweep(req, k,
+    energy, low): called after every sweep (low: it used the low-precision
+    products)."""
     base = memory(device)[0]
     dm = D.DMRG(H.heisenberg_model(1.0, float(req.delta)), neel(req.sites), device=device, flags=req.flags,
                 eigsolver=eigsolver)
@@ -119,6 +121,8 @@ def solve_dmrg(req: SolveRequest, eigsolver=None, device=0):
         e = dm.sweep(maxdim=req.chi_max, cutoff=req.cutoff, krylovdim=req.krylovdim, maxiter=req.maxiter, tol=1e-14)
         peak = max(peak, memory(device)[0] - base)
         es.append(e)
+        if on_sweep is not None:
+            on_sweep(req, k, e, low)
         settled = prev is not None and abs(e - prev) < (req.polish_tol * abs(e) if low else req.tol)
         prev = e
         if low and (settled or req.max_sweeps - 1 - k <= req.polish_sweeps):
