@@ -33,33 +33,71 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 static inline unsigned nb256(i64 n) { return (unsigned)((n + 255) / 256); }
 
 // ---------------------------------------------------------------- operand boxes
+// One workgroup per chunk of kScanChunk values of a block (a block of 100k
+// values is scanned by 13 workgroups, not one): per tail axis the largest
+// index of a nonzero, reduced in LDS and then into the block's box with one
+// atomic maximum per axis (box and flags cleared before the launch).  Blocks of
+// fewer than 2^24 values decode the column-major index in 32 bits with a float
+// reciprocal and one correction step, exact there (the quotient's rounding
+// error stays below 1); larger blocks divide in 64 bits.
+constexpr i64 kScanChunk = 8192;
+constexpr i64 kScanSmall = (i64)1 << 24;
+
+__device__ __forceinline__ int scan_div(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  const int r = n - q * d;
+  if (r >= d)
+    ++q;
+  else if (r < 0)
+    --q;
+  return q;
+}
+
+struct ScanInv {
+  float v[MAXR];
+};
+
 __global__ __launch_bounds__(256) void box_scan_kernel(const double* __restrict__ data, const i32* __restrict__ ids,
-                                                       i64 n, i64 bs, int N2, IArr td, i32* __restrict__ box,
-                                                       i32* __restrict__ flag) {
+                                                       i64 n, i64 bs, i64 cpb, int N2, IArr td, ScanInv inv,
+                                                       i32* __restrict__ box, i32* __restrict__ flag) {
   __shared__ int sm[MAXR];
   __shared__ int sf;
   const int tid = threadIdx.x;
-  for (i64 i = blockIdx.x; i < n; i += gridDim.x) {
+  const bool small = bs < kScanSmall;
+  for (i64 w = blockIdx.x; w < n * cpb; w += gridDim.x) {
+    const i64 i = w / cpb, c = w - i * cpb;
     if (tid < MAXR) sm[tid] = 0;
     if (tid == 0) sf = 0;
     __syncthreads();
     const double* b = data + (i64)(ids[i] - 1) * bs;
+    const i64 hi = min(bs, (c + 1) * kScanChunk);
     int lm[MAXR];
 #pragma unroll
     for (int a = 0; a < MAXR; ++a) lm[a] = 0;
     int lf = 0;
-    for (i64 e = tid; e < bs; e += 256) {
-      double v = b[e];
+    for (i64 e = c * kScanChunk + tid; e < hi; e += 256) {
+      const double v = b[e];
       if (v != 0.0) {  // NaN compares unequal, so it counts as nonzero
-        i64 x = e;
+        if (small) {
+          int x = (int)e;
 #pragma unroll
-        for (int a = 0; a < MAXR; ++a)
-          if (a < N2) {
-            i64 q = x / td.v[a];
-            int r = (int)(x - q * td.v[a]);
-            lm[a] = max(lm[a], r + 1);
-            x = q;
-          }
+          for (int a = 0; a < MAXR; ++a)
+            if (a < N2) {
+              const int d = (int)td.v[a];
+              const int q = scan_div(x, d, inv.v[a]);
+              lm[a] = max(lm[a], x - q * d + 1);
+              x = q;
+            }
+        } else {
+          i64 x = e;
+#pragma unroll
+          for (int a = 0; a < MAXR; ++a)
+            if (a < N2) {
+              const i64 q = x / td.v[a];
+              lm[a] = max(lm[a], (int)(x - q * td.v[a]) + 1);
+              x = q;
+            }
+        }
         if (!isfinite(v)) lf = 1;
       }
     }
@@ -68,8 +106,8 @@ __global__ __launch_bounds__(256) void box_scan_kernel(const double* __restrict_
       if (a < N2 && lm[a]) atomicMax(&sm[a], lm[a]);
     if (lf) atomicOr(&sf, 1);
     __syncthreads();
-    if (tid < N2) box[i * N2 + tid] = sm[tid];
-    if (tid == 0) flag[i] = sf;
+    if (tid < N2 && sm[tid]) atomicMax(&box[i * N2 + tid], sm[tid]);
+    if (tid == 0 && sf) atomicOr(&flag[i], 1);
     __syncthreads();
   }
 }
@@ -77,10 +115,19 @@ __global__ __launch_bounds__(256) void box_scan_kernel(const double* __restrict_
 void box_scan(Ctx* ctx, const double* data, const i32* ids, i64 n, i64 bs, int N2, const i64* tdims, i32* box,
               i32* flag) {
   if (n == 0) return;
+  k_zero(ctx, box, (size_t)(n * N2) * sizeof(i32));
+  k_zero(ctx, flag, (size_t)n * sizeof(i32));
+  if (bs == 0) return;
   IArr td{};
-  for (int a = 0; a < N2; ++a) td.v[a] = tdims[a];
-  unsigned g = (unsigned)std::min<i64>(n, 65536);
-  hipLaunchKernelGGL(box_scan_kernel, dim3(g), dim3(256), 0, ctx->stream, data, ids, n, bs, N2, td, box, flag);
+  ScanInv inv{};
+  for (int a = 0; a < N2; ++a) {
+    td.v[a] = tdims[a];
+    inv.v[a] = 1.0f / (float)std::max<i64>(tdims[a], 1);
+  }
+  const i64 cpb = (bs + kScanChunk - 1) / kScanChunk;
+  const unsigned g = (unsigned)std::min<i64>(n * cpb, 65536);
+  hipLaunchKernelGGL(box_scan_kernel, dim3(g), dim3(256), 0, ctx->stream, data, ids, n, bs, cpb, N2, td, inv, box,
+                     flag);
   HIP_CHECK(hipGetLastError());
 }
 
