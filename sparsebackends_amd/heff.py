@@ -397,6 +397,15 @@ class HeffProblem:
                 f[3] += 2.0 * (dl[ilp - 1] * d * d) * (dr[ir - 1] * w) * dr[irp - 1]
         return dict(total=sum(f), steps=f)
 
+    def useful_flops(self):
+        """algorithmic_flops with the two MPO steps counted only over the MPO's
+        nonzero entries: the chain multiplies every T1 block by the whole
+        dense (d w) x (d w) MPO block, most of whose entries are zero"""
+        f = self.algorithmic_flops()["steps"]
+        frac = float(np.count_nonzero(self.m.W)) / float(self.m.W.size)
+        steps = [f[0], f[1] * frac, f[2] * frac, f[3]]
+        return dict(total=sum(steps), steps=steps, mpo_nonzero_fraction=frac)
+
     def padded_flops(self):
         """the same sum with the padded extents the kernels execute"""
         d, w, Dl, Dr = self.m.d, self.m.w, self.Dl, self.Dr
