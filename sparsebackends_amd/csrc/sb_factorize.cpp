@@ -49,6 +49,39 @@ void down(Ctx* ctx, std::vector<T>& h, const T* d, size_t n) {
   h.resize(n);
   if (n) HIP_CHECK(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
 }
+
+// An orthonormal completion: vecs holds K vectors of length D (vector c at
+// vecs[c * vs + e * es]); those with null[c] are replaced, in order, by unit
+// vectors orthogonal to all the others (the kept ones and those completed
+// before): the unit vector e_j of the smallest projection onto the span so far,
+// orthogonalised twice (classical Gram-Schmidt) and normalised.
+void complete_orthonormal(std::vector<double>& vecs, i64 vs, i64 es, i64 D, i64 K, const std::vector<char>& null) {
+  std::vector<i64> basis;
+  for (i64 c = 0; c < K; ++c)
+    if (!null[c]) basis.push_back(c);
+  auto at = [&](i64 c, i64 e) -> double& { return vecs[(size_t)(c * vs + e * es)]; };
+  std::vector<double> w(D), proj(D);
+  for (i64 c = 0; c < K; ++c) {
+    if (!null[c]) continue;
+    std::fill(proj.begin(), proj.end(), 0.0);
+    for (i64 b : basis)
+      for (i64 e = 0; e < D; ++e) proj[e] += at(b, e) * at(b, e);
+    const i64 j = std::min_element(proj.begin(), proj.end()) - proj.begin();
+    std::fill(w.begin(), w.end(), 0.0);
+    w[j] = 1.0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (i64 b : basis) {
+        double dot = 0;
+        for (i64 e = 0; e < D; ++e) dot += at(b, e) * w[e];
+        for (i64 e = 0; e < D; ++e) w[e] -= dot * at(b, e);
+      }
+    double nw = 0;
+    for (double x : w) nw += x * x;
+    nw = std::sqrt(nw);
+    for (i64 e = 0; e < D; ++e) at(c, e) = w[e] / nw;
+    basis.push_back(c);
+  }
+}
 }  // namespace
 
 SvdResult bs_svd(Ctx* ctx, const Tensor& A, bool right, i64 maxdim, i64 mindim, double cutoff) {
@@ -401,10 +434,66 @@ SectSvdResult bs_svd_sectors(Ctx* ctx, const Tensor& psi, i64 nsec, const i64* r
   up(ctx, d_dl, std::vector<i64>(dl, dl + nl));
   up(ctx, d_dr, std::vector<i64>(dr, dr + nr));
   up(ctx, d_kept, kg);
+  // Kept triplets of (numerically) zero singular value: the Jacobi leaves null
+  // rows alone, so the factor built from normalised rows of X (U for a
+  // transposed sector, V^T for ortho right) would hold zero or noise there and
+  // not be an isometry, as LAPACK's factors are.  Those vectors are replaced by
+  // an orthonormal completion (only a state of lower rank than its kept bond,
+  // with cutoff = 0, reaches this).
+  DBuf<double> sigB;  // the singular values B's rows are divided by (1 for a completed V^T row)
+  {
+    std::vector<char> anynull(ng, 0);
+    bool need = false;
+    for (i64 g = 0; g < ng; ++g) {
+      double fro2 = 0;
+      for (i64 c = sgoff[g]; c < sgoff[g + 1]; ++c) fro2 += hs[c] * hs[c];
+      const double thr = 1e-13 * std::sqrt(fro2);
+      for (i64 c = 0; c < kg[g]; ++c)
+        if (!(hs[sgoff[g] + c] > thr)) anynull[g] = 1;
+      need = need || anynull[g];
+    }
+    if (need) {
+      std::vector<double> hsB(hs);
+      for (i64 g = 0; g < ng; ++g) {
+        if (!anynull[g]) continue;
+        double fro2 = 0;
+        for (i64 c = sgoff[g]; c < sgoff[g + 1]; ++c) fro2 += hs[c] * hs[c];
+        const double thr = 1e-13 * std::sqrt(fro2);
+        const i64 K = kg[g], n = sgoff[g + 1] - sgoff[g];
+        std::vector<char> null(K);
+        for (i64 c = 0; c < K; ++c) null[c] = !(hs[sgoff[g] + c] > thr);
+        if (!right) {  // A = U: columns c of the R x n column-major U of the sector
+          const i64 Rg = hR[g];
+          std::vector<double> u;
+          down(ctx, u, Uo.get() + uoff[g], (size_t)(Rg * n));
+          HIP_CHECK(hipStreamSynchronize(ctx->stream));
+          complete_orthonormal(u, Rg, 1, Rg, K, null);
+          HIP_CHECK(hipMemcpy(Uo.get() + uoff[g], u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {  // B = V^T: rows c of the n x C row-major sigma V^T, divided by sigma
+          const i64 Cg = hC[g];
+          std::vector<double> v;
+          down(ctx, v, SVo.get() + soff[g], (size_t)(n * Cg));
+          HIP_CHECK(hipStreamSynchronize(ctx->stream));
+          for (i64 c = 0; c < K; ++c)
+            if (!null[c])
+              for (i64 j = 0; j < Cg; ++j) v[(size_t)(c * Cg + j)] /= hs[sgoff[g] + c];
+          complete_orthonormal(v, Cg, 1, Cg, K, null);
+          for (i64 c = 0; c < K; ++c)
+            if (!null[c])
+              for (i64 j = 0; j < Cg; ++j) v[(size_t)(c * Cg + j)] *= hs[sgoff[g] + c];
+            else
+              hsB[sgoff[g] + c] = 1.0;
+          HIP_CHECK(hipMemcpy(SVo.get() + soff[g], v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+      }
+      if (right) up(ctx, sigB, hsB);
+    }
+  }
   k_sect_emit(ctx, true, nA, d1, Dl, Dm, nl, d_ablk.get(), d_rsec.get(), d_roff.get(), d_dl.get(), d_kept.get(),
               d_R.get(), d_uoff.get(), d_sgoff.get(), sig.get(), Uo.get(), right, out.A->data.get());
   k_sect_emit(ctx, false, nB, d2, Dm, Dr, nr, d_bblk.get(), d_csec.get(), d_coff.get(), d_dr.get(), d_kept.get(),
-              d_C.get(), d_soff.get(), d_sgoff.get(), sig.get(), SVo.get(), right, out.B->data.get());
+              d_C.get(), d_soff.get(), d_sgoff.get(), sigB.get() ? sigB.get() : sig.get(), SVo.get(), right,
+              out.B->data.get());
   HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return out;
 }

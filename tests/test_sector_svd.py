@@ -89,3 +89,45 @@ def test_gpu_sector_svd(name, prob, ortho, trunc):
     assert np.linalg.norm(R - Ro) <= 1e-12 * np.linalg.norm(Ro)
     iso = Ad.reshape(-1, Ad.shape[2]) if ortho == "left" else Bd.transpose(1, 0, 2).reshape(Bd.shape[1], -1).T
     np.testing.assert_allclose(iso.T @ iso, np.eye(iso.shape[1]), atol=1e-12)
+
+
+def rank_deficient(prob, keep_every=3):
+    """psi with most blocks zeroed: sectors of lower rank than their size, and
+    sectors whose whole matrix is zero (every singular value 0)"""
+    import copy
+    p = copy.copy(prob)
+    pp = dict(prob.psi_parts)
+    data = pp["data"].copy()
+    bs = len(data) // len(pp["ids"])
+    for i, bid in enumerate(pp["ids"]):
+        if i % keep_every:
+            data[(bid - 1) * bs:bid * bs] = 0.0
+    pp["data"] = data
+    p.psi_parts = pp
+    return p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,prob", list(problems())[:3], ids=["heisenberg", "hubbard", "heisenberg-mid"])
+@pytest.mark.parametrize("ortho", ["left", "right"])
+def test_gpu_sector_svd_rank_deficient_isometry(name, prob, ortho):
+    """cutoff = 0 keeps the triplets of zero singular value: the isometric
+    factor must still be an isometry (LAPACK's is), and A B = psi"""
+    from sparsebackends_amd import factorize as Fz
+    p = rank_deficient(prob)
+    rowsec, colsec, charges = D.fused_sectors(p.m, p.Bl, p.Br)
+    print(f"[sector svd rank-deficient] {name} {ortho}", flush=True)
+    A, B, svs, kept, spec = Fz.svd_sectors(D.upload(p.psi_parts), rowsec, colsec, p.Bl.dims, p.Br.dims,
+                                           len(charges), ortho=ortho)
+    Ao, Bo, so, ko, to = F.svd_sectors(p.psi_parts, rowsec, colsec, p.Bl.dims, p.Br.dims, len(charges), ortho=ortho)
+    assert np.array_equal(kept, ko)
+    assert np.sum(so <= 1e-13 * so.max()) > 0  # the case has null triplets to keep
+    assert np.abs(svs - so).max() <= 1e-13 * so.max()
+    a, b = A.parts(), B.parts()
+    for q, r in ((a, Ao), (b, Bo)):
+        assert np.array_equal(q["keys"], r["keys"]) and np.array_equal(q["ids"], r["ids"])
+    Ad, Bd = dense_factors(p, a, b, charges, kept)
+    psi, _ = expand(p.psi_parts, [p.Bl, p.Br], [None, None, 0, 1])
+    np.testing.assert_allclose(np.einsum("alm,bmr->ablr", Ad, Bd), psi, atol=1e-12 * np.abs(psi).max())
+    iso = Ad.reshape(-1, Ad.shape[2]) if ortho == "left" else Bd.transpose(1, 0, 2).reshape(Bd.shape[1], -1).T
+    np.testing.assert_allclose(iso.T @ iso, np.eye(iso.shape[1]), atol=1e-12)
