@@ -207,6 +207,42 @@ SvdResult bs_svd(Ctx* ctx, const Tensor& A, bool right, i64 maxdim, i64 mindim, 
   k_svd_emit(ctx, k, d1, d_kg.get(), d_kc.get(), d_soff.get(), d_gstart.get(), d2, d_woff.get(), W.get(), sig.get(),
              Uo.get(), perm.get(), right, R.U->data.get(), R.SV->data.get());
   HIP_CHECK(hipStreamSynchronize(ctx->stream));  // host staging vectors go out of scope
+  if (right && k) {
+    // ortho right: the SV rows of a group are V^T rows, an isometry in LAPACK's
+    // factorisation; a kept triplet of zero singular value (cutoff = 0 on a
+    // group of lower rank) has a zero row there, replaced by an orthonormal
+    // completion within its group
+    std::vector<char> gnull(ng, 0);
+    bool need = false;
+    std::vector<i64> nb0(ng + 1, 0);
+    for (i64 g = 0; g < ng; ++g) {
+      nb0[g + 1] = nb0[g] + kk[g];
+      const i64 nc = (gstart[g + 1] - gstart[g]) * d2;
+      double fro2 = 0;
+      for (int c = 0; c < (int)std::min<i64>(d1, nc); ++c) fro2 += hs[g * d1 + c] * hs[g * d1 + c];
+      for (i64 c = 0; c < kk[g]; ++c)
+        if (!(hs[g * d1 + c] > 1e-13 * std::sqrt(fro2))) gnull[g] = 1;
+      need = need || gnull[g];
+    }
+    if (need) {
+      std::vector<double> sv;
+      down(ctx, sv, R.SV->data.get(), (size_t)(nsv * d2));
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (i64 g = 0; g < ng; ++g) {
+        if (!gnull[g]) continue;
+        const i64 nc = (gstart[g + 1] - gstart[g]) * d2, K = kk[g];
+        const i64 base = soff[nb0[g]];  // the group's rows, contiguous: row c at base + c * nc
+        double fro2 = 0;
+        for (int c = 0; c < (int)std::min<i64>(d1, nc); ++c) fro2 += hs[g * d1 + c] * hs[g * d1 + c];
+        std::vector<char> null(K);
+        for (i64 c = 0; c < K; ++c) null[c] = !(hs[g * d1 + c] > 1e-13 * std::sqrt(fro2));
+        std::vector<double> v(sv.begin() + base, sv.begin() + base + K * nc);
+        complete_orthonormal(v, nc, 1, nc, K, null);
+        std::copy(v.begin(), v.end(), sv.begin() + base);
+      }
+      HIP_CHECK(hipMemcpy(R.SV->data.get(), sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
   return R;
 }
 

@@ -77,3 +77,31 @@ def test_errors():
     assert e.value.kind == "AssertionError"
     with pytest.raises(ValueError):
         sb.blocksparse_svd(dev(p), ortho="up")
+
+
+def test_rank_deficient_right_isometry():
+    """cutoff = 0 keeps the zero singular values of groups of lower rank: with
+    ortho right the SV rows of each group (V^T) are still orthonormal, as
+    LAPACK's are, and the factorisation still matches the oracle"""
+    rng = np.random.default_rng(3)
+    nL, nR, d1, d2 = 5, 6, 4, 3
+    D = rng.standard_normal((nL, nR, d1, d2))
+    D[rng.random((nL, nR)) > 0.5] = 0.0
+    D[:, :, 1, :] = D[:, :, 0, :]  # two equal rows in every group: rank <= d1 - 1
+    D[0] = 0.0
+    D[0, 0, 0, 0] = 1.0  # a group of rank 1
+    p = O.BS.from_dense(D, 2).parts()
+    svs = compare(p, "right")
+    assert np.sum(svs <= 1e-13 * svs.max()) >= nL  # null triplets kept
+    U, SV, svs, spec = sb.blocksparse_svd(dev(p), ortho="right")
+    u, s = U.parts(), SV.parts()
+    k = len(svs)
+    rows = np.zeros((k, nR * d2))
+    for (nb, R), i in zip(s["keys"], s["ids"]):
+        rows[nb - 1, (R - 1) * d2:R * d2] = s["data"][(i - 1) * d2:i * d2]
+    group = {}
+    for (L, nb) in u["keys"]:
+        group.setdefault(L, []).append(nb - 1)
+    for L, nbs in group.items():
+        V = rows[nbs]
+        np.testing.assert_allclose(V @ V.T, np.eye(len(nbs)), atol=1e-12)
