@@ -105,7 +105,10 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="internal: three fused applications and a 256 MB copy, for the profiled child runs")
     ap.add_argument("--no-traffic", action="store_true", help="skip the in-run rocprofv3 traffic measurement")
-    ap.add_argument("--only", default="", choices=["", "batch"],
+    ap.add_argument("--phase-sites", type=int, default=64, help="--only dmrg-phases: chain length")
+    ap.add_argument("--phase-chi", type=int, default=512, help="--only dmrg-phases: bond dimension cap")
+    ap.add_argument("--phase-sweeps", type=int, default=6, help="--only dmrg-phases: sweeps (the last one timed)")
+    ap.add_argument("--only", default="", choices=["", "batch", "sweep", "dmrg-phases"],
                     help="run one section alone (batch: the batching configuration, for profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="also run one plain device copy of this many MB (rocprof traffic calibration)")
@@ -253,6 +256,38 @@ def eig_section(sb, a, prob, plans, fused64, fused32):
     res["dmrg_step_solve"] = dict(krylovdim=3, maxiter=1, solves=a.solves, solves_per_s=len(lat) / lat.sum(),
                                   p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3))
     return res
+
+
+def dmrg_phases_section(sb, a):
+    """Where a two-site DMRG step at the serving sizes spends its time: an XXZ
+    chain (Delta = 0.9) of --phase-sites sites from the Neel state, bond
+    dimension cap --phase-chi, cutoff 1e-12, Krylov dimension 3; every sweep's
+    steps timed by phase (two-site contraction, the four H_eff plans'
+    creation, eigensolve, sector SVD, environment update), each phase ending
+    with a device synchronisation."""
+    from sparsebackends_amd import heff as H
+    from sparsebackends_amd import dmrg as D
+    from sparsebackends_amd import serve as S
+    N, chi = a.phase_sites, a.phase_chi
+    dm = D.DMRG(H.heisenberg_model(1.0, 0.9), S.neel(N))
+    out = dict(model="XXZ J=1 Delta=0.9, Neel start, U(1) Sz, MPO w=5", sites=N, chi_max=chi, cutoff=1e-12,
+               krylovdim=3, sweeps=[])
+    for k in range(a.phase_sweeps):
+        dm.phase_times = {}
+        sb.synchronize()
+        t0 = time.perf_counter()
+        e = dm.sweep(maxdim=chi, cutoff=1e-12, krylovdim=3, maxiter=1, tol=1e-14)
+        sb.synchronize()
+        dt = time.perf_counter() - t0
+        steps = 2 * (N - 1)
+        row = dict(sweep=k, seconds=dt, ms_per_step=1e3 * dt / steps, maxlinkdim=dm.maxlinkdim(), energy=e,
+                   largest_sector=max(max(b.dims) for b in dm.bonds),
+                   phase_ms_per_step={p: 1e3 * v / steps for p, v in dm.phase_times.items()})
+        out["sweeps"].append(row)
+        print(f"[dmrg-phases] sweep {k}: {row['ms_per_step']:.2f} ms/step, maxlink {row['maxlinkdim']}, "
+              + ", ".join(f"{p} {v:.2f}" for p, v in sorted(row["phase_ms_per_step"].items(), key=lambda x: -x[1])),
+              flush=True)
+    return out
 
 
 def dmrg_section(sb, a):
@@ -847,6 +882,12 @@ def main():
         return
     if a.serve_config5 > 0:
         print(json.dumps(dict(serving_config5=serve5_section(sb, a))), flush=True)
+        return
+    if a.only == "sweep":
+        print(json.dumps(dict(sweep=sweep_section(sb, a))), flush=True)
+        return
+    if a.only == "dmrg-phases":
+        print(json.dumps(dict(dmrg_phases=dmrg_phases_section(sb, a))), flush=True)
         return
     if a.only == "batch":
         res = dict(batch=batch_section(sb, a))

@@ -263,6 +263,9 @@ class DMRG:
                 self.R[j] = make_R(self.R[j + 1], self.A[j - 1], self.W, flags)[0]
         self.energies = []
         self.truncerr = 0.0
+        # phase timing of the two-site steps: a dict {phase: seconds} turns it
+        # on (each phase then ends with a device synchronisation)
+        self.phase_times = None
 
     @classmethod
     def from_mps(cls, model, A, bonds, device=0, flags=0, eigsolver=None):
@@ -276,18 +279,40 @@ class DMRG:
             self.R[j] = make_R(self.R[j + 1], self.A[j - 1], self.W, flags)[0]
         return self
 
+    def _phase(self, name, t0):
+        """with phase_times set (a dict), the device is synchronised and the
+        seconds since t0 are added to phase_times[name]; returns the new t0"""
+        if self.phase_times is None:
+            return t0
+        import time
+        from .storage import synchronize
+        synchronize(self.device)
+        t = time.perf_counter()
+        self.phase_times[name] = self.phase_times.get(name, 0.0) + (t - t0)
+        return t
+
     def step(self, b, direction, maxdim, cutoff, krylovdim, maxiter, tol, mindim=1):
         """optimise sites b, b+1 (1-based): eigsolve on H_eff, sector SVD, environment update"""
+        import time
         from .heff import Bond, heff_plans
         from .factorize import svd_sectors
+        t0 = 0.0
+        if self.phase_times is not None:
+            from .storage import synchronize
+            synchronize(self.device)
+            t0 = time.perf_counter()
         psi = two_site(self.A[b - 1], self.A[b])
+        t0 = self._phase("two_site", t0)
         plans, Hp = heff_plans(psi, self.L[b - 1], self.W, self.W, self.R[b + 2], self.flags)
+        t0 = self._phase("heff_plans", t0)
         vals, _, info = self.eigsolver(plans, psi, krylovdim=krylovdim, maxiter=maxiter, tol=tol)
+        t0 = self._phase("eigsolve", t0)
         Bl, Br = self.bonds[b - 1], self.bonds[b + 1]
         rowsec, colsec, charges = fused_sectors(self.m, Bl, Br)
         A, B, svs, kept, spec = svd_sectors(psi, rowsec, colsec, Bl.dims, Br.dims, len(charges),
                                             ortho="left" if direction > 0 else "right", maxdim=maxdim,
                                             mindim=mindim, cutoff=cutoff)
+        t0 = self._phase("svd", t0)
         live = [i for i in range(len(charges)) if kept[i] > 0]
         self.bonds[b] = Bond([charges[i] for i in live], [int(kept[i]) for i in live])
         self.A[b - 1], self.A[b] = A, B
@@ -302,6 +327,7 @@ class DMRG:
             self.R[b + 1] = make_R(self.R[b + 2], B, self.W, self.flags)[0]
             if b - 1 >= 1:
                 self.L.pop(b - 1, None)
+        self._phase("environment", t0)
         self.truncerr = max(self.truncerr, spec.truncerr)
         return vals[0], info
 
