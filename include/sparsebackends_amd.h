@@ -201,7 +201,8 @@ int sb_plan_execute(sb_ctx_t ctx, sb_plan_t plan);
  * Hpsi, made by sb_bs_new with psi's dims and N2 = 4, gets the block list the
  * chain's last contract! gives it (keys and ids bit for bit) and at every
  * execution the value of the chain for psi's current values (summation order
- * of its own: FP64 to rounding).  The sector extents are the environments'
+ * of its own: FP64 to rounding; a NaN or Inf in psi reaches the outputs its
+ * row feeds, which may be fewer than the chain's dense MPO steps spread it to).  The sector extents are the environments'
  * nonzero reach at creation: entries of psi beyond it are not read (the chain
  * multiplies them by zero environment entries); L, R, W1 and W2 must not change
  * afterwards (an execution after a library call changed them fails).  flags:

@@ -179,3 +179,37 @@ def test_fused_full_hubbard_chi2048():
     prob.make_fused_plan(sb.PREC_FP32)
     g32 = prob.Hpsi_fused.parts()
     assert rel_frobenius(g32["data"], chain["data"]) <= FP32_TOL
+
+
+def test_fused_zero_psi_then_values():
+    """psi of zeros gives Hpsi of zeros exactly (every block has nothing to
+    multiply and is cleared), and the next execution with values is whole"""
+    prob = small("hubbard")
+    print("[heff fused zero psi]", flush=True)
+    prob.upload()
+    prob.make_fused_plan(0)
+    pp = prob.psi_parts
+    prob.psi.set_data_(np.zeros_like(pp["data"]))
+    prob.fused.execute()
+    z = prob.Hpsi_fused.parts()
+    assert len(z["ids"]) > 0 and not np.any(z["data"])
+    prob.psi.set_data_(pp["data"])
+    prob.fused.execute()
+    got = prob.Hpsi_fused.parts()
+    q = HeffOracle(prob).run().parts()
+    same_blocks(got, q)
+    assert rel_frobenius(got["data"], q["data"]) <= FP64_TOL
+
+
+def test_fused_non_finite_psi_reaches_output():
+    """a NaN in psi reaches Hpsi (the rows holding it are multiplied), so an
+    eigensolve over the plan sees it; where it spreads may be narrower than in
+    the four-call chain, whose dense MPO steps multiply it by every MPO entry"""
+    prob = small("hubbard")
+    prob.upload()
+    prob.make_fused_plan(0)
+    data = prob.psi_parts["data"].copy()
+    data[int(np.flatnonzero(data)[3])] = np.nan
+    prob.psi.set_data_(data)
+    prob.fused.execute()
+    assert np.isnan(prob.Hpsi_fused.parts()["data"]).any()
