@@ -235,15 +235,21 @@ struct PackK {
 // mark the output blocks of the pairs that have work with these rows (a
 // chunk of rows meets the W12 columns of one of the pair's link items)
 template <class T, int P>
-__global__ __launch_bounds__(256) void heff_pack_psi(PackK<T> k) {
-  __shared__ int sm;
+__global__ __launch_bounds__(64) void heff_pack_psi(PackK<T> k) {
+  // one wave per psi block: the row mask is a wave reduction (no LDS, no
+  // barrier), and the pairs reading the block are fetched while psi is read
   const PsiJob jb = k.jobs[blockIdx.x];
-  if (threadIdx.x == 0) sm = 0;
-  __syncthreads();
+  const int lane = threadIdx.x;
   const int n = jb.dl * jb.dr;
   const i64 Dl = k.Dl;
+  int2 u0 = make_int2(0, 0);
+  HPair pr0{};
+  if (lane < jb.nu) {
+    u0 = k.uses[jb.u0 + lane];
+    pr0 = k.pairs[u0.x];
+  }
   int bits = 0;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+  for (int t = lane; t < n; t += 64) {
     const int r = t / jb.dl, l = t - r * jb.dl;
     // P consecutive values, 16-byte aligned (block payloads and rows are multiples of P >= 4 values)
     const double2* s = reinterpret_cast<const double2*>(k.psi + jb.src + (i64)P * (l + Dl * r));
@@ -256,15 +262,12 @@ __global__ __launch_bounds__(256) void heff_pack_psi(PackK<T> k) {
       if (v[h].y != 0.0) bits |= 1 << (2 * h + 1);
     }
   }
-  bits = wave_or(bits);
-  if ((threadIdx.x & 63) == 0 && bits) atomicOr(&sm, bits);
-  __syncthreads();
-  const int msk = sm, np = __popc(msk);
-  if (threadIdx.x == 0) k.pmask[jb.blk] = msk | (np << 16);
+  const int msk = wave_or(bits), np = __popc(msk);
+  if (lane == 0) k.pmask[jb.blk] = msk | (np << 16);
   const uint64_t lst = row_list(msk, P);
   const int K1 = up4(jb.dl);
   const i64 csz = (i64)K1 * jb.pstr;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+  for (int t = lane; t < n; t += 64) {  // the same values again, from this lane's cache lines
     const int r = t / jb.dl, l = t - r * jb.dl;
     const double* s = k.psi + jb.src + (i64)P * (l + Dl * r);
     T* d = k.pp + jb.dst + (i64)l * jb.pstr + 4 * r;
@@ -277,9 +280,13 @@ __global__ __launch_bounds__(256) void heff_pack_psi(PackK<T> k) {
       for (int j = 0; j < 4; ++j) dc[j] = v[j];
     }
   }
-  for (int t = threadIdx.x; t < jb.nu; t += blockDim.x) {
-    const int2 u = k.uses[jb.u0 + t];
-    const HPair pr = k.pairs[u.x];
+  for (int t = lane; t < jb.nu; t += 64) {
+    int2 u = u0;
+    HPair pr = pr0;
+    if (t != lane) {
+      u = k.uses[jb.u0 + t];
+      pr = k.pairs[u.x];
+    }
     bool act = false;
     for (int ch = 0; 4 * ch < np && !act; ++ch) {
       int cm = 0;
@@ -659,7 +666,7 @@ void launch_pack_psi_t(HeffState* s, hipStream_t st) {
   k.items = s->items.get();
   k.active = s->active.get();
   k.epoch = s->epoch;
-  const dim3 grid((unsigned)s->njobs), block(256);
+  const dim3 grid((unsigned)s->njobs), block(64);
   if (s->P == 4)
     hipLaunchKernelGGL((heff_pack_psi<T, 4>), grid, block, 0, st, k);
   else if (s->P == 8)
@@ -1259,6 +1266,9 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
                 "pairs=%lld  items=%lld  Hpsi.blocks=%lld  launches=%zu  flops=%.6g (psi's rows at planning)\n",
                 (long long)psi.n, (long long)L.n, (long long)R.n, (long long)pairs.size(), (long long)items.size(),
                 (long long)nout, st->launches.size(), flops);
+    for (size_t i = 0; i < st->launches.size(); ++i)
+      std::printf("[SB_TRACE]   heff launch %zu: %lld workgroups, %zu bytes of LDS each, %d A panels per batch\n", i,
+                  (long long)st->launches[i].n, st->launches[i].lds, st->launches[i].apb);
     std::fflush(stdout);
   }
   return pl;
