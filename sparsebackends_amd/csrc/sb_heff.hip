@@ -500,15 +500,14 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
   const T* const glp = k.lp;
   const T* const grp = k.rp;
   const T* const gw12 = k.w12;
-  // this lane's row of an entry's chunk (the k index of the W12 operand), -1 past the chunk's rows
+  // this lane's row of an entry's chunk (the k index of the W12 operand): the
+  // kq-th set bit of the chunk's row mask, -1 past the chunk's rows
   auto row_of = [=](int cm) {
-    int prow = -1, c = 0;
-    for (int b = 0; b < P; ++b)
-      if (cm >> b & 1) {
-        if (c == kq) prow = b;
-        ++c;
-      }
-    return prow;
+    int x = cm;
+    x = kq >= 1 ? x & (x - 1) : x;
+    x = kq >= 2 ? x & (x - 1) : x;
+    x = kq >= 3 ? x & (x - 1) : x;
+    return x ? __builtin_ctz(x) : -1;
   };
   const int lm_sh = LM > 16 ? 1 : 0;  // row tiles of A: LM / 16 = 1 or 2
   for (int e = 0; e < nE; ++e) {
@@ -524,10 +523,13 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
       if (!(((unsigned)amask >> b0) & ((1u << (b1 - b0)) - 1u))) continue;
       bar();  // the previous user of As is done
       // A_i[l'][c] = sum_l L_i[l][l'] psi[l][c], c = p_c + 4 r: operands straight from the panels
+      // tiles of the active panels of the batch, dealt to the team's waves in turn
       const int ntl = N1 >> 4, per = ntl << lm_sh;
-      for (int t = tw; t < (b1 - b0) * per; t += NW) {
-        const int ii = t / per, rem = t - ii * per, i = b0 + ii;
-        if (!(amask >> i & 1)) continue;
+      unsigned pbits = ((unsigned)amask >> b0) & ((1u << (b1 - b0)) - 1u);
+      for (int tk = 0; pbits; pbits &= pbits - 1) {
+        const int ii = __builtin_ctz(pbits), i = b0 + ii;
+        for (int rem = 0; rem < per; ++rem, ++tk) {
+        if (NW > 1 && (tk & 3) != tw) continue;
         const int mi = rem & lm_sh, ni = rem >> lm_sh;
         const T* Lg = glp + pr.lp + (i64)i * K1 * LSTR + mi * 16 + m;
         const T* Pt = Pg + ni * 16;
@@ -548,13 +550,24 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
           const int lq = mi * 16 + M::row(v, lane);
           if (lq < AR) At[lq * PSTR] = c[v];
         }
+        }
       }
       bar();
-      for (int it = 0; it < pr.nitem; ++it) {
-        const int2 c2 = its[pr.item0 - ib0 + it];
-        const int code = uni(c2.x), sup = uni(c2.y);
+      if (tw >= ng) continue;  // a wave of a team past the block's row groups has no output tiles
+      // the link items of the batch meeting the chunk's rows: one ballot per 64
+      // items, each lane decoding one item, the walk reading them with readlane
+      for (int c0 = 0; c0 < pr.nitem; c0 += 64) {
+        int code_l = 0;
+        bool on = false;
+        if (c0 + lane < pr.nitem) {
+          const int2 c2 = its[pr.item0 - ib0 + c0 + lane];
+          const int iL_l = c2.x & 31;
+          code_l = c2.x;
+          on = iL_l >= b0 && iL_l < b1 && (c2.y & cm);
+        }
+        for (uint64_t imask = __ballot(on); imask; imask &= imask - 1) {
+        const int code = __builtin_amdgcn_readlane(code_l, __builtin_ctzll(imask));
         const int iL = code & 31, iR = (code >> 5) & 31, lk = code >> 10;
-        if (iL < b0 || iL >= b1 || !(sup & cm)) continue;
         const T wv = prow >= 0 ? gw12[lk * P * P + (m % P) * P + prow] : (T)0;
         const T* Ai = As + (i64)(iL - b0) * AR * PSTR + zp;
         const T* Rg = grp + pr.rp + (i64)iR * K2 * RSTR + m;
@@ -579,6 +592,7 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
               acc[gi][ct][v / LPT] = M::run(wa, z[v], acc[gi][ct][v / LPT]);
             }
           }
+        }
         }
       }
     }
