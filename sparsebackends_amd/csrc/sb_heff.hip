@@ -156,11 +156,18 @@ struct HK {
 };
 
 // one active (sector pair, row chunk) of an output block, listed in LDS at
-// the start of its team's work: the pair, the chunk, its rows p and the L
-// panels an active item uses
+// the start of its team's work with what the walk needs of its pair (no
+// global load of the pair per entry): the chunk's psi panel, the pair's L
+// and R panels, its link items, its rows p and the L panels an active item uses
 struct HEnt {
-  int q, ch, cm, amask;
+  i64 pp;           // the chunk's psi panel
+  i64 lp, rp;       // the pair's first L / R link panel
+  int item0, nitem; // the pair's link items
+  int geo;          // dl | dr << 8 | nl << 16
+  int cm, amask;
+  int pad;
 };
+static_assert(sizeof(HEnt) == 48, "entry records are 48 bytes");
 
 template <class T>
 struct Mfma;
@@ -467,8 +474,12 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
             }
           }
           HEnt* en = ents + pos++;
-          en->q = q;
-          en->ch = ch;
+          en->pp = pr.pp + (i64)ch * up4(pr.dl) * lds_stride<T>(4 * up4(pr.dr));
+          en->lp = pr.lp;
+          en->rp = pr.rp;
+          en->item0 = pr.item0;
+          en->nitem = pr.nitem;
+          en->geo = pr.dl | (pr.dr << 8) | (pr.nl << 16);
           en->cm = cm;
           en->amask = am;
         }
@@ -511,13 +522,14 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
   };
   const int lm_sh = LM > 16 ? 1 : 0;  // row tiles of A: LM / 16 = 1 or 2
   for (int e = 0; e < nE; ++e) {
-    const HEnt en = ents[e];
-    const int q = uni(en.q), ch = uni(en.ch), cm = uni(en.cm), amask = uni(en.amask);
-    const HPair pr = k.pairs[q];
-    const int dl = pr.dl, dr = pr.dr, nl = pr.nl;
+    const HEnt& en = ents[e];
+    const int cm = uni(en.cm), amask = uni(en.amask), geo = uni(en.geo);
+    const int item0 = uni(en.item0), nitem = uni(en.nitem);
+    const i64 lp0 = uni(en.lp), rp0 = uni(en.rp);
+    const int dl = geo & 255, dr = (geo >> 8) & 255, nl = geo >> 16;
     const int K1 = up4(dl), K2 = up4(dr), N1 = 4 * K2, PSTR = lds_stride<T>(N1);
     const int prow = row_of(cm);
-    const T* Pg = gpp + pr.pp + (i64)ch * K1 * PSTR + m;
+    const T* Pg = gpp + uni(en.pp) + m;
     for (int b0 = 0; b0 < nl; b0 += k.apb) {
       const int b1 = min(nl, b0 + k.apb);
       if (!(((unsigned)amask >> b0) & ((1u << (b1 - b0)) - 1u))) continue;
@@ -531,7 +543,7 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
         for (int rem = 0; rem < per; ++rem, ++tk) {
         if (NW > 1 && (tk & 3) != tw) continue;
         const int mi = rem & lm_sh, ni = rem >> lm_sh;
-        const T* Lg = glp + pr.lp + (i64)i * K1 * LSTR + mi * 16 + m;
+        const T* Lg = glp + lp0 + (i64)i * K1 * LSTR + mi * 16 + m;
         const T* Pt = Pg + ni * 16;
         T la[KS], pa[KS];
 #pragma unroll
@@ -556,11 +568,11 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
       if (tw >= ng) continue;  // a wave of a team past the block's row groups has no output tiles
       // the link items of the batch meeting the chunk's rows: one ballot per 64
       // items, each lane decoding one item, the walk reading them with readlane
-      for (int c0 = 0; c0 < pr.nitem; c0 += 64) {
+      for (int c0 = 0; c0 < nitem; c0 += 64) {
         int code_l = 0;
         bool on = false;
-        if (c0 + lane < pr.nitem) {
-          const int2 c2 = its[pr.item0 - ib0 + c0 + lane];
+        if (c0 + lane < nitem) {
+          const int2 c2 = its[item0 - ib0 + c0 + lane];
           const int iL_l = c2.x & 31;
           code_l = c2.x;
           on = iL_l >= b0 && iL_l < b1 && (c2.y & cm);
@@ -570,7 +582,7 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
         const int iL = code & 31, iR = (code >> 5) & 31, lk = code >> 10;
         const T wv = prow >= 0 ? gw12[lk * P * P + (m % P) * P + prow] : (T)0;
         const T* Ai = As + (i64)(iL - b0) * AR * PSTR + zp;
-        const T* Rg = grp + pr.rp + (i64)iR * K2 * RSTR + m;
+        const T* Rg = grp + rp0 + (i64)iR * K2 * RSTR + m;
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           if (ct >= nct) break;
