@@ -533,6 +533,31 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
     for (int b0 = 0; b0 < nl; b0 += k.apb) {
       const int b1 = min(nl, b0 + k.apb);
       if (!(((unsigned)amask >> b0) & ((1u << (b1 - b0)) - 1u))) continue;
+      // the link items of the batch meeting the chunk's rows: one ballot per 64
+      // items, each lane decoding one item, the walk reading them with
+      // readlane; the first 64 here, so that (kernels for sectors of up to 16)
+      // the first item's W12 and R operands are fetched before the A products
+      int code0 = 0;
+      bool on0 = false;
+      if (lane < nitem) {
+        const int2 c2 = its[item0 - ib0 + lane];
+        const int iL_l = c2.x & 31;
+        code0 = c2.x;
+        on0 = iL_l >= b0 && iL_l < b1 && (c2.y & cm);
+      }
+      const uint64_t im0 = __ballot(on0);
+      constexpr bool PRE = KS == 4 && CT == 1;
+      T wv0 = (T)0, rb0[KS];
+      if constexpr (PRE) {
+        if (tw < ng && im0) {
+          const int code = __builtin_amdgcn_readlane(code0, __builtin_ctzll(im0));
+          const int iR = (code >> 5) & 31, lk = code >> 10;
+          wv0 = prow >= 0 ? gw12[lk * P * P + (m % P) * P + prow] : (T)0;
+          const T* Rg = grp + rp0 + (i64)iR * K2 * RSTR + m;
+#pragma unroll
+          for (int st = 0; st < KS; ++st) rb0[st] = 4 * st < K2 ? Rg[(4 * st + kq) * RSTR] : (T)0;
+        }
+      }
       bar();  // the previous user of As is done
       // A_i[l'][c] = sum_l L_i[l][l'] psi[l][c], c = p_c + 4 r: operands straight from the panels
       // tiles of the active panels of the batch, dealt to the team's waves in turn
@@ -566,29 +591,37 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
       }
       bar();
       if (tw >= ng) continue;  // a wave of a team past the block's row groups has no output tiles
-      // the link items of the batch meeting the chunk's rows: one ballot per 64
-      // items, each lane decoding one item, the walk reading them with readlane
+      bool first = PRE && im0 != 0;  // the first item's operands are in wv0 / rb0
       for (int c0 = 0; c0 < nitem; c0 += 64) {
-        int code_l = 0;
-        bool on = false;
-        if (c0 + lane < nitem) {
-          const int2 c2 = its[item0 - ib0 + c0 + lane];
-          const int iL_l = c2.x & 31;
-          code_l = c2.x;
-          on = iL_l >= b0 && iL_l < b1 && (c2.y & cm);
+        int code_l = code0;
+        uint64_t imask = im0;
+        if (c0) {
+          bool on = false;
+          if (c0 + lane < nitem) {
+            const int2 c2 = its[item0 - ib0 + c0 + lane];
+            const int iL_l = c2.x & 31;
+            code_l = c2.x;
+            on = iL_l >= b0 && iL_l < b1 && (c2.y & cm);
+          }
+          imask = __ballot(on);
         }
-        for (uint64_t imask = __ballot(on); imask; imask &= imask - 1) {
+        for (; imask; imask &= imask - 1) {
         const int code = __builtin_amdgcn_readlane(code_l, __builtin_ctzll(imask));
         const int iL = code & 31, iR = (code >> 5) & 31, lk = code >> 10;
-        const T wv = prow >= 0 ? gw12[lk * P * P + (m % P) * P + prow] : (T)0;
+        const T wv = first ? wv0 : prow >= 0 ? gw12[lk * P * P + (m % P) * P + prow] : (T)0;
         const T* Ai = As + (i64)(iL - b0) * AR * PSTR + zp;
         const T* Rg = grp + rp0 + (i64)iR * K2 * RSTR + m;
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           if (ct >= nct) break;
           T rb[KS];
+          if (first) {
 #pragma unroll
-          for (int st = 0; st < KS; ++st) rb[st] = 4 * st < K2 ? Rg[(4 * st + kq) * RSTR + ct * 16] : (T)0;
+            for (int st = 0; st < KS; ++st) rb[st] = rb0[st];
+          } else {
+#pragma unroll
+            for (int st = 0; st < KS; ++st) rb[st] = 4 * st < K2 ? Rg[(4 * st + kq) * RSTR + ct * 16] : (T)0;
+          }
 #pragma unroll
           for (int gi = 0; gi < GPW; ++gi) {
             const int gq = tw + NW * gi;
@@ -605,6 +638,7 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
             }
           }
         }
+        first = false;
         }
       }
     }
