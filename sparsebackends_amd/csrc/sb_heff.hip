@@ -137,6 +137,7 @@ template <class T>
 struct HK {
   const T *pp, *lp, *rp, *w12;  // psi chunks, L and R panels, W12 [lk][p'][p]
   const i32* pmask;             // per psi block: row mask | row count << 16
+  const int4* pinfo;            // per pair: its chunks with work, their rows and L panels (the pack kernel's)
   double* out;
   const int4* jobs;   // per workgroup: one block (y == -2) or up to 4 blocks, one per wave (-1: none)
   const i32 *o_dlp, *o_drp, *o_pb;
@@ -234,6 +235,7 @@ struct PackK {
   const int2* items;
   i32* active;  // per output block: the execution's epoch when a pair of it has work
   int epoch;
+  int4* pinfo;  // per pair: chunks with work (bits), rows of chunks 0 and 1 (16 bits each), L panels (8 bits per chunk), psi's row mask
 };
 
 // one workgroup per psi block: find the rows p holding a nonzero inside the
@@ -294,20 +296,30 @@ __global__ __launch_bounds__(64) void heff_pack_psi(PackK<T> k) {
       u = k.uses[jb.u0 + t];
       pr = k.pairs[u.x];
     }
-    bool act = false;
-    for (int ch = 0; 4 * ch < np && !act; ++ch) {
+    // per chunk of rows: the L panels of the items meeting it (those with work)
+    int cnt = 0, cms = 0, ams = 0;
+    for (int ch = 0; 4 * ch < np; ++ch) {
       int cm = 0;
       for (int j = 4 * ch; j < min(np, 4 * ch + 4); ++j) cm |= 1 << nib(lst, j);
+      int am = 0;
       if (pr.nl <= 4) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          if (i < pr.nl && (pr.pc[i] & cm)) act = true;
+          if (i < pr.nl && (pr.pc[i] & cm)) am |= 1 << i;
       } else {
-        for (int it = 0; it < pr.nitem; ++it)
-          if (k.items[pr.item0 + it].y & cm) act = true;
+        for (int it = 0; it < pr.nitem; ++it) {
+          const int2 c2 = k.items[pr.item0 + it];
+          if (c2.y & cm) am |= 1 << (c2.x & 31);
+        }
+      }
+      if (am) {
+        if (ch < 2) cms |= cm << (16 * ch);
+        ams |= (int)((unsigned)(am & 255) << (8 * ch));
+        cnt |= 1 << ch;
       }
     }
-    if (act) k.active[u.y] = k.epoch;
+    k.pinfo[u.x] = make_int4(cnt, cms, ams, msk | (np << 16));
+    if (cnt) k.active[u.y] = k.epoch;
   }
 }
 
@@ -419,31 +431,14 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
       const int q = qb + lane;
       int cnt = 0, cms = 0, ams = 0;  // per chunk: rows (16 bits, chunks 0 and 1), L panels (8 bits)
       HPair pr{};
+      int pm = 0;
       if (q < qe) {
         pr = k.pairs[q];
-        const int pm = k.pmask[pr.blk];
-        const int np = pm >> 16;
-        const uint64_t lst = row_list(pm & 0xffff, P);
-        for (int ch = 0; 4 * ch < np; ++ch) {
-          int cm = 0;
-          for (int j = 4 * ch; j < min(np, 4 * ch + 4); ++j) cm |= 1 << nib(lst, j);
-          int am = 0;
-          if (pr.nl <= 4) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-              if (i < pr.nl && (pr.pc[i] & cm)) am |= 1 << i;
-          } else {
-            for (int it = 0; it < pr.nitem; ++it) {
-              const int2 c2 = k.items[pr.item0 + it];
-              if (c2.y & cm) am |= 1 << (c2.x & 31);
-            }
-          }
-          if (am) {
-            if (ch < 2) cms |= cm << (16 * ch);
-            ams |= (int)((unsigned)(am & 255) << (8 * ch));
-            cnt |= 1 << ch;
-          }
-        }
+        const int4 pi = k.pinfo[q];
+        cnt = pi.x;
+        cms = pi.y;
+        ams = pi.z;
+        pm = pi.w;
       }
       const int ncnt = __popc(cnt);
       int inc = ncnt;  // inclusive scan of the counts over the wave
@@ -460,7 +455,7 @@ __global__ __launch_bounds__(256, KS == 4 ? 4 : 2) void heff_kernel(HK<T> k) {
           if (ch < 2) {
             cm = (cms >> (16 * ch)) & 0xffff;
           } else {  // chunks 2 and 3 (16 rows): the rows again
-            const int pm = k.pmask[pr.blk], np = pm >> 16;
+            const int np = pm >> 16;
             const uint64_t lst = row_list(pm & 0xffff, P);
             cm = 0;
             for (int j = 4 * ch; j < min(np, 4 * ch + 4); ++j) cm |= 1 << nib(lst, j);
@@ -682,6 +677,7 @@ struct HeffState {
   i64 nout = 0;
   DBuf<i32> o_dlp, o_drp, o_pb, o_ib, pmask, dirty, active;
   DBuf<int2> uses;
+  DBuf<int4> pinfo;  // per pair: its chunks with work, written by every pack
   int epoch = 0;  // executions so far: the pack kernel marks the blocks with work with it
   DBuf<int4> jobs;
   i64 vout = -1;  // Hpsi's version after this plan's last execution (another value: Hpsi written outside)
@@ -726,6 +722,7 @@ void launch_pack_psi_t(HeffState* s, hipStream_t st) {
   k.items = s->items.get();
   k.active = s->active.get();
   k.epoch = s->epoch;
+  k.pinfo = s->pinfo.get();
   const dim3 grid((unsigned)s->njobs), block(64);
   if (s->P == 4)
     hipLaunchKernelGGL((heff_pack_psi<T, 4>), grid, block, 0, st, k);
@@ -752,6 +749,7 @@ void launch_main(HeffState* s, const HeffState::Launch& la, hipStream_t st) {
   k.rp = (const T*)s->rp.get();
   k.w12 = (const T*)s->w12.get();
   k.pmask = s->pmask.get();
+  k.pinfo = s->pinfo.get();
   k.out = s->out->dptr();
   k.jobs = s->jobs.get() + la.first;
   k.o_dlp = s->o_dlp.get();
@@ -1086,6 +1084,7 @@ std::unique_ptr<Plan> heff_plan(Ctx* ctx, Tensor& out, const Tensor& psi, const 
   up(ctx, st->pjobs, pjobs);
   up(ctx, st->active, std::vector<i32>((size_t)std::max<i64>(nout, 1), -1));
   st->pmask.alloc(ctx, (size_t)std::max<i64>(st->njobs, 1));
+  st->pinfo.alloc(ctx, std::max<size_t>(pairs.size(), 1));
   // panels: zero (the padding stays zero), then filled from the operands
   auto zero_buf = [&](DBuf<unsigned char>& b, i64 elems) {
     const size_t bytes = (size_t)std::max<i64>(elems, 4) * esz;
