@@ -43,15 +43,6 @@ static inline unsigned nb256(i64 n) { return (unsigned)((n + 255) / 256); }
 constexpr i64 kScanChunk = 8192;
 constexpr i64 kScanSmall = (i64)1 << 24;
 
-__device__ __forceinline__ int scan_div(int n, int d, float inv) {
-  int q = (int)((float)n * inv);
-  const int r = n - q * d;
-  if (r >= d)
-    ++q;
-  else if (r < 0)
-    --q;
-  return q;
-}
 
 struct ScanInv {
   float v[MAXR];
@@ -84,7 +75,7 @@ __global__ __launch_bounds__(256) void box_scan_kernel(const double* __restrict_
           for (int a = 0; a < MAXR; ++a)
             if (a < N2) {
               const int d = (int)td.v[a];
-              const int q = scan_div(x, d, inv.v[a]);
+              const int q = div24(x, d, inv.v[a]);
               lm[a] = max(lm[a], x - q * d + 1);
               x = q;
             }

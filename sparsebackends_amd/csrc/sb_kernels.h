@@ -16,6 +16,21 @@ struct SArr {  // signed source selectors for key assembly: +a -> A axis a, -b -
   int v[MAXR];
 };
 
+// n / d for 0 <= n < 2^24 and d >= 1 by a float reciprocal (inv = 1.0f / d)
+// and one correction step: exact there, the quotient's rounding error staying
+// below 1 (device code that splits column-major indices of small blocks)
+#ifdef __HIPCC__
+__device__ __forceinline__ int div24(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  const int r = n - q * d;
+  if (r >= d)
+    ++q;
+  else if (r < 0)
+    --q;
+  return q;
+}
+#endif
+
 // ---- keys -------------------------------------------------------------
 // out_keys[i, j] = in_keys[i, sel[j]-1]  (j < Pout); lin[i] = col-major lin
 // of out_keys[i] over pdims_out (1-based); iota[i] = i

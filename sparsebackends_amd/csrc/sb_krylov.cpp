@@ -368,9 +368,10 @@ namespace {
 
 // One Lanczos solve as a resumable state, so that several solves can share
 // each operator application (a batch of plan chains) and each host
-// synchronisation.  Per iteration: [load(); the operator; applied(); a sync
-// when tracking(); store()] unless apply_needed() says H v is already in
-// place, then orth(); a sync; update() (true when the solve is over).
+// synchronisation.  Per iteration: [load(); the operator; applied(); store()]
+// unless apply_needed() says H v is already in place, then orth(); a sync;
+// redo() (when the operator's output reached beyond the compact layout:
+// orth() and a sync again); update() (true when the solve is over).
 class Lanczos {
  public:
   Lanczos(Ctx* ctx, Tensor& x, Tensor& y, int kdim, int maxiter, double tol)
@@ -390,7 +391,16 @@ class Lanczos {
     N2_ = x.N2;
     td_.assign(x.dims.begin() + x.P(), x.dims.end());
     ld_ = kdim + 1;
+    // pinned, so that the flag's copy to the host stays asynchronous (a copy
+    // into pageable memory would wait for the stream)
+    HIP_CHECK(hipHostMalloc((void**)&hv_, sizeof(i32), hipHostMallocDefault));
+    *hv_ = 0;
   }
+  ~Lanczos() {
+    if (hv_) (void)hipHostFree(hv_);
+  }
+  Lanczos(const Lanczos&) = delete;
+  Lanczos& operator=(const Lanczos&) = delete;
 
   // v0 = x / |x| in x's payload, and x's own nonzero extents per block
   void begin() {
@@ -464,20 +474,34 @@ class Lanczos {
     });
     x_.touch();
   }
-  // after the operator: y's boxes checked against S (read on the host after a sync)
+  // after the operator: y's boxes checked against S (read on the host after
+  // the step's sync, by redo())
   void applied() {
     ++R_.numops;
     if (track_) {
       k_box_exceeds(ctx_, nb_, N2_, S_.get(), y_.box.get(), yrow_.get(), viol_.get());
-      HIP_CHECK(hipMemcpyAsync(&hv_, viol_.get(), sizeof(i32), hipMemcpyDeviceToHost, ctx_->stream));
+      HIP_CHECK(hipMemcpyAsync(hv_, viol_.get(), sizeof(i32), hipMemcpyDeviceToHost, ctx_->stream));
+      pending_ = true;
     }
   }
-  // y's blocks by key into the compact layout, S widened first where needed
+  // y's blocks by key into the compact layout with the current S, queued at
+  // once: whether y reached beyond S is known after the step's sync
   void store() {
-    if (track_ && hv_) widen(j_ + 1);
     step(ctx_, "lanczos_gather", [&] {
       k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(j_ + 1));
     });
+  }
+  // after the sync that follows orth(): true when this step's operator output
+  // reached beyond S (rare: the support only grows while the Krylov space
+  // does); S is then widened and y gathered again, and the caller
+  // orthogonalises once more and syncs before update()
+  bool redo() {
+    if (!pending_) return false;
+    pending_ = false;
+    if (!*hv_) return false;
+    widen(j_ + 1);
+    store();
+    return true;
   }
 
   // classical Gram-Schmidt, twice, of vec(j+1) against v_0 .. v_j; the
@@ -624,8 +648,8 @@ class Lanczos {
   DBuf<i32> xb0_, yrow_, S_, viol_;
   DBuf<i64> off_, coff_, xoff_;
   DBuf<double> V_, part_, small_, dU_, tmp_;
-  bool track_ = false, have_first_ = false, converged_ = false;
-  i32 hv_ = 0;
+  bool track_ = false, have_first_ = false, converged_ = false, pending_ = false;
+  i32* hv_ = nullptr;  // pinned host word: y reached beyond S
   std::vector<double> T_, hh_, theta_, U_;
   int j_ = 0, cycles_ = 1, m_ = 0;
   double beta_ = 0, resid_ = 0;
@@ -666,11 +690,14 @@ EigResult eigsolve_lanczos(Ctx* ctx, const std::vector<const Plan*>& plans, Tens
       L.load();
       apply();
       L.applied();
-      if (L.tracking()) HIP_CHECK(hipStreamSynchronize(ctx->stream));
       L.store();
     }
     L.orth();
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (L.redo()) {
+      L.orth();
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
     if (L.update()) break;
   }
   return L.finish();
@@ -715,16 +742,20 @@ std::vector<EigResult> eigsolve_lanczos_batch(Ctx* ctx, Batch& batch, const std:
     if (!need.empty()) {
       for (int c : need) L[c]->load();
       apply(need);
-      bool any_track = false;
       for (int c : need) {
         L[c]->applied();
-        any_track = any_track || L[c]->tracking();
+        L[c]->store();
       }
-      if (any_track) HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      for (int c : need) L[c]->store();
     }
     for (int c : active) L[c]->orth();
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    bool again = false;
+    for (int c : active)
+      if (L[c]->redo()) {
+        L[c]->orth();
+        again = true;
+      }
+    if (again) HIP_CHECK(hipStreamSynchronize(ctx->stream));
     std::vector<int> still;
     for (int c : active)
       if (!L[c]->update()) still.push_back(c);

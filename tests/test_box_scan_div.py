@@ -1,6 +1,6 @@
-"""The box scan's 32-bit index decode (sb_boxgemm.hip:scan_div): quotient by
-a float reciprocal with one correction step, used for blocks of fewer than
-2^24 values.  Emulated here in IEEE single precision (numpy float32: the
+"""The 32-bit index decode of the box scan and of the Krylov vectors' compact
+layout (sb_kernels.h:div24): quotient by a float reciprocal with one
+correction step, used for blocks of fewer than 2^24 values.  Emulated here in IEEE single precision (numpy float32: the
 reciprocal rounded on the host, the product rounded to nearest, truncation
 toward zero, as the device's instructions do) and checked against integer
 division on every divisor up to 4096, powers of two and large divisors, at
