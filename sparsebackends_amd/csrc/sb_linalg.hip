@@ -35,12 +35,24 @@ __global__ __launch_bounds__(RT) void mdot_kernel(const double* __restrict__ V, 
   double acc[K + 1];
 #pragma unroll
   for (int i = 0; i <= K; ++i) acc[i] = 0.0;
-  for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
-    const double we = w[e];
+  if (k == K) {  // a full chunk: the K loads of an element issued together, no guard per vector
+    for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
+      const double we = w[e];
+      double v[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i)
-      if (i < k) acc[i] = fma(V[(i64)i * ldv + e], we, acc[i]);
-    acc[K] = fma(we, we, acc[K]);
+      for (int i = 0; i < K; ++i) v[i] = V[(i64)i * ldv + e];
+#pragma unroll
+      for (int i = 0; i < K; ++i) acc[i] = fma(v[i], we, acc[i]);
+      acc[K] = fma(we, we, acc[K]);
+    }
+  } else {
+    for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
+      const double we = w[e];
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (i < k) acc[i] = fma(V[(i64)i * ldv + e], we, acc[i]);
+      acc[K] = fma(we, we, acc[K]);
+    }
   }
 #pragma unroll
   for (int i = 0; i < K; ++i)
@@ -92,7 +104,15 @@ __global__ __launch_bounds__(RT) void orth_update_kernel(const double* __restric
   double a = 0.0;
   for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
     double we = w[e];
-    for (int i = 0; i < k; ++i) we = fma(-V[(i64)i * ldv + e], h[i], we);
+    int i = 0;
+    for (; i + 8 <= k; i += 8) {  // eight loads in flight, the updates in ascending i as below
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = V[(i64)(i + u) * ldv + e];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) we = fma(-v[u], h[i + u], we);
+    }
+    for (; i < k; ++i) we = fma(-V[(i64)i * ldv + e], h[i], we);
     w[e] = we;
     a = fma(we, we, a);
   }
