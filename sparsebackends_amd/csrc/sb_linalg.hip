@@ -81,8 +81,23 @@ void k_mdot(Ctx* ctx, const double* V, i64 ldv, int k, const double* w, i64 n, b
   for (int c = 0; c < k || (c == 0 && norm); c += MCH) {
     const int kc = std::min(MCH, k - c);
     const bool last = c + MCH >= k;
-    hipLaunchKernelGGL(mdot_kernel<MCH>, dim3(NB), dim3(RT), 0, ctx->stream, V + (i64)c * ldv, ldv, std::max(kc, 0),
-                       w, n, (norm && last) ? 1 : 0, part + (i64)c * NB);
+    // a kernel of the chunk's own width, so that its loads take the unguarded path
+    const void* fn = (const void*)mdot_kernel<MCH>;
+    switch (kc) {
+      case 1: fn = (const void*)mdot_kernel<1>; break;
+      case 2: fn = (const void*)mdot_kernel<2>; break;
+      case 3: fn = (const void*)mdot_kernel<3>; break;
+      case 4: fn = (const void*)mdot_kernel<4>; break;
+      case 5: fn = (const void*)mdot_kernel<5>; break;
+      case 6: fn = (const void*)mdot_kernel<6>; break;
+      case 7: fn = (const void*)mdot_kernel<7>; break;
+      default: break;
+    }
+    const double* vc = V + (i64)c * ldv;
+    int kk = std::max(kc, 0), nrm = (norm && last) ? 1 : 0;
+    double* pc = part + (i64)c * NB;
+    void* args[] = {(void*)&vc, (void*)&ldv, (void*)&kk, (void*)&w, (void*)&n, (void*)&nrm, (void*)&pc};
+    HIP_CHECK(hipLaunchKernel(fn, dim3(NB), dim3(RT), args, 0, ctx->stream));
     HIP_CHECK(hipGetLastError());
     if (last) break;
   }
