@@ -391,13 +391,6 @@ class Lanczos {
     N2_ = x.N2;
     td_.assign(x.dims.begin() + x.P(), x.dims.end());
     ld_ = kdim + 1;
-    // pinned, so that the flag's copy to the host stays asynchronous (a copy
-    // into pageable memory would wait for the stream)
-    HIP_CHECK(hipHostMalloc((void**)&hv_, sizeof(i32), hipHostMallocDefault));
-    *hv_ = 0;
-  }
-  ~Lanczos() {
-    if (hv_) (void)hipHostFree(hv_);
   }
   Lanczos(const Lanczos&) = delete;
   Lanczos& operator=(const Lanczos&) = delete;
@@ -446,15 +439,14 @@ class Lanczos {
     const int NBP = linalg_nb();
     V_.alloc(ctx_, (size_t)(kdim_ + 1) * n_);
     part_.alloc(ctx_, (size_t)(kdim_ + 2) * NBP);
-    small_.alloc(ctx_, 3 * (size_t)ld_ + 3);
+    small_.alloc(ctx_, 3 * (size_t)ld_ + 4);  // h1, h2, the norm, and the box flag's word
     dU_.alloc(ctx_, (size_t)kdim_ * kdim_);
-    viol_.alloc(ctx_, 1);
     // v0 and H v0 in the compact layout
     k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), x_.dptr(), vec(0));
     k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(1));
     have_first_ = true;  // H v0 is in vec(1) already
     T_.assign((size_t)ld_ * ld_, 0.0);
-    hh_.assign(3 * (size_t)ld_ + 3, 0.0);
+    hh_.assign(3 * (size_t)ld_ + 4, 0.0);
   }
 
   // whether this iteration applies the operator (the first one has H v0)
@@ -474,13 +466,13 @@ class Lanczos {
     });
     x_.touch();
   }
-  // after the operator: y's boxes checked against S (read on the host after
-  // the step's sync, by redo())
+  // after the operator: y's boxes checked against S; the flag sits in the
+  // last word of the small buffer that orth() copies to the host, and is read
+  // after the step's sync by redo()
   void applied() {
     ++R_.numops;
     if (track_) {
-      k_box_exceeds(ctx_, nb_, N2_, S_.get(), y_.box.get(), yrow_.get(), viol_.get());
-      HIP_CHECK(hipMemcpyAsync(hv_, viol_.get(), sizeof(i32), hipMemcpyDeviceToHost, ctx_->stream));
+      k_box_exceeds(ctx_, nb_, N2_, S_.get(), y_.box.get(), yrow_.get(), flag_word());
       pending_ = true;
     }
   }
@@ -498,7 +490,9 @@ class Lanczos {
   bool redo() {
     if (!pending_) return false;
     pending_ = false;
-    if (!*hv_) return false;
+    i32 hv = 0;
+    std::memcpy(&hv, &hh_[3 * (size_t)ld_ + 3], sizeof(i32));
+    if (!hv) return false;
     widen(j_ + 1);
     store();
     return true;
@@ -594,6 +588,7 @@ class Lanczos {
 
  private:
   double* vec(int i) { return V_.get() + (i64)i * n_; }
+  i32* flag_word() { return reinterpret_cast<i32*>(small_.get() + 3 * (size_t)ld_ + 3); }
 
   i64 layout() {  // coff from S; returns the length
     std::vector<i32> hS((size_t)nb_ * N2_);
@@ -645,11 +640,10 @@ class Lanczos {
   i64 nb_ = 0, bs_ = 0, nfull_ = 0, n_ = 0;
   int N2_ = 0, ld_ = 0;
   std::vector<i64> td_;
-  DBuf<i32> xb0_, yrow_, S_, viol_;
+  DBuf<i32> xb0_, yrow_, S_;
   DBuf<i64> off_, coff_, xoff_;
   DBuf<double> V_, part_, small_, dU_, tmp_;
   bool track_ = false, have_first_ = false, converged_ = false, pending_ = false;
-  i32* hv_ = nullptr;  // pinned host word: y reached beyond S
   std::vector<double> T_, hh_, theta_, U_;
   int j_ = 0, cycles_ = 1, m_ = 0;
   double beta_ = 0, resid_ = 0;

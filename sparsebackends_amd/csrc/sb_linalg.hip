@@ -309,58 +309,72 @@ void k_support(Ctx* ctx, i64 n, int N2, const i32* xbox, const i32* ybox, const 
 // compact layout: row i's box S[i, :] (column-major, axis 0 fastest) packed at
 // coff[i]; element t of the compact vector <-> entry of the row's block at
 // boff[i] (boff < 0: absent, reads as zero).  mode 0: gather full -> compact;
-// mode 1: scatter compact -> full (only inside the boxes).  One wave per row:
-// the row's extents, strides and offsets are the wave's alike, and an index
-// inside a box of fewer than 2^24 entries is split in 32 bits (div24).
+// mode 1: scatter compact -> full (only inside the boxes).  One wave per
+// chunk of kCompactChunk compact entries: it finds the row of the chunk's
+// first entry once (binary search), then walks the rows the chunk spans, each
+// row's extents, strides and offsets the wave's alike; an index inside a box
+// of fewer than 2^24 entries is split in 32 bits (div24).
+constexpr int kCompactChunk = 256;
 __global__ __launch_bounds__(64) void compact_kernel(int mode, i64 nrows, int N2, IArr td, const i32* __restrict__ S,
                                                      const i64* __restrict__ coff, const i64* __restrict__ boff,
                                                      const double* __restrict__ src, double* __restrict__ dst) {
   const int lane = threadIdx.x;
-  for (i64 i = blockIdx.x; i < nrows; i += gridDim.x) {
-    const i64 c0 = coff[i], n = coff[i + 1] - c0, bo = boff[i];
-    if (n == 0 || (mode == 1 && bo < 0)) continue;
-    int e[MAXR];
-    float inv[MAXR];
+  const i64 total = coff[nrows];
+  for (i64 c = blockIdx.x; c * kCompactChunk < total; c += gridDim.x) {
+    const i64 t0 = c * kCompactChunk, t1 = min(total, t0 + kCompactChunk);
+    i64 a = 0, b = nrows - 1;  // last row with coff <= t0
+    while (a < b) {
+      const i64 m = (a + b + 1) >> 1;
+      if (coff[m] <= t0)
+        a = m;
+      else
+        b = m - 1;
+    }
+    for (i64 i = a, t = t0; t < t1; ++i) {
+      const i64 r0 = coff[i], r1 = min(coff[i + 1], t1);
+      if (r1 <= t) continue;  // an empty row
+      const i64 bo = boff[i];
+      if (mode == 0 || bo >= 0) {
+        int e[MAXR];
+        float inv[MAXR];
 #pragma unroll
-    for (int q = 0; q < MAXR; ++q)
-      if (q < N2) {
-        e[q] = S[i * N2 + q];  // >= 1: the box holds n > 0 entries
-        inv[q] = 1.0f / (float)e[q];
-      }
-    if (n < ((i64)1 << 24)) {
+        for (int q = 0; q < MAXR; ++q)
+          if (q < N2) {
+            e[q] = S[i * N2 + q];  // >= 1: the row holds entries
+            inv[q] = 1.0f / (float)e[q];
+          }
+        const bool small = coff[i + 1] - r0 < ((i64)1 << 24);
 #pragma unroll 4
-      for (int x0 = lane; x0 < (int)n; x0 += 64) {
-        int x = x0;
-        i64 off = 0, st = 1;
+        for (i64 tt = t + lane; tt < r1; tt += 64) {
+          i64 off = 0, st = 1;
+          if (small) {
+            int x = (int)(tt - r0);
 #pragma unroll
-        for (int q = 0; q < MAXR; ++q)
-          if (q < N2) {
-            const int qq = div24(x, e[q], inv[q]);
-            off += (i64)(x - qq * e[q]) * st;
-            st *= td.v[q];
-            x = qq;
-          }
-        if (mode == 0)
-          dst[c0 + x0] = bo >= 0 ? src[bo + off] : 0.0;
-        else
-          dst[bo + off] = src[c0 + x0];
-      }
-    } else {
-      for (i64 x0 = lane; x0 < n; x0 += 64) {
-        i64 x = x0, off = 0, st = 1;
+            for (int q = 0; q < MAXR; ++q)
+              if (q < N2) {
+                const int qq = div24(x, e[q], inv[q]);
+                off += (i64)(x - qq * e[q]) * st;
+                st *= td.v[q];
+                x = qq;
+              }
+          } else {
+            i64 x = tt - r0;
 #pragma unroll
-        for (int q = 0; q < MAXR; ++q)
-          if (q < N2) {
-            const i64 qq = x / e[q];
-            off += (x - qq * e[q]) * st;
-            st *= td.v[q];
-            x = qq;
+            for (int q = 0; q < MAXR; ++q)
+              if (q < N2) {
+                const i64 qq = x / e[q];
+                off += (x - qq * e[q]) * st;
+                st *= td.v[q];
+                x = qq;
+              }
           }
-        if (mode == 0)
-          dst[c0 + x0] = bo >= 0 ? src[bo + off] : 0.0;
-        else
-          dst[bo + off] = src[c0 + x0];
+          if (mode == 0)
+            dst[tt] = bo >= 0 ? src[bo + off] : 0.0;
+          else
+            dst[bo + off] = src[tt];
+        }
       }
+      t = r1;
     }
   }
 }
@@ -369,7 +383,8 @@ void k_compact(Ctx* ctx, int mode, i64 nrows, int N2, const i64* tdims, const i3
   if (!total || !nrows) return;
   IArr td{};
   for (int q = 0; q < N2; ++q) td.v[q] = tdims[q];
-  hipLaunchKernelGGL(compact_kernel, dim3((unsigned)std::min<i64>(nrows, 65536)), dim3(64), 0, ctx->stream, mode,
+  const i64 chunks = (total + kCompactChunk - 1) / kCompactChunk;
+  hipLaunchKernelGGL(compact_kernel, dim3((unsigned)std::min<i64>(chunks, 65536)), dim3(64), 0, ctx->stream, mode,
                      nrows, N2, td, S, coff, boff, src, dst);
   HIP_CHECK(hipGetLastError());
 }
