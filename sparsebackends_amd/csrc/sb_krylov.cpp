@@ -28,7 +28,8 @@ void k_match_rows(Ctx*, const i32* xk, i64 nx, const i32* yk, i64 ny, int P, con
 void k_box_exceeds(Ctx*, i64 n, int N2, const i32* S, const i32* ybox, const i32* row, i32* flag);
 void k_support(Ctx*, i64 n, int N2, const i32* xbox, const i32* ybox, const i32* row, i32* S);
 void k_compact(Ctx*, int mode, i64 nrows, int N2, const i64* tdims, const i32* S, const i64* coff, i64 total,
-               const i64* boff, const double* src, double* dst);
+               const i64* boff, const double* src, double* dst, const i32* crow);
+int compact_chunk();
 
 namespace {
 
@@ -442,8 +443,8 @@ class Lanczos {
     small_.alloc(ctx_, 3 * (size_t)ld_ + 4);  // h1, h2, the norm, and the box flag's word
     dU_.alloc(ctx_, (size_t)kdim_ * kdim_);
     // v0 and H v0 in the compact layout
-    k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), x_.dptr(), vec(0));
-    k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(1));
+    k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), x_.dptr(), vec(0), crow_.get());
+    k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(1), crow_.get());
     have_first_ = true;  // H v0 is in vec(1) already
     T_.assign((size_t)ld_ * ld_, 0.0);
     hh_.assign(3 * (size_t)ld_ + 4, 0.0);
@@ -462,7 +463,7 @@ class Lanczos {
   // the basis vector into x's payload (entries outside S stay zero: x0 is zero there)
   void load() {
     step(ctx_, "lanczos_load", [&] {
-      k_compact(ctx_, 1, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), vec(j_), x_.dptr());
+      k_compact(ctx_, 1, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), vec(j_), x_.dptr(), crow_.get());
     });
     x_.touch();
   }
@@ -480,7 +481,8 @@ class Lanczos {
   // once: whether y reached beyond S is known after the step's sync
   void store() {
     step(ctx_, "lanczos_gather", [&] {
-      k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(j_ + 1));
+      k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, off_.get(), y_.dptr(), vec(j_ + 1),
+                crow_.get());
     });
   }
   // after the sync that follows orth(): true when this step's operator output
@@ -575,7 +577,8 @@ class Lanczos {
     });
     const double xn = norm_of(ctx_, ritzv.get(), n_);
     if (xn > 0) k_axpby(ctx_, n_, 1.0 / xn, ritzv.get(), 0.0, ritzv.get());
-    k_compact(ctx_, 1, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), ritzv.get(), x_.dptr());
+    k_compact(ctx_, 1, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), ritzv.get(), x_.dptr(),
+              crow_.get());
     x_.touch();
     HIP_CHECK(hipStreamSynchronize(ctx_->stream));
     R_.value = theta_[0];
@@ -602,6 +605,15 @@ class Lanczos {
       hc[i + 1] = hc[i] + c;
     }
     HIP_CHECK(hipMemcpyAsync(coff_.get(), hc.data(), (nb_ + 1) * sizeof(i64), hipMemcpyHostToDevice, ctx_->stream));
+    // per chunk of the compact kernel: the row holding its first entry
+    const i64 ch = compact_chunk(), nch = (hc[nb_] + ch - 1) / ch;
+    std::vector<i32> hr((size_t)std::max<i64>(nch, 1), 0);
+    for (i64 c = 0, i = 0; c < nch; ++c) {
+      while (i + 1 < nb_ && hc[i + 1] <= c * ch) ++i;
+      hr[(size_t)c] = (i32)i;
+    }
+    crow_.alloc(ctx_, hr.size());
+    HIP_CHECK(hipMemcpyAsync(crow_.get(), hr.data(), hr.size() * sizeof(i32), hipMemcpyHostToDevice, ctx_->stream));
     HIP_CHECK(hipStreamSynchronize(ctx_->stream));
     return hc[nb_];
   }
@@ -623,9 +635,9 @@ class Lanczos {
     for (int i = 0; i < nv; ++i) {
       k_zero(ctx_, full.get(), (size_t)nfull_ * sizeof(double));
       k_compact(ctx_, 1, nb_, N2_, td_.data(), S_old.get(), coff_old.get(), n_old, xoff_.get(),
-                V_.get() + (i64)i * n_old, full.get());
+                V_.get() + (i64)i * n_old, full.get(), nullptr);
       k_compact(ctx_, 0, nb_, N2_, td_.data(), S_.get(), coff_.get(), n_, xoff_.get(), full.get(),
-                V2.get() + (i64)i * n_);
+                V2.get() + (i64)i * n_, crow_.get());
     }
     V_ = std::move(V2);
     tmp_.release();
@@ -640,7 +652,7 @@ class Lanczos {
   i64 nb_ = 0, bs_ = 0, nfull_ = 0, n_ = 0;
   int N2_ = 0, ld_ = 0;
   std::vector<i64> td_;
-  DBuf<i32> xb0_, yrow_, S_;
+  DBuf<i32> xb0_, yrow_, S_, crow_;
   DBuf<i64> off_, coff_, xoff_;
   DBuf<double> V_, part_, small_, dU_, tmp_;
   bool track_ = false, have_first_ = false, converged_ = false, pending_ = false;

@@ -315,20 +315,26 @@ void k_support(Ctx* ctx, i64 n, int N2, const i32* xbox, const i32* ybox, const 
 // row's extents, strides and offsets the wave's alike; an index inside a box
 // of fewer than 2^24 entries is split in 32 bits (div24).
 constexpr int kCompactChunk = 256;
+int compact_chunk() { return kCompactChunk; }
 __global__ __launch_bounds__(64) void compact_kernel(int mode, i64 nrows, int N2, IArr td, const i32* __restrict__ S,
-                                                     const i64* __restrict__ coff, const i64* __restrict__ boff,
-                                                     const double* __restrict__ src, double* __restrict__ dst) {
+                                                     const i64* __restrict__ coff, const i32* __restrict__ crow,
+                                                     const i64* __restrict__ boff, const double* __restrict__ src,
+                                                     double* __restrict__ dst) {
   const int lane = threadIdx.x;
   const i64 total = coff[nrows];
   for (i64 c = blockIdx.x; c * kCompactChunk < total; c += gridDim.x) {
     const i64 t0 = c * kCompactChunk, t1 = min(total, t0 + kCompactChunk);
-    i64 a = 0, b = nrows - 1;  // last row with coff <= t0
-    while (a < b) {
-      const i64 m = (a + b + 1) >> 1;
-      if (coff[m] <= t0)
-        a = m;
-      else
-        b = m - 1;
+    i64 a = 0, b = nrows - 1;  // last row with coff <= t0: from the layout's table, else searched
+    if (crow) {
+      a = crow[c];
+    } else {
+      while (a < b) {
+        const i64 m = (a + b + 1) >> 1;
+        if (coff[m] <= t0)
+          a = m;
+        else
+          b = m - 1;
+      }
     }
     for (i64 i = a, t = t0; t < t1; ++i) {
       const i64 r0 = coff[i], r1 = min(coff[i + 1], t1);
@@ -379,13 +385,13 @@ __global__ __launch_bounds__(64) void compact_kernel(int mode, i64 nrows, int N2
   }
 }
 void k_compact(Ctx* ctx, int mode, i64 nrows, int N2, const i64* tdims, const i32* S, const i64* coff, i64 total,
-               const i64* boff, const double* src, double* dst) {
+               const i64* boff, const double* src, double* dst, const i32* crow) {
   if (!total || !nrows) return;
   IArr td{};
   for (int q = 0; q < N2; ++q) td.v[q] = tdims[q];
   const i64 chunks = (total + kCompactChunk - 1) / kCompactChunk;
   hipLaunchKernelGGL(compact_kernel, dim3((unsigned)std::min<i64>(chunks, 65536)), dim3(64), 0, ctx->stream, mode,
-                     nrows, N2, td, S, coff, boff, src, dst);
+                     nrows, N2, td, S, coff, crow, boff, src, dst);
   HIP_CHECK(hipGetLastError());
 }
 
