@@ -19,6 +19,8 @@ void k_mdot(Ctx*, const double* V, i64 ldv, int k, const double* w, i64 n, bool 
 void k_orth_update(Ctx*, const double* V, i64 ldv, int k, const double* h, double* w, i64 n, double* part,
                    double* norm_out);
 void k_combine(Ctx*, const double* V, i64 ldv, int kin, const double* U, int kout, double* out, i64 ldo, i64 n);
+bool k_orth_update_dot(Ctx*, const double* V, i64 ldv, int k, const double* h, double* w, i64 n, double* part,
+                       double* out);
 void k_axpby(Ctx*, i64 n, double a, const double* x, double b, double* y);
 void k_gather_blocks_by_key(Ctx*, i64 n, i64 bs, const i32* xids, const i64* off, const double* y, double* out);
 void k_match_keys(Ctx*, const i32* xk, i64 nx, const i32* yk, const i32* yids, i64 ny, int P, const i64* pdims,
@@ -509,8 +511,11 @@ class Lanczos {
     double* nr = h2 + ld_ + 1;
     step(ctx_, "lanczos_orth", [&] {
       k_mdot(ctx_, V_.get(), n_, j_ + 1, w, n_, true, part_.get(), h1);
-      k_orth_update(ctx_, V_.get(), n_, j_ + 1, h1, w, n_, part_.get(), nullptr);
-      k_mdot(ctx_, V_.get(), n_, j_ + 1, w, n_, false, part_.get(), h2);
+      // the first update and the second pass's dot products in one read of the basis
+      if (!k_orth_update_dot(ctx_, V_.get(), n_, j_ + 1, h1, w, n_, part_.get(), h2)) {
+        k_orth_update(ctx_, V_.get(), n_, j_ + 1, h1, w, n_, part_.get(), nullptr);
+        k_mdot(ctx_, V_.get(), n_, j_ + 1, w, n_, false, part_.get(), h2);
+      }
       k_orth_update(ctx_, V_.get(), n_, j_ + 1, h2, w, n_, part_.get(), nr);
     });
     HIP_CHECK(hipMemcpyAsync(hh_.data(), small_.get(), hh_.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -136,6 +136,59 @@ __global__ __launch_bounds__(RT) void orth_update_kernel(const double* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = t;
   }
 }
+// w -= sum_i V_i h[i] (i ascending, as orth_update_kernel) and, in the same
+// pass over the basis, part[i*NB + b] = block b's partial of V_i . w (the new
+// w; the element order and blocking of mdot_kernel), so that the second
+// Gram-Schmidt pass's dot products need no read of their own: the results
+// are those of the two kernels one after the other, bit for bit.  k <= 8 KC;
+// the basis values of an element stay in registers (index k-1 stands in for
+// the absent ones, whose terms are skipped).
+template <int KC>
+__global__ __launch_bounds__(RT) void orth_update_dot_kernel(const double* __restrict__ V, i64 ldv, int k,
+                                                             const double* __restrict__ h, double* __restrict__ w,
+                                                             i64 n, double* __restrict__ part) {
+  constexpr int K = 8 * KC;
+  __shared__ double s[RT];
+  const i64 per = (n + NB - 1) / NB;
+  const i64 lo = (i64)blockIdx.x * per, hi = min(n, lo + per);
+  double hv[K], acc[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    hv[i] = h[min(i, k - 1)];
+    acc[i] = 0.0;
+  }
+  for (i64 e = lo + threadIdx.x; e < hi; e += RT) {
+    double v[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) v[i] = V[(i64)min(i, k - 1) * ldv + e];
+    double we = w[e];
+#pragma unroll
+    for (int i = 0; i < K; ++i) we = i < k ? fma(-v[i], hv[i], we) : we;
+    w[e] = we;
+#pragma unroll
+    for (int i = 0; i < K; ++i) acc[i] = i < k ? fma(v[i], we, acc[i]) : acc[i];
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+    if (i < k) {
+      double t = block_sum(acc[i], s);
+      if (threadIdx.x == 0) part[(i64)i * NB + blockIdx.x] = t;
+    }
+}
+// the fused pass for k <= 32 (false: the caller runs the two kernels)
+bool k_orth_update_dot(Ctx* ctx, const double* V, i64 ldv, int k, const double* h, double* w, i64 n, double* part,
+                       double* out) {
+  if (k < 1 || k > 32) return false;
+  const void* fn = k <= 8 ? (const void*)orth_update_dot_kernel<1>
+                   : k <= 16 ? (const void*)orth_update_dot_kernel<2>
+                   : k <= 24 ? (const void*)orth_update_dot_kernel<3>
+                             : (const void*)orth_update_dot_kernel<4>;
+  void* args[] = {(void*)&V, (void*)&ldv, (void*)&k, (void*)&h, (void*)&w, (void*)&n, (void*)&part};
+  HIP_CHECK(hipLaunchKernel(fn, dim3(NB), dim3(RT), args, 0, ctx->stream));
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(RT), 0, ctx->stream, part, out);
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
 void k_orth_update(Ctx* ctx, const double* V, i64 ldv, int k, const double* h, double* w, i64 n, double* part,
                    double* norm_out) {
   hipLaunchKernelGGL(orth_update_kernel, dim3(NB), dim3(RT), 0, ctx->stream, V, ldv, k, h, w, n,
